@@ -1,0 +1,109 @@
+"""ctypes binding of libfaust_hip.so (the C ABI declared in include/faust_hip.h).
+
+The library is built in-tree (faust_amd/libfaust_hip.so).  There is no CPU
+fallback: importing this module without the built library raises.
+"""
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libfaust_hip.so")
+
+FH_OK, FH_ERR_ARG, FH_ERR_HIP, FH_ERR_NUMERIC, FH_ERR_UNSUPPORTED = 0, 1, 2, 3, 4
+FH_F64, FH_F32, FH_BF16 = 0, 1, 2
+FH_MAT_DENSE, FH_MAT_CSR, FH_MAT_BSR = 0, 1, 2
+PROX = {"sp": 0, "spcol": 1, "splin": 2, "splincol": 3, "supp": 4}
+
+
+class FaustHipError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[faust_hip error {code}] {msg}")
+        self.code = code
+
+
+class fh_constraint(C.Structure):
+    _fields_ = [("name", C.c_int), ("k", C.c_int64), ("nrows", C.c_int), ("ncols", C.c_int),
+                ("normalized", C.c_int), ("pos", C.c_int), ("supp", C.POINTER(C.c_double))]
+
+
+class fh_stop_crit(C.Structure):
+    _fields_ = [("num_its", C.c_int), ("is_criterion_error", C.c_int), ("error_threshold", C.c_double),
+                ("max_num_its", C.c_int)]
+
+
+class fh_palm_params(C.Structure):
+    _fields_ = [("is_update_way_R2L", C.c_int), ("factor_format", C.c_int), ("packing_RL", C.c_int),
+                ("no_normalization", C.c_int), ("no_lambda", C.c_int), ("norm2_max_iter", C.c_int),
+                ("norm2_threshold", C.c_double), ("constant_step_size", C.c_int), ("step_size", C.c_double),
+                ("is_fact_side_left", C.c_int), ("record_traj", C.c_int)]
+
+
+VP = C.c_void_p
+_SIGS = {
+    "fh_last_error": (C.c_char_p, []),
+    "fh_version": (C.c_char_p, []),
+    "fh_dev_count": (C.c_int, [C.POINTER(C.c_int)]),
+    "fh_set_dev": (C.c_int, [C.c_int]),
+    "fh_cur_dev": (C.c_int, [C.POINTER(C.c_int)]),
+    "fh_stream_sync": (C.c_int, [VP]),
+    "fh_mat_dense_create": (C.c_int, [C.c_int, C.c_int, C.c_int, VP, C.c_int, C.c_int, C.POINTER(VP)]),
+    "fh_mat_csr_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, VP, VP, VP, C.c_int, C.POINTER(VP)]),
+    "fh_mat_bsr_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, VP, VP, VP, C.c_int,
+                                    C.POINTER(VP)]),
+    "fh_mat_info": (C.c_int, [VP, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
+                              C.POINTER(C.c_int64)]),
+    "fh_mat_tocpu": (C.c_int, [VP, VP, VP, VP]),
+    "fh_mat_to_dense": (C.c_int, [VP, VP, VP]),
+    "fh_mat_free": (C.c_int, [VP]),
+    "fh_mat_mul_dense": (C.c_int, [VP, C.c_char, VP, C.c_int, C.c_int, VP, C.c_int, VP, VP, VP]),
+    "fh_gemm": (C.c_int, [C.c_int, C.c_char, C.c_char, C.c_int, C.c_int, C.c_int, VP, VP, C.c_int, VP, C.c_int, VP,
+                          VP, C.c_int, VP]),
+    "fh_faust_create": (C.c_int, [C.POINTER(VP)]),
+    "fh_faust_push_back": (C.c_int, [VP, VP]),
+    "fh_faust_size": (C.c_int, [VP, C.POINTER(C.c_int)]),
+    "fh_faust_free": (C.c_int, [VP]),
+    "fh_faust_multiply": (C.c_int, [VP, C.c_char, VP, C.c_int, VP, VP]),
+    "fh_faust_get_product": (C.c_int, [VP, VP, VP]),
+    "fh_faust_spectral_norm": (C.c_int, [VP, C.c_int, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_int), VP]),
+    "fh_prox": (C.c_int, [C.c_int, C.c_int, VP, C.c_int, C.c_int, C.c_int64, C.c_int, C.c_int, VP, VP]),
+    "fh_norm_frob": (C.c_int, [C.c_int, VP, C.c_int64, C.POINTER(C.c_double), VP]),
+    "fh_dot": (C.c_int, [C.c_int, VP, VP, C.c_int64, C.POINTER(C.c_double), VP]),
+    "fh_palm4msa": (C.c_int, [VP, C.c_int, C.c_int, C.POINTER(fh_constraint), C.c_int, C.POINTER(fh_stop_crit),
+                              C.POINTER(fh_palm_params), C.c_double, VP, C.POINTER(VP)]),
+    "fh_hierarchical": (C.c_int, [VP, C.c_int, C.c_int, C.POINTER(fh_constraint), C.POINTER(fh_constraint), C.c_int,
+                                  C.POINTER(fh_stop_crit), C.POINTER(fh_stop_crit), C.POINTER(fh_palm_params),
+                                  C.POINTER(VP)]),
+    "fh_palm_result_nfacts": (C.c_int, [VP, C.POINTER(C.c_int)]),
+    "fh_palm_result_lambda": (C.c_int, [VP, C.POINTER(C.c_double)]),
+    "fh_palm_result_factor": (C.c_int, [VP, C.c_int, C.POINTER(VP)]),
+    "fh_palm_result_traj": (C.c_int, [VP, VP, C.POINTER(C.c_int)]),
+    "fh_palm_result_free": (C.c_int, [VP]),
+}
+
+_lib = None
+
+
+def exported_names():
+    return list(_SIGS.keys())
+
+
+def lib():
+    """Load libfaust_hip.so (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: build it first (python -c 'import __graft_entry__ as g; "
+                              f"g.build()' or make -C faust_amd/csrc)")
+        l = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(l, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = l
+    return _lib
+
+
+def check(rc):
+    if rc != FH_OK:
+        raise FaustHipError(rc, lib().fh_last_error().decode())
+    return rc

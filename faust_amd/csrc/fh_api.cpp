@@ -1,0 +1,383 @@
+// fh_api.cpp -- C ABI: matrix handles, products, Faust chains, projectors.
+#include <algorithm>
+#include <vector>
+
+#include "fh_api_common.hpp"
+
+using namespace fh;
+
+namespace fh {
+
+// Host-side CSR -> CSC (counting sort by column, stable in row order, so rows
+// ascend inside each column).
+static void host_csc(int m, int n, const int32_t* rp, const int32_t* ci, const void* vals, size_t esz,
+                     std::vector<int>& cp, std::vector<int>& ri, std::vector<char>& cv) {
+  const int64_t nnz = rp[m];
+  cp.assign(n + 1, 0);
+  ri.resize(nnz);
+  cv.resize(nnz * esz);
+  for (int64_t p = 0; p < nnz; ++p) cp[ci[p] + 1]++;
+  for (int j = 0; j < n; ++j) cp[j + 1] += cp[j];
+  std::vector<int> fill(cp.begin(), cp.end() - 1);
+  const char* v = static_cast<const char*>(vals);
+  for (int i = 0; i < m; ++i)
+    for (int p = rp[i]; p < rp[i + 1]; ++p) {
+      int q = fill[ci[p]]++;
+      ri[q] = i;
+      std::memcpy(&cv[q * esz], v + p * esz, esz);
+    }
+}
+
+DMatP make_csr_from_host(int dtype, int m, int n, int nnz, const int32_t* rp, const int32_t* ci, const void* vals,
+                         hipStream_t st) {
+  FH_CHECK(m >= 0 && n >= 0 && nnz >= 0, "csr: negative size");
+  FH_CHECK(rp[0] == 0 && rp[m] == nnz, "csr: rowptr inconsistent with nnz");
+  // columns must be sorted and in range inside each row (canonical k order)
+  for (int i = 0; i < m; ++i) {
+    FH_CHECK(rp[i] <= rp[i + 1], "csr: rowptr not non-decreasing");
+    for (int p = rp[i]; p < rp[i + 1]; ++p) {
+      FH_CHECK(ci[p] >= 0 && ci[p] < n, "csr: column index out of range");
+      FH_CHECK(p == rp[i] || ci[p] > ci[p - 1], "csr: column indices must be strictly increasing in each row");
+    }
+  }
+  const size_t esz = dtype_size(dtype);
+  auto M = std::make_shared<DMat>();
+  M->kind = K_CSR;
+  M->dtype = dtype;
+  M->m = m;
+  M->n = n;
+  M->nnz = nnz;
+  std::vector<int> cp, ri;
+  std::vector<char> cv;
+  host_csc(m, n, rp, ci, vals, esz, cp, ri, cv);
+  M->rp.alloc(sizeof(int) * (m + 1));
+  M->ci.alloc(sizeof(int) * std::max(nnz, 1));
+  M->v.alloc(esz * std::max(nnz, 1));
+  M->cp.alloc(sizeof(int) * (n + 1));
+  M->ri.alloc(sizeof(int) * std::max(nnz, 1));
+  M->cv.alloc(esz * std::max(nnz, 1));
+  FH_HIP(hipMemcpyAsync(M->rp.p, rp, sizeof(int) * (m + 1), hipMemcpyHostToDevice, st));
+  FH_HIP(hipMemcpyAsync(M->cp.p, cp.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, st));
+  if (nnz > 0) {
+    FH_HIP(hipMemcpyAsync(M->ci.p, ci, sizeof(int) * nnz, hipMemcpyHostToDevice, st));
+    FH_HIP(hipMemcpyAsync(M->v.p, vals, esz * nnz, hipMemcpyHostToDevice, st));
+    FH_HIP(hipMemcpyAsync(M->ri.p, ri.data(), sizeof(int) * nnz, hipMemcpyHostToDevice, st));
+    FH_HIP(hipMemcpyAsync(M->cv.p, cv.data(), esz * nnz, hipMemcpyHostToDevice, st));
+  }
+  M->has_csc = true;
+  FH_HIP(hipStreamSynchronize(st));
+  return M;
+}
+
+DMatP make_dense(int dtype, int m, int n, const void* data, int ld, bool on_device, hipStream_t st) {
+  FH_CHECK(m >= 0 && n >= 0 && ld >= std::max(m, 1), "dense: bad shape or leading dimension");
+  const size_t esz = dtype_size(dtype);
+  auto M = std::make_shared<DMat>();
+  M->kind = K_DENSE;
+  M->dtype = dtype;
+  M->m = m;
+  M->n = n;
+  M->nnz = -1;
+  M->d.alloc(esz * std::max<size_t>((size_t)m * n, 1));
+  if (m > 0 && n > 0 && data)
+    FH_HIP(hipMemcpy2DAsync(M->d.p, esz * m, data, esz * ld, esz * m, n,
+                            on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st));
+  FH_HIP(hipStreamSynchronize(st));
+  return M;
+}
+
+// Y = op(F) X for an FP64 chain; X, Y col-major with ld = rows.
+void faust_multiply_f64(const std::vector<DMatP>& f, char op, const double* X, int p, double* Y, hipStream_t st) {
+  FH_CHECK(!f.empty(), "Faust::multiply: empty Faust");
+  const int J = (int)f.size();
+  int maxr = 0;
+  for (auto& x : f) maxr = std::max(maxr, op == 'N' ? x->m : x->n);
+  DBuf t1, t2;
+  if (J > 1) t1.alloc(sizeof(double) * std::max<size_t>((size_t)maxr * p, 1));
+  if (J > 2) t2.alloc(sizeof(double) * std::max<size_t>((size_t)maxr * p, 1));
+  const double* in = X;
+  int in_rows = (op == 'N') ? f.back()->n : f.front()->m;
+  for (int i = 0; i < J; ++i) {
+    const DMat& S = (op == 'N') ? *f[J - 1 - i] : *f[i];
+    const int out_rows = (op == 'N') ? S.m : S.n;
+    FH_CHECK(((op == 'N') ? S.n : S.m) == in_rows, "Faust::multiply: factor dimensions do not chain");
+    double* out = (i == J - 1) ? Y : ((i & 1) ? t2.as<double>() : t1.as<double>());
+    FH_CHECK(S.dtype == FH_F64, "Faust::multiply: FP64 chain expected");
+    if (S.kind == K_CSR)
+      csr_spmm_f64(S, op, in, in_rows, p, out, out_rows, 1.0, 0.0, st);
+    else if (S.kind == K_DENSE)
+      gemm_f64(op == 'N' ? 'N' : 'T', 'N', out_rows, p, in_rows, 1.0, S.d.as<double>(), S.m, in, in_rows, 0.0, out,
+               out_rows, st);
+    else
+      throw Error(FH_ERR_UNSUPPORTED, "Faust::multiply: BSR factors need the FP32/BF16 chain");
+    in = out;
+    in_rows = out_rows;
+  }
+  if (J > 1) FH_HIP(hipStreamSynchronize(st));  // the intermediates return to the pool
+}
+
+// Transform::get_product (faust_Transform.hpp:599-626)
+DMatP faust_get_product_f64(const std::vector<DMatP>& f, hipStream_t st) {
+  FH_CHECK(!f.empty(), "get_product: empty Faust");
+  const size_t n = f.size();
+  if (n == 1) return dense_of(*f[0], st);
+  auto mat = std::make_shared<DMat>();
+  matmul_f64(*f[n - 2], 'N', *f[n - 1], 'N', 1.0, 0.0, *mat, st);
+  for (int i = (int)n - 3; i >= 0; --i) {
+    auto nm = std::make_shared<DMat>();
+    matmul_f64(*f[i], 'N', *mat, 'N', 1.0, 0.0, *nm, st);
+    mat = nm;
+  }
+  return mat;
+}
+
+static bool chain_is_zero(const std::vector<DMatP>& f, hipStream_t st) {
+  for (auto& x : f)
+    if (nnz_of(*x, st) == 0) return true;
+  return false;
+}
+
+// Transform::spectralNorm (faust_Transform.hpp:684-709)
+double faust_spectral_norm_f64(const std::vector<DMatP>& f, int maxit, double thr, int* flag, hipStream_t st) {
+  if (f.empty()) return 1.0;
+  if (chain_is_zero(f, st)) return 0.0;
+  const int nr = f.front()->m, nc = f.back()->n;
+  std::vector<ChainOp> ops;
+  int n;
+  if (nc < nr) {
+    for (int i = (int)f.size() - 1; i >= 0; --i) ops.push_back({f[i].get(), 'T'});
+    for (auto& x : f) ops.push_back({x.get(), 'N'});
+    n = nc;
+  } else {
+    for (auto& x : f) ops.push_back({x.get(), 'N'});
+    for (int i = (int)f.size() - 1; i >= 0; --i) ops.push_back({f[i].get(), 'T'});
+    n = nr;
+  }
+  double lam = power_iteration_f64(ops, n, maxit, thr, flag, st);
+  return std::fabs(std::sqrt(lam));
+}
+
+}  // namespace fh
+
+extern "C" {
+
+int fh_mat_dense_create(int dtype, int nrows, int ncols, const void* data, int ld, int src_on_device, fh_mat_t* out) {
+  FH_API_BEGIN
+  FH_CHECK(out, "null output handle");
+  FH_CHECK(dtype == FH_F64 || dtype == FH_F32 || dtype == FH_BF16, "unknown dtype");
+  auto M = make_dense(dtype, nrows, ncols, data, ld, src_on_device != 0, nullptr);
+  *out = new fh_mat_s{M};
+  FH_API_END
+}
+
+int fh_mat_csr_create(int dtype, int nrows, int ncols, int nnz, const int32_t* rowptr, const int32_t* colind,
+                      const void* vals, int src_on_device, fh_mat_t* out) {
+  FH_API_BEGIN
+  FH_CHECK(out && rowptr && (nnz == 0 || (colind && vals)), "null argument");
+  FH_CHECK(dtype == FH_F64 || dtype == FH_F32 || dtype == FH_BF16, "unknown dtype");
+  const size_t esz = dtype_size(dtype);
+  if (src_on_device) {
+    std::vector<int32_t> rp(nrows + 1), ci(nnz);
+    std::vector<char> v(esz * nnz);
+    FH_HIP(hipMemcpy(rp.data(), rowptr, sizeof(int) * (nrows + 1), hipMemcpyDeviceToHost));
+    if (nnz) {
+      FH_HIP(hipMemcpy(ci.data(), colind, sizeof(int) * nnz, hipMemcpyDeviceToHost));
+      FH_HIP(hipMemcpy(v.data(), vals, esz * nnz, hipMemcpyDeviceToHost));
+    }
+    *out = new fh_mat_s{make_csr_from_host(dtype, nrows, ncols, nnz, rp.data(), ci.data(), v.data(), nullptr)};
+  } else {
+    *out = new fh_mat_s{make_csr_from_host(dtype, nrows, ncols, nnz, rowptr, colind, vals, nullptr)};
+  }
+  FH_API_END
+}
+
+int fh_mat_info(fh_mat_t M, int* kind, int* dtype, int* nrows, int* ncols, int64_t* nnz) {
+  FH_API_BEGIN
+  FH_CHECK(M, "null handle");
+  const DMat& d = *M->m;
+  if (kind) *kind = (int)d.kind;
+  if (dtype) *dtype = d.dtype;
+  if (nrows) *nrows = d.m;
+  if (ncols) *ncols = d.n;
+  if (nnz) {
+    if (d.kind == K_DENSE && d.dtype == FH_F64)
+      *nnz = nnz_of(*M->m, nullptr);
+    else if (d.kind == K_BSR)
+      *nnz = (int64_t)d.bnnz * d.bm * d.bn;
+    else
+      *nnz = d.nnz;
+  }
+  FH_API_END
+}
+
+int fh_mat_tocpu(fh_mat_t M, void* data_or_vals, int32_t* rowptr, int32_t* colind) {
+  FH_API_BEGIN
+  FH_CHECK(M, "null handle");
+  const DMat& d = *M->m;
+  const size_t esz = dtype_size(d.dtype);
+  if (d.kind == K_DENSE) {
+    if (d.m * (size_t)d.n)
+      FH_HIP(hipMemcpy(data_or_vals, d.d.p, esz * (size_t)d.m * d.n, hipMemcpyDeviceToHost));
+  } else if (d.kind == K_CSR) {
+    if (rowptr) FH_HIP(hipMemcpy(rowptr, d.rp.p, sizeof(int) * (d.m + 1), hipMemcpyDeviceToHost));
+    if (d.nnz) {
+      if (colind) FH_HIP(hipMemcpy(colind, d.ci.p, sizeof(int) * d.nnz, hipMemcpyDeviceToHost));
+      if (data_or_vals) FH_HIP(hipMemcpy(data_or_vals, d.v.p, esz * d.nnz, hipMemcpyDeviceToHost));
+    }
+  } else {
+    throw Error(FH_ERR_UNSUPPORTED, "tocpu: use fh_mat_to_dense for BSR handles");
+  }
+  FH_API_END
+}
+
+int fh_mat_to_dense(fh_mat_t M, void* dense_dev, void* stream) {
+  FH_API_BEGIN
+  FH_CHECK(M && dense_dev, "null argument");
+  hipStream_t st = (hipStream_t)stream;
+  const DMat& d = *M->m;
+  if (d.dtype == FH_F64) {
+    if (d.kind == K_DENSE)
+      FH_HIP(hipMemcpyAsync(dense_dev, d.d.p, sizeof(double) * (size_t)d.m * d.n, hipMemcpyDeviceToDevice, st));
+    else if (d.kind == K_CSR)
+      csr_to_dense_f64(d, (double*)dense_dev, st);
+    else
+      throw Error(FH_ERR_UNSUPPORTED, "to_dense: FP64 BSR not supported");
+  } else {
+    mixed_to_dense(d, dense_dev, st);
+  }
+  FH_API_END
+}
+
+int fh_mat_free(fh_mat_t M) {
+  FH_API_BEGIN
+  delete M;
+  FH_API_END
+}
+
+int fh_mat_mul_dense(fh_mat_t M, char op, const void* X, int ldx, int ncols, void* Y, int ldy, const void* alpha,
+                     const void* beta, void* stream) {
+  FH_API_BEGIN
+  FH_CHECK(M && X && Y, "null argument");
+  FH_CHECK(op == 'N' || op == 'T' || op == 'H', "op must be N, T or H");
+  hipStream_t st = (hipStream_t)stream;
+  const DMat& S = *M->m;
+  if (S.dtype == FH_F64) {
+    const double a = alpha ? *(const double*)alpha : 1.0, b = beta ? *(const double*)beta : 0.0;
+    const int orows = op == 'N' ? S.m : S.n, irows = op == 'N' ? S.n : S.m;
+    FH_CHECK(ldx >= std::max(irows, 1) && ldy >= std::max(orows, 1), "bad leading dimension");
+    if (S.kind == K_CSR)
+      csr_spmm_f64(S, op, (const double*)X, ldx, ncols, (double*)Y, ldy, a, b, st);
+    else if (S.kind == K_DENSE)
+      gemm_f64(op == 'N' ? 'N' : 'T', 'N', orows, ncols, irows, a, S.d.as<double>(), S.m, (const double*)X, ldx, b,
+               (double*)Y, ldy, st);
+    else
+      throw Error(FH_ERR_UNSUPPORTED, "FP64 BSR products are not supported");
+  } else {
+    mixed_mul_dense(S, op, X, ldx, ncols, Y, ldy, alpha, beta, st);
+  }
+  FH_API_END
+}
+
+int fh_gemm(int dtype, char opA, char opB, int m, int n, int k, const void* alpha, const void* A, int lda,
+            const void* B, int ldb, const void* beta, void* C, int ldc, void* stream) {
+  FH_API_BEGIN
+  FH_CHECK(dtype == FH_F64, "fh_gemm: FP64 only");
+  FH_CHECK(m >= 0 && n >= 0 && k >= 0, "fh_gemm: negative size");
+  gemm_f64(opA, opB, m, n, k, alpha ? *(const double*)alpha : 1.0, (const double*)A, lda, (const double*)B, ldb,
+           beta ? *(const double*)beta : 0.0, (double*)C, ldc, (hipStream_t)stream);
+  FH_API_END
+}
+
+int fh_faust_create(fh_faust_t* out) {
+  FH_API_BEGIN
+  FH_CHECK(out, "null output handle");
+  *out = new fh_faust_s();
+  FH_API_END
+}
+
+int fh_faust_push_back(fh_faust_t F, fh_mat_t M) {
+  FH_API_BEGIN
+  FH_CHECK(F && M, "null handle");
+  if (!F->f.empty()) FH_CHECK(F->f.back()->n == M->m->m, "push_back: dimensions do not chain");
+  F->f.push_back(M->m);
+  FH_API_END
+}
+
+int fh_faust_size(fh_faust_t F, int* n) {
+  FH_API_BEGIN
+  FH_CHECK(F && n, "null argument");
+  *n = (int)F->f.size();
+  FH_API_END
+}
+
+int fh_faust_free(fh_faust_t F) {
+  FH_API_BEGIN
+  delete F;
+  FH_API_END
+}
+
+int fh_faust_multiply(fh_faust_t F, char op, const void* X, int p, void* Y, void* stream) {
+  FH_API_BEGIN
+  FH_CHECK(F && X && Y, "null argument");
+  FH_CHECK(op == 'N' || op == 'T' || op == 'H', "op must be N, T or H");
+  FH_CHECK(!F->f.empty(), "Faust::multiply: empty Faust");
+  FH_CHECK(p >= 0, "negative column count");
+  hipStream_t st = (hipStream_t)stream;
+  if (F->f.front()->dtype == FH_F64)
+    faust_multiply_f64(F->f, op, (const double*)X, p, (double*)Y, st);
+  else
+    faust_multiply_mixed(F->f, op, X, p, Y, st);
+  FH_API_END
+}
+
+int fh_faust_get_product(fh_faust_t F, void* out_dev, void* stream) {
+  FH_API_BEGIN
+  FH_CHECK(F && out_dev, "null argument");
+  hipStream_t st = (hipStream_t)stream;
+  DMatP P = faust_get_product_f64(F->f, st);
+  FH_HIP(hipMemcpyAsync(out_dev, P->d.p, sizeof(double) * (size_t)P->m * P->n, hipMemcpyDeviceToDevice, st));
+  FH_HIP(hipStreamSynchronize(st));
+  FH_API_END
+}
+
+int fh_faust_spectral_norm(fh_faust_t F, int max_iter, double threshold, double* norm, int* flag, void* stream) {
+  FH_API_BEGIN
+  FH_CHECK(F && norm, "null argument");
+  *norm = faust_spectral_norm_f64(F->f, max_iter, threshold, flag, (hipStream_t)stream);
+  FH_API_END
+}
+
+int fh_prox(int dtype, int name, void* M, int nrows, int ncols, int64_t k, int normalized, int pos, const void* supp,
+            void* stream) {
+  FH_API_BEGIN
+  FH_CHECK(dtype == FH_F64, "fh_prox: FP64 only");
+  FH_CHECK(M && nrows >= 0 && ncols >= 0, "bad argument");
+  prox_f64(name, (double*)M, nrows, ncols, k, normalized != 0, pos != 0, (const double*)supp, (hipStream_t)stream);
+  FH_API_END
+}
+
+int fh_norm_frob(int dtype, const void* M, int64_t n, double* out, void* stream) {
+  FH_API_BEGIN
+  FH_CHECK(dtype == FH_F64 && out, "fh_norm_frob: FP64 only");
+  hipStream_t st = (hipStream_t)stream;
+  DBuf s(sizeof(double));
+  reduce_dot_f64((const double*)M, (const double*)M, n, s.as<double>(), st);
+  double h;
+  FH_HIP(hipMemcpyAsync(&h, s.p, sizeof(double), hipMemcpyDeviceToHost, st));
+  FH_HIP(hipStreamSynchronize(st));
+  *out = std::sqrt(h);
+  FH_API_END
+}
+
+int fh_dot(int dtype, const void* x, const void* y, int64_t n, double* out, void* stream) {
+  FH_API_BEGIN
+  FH_CHECK(dtype == FH_F64 && out, "fh_dot: FP64 only");
+  hipStream_t st = (hipStream_t)stream;
+  DBuf s(sizeof(double));
+  reduce_dot_f64((const double*)x, (const double*)y, n, s.as<double>(), st);
+  FH_HIP(hipMemcpyAsync(out, s.p, sizeof(double), hipMemcpyDeviceToHost, st));
+  FH_HIP(hipStreamSynchronize(st));
+  FH_API_END
+}
+
+}  // extern "C"

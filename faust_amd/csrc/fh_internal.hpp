@@ -1,0 +1,160 @@
+// fh_internal.hpp -- shared host/device declarations of libfaust_hip.
+//
+// Numerics of the FP64 path ("canonical order", shared with the oracle):
+//   * a product entry is acc = 0; for k ascending: acc = fma(a_ik, b_kj, acc);
+//     terms with an exact-zero operand may be skipped (fma(0,b,acc) == acc for
+//     finite b), so sparse and dense formulations give the same bits;
+//   * epilogue t = alpha*acc; out = (beta == 0) ? t : fma(beta, c_old, t);
+//   * sums/dots: 4096-element chunks, 256 lanes per chunk (lane t takes
+//     elements t, t+256, ...), a halving tree p[t] = p[t] + p[t+s] for
+//     s = 128..1, then chunk partials added in chunk order by one thread.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <hip/hip_bf16.h>
+
+#include <cstdint>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/faust_hip.h"
+
+namespace fh {
+
+constexpr int RED_CHUNK = 4096;
+constexpr int RED_LANES = 256;
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+void set_last_error(const std::string& s);
+
+#define FH_HIP(call)                                                                        \
+  do {                                                                                      \
+    hipError_t e_ = (call);                                                                 \
+    if (e_ != hipSuccess)                                                                   \
+      throw ::fh::Error(FH_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_));     \
+  } while (0)
+
+#define FH_CHECK(cond, msg)                                \
+  do {                                                     \
+    if (!(cond)) throw ::fh::Error(FH_ERR_ARG, (msg));     \
+  } while (0)
+
+#define FH_LAUNCH_CHECK() FH_HIP(hipGetLastError())
+
+// ------------------------------------------------------------ device memory
+// A size-class caching pool: freed blocks are kept for reuse so that the PALM
+// and chain loops do not call hipMalloc/hipFree per product.
+void* dev_alloc(size_t bytes);
+void dev_free(void* p);
+void pool_trim();
+
+struct DBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DBuf() = default;
+  explicit DBuf(size_t b) { alloc(b); }
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  DBuf(DBuf&& o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+  DBuf& operator=(DBuf&& o) noexcept {
+    if (this != &o) {
+      release();
+      p = o.p;
+      bytes = o.bytes;
+      o.p = nullptr;
+      o.bytes = 0;
+    }
+    return *this;
+  }
+  ~DBuf() { release(); }
+  void alloc(size_t b) {
+    release();
+    if (b) p = dev_alloc(b);
+    bytes = b;
+  }
+  void release() {
+    if (p) dev_free(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+inline size_t dtype_size(int dt) { return dt == FH_F64 ? 8 : dt == FH_F32 ? 4 : 2; }
+
+// ------------------------------------------------------------ matrices
+enum Kind { K_DENSE = 0, K_CSR = 1, K_BSR = 2 };
+
+// A device matrix.  Dense: column-major, ld = m.  CSR: int32 rowptr/colind,
+// columns sorted inside each row; its CSC twin (for op T/H products) is built
+// on the device the first time it is needed.  BSR: see k_bsr.hip.
+struct DMat {
+  Kind kind = K_DENSE;
+  int dtype = FH_F64;
+  int m = 0, n = 0;
+  DBuf d;                 // dense values
+  DBuf rp, ci, v;         // CSR
+  int64_t nnz = 0;        // CSR: stored entries; dense: -1 until counted
+  mutable bool has_csc = false;
+  mutable DBuf cp, ri, cv; // CSC
+  // BSR
+  int bm = 0, bn = 0, bnnz = 0;
+  DBuf brp, bci, bdat;    // bdat in the MFMA-ready layout of k_bsr.hip
+  bool is_identity = false;
+};
+using DMatP = std::shared_ptr<DMat>;
+
+// ------------------------------------------------------------ kernels (FP64 canonical)
+// Y = alpha*op(S)*X + beta*Y for CSR S, dense col-major X/Y.
+void csr_spmm_f64(const DMat& S, char op, const double* X, int ldx, int ncols, double* Y, int ldy, double alpha,
+                  double beta, hipStream_t st);
+// C = alpha*op(A)*op(S) + beta*C, A dense col-major, S CSR.
+void dense_csr_f64(const double* A, int lda, char opA, int m, const DMat& S, char opS, double* C, int ldc,
+                   double alpha, double beta, hipStream_t st);
+// C = alpha*op(A)*op(B) + beta*C, all dense col-major.
+void gemm_f64(char opA, char opB, int m, int n, int k, double alpha, const double* A, int lda, const double* B,
+              int ldb, double beta, double* C, int ldc, hipStream_t st);
+void build_csc(const DMat& S, hipStream_t st);
+
+// canonical reductions; result written to a device double
+void reduce_dot_f64(const double* x, const double* y, int64_t n, double* out, hipStream_t st);
+void reduce_sum_f64(const double* x, int64_t n, double* out, hipStream_t st);
+void diag_dot_f64(const double* A, const double* S, int m, int n, double* diag, hipStream_t st);
+// dense -> CSR (drops exact zeros), nnz returned on host (synchronizes)
+void dense_to_csr_f64(const double* D, int m, int n, DMat& out, hipStream_t st);
+void csr_to_dense_f64(const DMat& S, double* D, hipStream_t st);
+int64_t count_nnz_f64(const double* D, int64_t n, hipStream_t st);
+// elementwise helpers
+void axpby_f64(double* y, const double* x, int64_t n, double a, double b, hipStream_t st); // y = a*x + b*y canonical
+void scale_f64(double* x, int64_t n, const double* dscale, double hscale, int mode, hipStream_t st);
+void sub_f64(double* y, const double* x, int64_t n, hipStream_t st);  // y = y - x
+void fill_f64(double* x, int64_t n, double v, hipStream_t st);
+void eye_f64(double* x, int m, int n, hipStream_t st);
+// projectors on dense col-major M (k per row/col/global); exact partial_sort tie semantics
+void prox_f64(int name, double* M, int m, int n, int64_t k, bool normalized, bool pos, const double* supp,
+              hipStream_t st);
+
+// power iteration over a chain of (factor, op) applied last to first.
+struct ChainOp {
+  const DMat* M;
+  char op;
+};
+double power_iteration_f64(const std::vector<ChainOp>& ops, int n, int maxit, double thr, int* flag,
+                           hipStream_t st);
+
+// host-side start vector: srand(0xF4+'u'+57), Random on [-1, 1]
+const std::vector<double>& power_x0(int n);
+
+// generic FP64 product dispatch: out = alpha*op(A)*op(B) + beta*out (dense out)
+void matmul_f64(const DMat& A, char opA, const DMat& B, char opB, double alpha, double beta, DMat& out,
+                hipStream_t st);
+DMatP dense_of(const DMat& A, hipStream_t st);
+int64_t nnz_of(DMat& A, hipStream_t st);
+
+}  // namespace fh

@@ -1,0 +1,277 @@
+// k_reduce.hip -- canonical FP64 reductions, elementwise kernels and
+// dense <-> CSR/CSC conversions.
+#include <algorithm>
+#include <numeric>
+
+#include "fh_internal.hpp"
+
+namespace fh {
+
+// ---------------------------------------------------------------- reductions
+// One workgroup of 256 lanes per 4096-element chunk; lane t accumulates
+// elements c*4096 + t + 256*m (m = 0..15), then a halving tree.
+template <int MODE>  // 0: sum x, 1: dot x.y
+__global__ __launch_bounds__(256) void k_reduce_chunks(const double* __restrict__ x, const double* __restrict__ y,
+                                                       int64_t n, double* __restrict__ part) {
+  __shared__ double p[RED_LANES];
+  const int t = threadIdx.x;
+  const int64_t base = (int64_t)blockIdx.x * RED_CHUNK;
+  double acc = 0.0;
+#pragma unroll
+  for (int m = 0; m < RED_CHUNK / RED_LANES; ++m) {
+    int64_t i = base + (int64_t)m * RED_LANES + t;
+    if (i < n) acc = MODE ? fma(x[i], y[i], acc) : acc + x[i];
+  }
+  p[t] = acc;
+  __syncthreads();
+  for (int s = RED_LANES / 2; s >= 1; s >>= 1) {
+    if (t < s) p[t] = p[t] + p[t + s];
+    __syncthreads();
+  }
+  if (t == 0) part[blockIdx.x] = p[0];
+}
+
+__global__ void k_reduce_final(const double* __restrict__ part, int64_t nchunks, double* __restrict__ out) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    double tot = 0.0;
+    for (int64_t c = 0; c < nchunks; ++c) tot = tot + part[c];
+    *out = tot;
+  }
+}
+
+static void reduce_launch(int mode, const double* x, const double* y, int64_t n, double* out, hipStream_t st) {
+  if (n <= 0) {
+    FH_HIP(hipMemsetAsync(out, 0, sizeof(double), st));
+    return;
+  }
+  int64_t nch = (n + RED_CHUNK - 1) / RED_CHUNK;
+  DBuf part(sizeof(double) * nch);
+  if (mode)
+    hipLaunchKernelGGL(k_reduce_chunks<1>, dim3((unsigned)nch), dim3(RED_LANES), 0, st, x, y, n, part.as<double>());
+  else
+    hipLaunchKernelGGL(k_reduce_chunks<0>, dim3((unsigned)nch), dim3(RED_LANES), 0, st, x, y, n, part.as<double>());
+  FH_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(64), 0, st, part.as<double>(), nch, out);
+  FH_LAUNCH_CHECK();
+}
+
+void reduce_dot_f64(const double* x, const double* y, int64_t n, double* out, hipStream_t st) {
+  reduce_launch(1, x, y, n, out, st);
+}
+void reduce_sum_f64(const double* x, int64_t n, double* out, hipStream_t st) { reduce_launch(0, x, nullptr, n, out, st); }
+
+// diag[i] = sum_k A[k,i]*S[k,i] in ascending k (the diagonal of A^T S).
+__global__ void k_diag_dot(const double* __restrict__ A, const double* __restrict__ S, int m, int n,
+                           double* __restrict__ diag) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double* a = A + (size_t)i * m;
+  const double* s = S + (size_t)i * m;
+  double acc = 0.0;
+  for (int k = 0; k < m; ++k) acc = fma(a[k], s[k], acc);
+  diag[i] = acc;
+}
+
+void diag_dot_f64(const double* A, const double* S, int m, int n, double* diag, hipStream_t st) {
+  hipLaunchKernelGGL(k_diag_dot, dim3((n + 255) / 256), dim3(256), 0, st, A, S, m, n, diag);
+  FH_LAUNCH_CHECK();
+}
+
+// ---------------------------------------------------------------- elementwise
+__global__ void k_axpby(double* __restrict__ y, const double* __restrict__ x, int64_t n, double a, double b) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    double t = a * x[i];
+    y[i] = (b == 0.0) ? t : fma(b, y[i], t);
+  }
+}
+void axpby_f64(double* y, const double* x, int64_t n, double a, double b, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_axpby, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, y, x, n, a, b);
+  FH_LAUNCH_CHECK();
+}
+
+// mode 0: x *= hscale; mode 1: x *= 1/sqrt(*dscale) (Frobenius normalisation
+// from a device sum of squares: nrm = sqrt(s), inv = 1.0/nrm, x *= inv)
+__global__ void k_scale(double* __restrict__ x, int64_t n, const double* __restrict__ ds, double hs, int mode) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double s = hs;
+  if (mode == 1) {
+    double nrm = sqrt(*ds);
+    s = 1.0 / nrm;
+  }
+  x[i] = x[i] * s;
+}
+void scale_f64(double* x, int64_t n, const double* dscale, double hscale, int mode, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_scale, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, x, n, dscale, hscale, mode);
+  FH_LAUNCH_CHECK();
+}
+
+__global__ void k_sub(double* __restrict__ y, const double* __restrict__ x, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = y[i] - x[i];
+}
+void sub_f64(double* y, const double* x, int64_t n, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_sub, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, y, x, n);
+  FH_LAUNCH_CHECK();
+}
+
+__global__ void k_fill(double* __restrict__ x, int64_t n, double v) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = v;
+}
+void fill_f64(double* x, int64_t n, double v, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, x, n, v);
+  FH_LAUNCH_CHECK();
+}
+
+__global__ void k_eye(double* __restrict__ x, int m, int n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < (int64_t)m * n) {
+    int r = (int)(i % m), c = (int)(i / m);
+    x[i] = (r == c) ? 1.0 : 0.0;
+  }
+}
+void eye_f64(double* x, int m, int n, hipStream_t st) {
+  int64_t N = (int64_t)m * n;
+  if (N <= 0) return;
+  hipLaunchKernelGGL(k_eye, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st, x, m, n);
+  FH_LAUNCH_CHECK();
+}
+
+// ---------------------------------------------------------------- conversions
+// per-row and per-column nonzero counts of a dense col-major matrix
+__global__ void k_count_rows(const double* __restrict__ D, int m, int n, int* __restrict__ cnt) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  int c = 0;
+  for (int j = 0; j < n; ++j) c += (D[(size_t)j * m + i] != 0.0);
+  cnt[i] = c;
+}
+__global__ void k_count_cols(const double* __restrict__ D, int m, int n, int* __restrict__ cnt) {
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const double* col = D + (size_t)j * m;
+  int c = 0;
+  for (int i = 0; i < m; ++i) c += (col[i] != 0.0);
+  cnt[j] = c;
+}
+// CSR fill: row i scans columns ascending (coalesced across rows)
+__global__ void k_fill_csr(const double* __restrict__ D, int m, int n, const int* __restrict__ rp,
+                           int* __restrict__ ci, double* __restrict__ v) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  int p = rp[i];
+  for (int j = 0; j < n; ++j) {
+    double x = D[(size_t)j * m + i];
+    if (x != 0.0) {
+      ci[p] = j;
+      v[p] = x;
+      ++p;
+    }
+  }
+}
+// CSC fill: column j scans rows ascending
+__global__ void k_fill_csc(const double* __restrict__ D, int m, int n, const int* __restrict__ cp,
+                           int* __restrict__ ri, double* __restrict__ v) {
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const double* col = D + (size_t)j * m;
+  int p = cp[j];
+  for (int i = 0; i < m; ++i) {
+    double x = col[i];
+    if (x != 0.0) {
+      ri[p] = i;
+      v[p] = x;
+      ++p;
+    }
+  }
+}
+
+static std::vector<int> scan_counts(const int* dcnt, int len, hipStream_t st) {
+  std::vector<int> h(len + 1, 0);
+  FH_HIP(hipMemcpyAsync(h.data() + 1, dcnt, sizeof(int) * len, hipMemcpyDeviceToHost, st));
+  FH_HIP(hipStreamSynchronize(st));
+  for (int i = 0; i < len; ++i) h[i + 1] += h[i];
+  return h;
+}
+
+void dense_to_csr_f64(const double* D, int m, int n, DMat& out, hipStream_t st) {
+  out.kind = K_CSR;
+  out.dtype = FH_F64;
+  out.m = m;
+  out.n = n;
+  DBuf cr(sizeof(int) * std::max(m, 1)), cc(sizeof(int) * std::max(n, 1));
+  if (m > 0 && n > 0) {
+    hipLaunchKernelGGL(k_count_rows, dim3((m + 255) / 256), dim3(256), 0, st, D, m, n, cr.as<int>());
+    FH_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_count_cols, dim3((n + 255) / 256), dim3(256), 0, st, D, m, n, cc.as<int>());
+    FH_LAUNCH_CHECK();
+  }
+  std::vector<int> hr = m > 0 && n > 0 ? scan_counts(cr.as<int>(), m, st) : std::vector<int>(m + 1, 0);
+  std::vector<int> hc = m > 0 && n > 0 ? scan_counts(cc.as<int>(), n, st) : std::vector<int>(n + 1, 0);
+  int64_t nnz = hr[m];
+  out.nnz = nnz;
+  out.rp.alloc(sizeof(int) * (m + 1));
+  out.cp.alloc(sizeof(int) * (n + 1));
+  FH_HIP(hipMemcpyAsync(out.rp.p, hr.data(), sizeof(int) * (m + 1), hipMemcpyHostToDevice, st));
+  FH_HIP(hipMemcpyAsync(out.cp.p, hc.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, st));
+  out.ci.alloc(sizeof(int) * std::max<int64_t>(nnz, 1));
+  out.v.alloc(sizeof(double) * std::max<int64_t>(nnz, 1));
+  out.ri.alloc(sizeof(int) * std::max<int64_t>(nnz, 1));
+  out.cv.alloc(sizeof(double) * std::max<int64_t>(nnz, 1));
+  if (nnz > 0) {
+    hipLaunchKernelGGL(k_fill_csr, dim3((m + 255) / 256), dim3(256), 0, st, D, m, n, out.rp.as<int>(),
+                       out.ci.as<int>(), out.v.as<double>());
+    FH_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_fill_csc, dim3((n + 255) / 256), dim3(256), 0, st, D, m, n, out.cp.as<int>(),
+                       out.ri.as<int>(), out.cv.as<double>());
+    FH_LAUNCH_CHECK();
+  }
+  out.has_csc = true;
+  // the host vectors go out of scope: make sure the uploads have completed
+  FH_HIP(hipStreamSynchronize(st));
+}
+
+__global__ void k_csr_scatter(int m, const int* __restrict__ rp, const int* __restrict__ ci,
+                              const double* __restrict__ v, double* __restrict__ D) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  for (int p = rp[i]; p < rp[i + 1]; ++p) D[(size_t)ci[p] * m + i] = v[p];
+}
+
+void csr_to_dense_f64(const DMat& S, double* D, hipStream_t st) {
+  FH_HIP(hipMemsetAsync(D, 0, sizeof(double) * (size_t)S.m * S.n, st));
+  if (S.nnz > 0) {
+    hipLaunchKernelGGL(k_csr_scatter, dim3((S.m + 255) / 256), dim3(256), 0, st, S.m, S.rp.as<int>(),
+                       S.ci.as<int>(), S.v.as<double>(), D);
+    FH_LAUNCH_CHECK();
+  }
+}
+
+__global__ void k_count_nnz(const double* __restrict__ D, int64_t n, unsigned long long* __restrict__ out) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int c = (i < n && D[i] != 0.0) ? 1 : 0;
+  // wave-level ballot then one atomic per wave
+  unsigned long long b = __ballot(c);
+  if ((threadIdx.x & 63) == 0 && b) atomicAdd(out, (unsigned long long)__popcll(b));
+}
+
+int64_t count_nnz_f64(const double* D, int64_t n, hipStream_t st) {
+  if (n <= 0) return 0;
+  DBuf c(sizeof(unsigned long long));
+  FH_HIP(hipMemsetAsync(c.p, 0, sizeof(unsigned long long), st));
+  hipLaunchKernelGGL(k_count_nnz, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, D, n,
+                     c.as<unsigned long long>());
+  FH_LAUNCH_CHECK();
+  unsigned long long h = 0;
+  FH_HIP(hipMemcpyAsync(&h, c.p, sizeof(h), hipMemcpyDeviceToHost, st));
+  FH_HIP(hipStreamSynchronize(st));
+  return (int64_t)h;
+}
+
+}  // namespace fh

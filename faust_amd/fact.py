@@ -1,0 +1,122 @@
+"""palm4msa / hierarchical on the GPU (pyfaust.fact counterpart,
+wrapper/python/pyfaust/fact.py:798-879 and :1020-1335, backend 2020,
+on_gpu=True; C++ entry FaustFact.hpp:580-675,740-860)."""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, lib
+from .factparams import ParamsHierarchical, ParamsHierarchicalWHT, ParamsPalm4MSA
+from .faust import Factor, Faust
+
+
+class PalmResult:
+    """Raw result: factor handles, lambda and the (call, it, rel_err, lambda) trajectory."""
+
+    def __init__(self, ptr):
+        L = lib()
+        n, lam = C.c_int(), C.c_double()
+        check(L.fh_palm_result_nfacts(ptr, C.byref(n)))
+        check(L.fh_palm_result_lambda(ptr, C.byref(lam)))
+        self.lam = lam.value
+        self.factors = []
+        for i in range(n.value):
+            h = C.c_void_p()
+            check(L.fh_palm_result_factor(ptr, i, C.byref(h)))
+            self.factors.append(Factor(None, handle=h))
+        tl = C.c_int()
+        check(L.fh_palm_result_traj(ptr, None, C.byref(tl)))
+        t = np.zeros((tl.value, 4))
+        if tl.value:
+            check(L.fh_palm_result_traj(ptr, t.ctypes.data_as(C.c_void_p), C.byref(tl)))
+        self.traj = t
+        L.fh_palm_result_free(ptr)
+
+    def dense_factors(self):
+        return [f.toarray() for f in self.factors]
+
+    def kinds(self):
+        return [f.info()[0] for f in self.factors]
+
+
+def _nbytes(f):
+    kind, _, r, c, nnz = f.info()
+    return r * c * 8 if kind == _lib.FH_MAT_DENSE else nnz * 12 + (r + 1) * 4
+
+
+def _faust_times_lambda(res):
+    """TransformHelper(facts, lambda): lambda multiplies the factor of fewest
+    bytes (faust_Transform.hpp:188-214)."""
+    facs = res.dense_factors()
+    sizes = [_nbytes(f) for f in res.factors]
+    i = int(np.argmin(sizes))
+    kinds = res.kinds()
+    out = []
+    import scipy.sparse as sp
+    for j, (a, k) in enumerate(zip(facs, kinds)):
+        if j == i:
+            a = a * res.lam
+        out.append(sp.csr_matrix(a) if k == _lib.FH_MAT_CSR else a)
+    return Faust(out)
+
+
+def _palm_raw(M, p, record_traj=False):
+    M = np.asfortranarray(M, dtype=np.float64)
+    keep = []
+    cons = (_lib.fh_constraint * len(p.constraints))(*[c._c(keep) for c in p.constraints])
+    sc = p.stop_crit._c()
+    pc = p._c(record_traj)
+    init = None
+    if p.init_facts is not None:
+        init = Faust(list(p.init_facts)).h
+    out = C.c_void_p()
+    check(lib().fh_palm4msa(M.ctypes.data_as(C.c_void_p), M.shape[0], M.shape[1], cons, len(p.constraints),
+                            C.byref(sc), C.byref(pc), p.init_lambda, init, C.byref(out)))
+    return PalmResult(out)
+
+
+def _hier_raw(M, p, record_traj=False):
+    M = np.asfortranarray(M, dtype=np.float64)
+    keep = []
+    n = len(p.fact_constraints)
+    fc = (_lib.fh_constraint * n)(*[c._c(keep) for c in p.fact_constraints])
+    rc = (_lib.fh_constraint * n)(*[c._c(keep) for c in p.res_constraints])
+    s1, s2 = p.stop_crits[0]._c(), p.stop_crits[1]._c()
+    pc = p._c(record_traj, p.is_fact_side_left)
+    out = C.c_void_p()
+    check(lib().fh_hierarchical(M.ctypes.data_as(C.c_void_p), M.shape[0], M.shape[1], fc, rc, n, C.byref(s1),
+                                C.byref(s2), C.byref(pc), C.byref(out)))
+    return PalmResult(out)
+
+
+def palm4msa(M, p, ret_lambda=False, backend=2020, on_gpu=True, ret_raw=False):
+    """Factorize M with PALM4MSA (2020 backend) on the GPU."""
+    if backend != 2020:
+        raise ValueError("only backend=2020 is implemented")
+    if not isinstance(p, ParamsPalm4MSA):
+        raise TypeError("p must be a ParamsPalm4MSA")
+    res = _palm_raw(M, p, record_traj=ret_raw)
+    if ret_raw:
+        return res
+    F = _faust_times_lambda(res)
+    return (F, res.lam) if ret_lambda else F
+
+
+def hierarchical(M, p, ret_lambda=False, backend=2020, on_gpu=True, ret_raw=False):
+    """Hierarchical factorization (2020 backend) on the GPU.  p may be a
+    ParamsHierarchical or the string 'hadamard' / 'rectmat' shortcut for
+    'hadamard' only."""
+    if backend != 2020:
+        raise ValueError("only backend=2020 is implemented")
+    if isinstance(p, str):
+        if p.lower() != "hadamard":
+            raise ValueError("only the 'hadamard' shortcut is supported")
+        p = ParamsHierarchicalWHT(int(np.log2(M.shape[0])))
+    if not isinstance(p, ParamsHierarchical):
+        raise TypeError("p must be a ParamsHierarchical")
+    res = _hier_raw(M, p, record_traj=ret_raw)
+    if ret_raw:
+        return res
+    F = _faust_times_lambda(res)
+    return (F, res.lam) if ret_lambda else F

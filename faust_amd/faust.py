@@ -1,0 +1,236 @@
+"""The Faust class: a product of sparse/dense factors held on the MI355X.
+
+Mirrors the parts of pyfaust.Faust on the Faust x dense hot path
+(wrapper/python/pyfaust.py: __init__ :214, __matmul__ :1092-1216, T/H,
+toarray, norm(2), factors, numfactors, shape) on top of the C ABI of
+libfaust_hip.so.  Every product runs on the GPU; there is no CPU path.
+"""
+import ctypes as C
+
+import numpy as np
+import scipy.sparse as sp
+
+from . import _lib
+from ._lib import check, lib
+
+try:
+    import torch
+except ImportError:  # torch is the device-memory plumbing of the Python layer
+    torch = None
+
+_DT = {np.dtype(np.float64): _lib.FH_F64, np.dtype(np.float32): _lib.FH_F32}
+
+
+def _require_torch_cuda():
+    if torch is None or not torch.cuda.is_available():
+        raise RuntimeError("faust_amd needs a ROCm GPU visible to torch (no CPU fallback)")
+
+
+def _stream():
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _ptr(t):
+    return C.c_void_p(t.data_ptr())
+
+
+def colmajor_to_device(X, dtype=torch.float64 if torch else None):
+    """numpy/torch (n, p) -> device tensor holding X column-major (a (p, n) row-major tensor)."""
+    if torch is not None and isinstance(X, torch.Tensor):
+        t = X.to(device="cuda", dtype=dtype)
+        return t.t().contiguous()
+    a = np.asarray(X)
+    if a.ndim == 1:
+        a = a.reshape(-1, 1)
+    return torch.from_numpy(np.ascontiguousarray(a.T)).to(device="cuda", dtype=dtype)
+
+
+class Factor:
+    """One device matrix handle (MatDense / MatSparse / MatBSR on the GPU)."""
+
+    def __init__(self, M, dtype=np.float64, handle=None):
+        self.h = C.c_void_p()
+        if handle is not None:
+            self.h = handle
+            return
+        _require_torch_cuda()
+        L = lib()
+        dt = _DT[np.dtype(dtype)] if not (isinstance(dtype, str) and dtype == "bf16") else _lib.FH_BF16
+        if sp.issparse(M) and M.format == "bsr":
+            from .bsr import bsr_handle
+            self.h = bsr_handle(M, dt)
+        elif sp.issparse(M):
+            S = sp.csr_matrix(M)
+            S.sum_duplicates()
+            S.sort_indices()
+            rp = np.ascontiguousarray(S.indptr, dtype=np.int32)
+            ci = np.ascontiguousarray(S.indices, dtype=np.int32)
+            v = _host_values(S.data, dt)
+            check(L.fh_mat_csr_create(dt, S.shape[0], S.shape[1], int(S.nnz), rp.ctypes.data_as(C.c_void_p),
+                                      ci.ctypes.data_as(C.c_void_p), v.ctypes.data_as(C.c_void_p), 0,
+                                      C.byref(self.h)))
+        else:
+            a = np.asarray(M)
+            v = _host_values(np.asfortranarray(a), dt)
+            v = np.asfortranarray(v)
+            check(L.fh_mat_dense_create(dt, a.shape[0], a.shape[1], v.ctypes.data_as(C.c_void_p),
+                                        max(a.shape[0], 1), 0, C.byref(self.h)))
+
+    def info(self):
+        k, d, r, c, nnz = C.c_int(), C.c_int(), C.c_int(), C.c_int(), C.c_int64()
+        check(lib().fh_mat_info(self.h, C.byref(k), C.byref(d), C.byref(r), C.byref(c), C.byref(nnz)))
+        return k.value, d.value, r.value, c.value, nnz.value
+
+    @property
+    def shape(self):
+        return self.info()[2:4]
+
+    def toarray(self):
+        kind, dt, r, c, _ = self.info()
+        tdt = {_lib.FH_F64: torch.float64, _lib.FH_F32: torch.float32, _lib.FH_BF16: torch.bfloat16}[dt]
+        out = torch.empty((c, r), dtype=tdt, device="cuda")
+        check(lib().fh_mat_to_dense(self.h, _ptr(out), _stream()))
+        torch.cuda.current_stream().synchronize()
+        return out.t().to(torch.float64).cpu().numpy()
+
+    def __del__(self):
+        if getattr(self, "h", None) and self.h.value and _lib._lib is not None:
+            _lib._lib.fh_mat_free(self.h)
+            self.h = C.c_void_p()
+
+
+def _host_values(a, dt):
+    if dt == _lib.FH_F64:
+        return np.ascontiguousarray(a, dtype=np.float64) if a.ndim == 1 else np.asfortranarray(a, dtype=np.float64)
+    if dt == _lib.FH_F32:
+        return np.ascontiguousarray(a, dtype=np.float32) if a.ndim == 1 else np.asfortranarray(a, dtype=np.float32)
+    # bf16: round-to-nearest-even from float32, stored as uint16
+    f = np.asarray(a, dtype=np.float32)
+    u = f.view(np.uint32).astype(np.uint64)
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16)
+    return np.ascontiguousarray(r) if a.ndim == 1 else np.asfortranarray(r)
+
+
+class Faust:
+    """F = S_1 S_2 ... S_J on the GPU (pyfaust.Faust counterpart, pyfaust.py:214).
+
+    Faust(factors, dtype=np.float64): factors are numpy arrays (dense),
+    scipy CSR matrices (sparse) or scipy BSR matrices.
+    """
+
+    def __init__(self, factors=None, dtype=np.float64, _facs=None, _transposed=False):
+        _require_torch_cuda()
+        L = lib()
+        if _facs is None:
+            if factors is None or len(factors) == 0:
+                raise ValueError("a Faust needs at least one factor")
+            _facs = [f if isinstance(f, Factor) else Factor(f, dtype) for f in factors]
+        self._facs = _facs
+        self._transposed = _transposed
+        self.dtype = np.dtype(np.float64) if dtype == "bf16" else np.dtype(dtype)
+        self._dt = dtype
+        self.h = C.c_void_p()
+        check(L.fh_faust_create(C.byref(self.h)))
+        for f in self._facs:
+            check(L.fh_faust_push_back(self.h, f.h))
+
+    def __del__(self):
+        if getattr(self, "h", None) and self.h.value and _lib._lib is not None:
+            _lib._lib.fh_faust_free(self.h)
+            self.h = C.c_void_p()
+
+    # --- shape / structure -------------------------------------------------
+    @property
+    def shape(self):
+        r = self._facs[0].shape[0]
+        c = self._facs[-1].shape[1]
+        return (c, r) if self._transposed else (r, c)
+
+    def numfactors(self):
+        return len(self._facs)
+
+    def __len__(self):
+        return len(self._facs)
+
+    def factors(self, i):
+        """Dense copy of factor i (pyfaust Faust.factors returns the factor)."""
+        idx = len(self._facs) - 1 - i if self._transposed else i
+        a = self._facs[idx].toarray()
+        return a.T if self._transposed else a
+
+    def nnz_sum(self):
+        return sum(f.info()[4] for f in self._facs)
+
+    @property
+    def T(self):
+        return Faust(_facs=self._facs, dtype=self._dt, _transposed=not self._transposed)
+
+    @property
+    def H(self):
+        return self.T  # real scalars
+
+    def transpose(self):
+        return self.T
+
+    # --- products ----------------------------------------------------------
+    def _op(self):
+        return b"T" if self._transposed else b"N"
+
+    def matmul_device(self, Xd, out=None):
+        """Y = F @ X on device tensors in column-major form.
+
+        Xd: a (p, n) contiguous CUDA tensor = X (n x p) column-major.
+        Returns a (p, m) CUDA tensor = Y column-major.
+        """
+        m = self.shape[0]
+        p = Xd.shape[0]
+        if Xd.shape[1] != self.shape[1]:
+            raise ValueError("dimensions of the Faust and the matrix are not consistent")
+        if out is None:
+            out = torch.empty((p, m), dtype=Xd.dtype, device="cuda")
+        check(lib().fh_faust_multiply(self.h, self._op(), _ptr(Xd), p, _ptr(out), _stream()))
+        return out
+
+    def __matmul__(self, A):
+        """pyfaust Faust.__matmul__ for a dense ndarray / CUDA tensor (pyfaust.py:1177-1203)."""
+        if isinstance(A, Faust):
+            raise NotImplementedError("Faust @ Faust is outside the hot path")
+        is_torch = torch is not None and isinstance(A, torch.Tensor)
+        if not is_torch and sp.issparse(A):
+            A = A.toarray()
+        vec = (not is_torch and np.ndim(A) == 1) or (is_torch and A.dim() == 1)
+        tdt = torch.float64 if self.dtype == np.float64 else torch.float32
+        Xd = colmajor_to_device(A, tdt)
+        Y = self.matmul_device(Xd)
+        if is_torch:
+            return Y.t()
+        torch.cuda.current_stream().synchronize()
+        y = Y.cpu().numpy().T
+        return y.reshape(-1) if vec else np.asfortranarray(y)
+
+    def dot(self, A):
+        return self @ A
+
+    def toarray(self):
+        """Dense product (Transform::get_product)."""
+        m, n = self._facs[0].shape[0], self._facs[-1].shape[1]
+        out = torch.empty((n, m), dtype=torch.float64, device="cuda")
+        check(lib().fh_faust_get_product(self.h, _ptr(out), _stream()))
+        a = out.t().cpu().numpy()
+        return np.asfortranarray(a.T if self._transposed else a)
+
+    def norm(self, ord="fro", **kwargs):
+        """norm(2): power iteration on the chain (TransformHelper::spectralNorm);
+        'fro': Frobenius norm of the dense product."""
+        if ord == 2:
+            nrm, flag = C.c_double(), C.c_int()
+            check(lib().fh_faust_spectral_norm(self.h, int(kwargs.get("max_num_its", 100)),
+                                                float(kwargs.get("threshold", 1e-6)), C.byref(nrm), C.byref(flag),
+                                                _stream()))
+            return nrm.value
+        if ord == "fro":
+            return float(np.linalg.norm(self.toarray()))
+        raise ValueError("norm: only 2 and 'fro' are supported")
+
+    def __repr__(self):
+        return f"Faust size {self.shape[0]}x{self.shape[1]}, {self.numfactors()} factors, on MI355X"

@@ -1,0 +1,170 @@
+/*
+ * faust_hip.h -- C ABI of libfaust_hip.so, the MI355X (gfx950) backend of the
+ * FAuST hot paths.  Plain pointers and sizes only; every entry point returns
+ * an int status (FH_OK == 0) and fh_last_error() gives the message of the
+ * last failure on the calling thread.
+ *
+ * The roles mirror the gpu_mod C function tables that the reference's GPU2
+ * layer dlopens (src/faust_linear_operator/GPU2/faust_gpu_mod_utils.h:24-102,
+ * faust_gpu_mod_utils.cpp:40-90): dense / sparse (CSR) / BSR matrix handles
+ * with togpu/tocpu/free, `mul_gpu_dsm_ext`-style products with op flags and
+ * alpha/beta passed by pointer, prox projectors, norms and device selection.
+ * On top of that it exports the two hot-path drivers the reference's C++
+ * core runs on the CPU: the Faust chain product (TransformHelper::multiply)
+ * and palm4msa2 / hierarchical.
+ *
+ * Conventions (as at the reference call sites):
+ *   - dense matrices are column-major with a leading dimension (ld >= nrows);
+ *   - sparse matrices are CSR with int32 rowptr/colind (faust_MatSparse.h:261-264);
+ *   - BSR: int32 browptr/bcolinds, blocks of bm x bn stored column-major
+ *     (faust_MatBSR.h:120-150);
+ *   - op: 'N', 'T' or 'H' (gm_Op OP_NOTRANSP/OP_TRANSP/OP_CONJTRANSP,
+ *     faust_gpu_mod_utils.cpp:189-199); the scalars here are real, so 'H' == 'T';
+ *   - `stream` is a hipStream_t (NULL = the null stream); results are ordered
+ *     on that stream and the call does not wait unless stated.
+ */
+#ifndef FAUST_HIP_H
+#define FAUST_HIP_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { FH_OK = 0, FH_ERR_ARG = 1, FH_ERR_HIP = 2, FH_ERR_NUMERIC = 3, FH_ERR_UNSUPPORTED = 4 };
+enum { FH_F64 = 0, FH_F32 = 1, FH_BF16 = 2 };
+enum { FH_MAT_DENSE = 0, FH_MAT_CSR = 1, FH_MAT_BSR = 2 };
+
+typedef struct fh_mat_s *fh_mat_t;     /* device matrix handle (dense, CSR or BSR) */
+typedef struct fh_faust_s *fh_faust_t; /* chain of factor handles (a Faust) */
+
+/* Message of the last failed call on this thread (never NULL). */
+const char *fh_last_error(void);
+/* Library version string. */
+const char *fh_version(void);
+
+/* ---- device selection: gm_GenPurposeFunc dev_count/set_dev/cur_dev
+ *      (src/faust_linear_operator/GPU2/faust_cuda_device.cpp:6-30) */
+int fh_dev_count(int *count);
+int fh_set_dev(int dev);
+int fh_cur_dev(int *dev);
+/* Wait for all work on `stream`. */
+int fh_stream_sync(void *stream);
+
+/* ---- matrix handles: togpu / tocpu / info / free
+ *      (dsm_funcs->togpu, spm_funcs->togpu, bsr_funcs->togpu;
+ *       faust_MatDense_gpu.cpp.in:115, faust_MatSparse_gpu.cpp.in, faust_MatBSR_gpu.cpp.in).
+ * `src_on_device` != 0 means the pointers are device pointers (copied D2D). */
+int fh_mat_dense_create(int dtype, int nrows, int ncols, const void *data, int ld, int src_on_device,
+                        fh_mat_t *out);
+int fh_mat_csr_create(int dtype, int nrows, int ncols, int nnz, const int32_t *rowptr, const int32_t *colind,
+                      const void *vals, int src_on_device, fh_mat_t *out);
+int fh_mat_bsr_create(int dtype, int nrows, int ncols, int bm, int bn, int bnnz, const int32_t *browptr,
+                      const int32_t *bcolinds, const void *bdata, int src_on_device, fh_mat_t *out);
+int fh_mat_info(fh_mat_t M, int *kind, int *dtype, int *nrows, int *ncols, int64_t *nnz);
+/* Copy back to host buffers (dense: `data` nrows*ncols col-major; CSR:
+ * rowptr[nrows+1], colind[nnz], vals[nnz]; BSR: browptr, bcolinds, bdata). */
+int fh_mat_tocpu(fh_mat_t M, void *data_or_vals, int32_t *rowptr, int32_t *colind);
+/* Densify into a device buffer (col-major, ld = nrows). */
+int fh_mat_to_dense(fh_mat_t M, void *dense_dev, void *stream);
+int fh_mat_free(fh_mat_t M);
+
+/* ---- products
+ * Y = alpha*op(M)*X + beta*Y with M a matrix handle and X, Y device dense
+ * col-major.  Replaces spm_funcs->mul_gpu_dsm_ext / bsr_funcs->mul_gpu_dsm_ext
+ * / dsm_funcs->mul_gpu_dsm_ext (faust_MatSparse_gpu.cpp.in:456-472,
+ * faust_MatBSR_gpu.cpp.in:109-152, faust_MatDense_gpu.cpp.in:478).
+ * alpha/beta point to host scalars of the matrix dtype (FP32 for BF16). */
+int fh_mat_mul_dense(fh_mat_t M, char op, const void *X, int ldx, int ncols, void *Y, int ldy,
+                     const void *alpha, const void *beta, void *stream);
+/* C = alpha*op(A)*op(B) + beta*C, dense device operands (dsm_funcs->gemm,
+ * faust_MatDense_gpu.cpp.in:518-551; FP64 follows the canonical order). */
+int fh_gemm(int dtype, char opA, char opB, int m, int n, int k, const void *alpha, const void *A, int lda,
+            const void *B, int ldb, const void *beta, void *C, int ldc, void *stream);
+
+/* ---- Faust (factor chain).  Handles are borrowed: freeing the Faust does
+ * not free the factors (faust_TransformHelperGen.h:140 shares factors). */
+int fh_faust_create(fh_faust_t *out);
+int fh_faust_push_back(fh_faust_t F, fh_mat_t M);
+int fh_faust_size(fh_faust_t F, int *nfactors);
+int fh_faust_free(fh_faust_t F);
+/* Y = op(F)*X, X (ncols(op(F)) x p) and Y device col-major with ld = rows.
+ * TransformHelper<FPP,Cpu>::multiply(const FPP*, int, FPP*)
+ * (faust_TransformHelper.hpp:407-434) -> Transform::multiply
+ * (faust_Transform.hpp:1714-1812): right-to-left for 'N', left-to-right with
+ * per-factor op for 'T'/'H', two ping-pong intermediates. */
+int fh_faust_multiply(fh_faust_t F, char op, const void *X, int p, void *Y, void *stream);
+/* Dense product of the chain into a device buffer (Transform::get_product,
+ * faust_Transform.hpp:599-652). */
+int fh_faust_get_product(fh_faust_t F, void *out_dev, void *stream);
+/* 2-norm by power iteration (TransformHelper::spectralNorm,
+ * faust_TransformHelper.hpp:1201-1218; power_iteration,
+ * faust_linear_algebra.hpp:808-898).  Synchronizes. */
+int fh_faust_spectral_norm(fh_faust_t F, int max_iter, double threshold, double *norm, int *flag, void *stream);
+
+/* ---- projectors on a dense device matrix, in place
+ * (dsm_funcs->prox_sp/prox_spcol/prox_splin, faust_MatDense_gpu.cpp.in:746-765;
+ *  splincol and supp are native here, the reference falls back to the CPU:
+ *  faust_prox_gpu.hpp:103-108).  Tie handling reproduces std::partial_sort
+ *  with comparator |a| > |b| (faust_prox.hpp:53-69). */
+enum { FH_PROX_SP = 0, FH_PROX_SPCOL = 1, FH_PROX_SPLIN = 2, FH_PROX_SPLINCOL = 3, FH_PROX_SUPP = 4 };
+int fh_prox(int dtype, int name, void *M, int nrows, int ncols, int64_t k, int normalized, int pos,
+            const void *supp, void *stream);
+
+/* ---- canonical FP64 reductions (dsm_funcs->norm_frob / dot); synchronize. */
+int fh_norm_frob(int dtype, const void *M, int64_t n, double *out, void *stream);
+int fh_dot(int dtype, const void *x, const void *y, int64_t n, double *out, void *stream);
+
+/* ---- PALM4MSA 2020 / hierarchical (faust_palm4msa2020.hpp:47-300,
+ * faust_hierarchical.hpp:1-150; Python entry FaustFact.hpp:580-675,740-860). */
+typedef struct {
+  int name;          /* FH_PROX_* */
+  int64_t k;
+  int nrows, ncols;
+  int normalized;    /* ConstraintInt default: 1 */
+  int pos;           /* default: 0 */
+  const double *supp;/* FH_PROX_SUPP: host nrows*ncols col-major */
+} fh_constraint;
+
+typedef struct {
+  int num_its;            /* StoppingCriterion num_its (factparams.py:20-31) */
+  int is_criterion_error; /* stop on error < error_threshold */
+  double error_threshold;
+  int max_num_its;
+} fh_stop_crit;
+
+typedef struct {
+  int is_update_way_R2L;  /* factparams.py:955-962 defaults */
+  int factor_format;      /* 0 dense, 1 sparse, 2 dynamic */
+  int packing_RL;
+  int no_normalization;
+  int no_lambda;
+  int norm2_max_iter;
+  double norm2_threshold;
+  int constant_step_size;
+  double step_size;
+  int is_fact_side_left;  /* hierarchical */
+  int record_traj;        /* record (call, it, rel_err, lambda) after each iteration */
+} fh_palm_params;
+
+typedef struct fh_palm_result_s *fh_palm_result_t;
+
+/* A is a host FP64 nrows x ncols col-major array. init_facts may be NULL. */
+int fh_palm4msa(const double *A, int nrows, int ncols, const fh_constraint *cons, int ncons,
+                const fh_stop_crit *sc, const fh_palm_params *p, double lambda_init, fh_faust_t init_facts,
+                fh_palm_result_t *out);
+int fh_hierarchical(const double *A, int nrows, int ncols, const fh_constraint *fac_cons,
+                    const fh_constraint *res_cons, int nlevels, const fh_stop_crit *sc_local,
+                    const fh_stop_crit *sc_global, const fh_palm_params *p, fh_palm_result_t *out);
+int fh_palm_result_nfacts(fh_palm_result_t r, int *n);
+int fh_palm_result_lambda(fh_palm_result_t r, double *lambda);
+/* factor i as a new matrix handle (owned by the caller) */
+int fh_palm_result_factor(fh_palm_result_t r, int i, fh_mat_t *out);
+int fh_palm_result_traj(fh_palm_result_t r, double *out4, int *len);
+int fh_palm_result_free(fh_palm_result_t r);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

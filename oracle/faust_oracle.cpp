@@ -1,0 +1,1123 @@
+// faust_oracle.cpp -- TEST INFRASTRUCTURE ONLY.
+//
+// CPU FP64 restatement of the reference FAuST algorithms on the two hot
+// paths.  It is the parity checker for the HIP path (tests/, smoke()) and the
+// timed CPU baseline (bench.py).  The product (faust_amd/) never links it.
+//
+// Follows (file:line in the reference tree):
+//   chain product          src/faust_linear_operator/CPU/faust_Transform.hpp:1714-1812
+//   get_product            src/faust_linear_operator/CPU/faust_Transform.hpp:599-652
+//   spectral norm          src/faust_linear_operator/CPU/faust_Transform.hpp:684-709
+//   power iteration        src/faust_linear_operator/CPU/faust_linear_algebra.hpp:808-898
+//   Random start vector    src/faust_linear_operator/CPU/faust_Vect.hpp:364-367 (Eigen 3.4 Random)
+//   projectors             src/faust_linear_operator/CPU/faust_prox.hpp:53-200,309-320,374-383
+//   dense/sparse choice    src/algorithm/constraint/faust_prox_gen.hpp:118-210
+//   products ordering      src/faust_linear_operator/CPU/faust_prod_opt.hpp:49-86
+//   dynprog                src/faust_linear_operator/faust_prod_opt_gen.hpp:8-200,
+//                          src/faust_linear_operator/CPU/faust_prod_opt.cpp:6-86
+//   palm4msa2              src/algorithm/factorization/faust_palm4msa2020.hpp:1-676
+//   hierarchical           src/algorithm/factorization/faust_hierarchical.hpp:1-150
+//   stopping criterion     src/algorithm/factorization/faust_StoppingCriterion.hpp:113-150
+//
+// Arithmetic follows the canonical order documented in faust_oracle.h.
+#include "faust_oracle.h"
+
+#include <algorithm>
+#include <climits>
+#include <functional>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+thread_local std::string g_err;
+
+constexpr int64_t RED_CHUNK = 4096;
+constexpr int RED_LANES = 256;
+
+// ---------------------------------------------------------------- reductions
+template <class F>
+double reduce_generic(int64_t n, F term_update) {
+  if (n <= 0) return 0.0;
+  int64_t nchunks = (n + RED_CHUNK - 1) / RED_CHUNK;
+  std::vector<double> part(nchunks);
+  for (int64_t c = 0; c < nchunks; ++c) {
+    double p[RED_LANES];
+    for (int t = 0; t < RED_LANES; ++t) {
+      double acc = 0.0;
+      for (int m = 0; m < RED_CHUNK / RED_LANES; ++m) {
+        int64_t i = c * RED_CHUNK + (int64_t)m * RED_LANES + t;
+        if (i < n) acc = term_update(acc, i);
+      }
+      p[t] = acc;
+    }
+    for (int s = RED_LANES / 2; s >= 1; s >>= 1)
+      for (int t = 0; t < s; ++t) p[t] = p[t] + p[t + s];
+    part[c] = p[0];
+  }
+  double tot = 0.0;
+  for (int64_t c = 0; c < nchunks; ++c) tot = tot + part[c];
+  return tot;
+}
+
+double rsum(const double* x, int64_t n) {
+  return reduce_generic(n, [x](double acc, int64_t i) { return acc + x[i]; });
+}
+double rdot(const double* x, const double* y, int64_t n) {
+  return reduce_generic(n, [x, y](double acc, int64_t i) { return std::fma(x[i], y[i], acc); });
+}
+
+// ---------------------------------------------------------------- matrices
+struct Mat {
+  enum Kind { DENSE = 0, SPARSE = 1, BSR = 2 } kind = DENSE;
+  int m = 0, n = 0;
+  std::vector<double> d;               // dense col-major
+  std::vector<int> rp, ci;             // CSR
+  std::vector<double> v;
+  int bm = 0, bn = 0;                  // BSR
+  std::vector<int> brp, bci;
+  std::vector<double> bd;
+
+  int64_t nnz() const {
+    if (kind == SPARSE) return (int64_t)v.size();
+    if (kind == BSR) {  // stored values count (MatBSR::getNonZeros counts block entries)
+      return (int64_t)bd.size();
+    }
+    int64_t c = 0;
+    for (double x : d) c += (x != 0.0);
+    return c;
+  }
+};
+using MatP = std::shared_ptr<const Mat>;
+
+Mat dense_of(const Mat& A) {
+  if (A.kind == Mat::DENSE) return A;
+  Mat D;
+  D.kind = Mat::DENSE;
+  D.m = A.m;
+  D.n = A.n;
+  D.d.assign((size_t)A.m * A.n, 0.0);
+  if (A.kind == Mat::SPARSE) {
+    for (int i = 0; i < A.m; ++i)
+      for (int p = A.rp[i]; p < A.rp[i + 1]; ++p) D.d[(size_t)A.ci[p] * A.m + i] += A.v[p];
+  } else {
+    int nbr = A.m / A.bm;
+    for (int br = 0; br < nbr; ++br)
+      for (int b = A.brp[br]; b < A.brp[br + 1]; ++b) {
+        const double* blk = &A.bd[(size_t)b * A.bm * A.bn];
+        for (int c = 0; c < A.bn; ++c)
+          for (int r = 0; r < A.bm; ++r)
+            D.d[(size_t)(A.bci[b] * A.bn + c) * A.m + br * A.bm + r] += blk[(size_t)c * A.bm + r];
+      }
+  }
+  return D;
+}
+
+// Dense -> CSR dropping exact zeros (MatSparse(const MatDense&)).
+Mat sparse_of_dense(const Mat& D) {
+  Mat S;
+  S.kind = Mat::SPARSE;
+  S.m = D.m;
+  S.n = D.n;
+  S.rp.assign(D.m + 1, 0);
+  for (int j = 0; j < D.n; ++j)
+    for (int i = 0; i < D.m; ++i)
+      if (D.d[(size_t)j * D.m + i] != 0.0) S.rp[i + 1]++;
+  for (int i = 0; i < D.m; ++i) S.rp[i + 1] += S.rp[i];
+  S.ci.resize(S.rp[D.m]);
+  S.v.resize(S.rp[D.m]);
+  std::vector<int> fill(S.rp.begin(), S.rp.end() - 1);
+  for (int j = 0; j < D.n; ++j)
+    for (int i = 0; i < D.m; ++i) {
+      double x = D.d[(size_t)j * D.m + i];
+      if (x != 0.0) {
+        S.ci[fill[i]] = j;
+        S.v[fill[i]] = x;
+        fill[i]++;
+      }
+    }
+  return S;
+}
+
+// Column-compressed view of op(A): for each column k of op(A), the list of
+// (row, value) pairs in ascending row order.  Exact zeros are dropped (they
+// are no-ops under fma).
+struct CSC {
+  int m = 0, n = 0;
+  std::vector<int> cp, ri;
+  std::vector<double> v;
+};
+
+CSC csc_of_op(const Mat& A, char op) {
+  // op(A) dims
+  int om = (op == 'N') ? A.m : A.n;
+  int on = (op == 'N') ? A.n : A.m;
+  CSC C;
+  C.m = om;
+  C.n = on;
+  C.cp.assign(on + 1, 0);
+  // gather triplets (row_of_op, col_of_op, val) then bucket by column
+  std::vector<int> tr, tc;
+  std::vector<double> tv;
+  auto push = [&](int i, int j, double x) {  // (i,j) in A coordinates
+    if (x == 0.0) return;
+    if (op == 'N') {
+      tr.push_back(i);
+      tc.push_back(j);
+    } else {
+      tr.push_back(j);
+      tc.push_back(i);
+    }
+    tv.push_back(x);
+  };
+  if (A.kind == Mat::DENSE) {
+    for (int j = 0; j < A.n; ++j)
+      for (int i = 0; i < A.m; ++i) push(i, j, A.d[(size_t)j * A.m + i]);
+  } else if (A.kind == Mat::SPARSE) {
+    for (int i = 0; i < A.m; ++i)
+      for (int p = A.rp[i]; p < A.rp[i + 1]; ++p) push(i, A.ci[p], A.v[p]);
+  } else {
+    Mat D = dense_of(A);
+    for (int j = 0; j < D.n; ++j)
+      for (int i = 0; i < D.m; ++i) push(i, j, D.d[(size_t)j * D.m + i]);
+  }
+  for (size_t t = 0; t < tc.size(); ++t) C.cp[tc[t] + 1]++;
+  for (int j = 0; j < on; ++j) C.cp[j + 1] += C.cp[j];
+  C.ri.resize(tv.size());
+  C.v.resize(tv.size());
+  std::vector<int> fill(C.cp.begin(), C.cp.end() - 1);
+  // stable counting sort by column; then sort rows inside each column
+  for (size_t t = 0; t < tc.size(); ++t) {
+    int j = tc[t];
+    C.ri[fill[j]] = tr[t];
+    C.v[fill[j]] = tv[t];
+    fill[j]++;
+  }
+  for (int j = 0; j < on; ++j) {
+    int b = C.cp[j], e = C.cp[j + 1];
+    std::vector<std::pair<int, double>> col;
+    col.reserve(e - b);
+    for (int p = b; p < e; ++p) col.emplace_back(C.ri[p], C.v[p]);
+    std::sort(col.begin(), col.end(),
+              [](const std::pair<int, double>& a, const std::pair<int, double>& b2) { return a.first < b2.first; });
+    for (int p = b; p < e; ++p) {
+      C.ri[p] = col[p - b].first;
+      C.v[p] = col[p - b].second;
+    }
+  }
+  return C;
+}
+
+// Dense op(B) as a col-major array.
+std::vector<double> dense_op(const Mat& B, char op, int& rows, int& cols) {
+  Mat D = dense_of(B);
+  if (op == 'N') {
+    rows = D.m;
+    cols = D.n;
+    return D.d;
+  }
+  rows = D.n;
+  cols = D.m;
+  std::vector<double> T((size_t)rows * cols);
+  for (int j = 0; j < D.n; ++j)
+    for (int i = 0; i < D.m; ++i) T[(size_t)i * rows + j] = D.d[(size_t)j * D.m + i];
+  return T;
+}
+
+int g_threads = 0;
+
+// out = alpha * op(A) op(B) + beta * out (gemm_gen, faust_linear_algebra.hpp:922).
+// Canonical: per entry acc over k ascending with fma; epilogue t = alpha*acc,
+// out = beta==0 ? t : fma(beta, c_old, t).
+void matmul(const Mat& A, char opA, const Mat& B, char opB, double alpha, double beta, Mat& out) {
+  CSC a = csc_of_op(A, opA);
+  int br, bc;
+  std::vector<double> b = dense_op(B, opB, br, bc);
+  if (a.n != br) throw std::runtime_error("matmul: dimension mismatch");
+  int M = a.m, N = bc;
+  std::vector<double> old;
+  bool use_beta = (beta != 0.0);
+  if (use_beta) {
+    if (out.kind != Mat::DENSE || out.m != M || out.n != N)
+      throw std::runtime_error("matmul: beta != 0 needs a dense output of the right size");
+    old = out.d;
+  }
+  std::vector<double> res((size_t)M * N);
+#pragma omp parallel for schedule(static) num_threads(g_threads > 0 ? g_threads : 1)
+  for (int j = 0; j < N; ++j) {
+    std::vector<double> acc(M, 0.0);
+    const double* bj = &b[(size_t)j * br];
+    for (int k = 0; k < br; ++k) {
+      double bk = bj[k];
+      if (bk == 0.0) continue;
+      for (int p = a.cp[k]; p < a.cp[k + 1]; ++p) acc[a.ri[p]] = std::fma(a.v[p], bk, acc[a.ri[p]]);
+    }
+    double* rj = &res[(size_t)j * M];
+    for (int i = 0; i < M; ++i) {
+      double t = alpha * acc[i];
+      rj[i] = use_beta ? std::fma(beta, old[(size_t)j * M + i], t) : t;
+    }
+  }
+  out.kind = Mat::DENSE;
+  out.m = M;
+  out.n = N;
+  out.d.swap(res);
+  out.rp.clear();
+  out.ci.clear();
+  out.v.clear();
+}
+
+Mat matmul_new(const Mat& A, char opA, const Mat& B, char opB, double alpha = 1.0) {
+  Mat out;
+  matmul(A, opA, B, opB, alpha, 0.0, out);
+  return out;
+}
+
+double fro_norm(const Mat& D) {
+  Mat X = dense_of(D);
+  return std::sqrt(rdot(X.d.data(), X.d.data(), (int64_t)X.d.size()));
+}
+
+// ---------------------------------------------------------------- chain
+struct Chain {
+  std::vector<MatP> f;
+  int nrows() const { return f.empty() ? 0 : f.front()->m; }
+  int ncols() const { return f.empty() ? 0 : f.back()->n; }
+  bool is_zero() const {
+    for (auto& x : f)
+      if (x->nnz() == 0) return true;
+    return false;
+  }
+};
+
+// Transform::get_product 'N' (faust_Transform.hpp:599-626).
+Mat get_product(const Chain& c) {
+  if (c.f.empty()) throw std::runtime_error("get_product: empty Faust");
+  size_t n = c.f.size();
+  if (n == 1) return dense_of(*c.f[0]);
+  Mat mat = matmul_new(*c.f[n - 2], 'N', *c.f[n - 1], 'N');
+  for (int i = (int)n - 3; i >= 0; --i) mat = matmul_new(*c.f[i], 'N', mat, 'N');
+  return mat;
+}
+
+// y = op-chain applied right to left to a vector, each step a canonical mat-vec.
+std::vector<double> matvec(const Mat& A, char op, const std::vector<double>& x) {
+  Mat X;
+  X.kind = Mat::DENSE;
+  X.m = (int)x.size();
+  X.n = 1;
+  X.d = x;
+  Mat y = matmul_new(A, op, X, 'N');
+  return y.d;
+}
+
+void power_x0(int n, std::vector<double>& x) {
+  // power_iteration: srand(0xF4+'u'+57); xk.setRand()  (faust_linear_algebra.hpp:836-837)
+  // Eigen 3.4 Random for a real scalar on [-1,1]: x + (y-x)*Scalar(std::rand())/Scalar(RAND_MAX)
+  srand(0xF4 + 'u' + 57);
+  x.resize(n);
+  for (int i = 0; i < n; ++i) x[i] = -1.0 + (2.0 * (double)std::rand()) / (double)RAND_MAX;
+}
+
+// power_iteration over the chain ops (applied last to first), returns lambda.
+// faust_linear_algebra.hpp:808-898
+double power_iteration(const std::vector<std::pair<MatP, char>>& ops, int n, int maxit, double thr, int& flag) {
+  if (maxit <= 0) throw std::runtime_error("power_iteration: nbr_iter_max <= 0");
+  std::vector<double> xk, xk_norm;
+  power_x0(n, xk);
+  double lambda_old = 1.0, lambda = 0.0;
+  int i = 0;
+  flag = 0;
+  while ((std::fabs(lambda_old - lambda) > std::fabs(thr) || std::fabs(lambda) <= std::fabs(thr)) && i < maxit) {
+    i++;
+    lambda_old = lambda;
+    xk_norm = xk;
+    double nrm = std::sqrt(rdot(xk_norm.data(), xk_norm.data(), n));
+    if (nrm == 0.0) throw std::domain_error("Can't normalize a zero-norm vector.");
+    double inv = 1 / nrm;
+    for (auto& v : xk_norm) v *= inv;
+    std::vector<double> y = xk_norm;
+    for (int s = (int)ops.size() - 1; s >= 0; --s) y = matvec(*ops[s].first, ops[s].second, y);
+    xk = y;
+    bool zero = true;
+    for (double v : xk)
+      if (v != 0.0) {
+        zero = false;
+        break;
+      }
+    if (zero) return 0.0;
+    lambda = rdot(xk_norm.data(), xk.data(), n);
+  }
+  flag = (i < maxit) ? i : -1;
+  return lambda;
+}
+
+// Transform::spectralNorm (faust_Transform.hpp:684-709): the factors are not
+// flagged orthogonal on this path, so the whole chain is used.
+double spectral_norm(const Chain& c, int maxit, double thr, int& flag) {
+  if (c.f.empty()) return 1.0;
+  if (c.is_zero()) return 0.0;
+  std::vector<std::pair<MatP, char>> ops;
+  int n;
+  if (c.ncols() < c.nrows()) {
+    // AtA = F^H F : list [S_J^H..S_1^H, S_1..S_J]
+    for (int i = (int)c.f.size() - 1; i >= 0; --i) ops.emplace_back(c.f[i], 'T');
+    for (auto& x : c.f) ops.emplace_back(x, 'N');
+    n = c.ncols();
+  } else {
+    // AtA = F F^H : list [S_1..S_J, S_J^H..S_1^H]
+    for (auto& x : c.f) ops.emplace_back(x, 'N');
+    for (int i = (int)c.f.size() - 1; i >= 0; --i) ops.emplace_back(c.f[i], 'T');
+    n = c.nrows();
+  }
+  double lam = power_iteration(ops, n, maxit, thr, flag);
+  return std::fabs(std::sqrt(lam));
+}
+
+// ---------------------------------------------------------------- projectors
+bool partial_sort_comp(const std::pair<int, double>& a, const std::pair<int, double>& b) {
+  return std::fabs(a.second) > std::fabs(b.second);
+}
+
+// sort_idx (faust_prox.hpp:59-69): std::partial_sort of (index,value) pairs.
+std::vector<int> sort_idx(const std::vector<double>& v, int s) {
+  std::vector<std::pair<int, double>> vp(v.size());
+  for (size_t i = 0; i < v.size(); ++i) vp[i] = std::make_pair((int)i, v[i]);
+  std::partial_sort(vp.begin(), vp.begin() + s, vp.end(), partial_sort_comp);
+  std::vector<int> idx(s);
+  for (int i = 0; i < s; ++i) idx[i] = vp[i].first;
+  return idx;
+}
+
+void pre_prox_pos(std::vector<double>& M) {
+  // faust_prox.hpp:309-320: the test goes through a complex<float> conversion
+  for (auto& x : M)
+    if ((float)x < 0.0f) x = 0.0;
+}
+
+void normalize(std::vector<double>& M) {
+  // MatDense::normalize (faust_MatDense.hpp:1226-1250): M *= 1/||M||_F
+  double n = std::sqrt(rdot(M.data(), M.data(), (int64_t)M.size()));
+  if (n != 0.0) {
+    double inv = 1.0 / n;
+    for (auto& x : M) x *= inv;
+  } else
+    throw std::domain_error("the norm is zero, can't normalize.");
+}
+
+void prox_sp(std::vector<double>& M, int m, int n, int64_t k, bool normalized, bool pos) {
+  int64_t N = (int64_t)m * n;
+  if (pos) pre_prox_pos(M);
+  if (k <= 0)
+    std::fill(M.begin(), M.end(), 0.0);
+  else if (k < N) {
+    std::vector<double> vec(M);
+    std::vector<int> idx = sort_idx(vec, (int)k);
+    std::fill(M.begin(), M.end(), 0.0);
+    for (int i : idx) M[i] = vec[i];
+  }
+  if (normalized) normalize(M);
+}
+
+void prox_spcol(std::vector<double>& M, int m, int n, int k, bool normalized, bool pos) {
+  if (pos) pre_prox_pos(M);
+  if (k <= 0)
+    std::fill(M.begin(), M.end(), 0.0);
+  else if (k < m) {
+    std::vector<double> out((size_t)m * n, 0.0);
+    for (int j = 0; j < n; ++j) {
+      std::vector<double> col(M.begin() + (size_t)j * m, M.begin() + (size_t)(j + 1) * m);
+      for (int i : sort_idx(col, k)) out[(size_t)j * m + i] = col[i];
+    }
+    M.swap(out);
+  }
+  if (normalized) normalize(M);
+}
+
+void prox_splin(std::vector<double>& M, int m, int n, int k, bool normalized, bool pos) {
+  if (pos) pre_prox_pos(M);
+  if (k <= 0)
+    std::fill(M.begin(), M.end(), 0.0);
+  else if (k < n) {
+    std::vector<double> out((size_t)m * n, 0.0);
+    std::vector<double> row(n);
+    for (int i = 0; i < m; ++i) {
+      for (int j = 0; j < n; ++j) row[j] = M[(size_t)j * m + i];
+      for (int j : sort_idx(row, k)) out[(size_t)j * m + i] = row[j];
+    }
+    M.swap(out);
+  }
+  if (normalized) normalize(M);
+}
+
+void prox_splincol(std::vector<double>& M, int m, int n, int k, bool normalized, bool pos) {
+  // faust_prox.hpp:180-200: pos only touches M, which is overwritten (copies taken first)
+  std::vector<double> Mc(M), Ml(M);
+  if (pos) pre_prox_pos(M);
+  prox_spcol(Mc, m, n, k, false, false);
+  prox_splin(Ml, m, n, k, false, false);
+  for (size_t i = 0; i < M.size(); ++i)
+    if (Mc[i] != 0.0) Ml[i] = 0;
+  for (size_t i = 0; i < M.size(); ++i) Mc[i] += Ml[i];
+  M.swap(Mc);
+  if (normalized) normalize(M);
+}
+
+void prox_supp(std::vector<double>& M, const double* S, bool normalized, bool pos) {
+  if (pos) pre_prox_pos(M);
+  for (size_t i = 0; i < M.size(); ++i) M[i] *= S[i];
+  if (normalized) normalize(M);
+}
+
+void apply_prox(const or_cons& c, std::vector<double>& M) {
+  switch (c.name) {
+    case OR_SP: prox_sp(M, c.nrows, c.ncols, c.k, c.normalized, c.pos); break;
+    case OR_SPCOL: prox_spcol(M, c.nrows, c.ncols, c.k, c.normalized, c.pos); break;
+    case OR_SPLIN: prox_splin(M, c.nrows, c.ncols, c.k, c.normalized, c.pos); break;
+    case OR_SPLINCOL: prox_splincol(M, c.nrows, c.ncols, c.k, c.normalized, c.pos); break;
+    case OR_SUPP:
+      if (!c.supp) throw std::runtime_error("supp constraint without support matrix");
+      prox_supp(M, c.supp, c.normalized, c.pos);
+      break;
+    default: throw std::runtime_error("unknown constraint");
+  }
+}
+
+int64_t sparse_size(int64_t nnz, int64_t nrows) { return nnz * (8 + 4) + (nrows + 1) * 4; }
+int64_t dense_size(int64_t nrows, int64_t ncols) { return nrows * ncols * 8; }
+
+// ConstraintInt/ConstraintMat::project_gen (faust_prox_gen.hpp:118-210)
+MatP project_gen(const or_cons& c, Mat D) {
+  int64_t m = c.nrows, n = c.ncols;
+  bool out_dense;
+  std::vector<double>& M = D.d;
+  switch (c.name) {
+    case OR_SP: out_dense = sparse_size(c.k, m) > dense_size(m, n); break;
+    case OR_SPCOL:
+    case OR_SPLINCOL: out_dense = sparse_size((int64_t)c.k * m, m) > dense_size(m, n); break;
+    case OR_SPLIN: out_dense = sparse_size((int64_t)c.k * n, m) > dense_size(m, n); break;
+    default: out_dense = false; break;  // decided after projection from the real nnz
+  }
+  apply_prox(c, M);
+  if (c.name == OR_SUPP) out_dense = sparse_size(D.nnz(), m) > dense_size(m, n);
+  if (out_dense) return std::make_shared<Mat>(D);
+  return std::make_shared<Mat>(sparse_of_dense(D));
+}
+
+// ---------------------------------------------------------------- dynprog
+// faust_prod_opt.cpp:6-86 (int arithmetic; the wrap-around of large products
+// is reproduced with unsigned 32-bit arithmetic)
+int32_t imul(int64_t a, int64_t b) { return (int32_t)(uint32_t)((uint64_t)a * (uint64_t)b); }
+int32_t iadd(int32_t a, int32_t b) { return (int32_t)((uint32_t)a + (uint32_t)b); }
+int32_t cost_dense_dense(int a_nr, int a_nc, int b_nr, int b_nc, bool b_tr) {
+  int32_t c = imul(a_nr, a_nc);
+  return b_tr ? imul(c, b_nr) : imul(c, b_nc);
+}
+int32_t cost_dense_sparse(int a_nr, int a_nc, bool a_tr, int b_nnz) { return a_tr ? imul(a_nc, b_nnz) : imul(a_nr, b_nnz); }
+int32_t cost_sparse_dense(int a_nnz, int b_nr, int b_nc, bool b_tr) { return b_tr ? imul(a_nnz, b_nr) : imul(a_nnz, b_nc); }
+int32_t cost_sparse_sparse(int a_nr, int a_nc, int a_nnz, bool a_tr, int b_nnz) {
+  int32_t p = imul(a_nnz, b_nnz);
+  return a_tr ? p / a_nc : p / a_nr;
+}
+
+// dynprog_multiply (faust_prod_opt_gen.hpp:81-200), op 'N' only (the PALM
+// packing path); BSR factors are not produced by PALM.
+Mat dynprog_multiply(const std::vector<MatP>& f) {
+  int n = (int)f.size();
+  if (n == 1) return dense_of(*f[0]);
+  std::vector<std::vector<int32_t>> c(n, std::vector<int32_t>(n, 0));
+  std::vector<std::vector<int>> inds(n, std::vector<int>(n, 0));
+  for (int p = 1; p < n; ++p)
+    for (int i = 0; i < n - p; ++i) {
+      int j = p + i;
+      bool lf_dense = f[i]->kind == Mat::DENSE, rf_dense = f[j]->kind == Mat::DENSE;
+      c[i][j] = INT_MAX;
+      for (int k = i; k < j; ++k) {
+        int32_t cost = iadd(c[i][k], c[k + 1][j]);
+        int mf_nc = f[k]->n;
+        int32_t add;
+        if (lf_dense && rf_dense)
+          add = cost_dense_dense(f[i]->m, mf_nc, f[j]->m, f[j]->n, false);
+        else if (lf_dense)
+          add = cost_dense_sparse(f[i]->m, mf_nc, false, (int)f[j]->nnz());
+        else if (rf_dense)
+          add = cost_sparse_dense((int)f[i]->nnz(), f[j]->m, f[j]->n, false);
+        else
+          add = cost_sparse_sparse(f[i]->m, f[i]->n, (int)f[i]->nnz(), false, (int)f[j]->nnz());
+        cost = iadd(cost, add);
+        if (cost < c[i][j]) {
+          c[i][j] = cost;
+          inds[i][j] = k;
+        }
+      }
+    }
+  std::function<Mat(int, int)> rec = [&](int i, int j) -> Mat {
+    if (i == j) return dense_of(*f[i]);
+    if (j - i == 1) return matmul_new(*f[i], 'N', *f[j], 'N');
+    int k = inds[i][j];
+    Mat p1 = (k == i) ? Mat() : rec(i, k);
+    Mat p2 = (k + 1 == j) ? Mat() : rec(k + 1, j);
+    const Mat& l = (k == i) ? *f[i] : p1;
+    const Mat& r = (k + 1 == j) ? *f[j] : p2;
+    return matmul_new(l, 'N', r, 'N');
+  };
+  return rec(0, n - 1);
+}
+
+// pack_factors(DYNPROG) (faust_TransformHelper.hpp:1666-1716) of a whole chain
+Chain packed(const Chain& c) {
+  Chain out;
+  if (c.f.empty()) return out;
+  if (c.f.size() == 1) {
+    if (c.f[0]->kind == Mat::DENSE) {
+      out.f.push_back(c.f[0]);
+      return out;
+    }
+    out.f.push_back(std::make_shared<Mat>(dense_of(*c.f[0])));
+    return out;
+  }
+  out.f.push_back(std::make_shared<Mat>(dynprog_multiply(c.f)));
+  return out;
+}
+
+// ---------------------------------------------------------------- PALM4MSA 2020
+struct TrajPoint {
+  double call, it, relerr, lambda;
+};
+
+struct Palm {
+  // inputs
+  const Mat* A;
+  std::vector<or_cons> cons;
+  or_stop sc;
+  or_params p;
+  int call_id;
+  std::vector<TrajPoint>* traj;
+};
+
+// multiply_order_opt_all_ends (faust_prod_opt.hpp:49-86), MatGeneric version
+void mul_all_ends(std::vector<const Mat*> facts, Mat& out, double alpha, double beta, std::vector<char> flags) {
+  int nf = (int)facts.size();
+  int ri = 0, li = nf - 1;
+  Mat tmpr, tmpl;
+  auto flag = [&](int i) { return flags[(int)flags.size() > i ? i : 0]; };
+  while (li - ri > 1) {
+    const Mat *R1 = facts[ri], *R2 = facts[ri + 1], *L1 = facts[li - 1], *L2 = facts[li];
+    int64_t lhs = (int64_t)R1->m * R1->n * R2->n;
+    int64_t rhs = (int64_t)L1->m * L1->n * L2->n;
+    if (lhs < rhs) {
+      Mat t;
+      matmul(*R1, flag(ri), *R2, flag(ri + 1), 1.0, 0.0, t);
+      tmpr = std::move(t);
+      ri++;
+      facts[ri] = &tmpr;
+      if ((int)flags.size() > ri) flags[ri] = 'N';
+    } else {
+      Mat t;
+      matmul(*L1, flag(li - 1), *L2, flag(li), 1.0, 0.0, t);
+      tmpl = std::move(t);
+      li--;
+      facts[li] = &tmpl;
+      if ((int)flags.size() > li) flags[li] = 'N';
+    }
+  }
+  char opl = ri == 0 ? flags[0] : 'N';
+  char opr = (li == nf - 1 && (int)flags.size() > li) ? flags[li] : 'N';
+  matmul(*facts[ri], opl, *facts[li], opr, alpha, beta, out);
+}
+
+double calc_rel_err(const Chain& S, const Mat& A, double lambda) {
+  Mat err = get_product(S);
+  for (auto& x : err.d) x *= lambda;
+  for (size_t i = 0; i < err.d.size(); ++i) err.d[i] -= A.d[i];
+  double ne = std::sqrt(rdot(err.d.data(), err.d.data(), (int64_t)err.d.size()));
+  double na = std::sqrt(rdot(A.d.data(), A.d.data(), (int64_t)A.d.size()));
+  return ne / na;
+}
+
+bool sc_continue(const or_stop& sc, int it, double err) {
+  if (!sc.is_criterion_error) return it < sc.num_its;
+  if (err >= 0) {
+    if (err < sc.error_threshold) return false;
+    return it < sc.max_num_its;
+  }
+  return true;
+}
+
+// grad2: compute_n_apply_grad2 (faust_palm4msa2020.hpp:363-439)
+void grad2(const Chain& pL, const Chain& pR, const Mat& A, double lambda, double c, Mat& D, const or_stop& sc,
+           double& error) {
+  Mat tmp = A;
+  const Mat *L = nullptr, *R = nullptr;
+  Mat Ld, Rd;
+  if (!pR.f.empty()) {
+    if (pR.f.size() == 1)
+      R = pR.f[0].get();
+    else {
+      Rd = get_product(pR);
+      R = &Rd;
+    }
+  }
+  if (!pL.f.empty()) {
+    if (pL.f.size() == 1)
+      L = pL.f[0].get();
+    else {
+      Ld = get_product(pL);
+      L = &Ld;
+    }
+  }
+  std::vector<const Mat*> facts;
+  std::vector<char> fl;
+  if (R && L) {
+    mul_all_ends({L, &D, R}, tmp, lambda, -1.0, {'N'});
+    if (sc.is_criterion_error) error = fro_norm(tmp);
+    facts = {L, &tmp, R};
+    fl = {'T', 'N', 'T'};
+  } else if (R) {
+    mul_all_ends({&D, R}, tmp, lambda, -1.0, {'N'});
+    if (sc.is_criterion_error) error = fro_norm(tmp);
+    facts = {&tmp, R};
+    fl = {'N', 'T'};
+  } else {
+    mul_all_ends({L, &D}, tmp, lambda, -1.0, {'N'});
+    if (sc.is_criterion_error) error = fro_norm(tmp);
+    facts = {L, &tmp};
+    fl = {'T', 'N'};
+  }
+  Mat g;
+  mul_all_ends(facts, g, lambda / c, 0.0, fl);
+  for (size_t i = 0; i < D.d.size(); ++i) D.d[i] -= g.d[i];
+}
+
+// update_lambda (faust_palm4msa2020.hpp:442-493)
+void update_lambda(const Chain& S, const std::vector<Chain>& pL, const std::vector<Chain>& pR, const Mat& A,
+                   double& lambda) {
+  size_t n = S.f.size();
+  bool packing = pR[0].f.size() == 1 && pL[n - 1].f.size() == 1;
+  Mat Smat;
+  if (packing) {
+    if ((int64_t)S.f[0]->n * pR[0].nrows() < (int64_t)pL[n - 1].ncols() * S.f[n - 1]->m) {
+      Chain c;
+      c.f.push_back(S.f[0]);
+      for (auto& x : pR[0].f) c.f.push_back(x);
+      Smat = get_product(c);
+    } else {
+      Chain c = pL[n - 1];
+      c.f.push_back(S.f[n - 1]);
+      Smat = get_product(c);
+    }
+  } else
+    Smat = get_product(S);
+  // trace(A^H S): diagonal entries by canonical fma chains, then a canonical sum
+  int m = A.m, nc = A.n;
+  std::vector<double> diag(nc);
+  for (int i = 0; i < nc; ++i) {
+    double acc = 0.0;
+    for (int k = 0; k < m; ++k) {
+      double a = A.d[(size_t)i * m + k], s = Smat.d[(size_t)i * m + k];
+      if (a != 0.0 && s != 0.0) acc = std::fma(a, s, acc);
+    }
+    diag[i] = acc;
+  }
+  double tr = rsum(diag.data(), nc);
+  double nS = std::sqrt(rdot(Smat.d.data(), Smat.d.data(), (int64_t)Smat.d.size()));
+  if (nS == 0.0)
+    throw std::runtime_error("Error in update_lambda: S Frobenius norm is zero, can't compute lambda.");
+  if (std::isnan(tr) || std::isnan(nS))
+    throw std::runtime_error("Error in update_lambda: S (the Faust) contains nan elements");
+  lambda = tr / (nS * nS);
+}
+
+void palm4msa2(Palm& P, Chain& S, double& lambda) {
+  const Mat& A = *P.A;
+  const or_params& p = P.p;
+  int nf = (int)P.cons.size();
+  if (nf == 0) throw std::runtime_error("No constraint passed to palm4msa.");
+  std::vector<std::pair<int, int>> dims;
+  for (auto& c : P.cons) dims.emplace_back(c.nrows, c.ncols);
+  if (p.no_normalization)
+    for (auto& c : P.cons) c.normalized = 0;
+  const int AllDense = 0, AllSparse = 1;
+  if ((int)S.f.size() != nf) {
+    S.f.clear();
+    auto zero = std::make_shared<Mat>();
+    int zr = p.is_update_way_R2L ? dims.back().first : dims[0].first;
+    int zc = p.is_update_way_R2L ? dims.back().second : dims[0].second;
+    zero->m = zr;
+    zero->n = zc;
+    if (p.factor_format != AllDense) {
+      zero->kind = Mat::SPARSE;
+      zero->rp.assign(zr + 1, 0);
+    } else {
+      zero->kind = Mat::DENSE;
+      zero->d.assign((size_t)zr * zc, 0.0);
+    }
+    auto eye = [&](int r, int c) {
+      auto e = std::make_shared<Mat>();
+      e->m = r;
+      e->n = c;
+      int mn = std::min(r, c);
+      if (p.factor_format != AllDense) {
+        e->kind = Mat::SPARSE;
+        e->rp.assign(r + 1, 0);
+        for (int i = 0; i < r; ++i) e->rp[i + 1] = e->rp[i] + (i < mn ? 1 : 0);
+        for (int i = 0; i < mn; ++i) {
+          e->ci.push_back(i);
+          e->v.push_back(1.0);
+        }
+      } else {
+        e->kind = Mat::DENSE;
+        e->d.assign((size_t)r * c, 0.0);
+        for (int i = 0; i < mn; ++i) e->d[(size_t)i * r + i] = 1.0;
+      }
+      return e;
+    };
+    if (p.is_update_way_R2L) {
+      for (int i = 0; i < nf - 1; ++i) S.f.push_back(eye(dims[i].first, dims[i].second));
+      S.f.push_back(zero);
+    } else {
+      S.f.push_back(zero);
+      for (int i = 1; i < nf; ++i) S.f.push_back(eye(dims[i].first, dims[i].second));
+    }
+  } else if (p.factor_format == AllSparse) {
+    for (auto& x : S.f)
+      if (x->kind != Mat::SPARSE) x = std::make_shared<Mat>(sparse_of_dense(dense_of(*x)));
+  } else if (p.factor_format == AllDense) {
+    for (auto& x : S.f)
+      if (x->kind != Mat::DENSE) x = std::make_shared<Mat>(dense_of(*x));
+  }
+  std::vector<Chain> pL(nf), pR(nf);
+  int f_id = 0;
+  auto pack = [&](Chain c) { return p.packing_RL ? packed(c) : c; };
+  double error = -1;
+  int it = 0;
+  while (sc_continue(P.sc, it, error)) {
+    // init_ite
+    if (p.is_update_way_R2L) {
+      pL[0] = Chain();
+      for (int i = 1; i < nf; ++i) {
+        Chain c = pL[i - 1];
+        c.f.push_back(S.f[i - 1]);
+        pL[i] = pack(c);
+      }
+      f_id = nf - 1;
+    } else {
+      pR[nf - 1] = Chain();
+      for (int i = nf - 2; i >= 0; --i) {
+        Chain c;
+        c.f.push_back(S.f[i + 1]);
+        for (auto& x : pR[i + 1].f) c.f.push_back(x);
+        pR[i] = pack(c);
+      }
+      f_id = 0;
+    }
+    while (p.is_update_way_R2L ? f_id >= 0 : f_id < nf) {
+      // update_fact (faust_palm4msa2020.hpp:528-676)
+      double nR = 1, nL = 1, c;
+      int flag;
+      if (p.constant_step_size)
+        c = 1 / p.step_size;
+      else {
+        if (!pR[f_id].f.empty()) nR = spectral_norm(pR[f_id], p.norm2_max_iter, p.norm2_threshold, flag);
+        if (!pL[f_id].f.empty()) nL = spectral_norm(pL[f_id], p.norm2_max_iter, p.norm2_threshold, flag);
+        if (std::isnan(nL) || std::isnan(nR)) throw std::runtime_error("2-norm computation error: R or L 2-norm is NaN.");
+        if ((nR == 0 && (it > 0 || !p.is_update_way_R2L)) || (nL == 0 && (it > 0 || p.is_update_way_R2L)))
+          throw std::runtime_error("2-norm computation error in computation of descent step inverse: R or L 2-norm is zero.");
+        c = 1.001 * lambda * lambda * nR * nR * nL * nL;
+      }
+      Mat D = dense_of(*S.f[f_id]);
+      grad2(pL[f_id], pR[f_id], A, lambda, c, D, P.sc, error);
+      const or_cons& cons = P.cons[f_id];
+      if (p.factor_format == 2) {
+        S.f[f_id] = project_gen(cons, std::move(D));
+      } else {
+        apply_prox(cons, D.d);
+        if (p.factor_format == AllSparse)
+          S.f[f_id] = std::make_shared<Mat>(sparse_of_dense(D));
+        else
+          S.f[f_id] = std::make_shared<Mat>(D);
+      }
+      // next_fid
+      if (p.is_update_way_R2L) {
+        if (f_id > 0) {
+          Chain c2;
+          c2.f.push_back(S.f[f_id]);
+          for (auto& x : pR[f_id].f) c2.f.push_back(x);
+          pR[f_id - 1] = pack(c2);
+        }
+        f_id--;
+      } else {
+        if (f_id < nf - 1) {
+          Chain c2 = pL[f_id];
+          c2.f.push_back(S.f[f_id]);
+          pL[f_id + 1] = pack(c2);
+        }
+        f_id++;
+      }
+    }
+    if (!p.no_lambda) update_lambda(S, pL, pR, A, lambda);
+    if (P.traj) P.traj->push_back({(double)P.call_id, (double)it, calc_rel_err(S, A, lambda), lambda});
+    it++;
+  }
+}
+
+Mat dense_from_ptr(const double* A, int m, int n) {
+  Mat M;
+  M.kind = Mat::DENSE;
+  M.m = m;
+  M.n = n;
+  M.d.assign(A, A + (size_t)m * n);
+  return M;
+}
+
+MatP mat_of_fac(const or_fac& f) {
+  auto M = std::make_shared<Mat>();
+  M->m = f.nrows;
+  M->n = f.ncols;
+  if (f.kind == OR_DENSE) {
+    M->kind = Mat::DENSE;
+    M->d.assign(f.dense, f.dense + (size_t)f.nrows * f.ncols);
+  } else if (f.kind == OR_SPARSE) {
+    M->kind = Mat::SPARSE;
+    M->rp.assign(f.rowptr, f.rowptr + f.nrows + 1);
+    M->ci.assign(f.colind, f.colind + f.nnz);
+    M->v.assign(f.vals, f.vals + f.nnz);
+  } else {
+    M->kind = Mat::BSR;
+    M->bm = f.bm;
+    M->bn = f.bn;
+    int nbr = f.nrows / f.bm;
+    M->brp.assign(f.browptr, f.browptr + nbr + 1);
+    M->bci.assign(f.bcolinds, f.bcolinds + f.bnnz);
+    M->bd.assign(f.bdata, f.bdata + (size_t)f.bnnz * f.bm * f.bn);
+  }
+  return M;
+}
+
+Chain chain_of(const or_fac* facs, int n) {
+  Chain c;
+  for (int i = 0; i < n; ++i) c.f.push_back(mat_of_fac(facs[i]));
+  return c;
+}
+
+}  // namespace
+
+struct or_result {
+  Chain S;
+  double lambda = 1.0;
+  std::vector<TrajPoint> traj;
+};
+
+#define OR_TRY(...)                  \
+  try {                              \
+    __VA_ARGS__;                     \
+    return 0;                        \
+  } catch (const std::exception& e) { \
+    g_err = e.what();                \
+    return 1;                        \
+  }
+
+extern "C" {
+
+const char* or_last_error(void) { return g_err.c_str(); }
+
+double or_reduce_sum(const double* x, int64_t n) { return rsum(x, n); }
+double or_reduce_dot(const double* x, const double* y, int64_t n) { return rdot(x, y, n); }
+
+int or_gemm(char opA, char opB, int m, int n, int k, double alpha, const double* A, int lda, const double* B, int ldb,
+            double beta, double* C, int ldc) {
+  OR_TRY({
+    int ar = opA == 'N' ? m : k, ac = opA == 'N' ? k : m;
+    int br = opB == 'N' ? k : n, bc = opB == 'N' ? n : k;
+    Mat a, b, c;
+    a.m = ar;
+    a.n = ac;
+    a.d.resize((size_t)ar * ac);
+    for (int j = 0; j < ac; ++j)
+      for (int i = 0; i < ar; ++i) a.d[(size_t)j * ar + i] = A[(size_t)j * lda + i];
+    b.m = br;
+    b.n = bc;
+    b.d.resize((size_t)br * bc);
+    for (int j = 0; j < bc; ++j)
+      for (int i = 0; i < br; ++i) b.d[(size_t)j * br + i] = B[(size_t)j * ldb + i];
+    c.m = m;
+    c.n = n;
+    c.d.resize((size_t)m * n);
+    for (int j = 0; j < n; ++j)
+      for (int i = 0; i < m; ++i) c.d[(size_t)j * m + i] = C[(size_t)j * ldc + i];
+    char oa = opA == 'H' ? 'T' : opA, ob = opB == 'H' ? 'T' : opB;
+    matmul(a, oa, b, ob, alpha, beta, c);
+    for (int j = 0; j < n; ++j)
+      for (int i = 0; i < m; ++i) C[(size_t)j * ldc + i] = c.d[(size_t)j * m + i];
+  })
+}
+
+int or_chain_multiply(const or_fac* facs, int nfacs, char op, const double* X, int p, double* Y, int nthreads) {
+  OR_TRY({
+    if (nfacs <= 0) throw std::runtime_error("empty Faust");
+    g_threads = nthreads;
+    Chain c = chain_of(facs, nfacs);
+    int rows_in = op == 'N' ? c.ncols() : c.nrows();
+    Mat cur = dense_from_ptr(X, rows_in, p);
+    char o = op == 'H' ? 'T' : op;
+    if (op == 'N')
+      for (int i = nfacs - 1; i >= 0; --i) cur = matmul_new(*c.f[i], 'N', cur, 'N');
+    else
+      for (int i = 0; i < nfacs; ++i) cur = matmul_new(*c.f[i], o, cur, 'N');
+    std::memcpy(Y, cur.d.data(), sizeof(double) * cur.d.size());
+    g_threads = 0;
+  })
+}
+
+int or_get_product(const or_fac* facs, int nfacs, double* out) {
+  OR_TRY({
+    Chain c = chain_of(facs, nfacs);
+    Mat P = get_product(c);
+    std::memcpy(out, P.d.data(), sizeof(double) * P.d.size());
+  })
+}
+
+int or_spectral_norm(const or_fac* facs, int nfacs, int maxit, double thr, double* out_norm, int* out_flag) {
+  OR_TRY({
+    Chain c = chain_of(facs, nfacs);
+    int flag = 0;
+    *out_norm = spectral_norm(c, maxit, thr, flag);
+    if (out_flag) *out_flag = flag;
+  })
+}
+
+void or_power_iteration_x0(int n, double* out) {
+  std::vector<double> x;
+  power_x0(n, x);
+  std::memcpy(out, x.data(), sizeof(double) * n);
+}
+
+int or_prox(const or_cons* c, double* M) {
+  OR_TRY({
+    std::vector<double> v(M, M + (size_t)c->nrows * c->ncols);
+    apply_prox(*c, v);
+    std::memcpy(M, v.data(), sizeof(double) * v.size());
+  })
+}
+
+int or_palm4msa(const double* A, int m, int n, const or_cons* cons, int ncons, const or_stop* sc, const or_params* p,
+                double lambda_in, const or_fac* init_facs, int n_init, int record_traj, or_result** out) {
+  OR_TRY({
+    auto r = std::make_unique<or_result>();
+    Mat Am = dense_from_ptr(A, m, n);
+    Palm P{&Am, std::vector<or_cons>(cons, cons + ncons), *sc, *p, 0, record_traj ? &r->traj : nullptr};
+    if (init_facs && n_init > 0) r->S = chain_of(init_facs, n_init);
+    r->lambda = lambda_in;
+    palm4msa2(P, r->S, r->lambda);
+    *out = r.release();
+  })
+}
+
+int or_hierarchical(const double* A, int m, int n, const or_cons* fac_cons, const or_cons* res_cons, int nlevels,
+                    const or_stop* sc_local, const or_stop* sc_global, const or_params* p, int record_traj,
+                    or_result** out) {
+  OR_TRY({
+    // faust_hierarchical.hpp:1-150
+    auto r = std::make_unique<or_result>();
+    Mat Am = dense_from_ptr(A, m, n);
+    Chain& S = r->S;
+    S.f.push_back(std::make_shared<Mat>(Am));
+    double glo_lambda = 1.0;
+    bool R2L = p->is_update_way_R2L, left = p->is_fact_side_left;
+    for (int i = 0; i < nlevels; ++i) {
+      MatP Si = left ? S.f[0] : S.f[i];
+      const or_cons& fc = fac_cons[i];
+      const or_cons& rc = res_cons[i];
+      int zd[2], idd[2];
+      if ((!R2L && !left) || (R2L && left)) {
+        zd[0] = fc.nrows; zd[1] = fc.ncols; idd[0] = rc.nrows; idd[1] = rc.ncols;
+      } else {
+        zd[0] = rc.nrows; zd[1] = rc.ncols; idd[0] = fc.nrows; idd[1] = fc.ncols;
+      }
+      auto zero = std::make_shared<Mat>();
+      auto id = std::make_shared<Mat>();
+      zero->m = zd[0]; zero->n = zd[1];
+      id->m = idd[0]; id->n = idd[1];
+      int mn = std::min(idd[0], idd[1]);
+      if (p->factor_format == 0) {
+        zero->kind = Mat::DENSE;
+        zero->d.assign((size_t)zd[0] * zd[1], 0.0);
+        id->kind = Mat::DENSE;
+        id->d.assign((size_t)idd[0] * idd[1], 0.0);
+        for (int t = 0; t < mn; ++t) id->d[(size_t)t * idd[0] + t] = 1.0;
+      } else {
+        zero->kind = Mat::SPARSE;
+        zero->rp.assign(zd[0] + 1, 0);
+        id->kind = Mat::SPARSE;
+        id->rp.assign(idd[0] + 1, 0);
+        for (int t = 0; t < idd[0]; ++t) id->rp[t + 1] = id->rp[t] + (t < mn ? 1 : 0);
+        for (int t = 0; t < mn; ++t) { id->ci.push_back(t); id->v.push_back(1.0); }
+      }
+      Chain Sith;
+      if (R2L) Sith.f = {id, zero}; else Sith.f = {zero, id};
+      std::vector<or_cons> sic = left ? std::vector<or_cons>{rc, fc} : std::vector<or_cons>{fc, rc};
+      double lambda_ = 1.0;
+      Mat Sid = dense_of(*Si);
+      Palm P1{&Sid, sic, *sc_local, *p, 2 * i, record_traj ? &r->traj : nullptr};
+      palm4msa2(P1, Sith, lambda_);
+      glo_lambda *= lambda_;
+      if (left) {
+        S.f.erase(S.f.begin());
+        S.f.insert(S.f.begin(), Sith.f[1]);
+        S.f.insert(S.f.begin(), Sith.f[0]);
+      } else {
+        S.f.pop_back();
+        S.f.push_back(Sith.f[0]);
+        S.f.push_back(Sith.f[1]);
+      }
+      std::vector<or_cons> glo;
+      for (int t = 0; t <= i; ++t) {
+        if (left) glo.insert(glo.begin(), fac_cons[t]); else glo.push_back(fac_cons[t]);
+      }
+      if (left) glo.insert(glo.begin(), rc); else glo.push_back(rc);
+      Palm P2{&Am, glo, *sc_global, *p, 2 * i + 1, record_traj ? &r->traj : nullptr};
+      palm4msa2(P2, S, glo_lambda);
+    }
+    r->lambda = glo_lambda;
+    *out = r.release();
+  })
+}
+
+int or_result_nfacts(const or_result* r) { return (int)r->S.f.size(); }
+double or_result_lambda(const or_result* r) { return r->lambda; }
+int or_result_fact_info(const or_result* r, int i, int* kind, int* nrows, int* ncols, int* nnz) {
+  if (i < 0 || i >= (int)r->S.f.size()) return 1;
+  const Mat& M = *r->S.f[i];
+  *kind = (int)M.kind;
+  *nrows = M.m;
+  *ncols = M.n;
+  *nnz = (int)M.nnz();
+  return 0;
+}
+int or_result_fact_dense(const or_result* r, int i, double* out) {
+  if (i < 0 || i >= (int)r->S.f.size()) return 1;
+  Mat D = dense_of(*r->S.f[i]);
+  std::memcpy(out, D.d.data(), sizeof(double) * D.d.size());
+  return 0;
+}
+int or_result_traj_len(const or_result* r) { return (int)r->traj.size(); }
+int or_result_traj(const or_result* r, double* out) {
+  for (size_t i = 0; i < r->traj.size(); ++i) {
+    out[4 * i + 0] = r->traj[i].call;
+    out[4 * i + 1] = r->traj[i].it;
+    out[4 * i + 2] = r->traj[i].relerr;
+    out[4 * i + 3] = r->traj[i].lambda;
+  }
+  return 0;
+}
+void or_result_free(or_result* r) { delete r; }
+
+}  // extern "C"

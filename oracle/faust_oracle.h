@@ -1,0 +1,131 @@
+/*
+ * faust_oracle.h -- C interface of the CPU FP64 oracle.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is a plain-C++ restatement of the
+ * reference FAuST algorithms on the two hot paths (Faust x dense products and
+ * the PALM4MSA / hierarchical factorization loop).  It is loaded by the test
+ * suite, by __graft_entry__.smoke() and by bench.py's cpu_baseline leg as the
+ * checker.  Nothing in the product (faust_amd/) links or calls it.
+ *
+ * Arithmetic convention shared with the HIP kernels (the "canonical order"):
+ *   - every matrix-product entry is acc = 0; for k ascending: acc = fma(a,b,acc)
+ *     (terms with an exact-zero operand are skipped, which leaves acc unchanged);
+ *   - alpha/beta epilogue: t = alpha*acc; out = beta==0 ? t : fma(beta, c, t);
+ *   - sums/dots over n values: 4096-element chunks, 256 lanes per chunk, lane t
+ *     takes elements t, t+256, ..., a fixed halving tree over the 256 lanes,
+ *     then the chunk partials are added sequentially in chunk order.
+ * Eigen's own summation order cannot be reproduced without Eigen (absent);
+ * this convention replaces it, and the GPU path follows it bit for bit.
+ */
+#ifndef FAUST_ORACLE_H
+#define FAUST_ORACLE_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* factor kinds (MatDense / MatSparse / MatBSR) */
+enum { OR_DENSE = 0, OR_SPARSE = 1, OR_BSR = 2 };
+
+/* A factor handed across the C boundary: CSR int32 / col-major dense /
+ * BSR with column-major bm x bn blocks (faust_MatBSR.h:120-150). */
+typedef struct {
+  int kind;
+  int nrows, ncols;
+  /* OR_DENSE */
+  const double *dense;           /* nrows*ncols col-major */
+  /* OR_SPARSE */
+  int nnz;
+  const int *rowptr, *colind;
+  const double *vals;
+  /* OR_BSR */
+  int bm, bn, bnnz;
+  const int *browptr, *bcolinds;
+  const double *bdata;
+} or_fac;
+
+/* constraint names (subset of faust_ConstraintGeneric.h:49-70) */
+enum { OR_SP = 0, OR_SPCOL = 1, OR_SPLIN = 2, OR_SPLINCOL = 3, OR_SUPP = 4 };
+
+typedef struct {
+  int name;
+  int k;
+  int nrows, ncols;
+  int normalized;
+  int pos;
+  const double *supp; /* OR_SUPP only, nrows*ncols col-major */
+} or_cons;
+
+typedef struct {
+  int num_its;
+  int is_criterion_error;
+  double error_threshold;
+  int max_num_its;
+} or_stop;
+
+typedef struct {
+  int is_update_way_R2L;
+  int factor_format;    /* 0 dense, 1 sparse, 2 dynamic */
+  int packing_RL;
+  int no_normalization;
+  int no_lambda;
+  int norm2_max_iter;
+  double norm2_threshold;
+  int constant_step_size;
+  double step_size;
+  int is_fact_side_left; /* hierarchical only */
+} or_params;
+
+const char *or_last_error(void);
+
+/* canonical reductions */
+double or_reduce_sum(const double *x, int64_t n);
+double or_reduce_dot(const double *x, const double *y, int64_t n);
+
+/* C = alpha*op(A)*op(B) + beta*C, dense col-major operands */
+int or_gemm(char opA, char opB, int m, int n, int k, double alpha,
+            const double *A, int lda, const double *B, int ldb, double beta,
+            double *C, int ldc);
+
+/* Y = op(F) X, F = chain of factors (faust_Transform.hpp:1714-1812).
+ * X: (op=='N' ? ncols(F) : nrows(F)) x p col-major. */
+int or_chain_multiply(const or_fac *facs, int nfacs, char op, const double *X,
+                      int p, double *Y, int nthreads);
+
+/* dense product of the chain (faust_Transform.hpp:599-652, op 'N') */
+int or_get_product(const or_fac *facs, int nfacs, double *out);
+
+/* spectral norm by power iteration on the F F^H / F^H F chain
+ * (faust_Transform.hpp:684-709, faust_linear_algebra.hpp:808-898) */
+int or_spectral_norm(const or_fac *facs, int nfacs, int maxit, double thr,
+                     double *out_norm, int *out_flag);
+
+/* the start vector power_iteration draws: srand(0xF4+'u'+57); Random(n) */
+void or_power_iteration_x0(int n, double *out);
+
+/* proximal projectors on a dense col-major matrix (faust_prox.hpp) */
+int or_prox(const or_cons *c, double *M);
+
+/* PALM4MSA 2020 / hierarchical results */
+typedef struct or_result or_result;
+int or_palm4msa(const double *A, int m, int n, const or_cons *cons, int ncons,
+                const or_stop *sc, const or_params *p, double lambda_in,
+                const or_fac *init_facs, int n_init, int record_traj,
+                or_result **out);
+int or_hierarchical(const double *A, int m, int n, const or_cons *fac_cons,
+                    const or_cons *res_cons, int nlevels, const or_stop *sc_local,
+                    const or_stop *sc_global, const or_params *p,
+                    int record_traj, or_result **out);
+int or_result_nfacts(const or_result *r);
+double or_result_lambda(const or_result *r);
+int or_result_fact_info(const or_result *r, int i, int *kind, int *nrows,
+                        int *ncols, int *nnz);
+int or_result_fact_dense(const or_result *r, int i, double *out);
+int or_result_traj_len(const or_result *r);
+int or_result_traj(const or_result *r, double *out);
+void or_result_free(or_result *r);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
