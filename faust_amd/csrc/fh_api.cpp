@@ -87,14 +87,21 @@ DMatP make_dense(int dtype, int m, int n, const void* data, int ld, bool on_devi
 }
 
 // Y = op(F) X for an FP64 chain; X, Y col-major with ld = rows.
-void faust_multiply_f64(const std::vector<DMatP>& f, char op, const double* X, int p, double* Y, hipStream_t st) {
+// ws: two reusable intermediates, grown on demand (hipMalloc only when a
+// larger product than any before is requested).
+void faust_multiply_f64(const std::vector<DMatP>& f, char op, const double* X, int p, double* Y, hipStream_t st,
+                        DBuf* ws) {
   FH_CHECK(!f.empty(), "Faust::multiply: empty Faust");
   const int J = (int)f.size();
   int maxr = 0;
   for (auto& x : f) maxr = std::max(maxr, op == 'N' ? x->m : x->n);
-  DBuf t1, t2;
-  if (J > 1) t1.alloc(sizeof(double) * std::max<size_t>((size_t)maxr * p, 1));
-  if (J > 2) t2.alloc(sizeof(double) * std::max<size_t>((size_t)maxr * p, 1));
+  const size_t need = sizeof(double) * std::max<size_t>((size_t)maxr * p, 1);
+  for (int b = 0; b < 2 && b < J - 1; ++b)
+    if (ws[b].bytes < need) {
+      FH_HIP(hipStreamSynchronize(st));  // the old buffer may still be in use
+      ws[b].alloc(need);
+    }
+  DBuf &t1 = ws[0], &t2 = ws[1];
   const double* in = X;
   int in_rows = (op == 'N') ? f.back()->n : f.front()->m;
   for (int i = 0; i < J; ++i) {
@@ -113,7 +120,6 @@ void faust_multiply_f64(const std::vector<DMatP>& f, char op, const double* X, i
     in = out;
     in_rows = out_rows;
   }
-  if (J > 1) FH_HIP(hipStreamSynchronize(st));  // the intermediates return to the pool
 }
 
 // Transform::get_product (faust_Transform.hpp:599-626)
@@ -324,9 +330,9 @@ int fh_faust_multiply(fh_faust_t F, char op, const void* X, int p, void* Y, void
   FH_CHECK(p >= 0, "negative column count");
   hipStream_t st = (hipStream_t)stream;
   if (F->f.front()->dtype == FH_F64)
-    faust_multiply_f64(F->f, op, (const double*)X, p, (double*)Y, st);
+    faust_multiply_f64(F->f, op, (const double*)X, p, (double*)Y, st, F->ws);
   else
-    faust_multiply_mixed(F->f, op, X, p, Y, st);
+    faust_multiply_mixed(F->f, op, X, p, Y, st, F->ws);
   FH_API_END
 }
 

@@ -5,7 +5,7 @@ void mixed_to_dense(const DMat&, void*, hipStream_t) { throw Error(FH_ERR_UNSUPP
 void mixed_mul_dense(const DMat&, char, const void*, int, int, void*, int, const void*, const void*, hipStream_t) {
   throw Error(FH_ERR_UNSUPPORTED, "FP32/BF16 not built");
 }
-void faust_multiply_mixed(const std::vector<DMatP>&, char, const void*, int, void*, hipStream_t) {
+void faust_multiply_mixed(const std::vector<DMatP>&, char, const void*, int, void*, hipStream_t, DBuf*) {
   throw Error(FH_ERR_UNSUPPORTED, "FP32/BF16 not built");
 }
 }  // namespace fh
