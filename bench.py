@@ -5,8 +5,9 @@ python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME] [--no-cpu-
 Prints ONE JSON line.  `value` is the algorithmic HBM throughput (GB/s) of
 one step = one Faust x dense product (all factors) with inputs resident in
 HBM.  `roofline` reports the dominant kernel's algorithmic bytes per launch
-divided by its average launch duration (HIP events around a HIP-graph replay
-of back-to-back launches on the launch stream).  `cpu_baseline` is the
+divided by its average launch duration, measured with HIP events recorded on
+the launch stream around every factor kernel (the library's per-factor
+timers) over a second run of the same steps.  `cpu_baseline` is the
 oracle (CPU FP64 restatement of the reference) timed on this host on a
 bounded sample of the same workload.
 """
@@ -30,11 +31,10 @@ def _parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="chain_csr_f64")
+    ap.add_argument("--workload", default="chain_bsr_bf16")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--palm", action="store_true", help="also time hierarchical PALM4MSA (Hadamard 1024)")
-    ap.add_argument("--kernel-reps", type=int, default=200)
     return ap.parse_args()
 
 
@@ -52,45 +52,58 @@ def workload_chain_csr_f64():
                 flops_per_step=2 * J * 2 * n * nnz * p, kernel="k_csr_spmm_f64")
 
 
-WORKLOADS = {"chain_csr_f64": workload_chain_csr_f64}
+def _bsr_factors(seed, J=6, n=8192, bs=32, bpr=4):
+    from faust_amd.bsr import rand_bsr
+    return [rand_bsr(n, n, bs, bpr, np.random.default_rng(seed + f), scale=1 / np.sqrt(128)) for f in range(J)]
+
+
+def _bsr_workload(values):
+    """6 BSR factors 8192x8192, 32x32 blocks, 4 distinct uniform block columns
+    per block-row, values N(0, 1/128) stored BF16 (or FP32 as BF16 hi/lo
+    pairs); X 8192x512 BF16 N(0,1); FP32 accumulation on MFMA."""
+    n, J, bs, bpr, p = 8192, 6, 32, 4, 512
+    facs = _bsr_factors(0xB5B, J, n, bs, bpr)
+    X = np.asfortranarray(np.random.default_rng(0xB5B + 100).standard_normal((n, p)))
+    vbytes = 2 if values == "bf16" else 4
+    nb = (n // bs) * bpr
+    per_factor = nb * bs * bs * vbytes + nb * 4 + (n // bs + 1) * 4 + n * p * 2 + n * p * 2
+    return dict(name=f"faust_bsr_{values}_8192x8192_J6_b32x32_4blk_X8192x512_bf16", facs=facs, X=X,
+                dtype="bf16", fac_dtype=("bf16" if values == "bf16" else np.float32), ops=("N",),
+                bytes_per_launch=per_factor, launches_per_step=J, bytes_per_step=J * per_factor,
+                flops_per_step=J * 2 * nb * bs * bs * p, kernel="k_bsr_mfma<%s>" % ("false" if values == "bf16" else "true"),
+                cpu_reps_cap=1)
+
+
+def workload_chain_bsr_bf16():
+    return _bsr_workload("bf16")
+
+
+def workload_chain_bsr_f32():
+    return _bsr_workload("f32")
+
+
+def workload_chain_csr_f32():
+    """6 CSR factors 16384x16384, exactly 12 distinct uniform columns per row,
+    N(0,1) FP32; X 16384x256 FP32 N(0,1).  (The number of factors is not
+    fixed by the config text; 6 as for the BSR Faust.)"""
+    import faust_amd
+    n, J, nnz, p = 16384, 6, 12, 256
+    facs = [faust_amd.rand_csr(n, n, nnz, np.random.default_rng(0xC5F + f)) for f in range(J)]
+    facs = [f.astype(np.float32).astype(np.float64) for f in facs]
+    X = np.asfortranarray(np.random.default_rng(0xC5F + 100).standard_normal((n, p)).astype(np.float32)
+                          .astype(np.float64))
+    per_factor = nnz * n * (4 + 4) + (n + 1) * 4 + n * p * 4 + n * p * 4
+    return dict(name="faust_csr_fp32_16384x16384_J6_nnz12_X16384x256", facs=facs, X=X, dtype="f32",
+                fac_dtype=np.float32, ops=("N",), bytes_per_launch=per_factor, launches_per_step=J,
+                bytes_per_step=J * per_factor, flops_per_step=J * 2 * n * nnz * p, kernel="k_csr_rm_f32<false>",
+                cpu_reps_cap=3)
+
+
+WORKLOADS = {"chain_csr_f64": workload_chain_csr_f64, "chain_bsr_bf16": workload_chain_bsr_bf16,
+             "chain_bsr_f32": workload_chain_bsr_f32, "chain_csr_f32": workload_chain_csr_f32}
 
 
 # ----------------------------------------------------------------- helpers
-def _graph_time(fn, reps, warm=3):
-    """Average time (ms) of fn() replayed from a captured HIP graph, timed with
-    events on the capture stream.  Falls back to an eager loop."""
-    import torch
-    s = torch.cuda.Stream()
-    with torch.cuda.stream(s):
-        for _ in range(warm):
-            fn()
-    torch.cuda.synchronize()
-    g = torch.cuda.CUDAGraph()
-    try:
-        with torch.cuda.graph(g, stream=s):
-            for _ in range(reps):
-                fn()
-        graphed = True
-    except Exception:
-        graphed = False
-    torch.cuda.synchronize()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    with torch.cuda.stream(s):
-        if graphed:
-            g.replay()  # warm replay
-            torch.cuda.current_stream().synchronize()
-            e0.record()
-            g.replay()
-            e1.record()
-        else:
-            e0.record()
-            for _ in range(reps):
-                fn()
-            e1.record()
-    torch.cuda.synchronize()
-    return e0.elapsed_time(e1) / reps, graphed
-
-
 def cpu_baseline(wl, budget_s):
     import oracle
     nthreads = len(os.sched_getaffinity(0))
@@ -100,7 +113,7 @@ def cpu_baseline(wl, budget_s):
     for op in wl["ops"]:
         oracle.chain_multiply(wl["facs"], X, op, nthreads=nthreads)
     one = time.perf_counter() - t0
-    reps = max(1, min(200, int(budget_s / max(one, 1e-6))))
+    reps = max(1, min(wl.get("cpu_reps_cap", 200), int(budget_s / max(one, 1e-6))))
     t0 = time.perf_counter()
     for _ in range(reps):
         for op in wl["ops"]:
@@ -124,7 +137,7 @@ def main():
     torch.cuda.set_device(0)
     wl = WORKLOADS[args.workload]()
     tdt = {"f64": torch.float64, "f32": torch.float32, "bf16": torch.bfloat16}[wl["dtype"]]
-    F = faust_amd.Faust(wl["facs"], dtype={"f64": np.float64, "f32": np.float32, "bf16": "bf16"}[wl["dtype"]])
+    F = faust_amd.Faust(wl["facs"], dtype=wl.get("fac_dtype", np.float64))
     Xd = colmajor_to_device(wl["X"], tdt)
     FT = F.T
     outs = {op: torch.empty((Xd.shape[0], (F if op == "N" else FT).shape[0]), dtype=tdt, device="cuda")
@@ -137,43 +150,69 @@ def main():
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
+    # timed region: the K steps captured once in a HIP graph and replayed (no
+    # host launch gaps); eager launches behind a GPU spin are the fallback
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    graphed = False
+    try:
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            for _ in range(args.steps):
+                step()
+        g.replay()
+        torch.cuda.synchronize()
+        graphed = True
+    except Exception:
+        torch.cuda.synchronize()
     t0 = time.perf_counter()
-    e0.record()
-    for _ in range(args.steps):
-        step()
-    e1.record()
+    if graphed:
+        e0.record()
+        g.replay()
+        e1.record()
+    else:
+        torch.cuda._sleep(50_000_000)
+        e0.record()
+        for _ in range(args.steps):
+            step()
+        e1.record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ms_step = e0.elapsed_time(e1) / args.steps
     value = wl["bytes_per_step"] / (ms_step * 1e-3) / 1e9
 
-    # dominant kernel: one factor product, back-to-back launches in a HIP graph
-    f0 = F._facs[0]
-    p = Xd.shape[0]
-    r, c = f0.shape
-    xin = torch.randn((p, c), dtype=tdt, device="cuda")
-    yout = torch.empty((p, r), dtype=tdt, device="cuda")
+    # dominant kernel: per-factor HIP events on the launch stream around each
+    # factor's kernel, over a second run of the same steps
     L = _lib.lib()
-    one, zero = (C.c_double(1.0), C.c_double(0.0)) if wl["dtype"] == "f64" else (C.c_float(1.0), C.c_float(0.0))
-
-    def launch():
-        _lib.check(L.fh_mat_mul_dense(f0.h, b"N", C.c_void_p(xin.data_ptr()), c, p, C.c_void_p(yout.data_ptr()), r,
-                                      C.byref(one), C.byref(zero),
-                                      C.c_void_p(torch.cuda.current_stream().cuda_stream)))
-
-    k_ms, graphed = _graph_time(launch, args.kernel_reps)
+    J = F.numfactors()
+    for G in {id(F): F, id(FT): FT}.values():
+        _lib.check(L.fh_faust_enable_timers(G.h, 1))
+    torch.cuda._sleep(50_000_000)  # let the host queue every launch before the GPU starts
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    ms = np.zeros(J + 1)
+    cnt = np.zeros(J + 1, dtype=np.int64)
+    for G in {id(F): F, id(FT): FT}.values():
+        m1, c1 = np.zeros(J + 1), np.zeros(J + 1, dtype=np.int64)
+        _lib.check(L.fh_faust_timers(G.h, m1.ctypes.data_as(C.c_void_p), c1.ctypes.data_as(C.c_void_p)))
+        _lib.check(L.fh_faust_enable_timers(G.h, 0))
+        ms += m1
+        cnt += c1
+    k_ms = ms[:J].sum() / max(cnt[:J].sum(), 1)
     achieved = wl["bytes_per_launch"] / (k_ms * 1e-3) / 1e9
     roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": wl["kernel"],
-            "avg_launch_us": round(k_ms * 1e3, 3), "graph_timed": graphed}
+            "avg_launch_us": round(k_ms * 1e3, 3), "launches_timed": int(cnt[:J].sum()),
+            "bytes_per_launch": wl["bytes_per_launch"],
+            "layout_conversion_us_per_step": round(ms[J] / max(args.steps, 1) * 1e3, 3)}
 
     out = {"metric": "Faust·X HBM GB/s (% peak); PALM4MSA time-to-rel-err, tuned vs plain FP64",
            "value": round(value, 2), "unit": "GB/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
            "ms_per_step": round(ms_step, 5), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
            "dtype": wl["dtype"], "data": "synthetic (seeded N(0,1) values, uniform random supports)",
            "config": {"workload": wl["name"], "ops": list(wl["ops"]), "bytes_per_step": wl["bytes_per_step"],
-                      "frac_of_hbm_peak": round(value / HBM_PEAK_GBS, 4), "wall_s": round(wall, 4)},
+                      "frac_of_hbm_peak": round(value / HBM_PEAK_GBS, 4), "wall_s": round(wall, 4),
+                      "timed_as_hip_graph": graphed},
            "roofline": roof}
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(wl, args.cpu_seconds)

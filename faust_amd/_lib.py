@@ -64,6 +64,8 @@ _SIGS = {
     "fh_faust_free": (C.c_int, [VP]),
     "fh_faust_multiply": (C.c_int, [VP, C.c_char, VP, C.c_int, VP, VP]),
     "fh_faust_get_product": (C.c_int, [VP, VP, VP]),
+    "fh_faust_enable_timers": (C.c_int, [VP, C.c_int]),
+    "fh_faust_timers": (C.c_int, [VP, VP, VP]),
     "fh_faust_spectral_norm": (C.c_int, [VP, C.c_int, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_int), VP]),
     "fh_prox": (C.c_int, [C.c_int, C.c_int, VP, C.c_int, C.c_int, C.c_int64, C.c_int, C.c_int, VP, VP]),
     "fh_norm_frob": (C.c_int, [C.c_int, VP, C.c_int64, C.POINTER(C.c_double), VP]),

@@ -90,7 +90,7 @@ DMatP make_dense(int dtype, int m, int n, const void* data, int ld, bool on_devi
 // ws: two reusable intermediates, grown on demand (hipMalloc only when a
 // larger product than any before is requested).
 void faust_multiply_f64(const std::vector<DMatP>& f, char op, const double* X, int p, double* Y, hipStream_t st,
-                        DBuf* ws) {
+                        DBuf* ws, FaustTimers* tm) {
   FH_CHECK(!f.empty(), "Faust::multiply: empty Faust");
   const int J = (int)f.size();
   int maxr = 0;
@@ -110,6 +110,8 @@ void faust_multiply_f64(const std::vector<DMatP>& f, char op, const double* X, i
     FH_CHECK(((op == 'N') ? S.n : S.m) == in_rows, "Faust::multiply: factor dimensions do not chain");
     double* out = (i == J - 1) ? Y : ((i & 1) ? t2.as<double>() : t1.as<double>());
     FH_CHECK(S.dtype == FH_F64, "Faust::multiply: FP64 chain expected");
+    const size_t slot = (op == 'N') ? (size_t)(J - 1 - i) : (size_t)i;
+    if (tm) tm->begin(slot, st);
     if (S.kind == K_CSR)
       csr_spmm_f64(S, op, in, in_rows, p, out, out_rows, 1.0, 0.0, st);
     else if (S.kind == K_DENSE)
@@ -117,6 +119,7 @@ void faust_multiply_f64(const std::vector<DMatP>& f, char op, const double* X, i
                out_rows, st);
     else
       throw Error(FH_ERR_UNSUPPORTED, "Faust::multiply: BSR factors need the FP32/BF16 chain");
+    if (tm) tm->end(slot, st);
     in = out;
     in_rows = out_rows;
   }
@@ -330,9 +333,32 @@ int fh_faust_multiply(fh_faust_t F, char op, const void* X, int p, void* Y, void
   FH_CHECK(p >= 0, "negative column count");
   hipStream_t st = (hipStream_t)stream;
   if (F->f.front()->dtype == FH_F64)
-    faust_multiply_f64(F->f, op, (const double*)X, p, (double*)Y, st, F->ws);
+    faust_multiply_f64(F->f, op, (const double*)X, p, (double*)Y, st, F->ws, &F->timers);
   else
-    faust_multiply_mixed(F->f, op, X, p, Y, st, F->ws);
+    faust_multiply_mixed(F->f, op, X, p, Y, st, F->ws, &F->timers);
+  FH_API_END
+}
+
+int fh_faust_enable_timers(fh_faust_t F, int on) {
+  FH_API_BEGIN
+  FH_CHECK(F, "null handle");
+  if (!on) F->timers.collect();
+  F->timers.on = on != 0;
+  F->timers.resize(F->f.size() + 1);
+  FH_API_END
+}
+
+int fh_faust_timers(fh_faust_t F, double* ms, int64_t* counts) {
+  FH_API_BEGIN
+  FH_CHECK(F && ms && counts, "null argument");
+  F->timers.resize(F->f.size() + 1);
+  F->timers.collect();
+  for (size_t s = 0; s <= F->f.size(); ++s) {
+    ms[s] = F->timers.ms[s];
+    counts[s] = F->timers.count[s];
+    F->timers.ms[s] = 0.0;
+    F->timers.count[s] = 0;
+  }
   FH_API_END
 }
 

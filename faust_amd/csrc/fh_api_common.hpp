@@ -7,6 +7,72 @@
 struct fh_mat_s {
   fh::DMatP m;
 };
+// Per-factor device timers (the reference's per-factor Transform::multiply
+// timers, faust_Transform.hpp:1499-1536): HIP events recorded on the launch
+// stream around each factor's kernel; slot nfactors collects layout
+// conversions.
+namespace fh {
+struct FaustTimers {
+  bool on = false;
+  std::vector<std::vector<std::pair<hipEvent_t, hipEvent_t>>> pending;
+  std::vector<double> ms;
+  std::vector<int64_t> count;
+  std::vector<hipEvent_t> spare;
+  void resize(size_t slots) {
+    if (pending.size() < slots) {
+      pending.resize(slots);
+      ms.resize(slots, 0.0);
+      count.resize(slots, 0);
+    }
+  }
+  hipEvent_t get() {
+    if (!spare.empty()) {
+      hipEvent_t e = spare.back();
+      spare.pop_back();
+      return e;
+    }
+    hipEvent_t e;
+    FH_HIP(hipEventCreate(&e));
+    return e;
+  }
+  void begin(size_t slot, hipStream_t st) {
+    if (!on) return;
+    resize(slot + 1);
+    hipEvent_t a = get();
+    FH_HIP(hipEventRecord(a, st));
+    pending[slot].push_back({a, nullptr});
+  }
+  void end(size_t slot, hipStream_t st) {
+    if (!on) return;
+    hipEvent_t b = get();
+    FH_HIP(hipEventRecord(b, st));
+    pending[slot].back().second = b;
+  }
+  void collect() {
+    for (size_t s = 0; s < pending.size(); ++s) {
+      for (auto& pr : pending[s]) {
+        FH_HIP(hipEventSynchronize(pr.second));
+        float t = 0.f;
+        FH_HIP(hipEventElapsedTime(&t, pr.first, pr.second));
+        ms[s] += t;
+        count[s] += 1;
+        spare.push_back(pr.first);
+        spare.push_back(pr.second);
+      }
+      pending[s].clear();
+    }
+  }
+  ~FaustTimers() {
+    for (auto& v : pending)
+      for (auto& pr : v) {
+        (void)hipEventDestroy(pr.first);
+        if (pr.second) (void)hipEventDestroy(pr.second);
+      }
+    for (auto e : spare) (void)hipEventDestroy(e);
+  }
+};
+}  // namespace fh
+
 // A Faust keeps its two ping-pong intermediates (Transform::multiply's
 // tmp_buf1/tmp_buf2, faust_Transform.hpp:1780-1782) between calls, so a
 // repeated product allocates nothing, synchronizes nothing and can be
@@ -14,6 +80,7 @@ struct fh_mat_s {
 struct fh_faust_s {
   std::vector<fh::DMatP> f;
   fh::DBuf ws[2];
+  fh::FaustTimers timers;
 };
 
 #define FH_API_BEGIN try {
@@ -34,7 +101,7 @@ DMatP make_csr_from_host(int dtype, int m, int n, int nnz, const int32_t* rp, co
                          hipStream_t st);
 DMatP make_dense(int dtype, int m, int n, const void* data, int ld, bool on_device, hipStream_t st);
 void faust_multiply_f64(const std::vector<DMatP>& f, char op, const double* X, int p, double* Y, hipStream_t st,
-                        DBuf* ws);
+                        DBuf* ws, FaustTimers* tm = nullptr);
 DMatP faust_get_product_f64(const std::vector<DMatP>& f, hipStream_t st);
 double faust_spectral_norm_f64(const std::vector<DMatP>& f, int maxit, double thr, int* flag, hipStream_t st);
 // FP32 / BF16 paths (k_mixed.hip, k_bsr.hip)
@@ -42,5 +109,5 @@ void mixed_to_dense(const DMat& M, void* dense_dev, hipStream_t st);
 void mixed_mul_dense(const DMat& S, char op, const void* X, int ldx, int ncols, void* Y, int ldy, const void* alpha,
                      const void* beta, hipStream_t st);
 void faust_multiply_mixed(const std::vector<DMatP>& f, char op, const void* X, int p, void* Y, hipStream_t st,
-                          DBuf* ws);
+                          DBuf* ws, FaustTimers* tm = nullptr);
 }  // namespace fh

@@ -92,8 +92,9 @@ inline size_t dtype_size(int dt) { return dt == FH_F64 ? 8 : dt == FH_F32 ? 4 : 
 enum Kind { K_DENSE = 0, K_CSR = 1, K_BSR = 2 };
 
 // A device matrix.  Dense: column-major, ld = m.  CSR: int32 rowptr/colind,
-// columns sorted inside each row; its CSC twin (for op T/H products) is built
-// on the device the first time it is needed.  BSR: see k_bsr.hip.
+// columns sorted inside each row, with its CSC twin (rows ascending inside
+// each column) built when the matrix is created; op T/H products walk the
+// twin.  BSR: see below and k_mixed.hip.
 struct DMat {
   Kind kind = K_DENSE;
   int dtype = FH_F64;
@@ -103,9 +104,13 @@ struct DMat {
   int64_t nnz = 0;        // CSR: stored entries; dense: -1 until counted
   mutable bool has_csc = false;
   mutable DBuf cp, ri, cv; // CSC
-  // BSR
+  // BSR (32x32 blocks): bdat holds each block row-major in BF16; FP32 values
+  // are stored as a BF16 pair per entry (hi plane then lo plane per block,
+  // hi = bf16(v), lo = bf16(v - hi)).  t* is the BSR of the transpose.
   int bm = 0, bn = 0, bnnz = 0;
-  DBuf brp, bci, bdat;    // bdat in the MFMA-ready layout of k_bsr.hip
+  bool split = false;
+  DBuf brp, bci, bdat;
+  DBuf tbrp, tbci, tbdat;
   bool is_identity = false;
 };
 using DMatP = std::shared_ptr<DMat>;
@@ -120,7 +125,6 @@ void dense_csr_f64(const double* A, int lda, char opA, int m, const DMat& S, cha
 // C = alpha*op(A)*op(B) + beta*C, all dense col-major.
 void gemm_f64(char opA, char opB, int m, int n, int k, double alpha, const double* A, int lda, const double* B,
               int ldb, double beta, double* C, int ldc, hipStream_t st);
-void build_csc(const DMat& S, hipStream_t st);
 
 // canonical reductions; result written to a device double
 void reduce_dot_f64(const double* x, const double* y, int64_t n, double* out, hipStream_t st);

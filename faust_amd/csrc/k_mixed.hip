@@ -1,0 +1,503 @@
+// k_mixed.hip -- FP32 / BF16 Faust x dense: BSR on MFMA and FP32 CSR SpMM.
+//
+// BSR (32x32 blocks, bm = bn = 32): Y = op(B) X with X, Y BF16 column-major,
+// FP32 accumulation in v_mfma_f32_32x32x16_bf16.  Each wave owns a 32-row x
+// 32-column output tile (one block-row, 32 columns) and, for every block of
+// its block-row, issues two MFMAs (K = 16 + 16).  Operand lanes:
+//   A: lane l holds block row (l % 32), 8 consecutive k at 8*(l / 32);
+//   B: lane l holds X column (l % 32), the same 8 consecutive k;
+// so with blocks stored row-major and X column-major every operand is one
+// 16-byte load.  FP32 block values are stored as a BF16 (hi, lo) pair and
+// cost one extra MFMA per step (error ~2^-16 relative per value, below the
+// BF16 rounding of the activations).
+// Work mapping: workgroup = 2 block-rows x 64 columns; consecutive workgroup
+// ids walk the 64-column panels first, so with 512 columns the 8 panels land
+// on the 8 XCDs and each XCD's L2 holds its 1 MiB X panel for all the
+// gathers of its block-rows.
+//
+// FP32 CSR: intermediates are row-major inside the chain so that the
+// gather of one X row segment is a contiguous 128-byte read.  A workgroup
+// computes 32 rows x 32 columns (8 lanes x float4 per row); panels of 32
+// columns cycle over the XCDs the same way.  The chain input is transposed
+// once into row-major and the last factor writes column-major through LDS.
+#include <algorithm>
+#include <vector>
+
+#include "fh_api_common.hpp"
+
+namespace fh {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef unsigned short u16;
+
+__device__ __forceinline__ float bf2f(u16 h) { return __uint_as_float(((unsigned)h) << 16); }
+__device__ __forceinline__ u16 f2bf(float f) {
+  unsigned u = __float_as_uint(f);
+  if ((u & 0x7f800000u) == 0x7f800000u) return (u16)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));  // inf / nan
+  u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even
+  return (u16)(u >> 16);
+}
+static inline u16 host_f2bf(float f) {
+  unsigned u;
+  std::memcpy(&u, &f, 4);
+  if ((u & 0x7f800000u) == 0x7f800000u) return (u16)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (u16)(u >> 16);
+}
+static inline float host_bf2f(u16 h) {
+  unsigned u = ((unsigned)h) << 16;
+  float f;
+  std::memcpy(&f, &u, 4);
+  return f;
+}
+
+// ---------------------------------------------------------------- BSR MFMA
+template <bool SPLIT>
+__global__ __launch_bounds__(256) void k_bsr_mfma(int nbr, int p, const int* __restrict__ brp,
+                                                  const int* __restrict__ bci, const u16* __restrict__ bdat,
+                                                  const u16* __restrict__ X, int ldx, u16* __restrict__ Y, int ldy,
+                                                  float alpha, float beta, int npanels) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int panel = blockIdx.x % npanels, pair = blockIdx.x / npanels;
+  const int br = pair * 2 + (wave >> 1);
+  if (br >= nbr) return;
+  const int j = panel * 64 + (wave & 1) * 32 + (lane & 31);
+  const int kh = lane >> 5;  // which 8-wide k group of the 16
+  const bool jok = j < p;
+  const u16* xcol = X + (size_t)(jok ? j : 0) * ldx;
+  constexpr int BLK = 1024;                      // entries per 32x32 block
+  constexpr int PLANES = SPLIT ? 2 : 1;
+  f32x16 acc = {};
+  const int b0 = brp[br], b1 = brp[br + 1];
+  for (int b = b0; b < b1; b += 4) {
+    bf16x8 a[4][2][PLANES], x[4][2];
+    const int nb = min(4, b1 - b);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      if (q < nb) {
+        const int bc = bci[b + q];
+        const u16* blk = bdat + (size_t)(b + q) * BLK * PLANES + (lane & 31) * 32 + kh * 8;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+#pragma unroll
+          for (int s = 0; s < PLANES; ++s) a[q][h][s] = *reinterpret_cast<const bf16x8*>(blk + s * BLK + h * 16);
+          if (jok)
+            x[q][h] = *reinterpret_cast<const bf16x8*>(xcol + bc * 32 + h * 16 + kh * 8);
+          else
+            x[q][h] = bf16x8{};
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      if (q < nb) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[q][h][0], x[q][h], acc, 0, 0, 0);
+          if (SPLIT) acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[q][h][PLANES - 1], x[q][h], acc, 0, 0, 0);
+        }
+      }
+    }
+  }
+  if (!jok) return;
+  u16* ycol = Y + (size_t)j * ldy + br * 32;
+#pragma unroll
+  for (int g = 0; g < 4; ++g) {
+    const int r0 = 8 * g + 4 * kh;
+    u16 o[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      float v = alpha * acc[4 * g + t];
+      if (beta != 0.0f) v = fmaf(beta, bf2f(ycol[r0 + t]), v);
+      o[t] = f2bf(v);
+    }
+    uint2 w;
+    w.x = (unsigned)o[0] | ((unsigned)o[1] << 16);
+    w.y = (unsigned)o[2] | ((unsigned)o[3] << 16);
+    *reinterpret_cast<uint2*>(ycol + r0) = w;
+  }
+}
+
+static void bsr_mul(const DMat& B, char op, const void* X, int ldx, int p, void* Y, int ldy, float alpha,
+                    float beta, hipStream_t st) {
+  const bool tr = op != 'N';
+  const int rows = tr ? B.n : B.m;
+  const int nbr = rows / 32;
+  FH_CHECK(ldx % 8 == 0 && ldy % 4 == 0, "BSR MFMA path: ldx must be a multiple of 8 and ldy of 4");
+  FH_CHECK(((uintptr_t)X & 15) == 0 && ((uintptr_t)Y & 7) == 0, "BSR MFMA path: X must be 16-byte aligned");
+  if (nbr == 0 || p == 0) return;
+  const int npanels = (p + 63) / 64;
+  const int pairs = (nbr + 1) / 2;
+  const int* rp = tr ? B.tbrp.as<int>() : B.brp.as<int>();
+  const int* ci = tr ? B.tbci.as<int>() : B.bci.as<int>();
+  const u16* dat = tr ? B.tbdat.as<u16>() : B.bdat.as<u16>();
+  dim3 grid((unsigned)(npanels * pairs));
+  if (B.split)
+    hipLaunchKernelGGL(k_bsr_mfma<true>, grid, dim3(256), 0, st, nbr, p, rp, ci, dat, (const u16*)X, ldx, (u16*)Y,
+                       ldy, alpha, beta, npanels);
+  else
+    hipLaunchKernelGGL(k_bsr_mfma<false>, grid, dim3(256), 0, st, nbr, p, rp, ci, dat, (const u16*)X, ldx, (u16*)Y,
+                       ldy, alpha, beta, npanels);
+  FH_LAUNCH_CHECK();
+}
+
+// ---------------------------------------------------------------- FP32 CSR
+// Row-major gather SpMM: Y[r, c] = alpha * sum_e v[e] * X[ci[e], c] + beta*Y.
+// OUT_COL: write Y column-major (ld = ldy) through an LDS transpose.
+template <bool OUT_COL>
+__global__ __launch_bounds__(256) void k_csr_rm_f32(int rows, int p, const int* __restrict__ rp,
+                                                    const int* __restrict__ ci, const float* __restrict__ v,
+                                                    const float* __restrict__ X, int ldx, float* __restrict__ Y,
+                                                    int ldy, float alpha, float beta, int npanels) {
+  __shared__ float tile[32][33];
+  const int t = threadIdx.x;
+  const int panel = blockIdx.x % npanels, rblk = blockIdx.x / npanels;
+  const int r = rblk * 32 + (t >> 3);
+  const int c0 = panel * 32 + (t & 7) * 4;
+  float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (r < rows && c0 < p) {
+    const int e0 = rp[r], e1 = rp[r + 1];
+    for (int e = e0; e < e1; ++e) {
+      const float w = v[e];
+      const float4 x = *reinterpret_cast<const float4*>(X + (size_t)ci[e] * ldx + c0);
+      acc.x = fmaf(w, x.x, acc.x);
+      acc.y = fmaf(w, x.y, acc.y);
+      acc.z = fmaf(w, x.z, acc.z);
+      acc.w = fmaf(w, x.w, acc.w);
+    }
+  }
+  if (!OUT_COL) {
+    if (r < rows && c0 < p) {
+      float4* y = reinterpret_cast<float4*>(Y + (size_t)r * ldy + c0);
+      float4 o = make_float4(alpha * acc.x, alpha * acc.y, alpha * acc.z, alpha * acc.w);
+      if (beta != 0.0f) {
+        const float4 old = *y;
+        o.x = fmaf(beta, old.x, o.x);
+        o.y = fmaf(beta, old.y, o.y);
+        o.z = fmaf(beta, old.z, o.z);
+        o.w = fmaf(beta, old.w, o.w);
+      }
+      *y = o;
+    }
+    return;
+  }
+  const int lr = t >> 3, lc = (t & 7) * 4;
+  tile[lr][lc + 0] = acc.x;
+  tile[lr][lc + 1] = acc.y;
+  tile[lr][lc + 2] = acc.z;
+  tile[lr][lc + 3] = acc.w;
+  __syncthreads();
+  // thread -> column (t >> 3), 4 consecutive rows (t & 7) * 4
+  const int cc = t >> 3, rr = (t & 7) * 4;
+  const int gc = panel * 32 + cc;
+  if (gc >= p) return;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int gr = rblk * 32 + rr + q;
+    if (gr < rows) {
+      float o = alpha * tile[rr + q][cc];
+      float* y = Y + (size_t)gc * ldy + gr;
+      if (beta != 0.0f) o = fmaf(beta, *y, o);
+      *y = o;
+    }
+  }
+}
+
+// column-major (rows x p, ld) -> row-major (rows x ldr), zero-padding columns p..ldr
+__global__ __launch_bounds__(256) void k_col2row_f32(int rows, int p, const float* __restrict__ X, int ldx,
+                                                     float* __restrict__ R, int ldr) {
+  __shared__ float tile[32][33];
+  const int t = threadIdx.x;
+  const int r0 = blockIdx.x * 32, c0 = blockIdx.y * 32;
+  // read: thread -> column c0 + (t >> 3), rows r0 + (t & 7)*4 .. +3
+  const int cc = t >> 3, rr = (t & 7) * 4;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int gr = r0 + rr + q, gc = c0 + cc;
+    tile[rr + q][cc] = (gr < rows && gc < p) ? X[(size_t)gc * ldx + gr] : 0.0f;
+  }
+  __syncthreads();
+  const int lr = t >> 3, lc = (t & 7) * 4;
+  const int gr = r0 + lr;
+  if (gr >= rows) return;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int gc = c0 + lc + q;
+    if (gc < ldr) R[(size_t)gr * ldr + gc] = tile[lr][lc + q];
+  }
+}
+
+static void csr_f32_mul_rm(const DMat& S, char op, const float* X, int ldx, int p, float* Y, int ldy, bool out_col,
+                           float alpha, float beta, hipStream_t st) {
+  const bool tr = op != 'N';
+  const int rows = tr ? S.n : S.m;
+  if (rows == 0 || p == 0) return;
+  const int* rp = tr ? S.cp.as<int>() : S.rp.as<int>();
+  const int* ci = tr ? S.ri.as<int>() : S.ci.as<int>();
+  const float* v = tr ? S.cv.as<float>() : S.v.as<float>();
+  const int npanels = (p + 31) / 32;
+  dim3 grid((unsigned)(npanels * ((rows + 31) / 32)));
+  if (out_col)
+    hipLaunchKernelGGL(k_csr_rm_f32<true>, grid, dim3(256), 0, st, rows, p, rp, ci, v, X, ldx, Y, ldy, alpha, beta,
+                       npanels);
+  else
+    hipLaunchKernelGGL(k_csr_rm_f32<false>, grid, dim3(256), 0, st, rows, p, rp, ci, v, X, ldx, Y, ldy, alpha, beta,
+                       npanels);
+  FH_LAUNCH_CHECK();
+}
+
+static void col2row(int rows, int p, const float* X, int ldx, float* R, int ldr, hipStream_t st) {
+  if (rows == 0 || p == 0) return;
+  dim3 grid((rows + 31) / 32, (ldr + 31) / 32);
+  hipLaunchKernelGGL(k_col2row_f32, grid, dim3(256), 0, st, rows, p, X, ldx, R, ldr);
+  FH_LAUNCH_CHECK();
+}
+
+// ---------------------------------------------------------------- conversions
+__global__ void k_bsr_to_dense(int nbr, const int* __restrict__ brp, const int* __restrict__ bci,
+                               const u16* __restrict__ bdat, int split, int m, float* __restrict__ D) {
+  const int br = blockIdx.x;
+  for (int b = brp[br]; b < brp[br + 1]; ++b) {
+    const u16* blk = bdat + (size_t)b * 1024 * (split ? 2 : 1);
+    for (int e = threadIdx.x; e < 1024; e += blockDim.x) {
+      const int i = e >> 5, k = e & 31;
+      float v = bf2f(blk[e]);
+      if (split) v += bf2f(blk[1024 + e]);
+      D[(size_t)(bci[b] * 32 + k) * m + br * 32 + i] += v;
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void k_csr_scatter_f32(int m, const int* __restrict__ rp, const int* __restrict__ ci,
+                                  const float* __restrict__ v, float* __restrict__ D) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  for (int p = rp[i]; p < rp[i + 1]; ++p) D[(size_t)ci[p] * m + i] = v[p];
+}
+
+void mixed_to_dense(const DMat& M, void* dense_dev, hipStream_t st) {
+  if (M.kind == K_DENSE) {
+    FH_HIP(hipMemcpyAsync(dense_dev, M.d.p, dtype_size(M.dtype) * (size_t)M.m * M.n, hipMemcpyDeviceToDevice, st));
+    return;
+  }
+  FH_CHECK(M.dtype == FH_F32 || M.kind == K_BSR, "to_dense: BF16 CSR not supported");
+  FH_HIP(hipMemsetAsync(dense_dev, 0, sizeof(float) * (size_t)M.m * M.n, st));
+  if (M.kind == K_BSR) {
+    if (M.m > 0)
+      hipLaunchKernelGGL(k_bsr_to_dense, dim3(M.m / 32), dim3(256), 0, st, M.m / 32, M.brp.as<int>(),
+                         M.bci.as<int>(), M.bdat.as<u16>(), M.split ? 1 : 0, M.m, (float*)dense_dev);
+  } else if (M.nnz > 0) {
+    hipLaunchKernelGGL(k_csr_scatter_f32, dim3((M.m + 255) / 256), dim3(256), 0, st, M.m, M.rp.as<int>(),
+                       M.ci.as<int>(), M.v.as<float>(), (float*)dense_dev);
+  }
+  FH_LAUNCH_CHECK();
+}
+
+// Single product with column-major X and Y (fh_mat_mul_dense).
+void mixed_mul_dense(const DMat& S, char op, const void* X, int ldx, int ncols, void* Y, int ldy, const void* alpha,
+                     const void* beta, hipStream_t st) {
+  const float a = alpha ? *(const float*)alpha : 1.0f, b = beta ? *(const float*)beta : 0.0f;
+  if (S.kind == K_BSR) {
+    bsr_mul(S, op, X, ldx, ncols, Y, ldy, a, b, st);
+    return;
+  }
+  FH_CHECK(S.kind == K_CSR && S.dtype == FH_F32, "FP32/BF16 products: BSR or FP32 CSR factors only");
+  const int in_rows = op == 'N' ? S.n : S.m;
+  const int ldr = (ncols + 3) & ~3;
+  DBuf R(sizeof(float) * std::max<size_t>((size_t)in_rows * ldr, 1));
+  col2row(in_rows, ncols, (const float*)X, ldx, R.as<float>(), ldr, st);
+  csr_f32_mul_rm(S, op, R.as<float>(), ldr, ncols, (float*)Y, ldy, true, a, b, st);
+  FH_HIP(hipStreamSynchronize(st));  // R returns to the pool
+}
+
+// Chain product for BSR (BF16 activations) or FP32 CSR chains.
+void faust_multiply_mixed(const std::vector<DMatP>& f, char op, const void* X, int p, void* Y, hipStream_t st,
+                          DBuf* ws, FaustTimers* tm) {
+  const int J = (int)f.size();
+  const bool bsr = f[0]->kind == K_BSR;
+  for (auto& x : f) {
+    FH_CHECK((x->kind == K_BSR) == bsr, "Faust::multiply: BSR and CSR factors cannot be mixed in one FP32/BF16 chain");
+    if (!bsr) FH_CHECK(x->kind == K_CSR && x->dtype == FH_F32, "Faust::multiply: FP32 chains need FP32 CSR factors");
+  }
+  int maxr = 0;
+  for (auto& x : f) maxr = std::max(maxr, op == 'N' ? x->m : x->n);
+  const int in_rows0 = (op == 'N') ? f.back()->n : f.front()->m;
+  maxr = std::max(maxr, in_rows0);
+  const int ldr = bsr ? 0 : (p + 3) & ~3;
+  const size_t need = bsr ? sizeof(u16) * std::max<size_t>((size_t)maxr * p, 1)
+                          : sizeof(float) * std::max<size_t>((size_t)maxr * ldr, 1);
+  const int nbuf = bsr ? std::min(2, J - 1) : std::min(2, J);
+  for (int b = 0; b < nbuf; ++b)
+    if (ws[b].bytes < need) {
+      FH_HIP(hipStreamSynchronize(st));
+      ws[b].alloc(need);
+    }
+  int in_rows = in_rows0;
+  if (bsr) {
+    const void* in = X;
+    for (int i = 0; i < J; ++i) {
+      const DMat& S = (op == 'N') ? *f[J - 1 - i] : *f[i];
+      const int out_rows = (op == 'N') ? S.m : S.n;
+      FH_CHECK(((op == 'N') ? S.n : S.m) == in_rows, "Faust::multiply: factor dimensions do not chain");
+      void* out = (i == J - 1) ? Y : ws[i & 1].p;
+      const size_t slot = (op == 'N') ? (size_t)(J - 1 - i) : (size_t)i;
+      if (tm) tm->begin(slot, st);
+      bsr_mul(S, op, in, in_rows, p, out, out_rows, 1.0f, 0.0f, st);
+      if (tm) tm->end(slot, st);
+      in = out;
+      in_rows = out_rows;
+    }
+    return;
+  }
+  // FP32 CSR: transpose the input once, chain in row-major, last factor writes column-major
+  if (tm) tm->begin(J, st);
+  col2row(in_rows, p, (const float*)X, in_rows, ws[0].as<float>(), ldr, st);
+  if (tm) tm->end(J, st);
+  const float* in = ws[0].as<float>();
+  for (int i = 0; i < J; ++i) {
+    const DMat& S = (op == 'N') ? *f[J - 1 - i] : *f[i];
+    const int out_rows = (op == 'N') ? S.m : S.n;
+    FH_CHECK(((op == 'N') ? S.n : S.m) == in_rows, "Faust::multiply: factor dimensions do not chain");
+    const bool last = (i == J - 1);
+    float* out = last ? (float*)Y : ws[(i + 1) & 1].as<float>();
+    const size_t slot = (op == 'N') ? (size_t)(J - 1 - i) : (size_t)i;
+    if (tm) tm->begin(slot, st);
+    csr_f32_mul_rm(S, op, in, ldr, p, out, last ? out_rows : ldr, last, 1.0f, 0.0f, st);
+    if (tm) tm->end(slot, st);
+    in = out;
+    in_rows = out_rows;
+  }
+}
+
+// ---------------------------------------------------------------- BSR handle
+// Host: blocks arrive column-major (faust_MatBSR.h:120-150); the device keeps
+// them row-major per block, BF16 (or BF16 hi/lo planes for FP32), plus the
+// BSR of the transpose for op T/H.
+static void put_block(const void* src, int dtype, size_t b, bool transpose, bool split, std::vector<u16>& dst,
+                      size_t slot) {
+  const size_t planes = split ? 2 : 1;
+  u16* out = &dst[slot * 1024 * planes];
+  for (int i = 0; i < 32; ++i)
+    for (int k = 0; k < 32; ++k) {
+      // entry (i, k) of the stored block (of B or of B^T)
+      const int r = transpose ? k : i, c = transpose ? i : k;
+      const size_t e = b * 1024 + (size_t)c * 32 + r;  // column-major source
+      float v;
+      if (dtype == FH_BF16)
+        v = host_bf2f(static_cast<const u16*>(src)[e]);
+      else
+        v = static_cast<const float*>(src)[e];
+      const u16 hi = host_f2bf(v);
+      out[i * 32 + k] = hi;
+      if (split) out[1024 + i * 32 + k] = host_f2bf(v - host_bf2f(hi));
+    }
+}
+
+DMatP make_bsr(int dtype, int m, int n, int bnnz, const int32_t* browptr, const int32_t* bcolinds,
+               const void* bdata, hipStream_t st) {
+  FH_CHECK(dtype == FH_BF16 || dtype == FH_F32, "BSR factors are BF16 or FP32");
+  FH_CHECK(m % 32 == 0 && n % 32 == 0, "BSR: dimensions must be multiples of the block size");
+  const int nbr = m / 32, nbc = n / 32;
+  FH_CHECK(browptr[0] == 0 && browptr[nbr] == bnnz, "BSR: browptr inconsistent with bnnz");
+  for (int br = 0; br < nbr; ++br) {
+    FH_CHECK(browptr[br] <= browptr[br + 1], "BSR: browptr not non-decreasing");
+    for (int b = browptr[br]; b < browptr[br + 1]; ++b)
+      FH_CHECK(bcolinds[b] >= 0 && bcolinds[b] < nbc, "BSR: block column index out of range");
+  }
+  const bool split = dtype == FH_F32;
+  const size_t planes = split ? 2 : 1;
+  auto M = std::make_shared<DMat>();
+  M->kind = K_BSR;
+  M->dtype = dtype;
+  M->m = m;
+  M->n = n;
+  M->bm = M->bn = 32;
+  M->bnnz = bnnz;
+  M->split = split;
+  M->nnz = (int64_t)bnnz * 1024;
+  std::vector<u16> dat((size_t)std::max(bnnz, 1) * 1024 * planes), tdat(dat.size());
+  for (int b = 0; b < bnnz; ++b) put_block(bdata, dtype, b, false, split, dat, b);
+  // transposed BSR: block (br, bc) -> (bc, br) with the block transposed
+  std::vector<int32_t> tbrp(nbc + 1, 0), tbci(std::max(bnnz, 1));
+  for (int b = 0; b < bnnz; ++b) tbrp[bcolinds[b] + 1]++;
+  for (int c = 0; c < nbc; ++c) tbrp[c + 1] += tbrp[c];
+  std::vector<int32_t> fill(tbrp.begin(), tbrp.end() - 1);
+  for (int br = 0; br < nbr; ++br)
+    for (int b = browptr[br]; b < browptr[br + 1]; ++b) {
+      const int slot = fill[bcolinds[b]]++;
+      tbci[slot] = br;
+      put_block(bdata, dtype, b, true, split, tdat, slot);
+    }
+  auto up = [&](DBuf& d, const void* h, size_t bytes) {
+    d.alloc(std::max<size_t>(bytes, 4));
+    if (bytes) FH_HIP(hipMemcpyAsync(d.p, h, bytes, hipMemcpyHostToDevice, st));
+  };
+  up(M->brp, browptr, sizeof(int) * (nbr + 1));
+  up(M->bci, bcolinds, sizeof(int) * bnnz);
+  up(M->bdat, dat.data(), sizeof(u16) * (size_t)bnnz * 1024 * planes);
+  up(M->tbrp, tbrp.data(), sizeof(int) * (nbc + 1));
+  up(M->tbci, tbci.data(), sizeof(int) * bnnz);
+  up(M->tbdat, tdat.data(), sizeof(u16) * (size_t)bnnz * 1024 * planes);
+  FH_HIP(hipStreamSynchronize(st));
+  return M;
+}
+
+// BSR with other block sizes: expanded to an FP32 CSR factor.
+static DMatP bsr_as_csr(int dtype, int m, int n, int bm, int bn, int bnnz, const int32_t* browptr,
+                        const int32_t* bcolinds, const void* bdata, hipStream_t st) {
+  FH_CHECK(bm > 0 && bn > 0 && m % bm == 0 && n % bn == 0, "BSR: dimensions must be multiples of the block size");
+  const int nbr = m / bm;
+  std::vector<int32_t> rp(m + 1, 0), ci;
+  std::vector<float> v;
+  for (int br = 0; br < nbr; ++br) {
+    std::vector<int> order;
+    for (int b = browptr[br]; b < browptr[br + 1]; ++b) order.push_back(b);
+    std::sort(order.begin(), order.end(), [&](int x, int y) { return bcolinds[x] < bcolinds[y]; });
+    for (int i = 0; i < bm; ++i) {
+      for (int b : order)
+        for (int k = 0; k < bn; ++k) {
+          const size_t e = (size_t)b * bm * bn + (size_t)k * bm + i;
+          float x = dtype == FH_BF16 ? host_bf2f(static_cast<const u16*>(bdata)[e]) : static_cast<const float*>(bdata)[e];
+          ci.push_back(bcolinds[b] * bn + k);
+          v.push_back(x);
+        }
+      rp[br * bm + i + 1] = (int32_t)ci.size();
+    }
+  }
+  return make_csr_from_host(FH_F32, m, n, (int)ci.size(), rp.data(), ci.data(), v.data(), st);
+}
+
+}  // namespace fh
+
+using namespace fh;
+
+extern "C" int fh_mat_bsr_create(int dtype, int nrows, int ncols, int bm, int bn, int bnnz, const int32_t* browptr,
+                                 const int32_t* bcolinds, const void* bdata, int src_on_device, fh_mat_t* out) {
+  FH_API_BEGIN
+  FH_CHECK(out && browptr && (bnnz == 0 || (bcolinds && bdata)), "null argument");
+  FH_CHECK(bm > 0 && bn > 0 && nrows % bm == 0 && ncols % bn == 0, "BSR: bad block size");
+  const size_t esz = dtype_size(dtype);
+  std::vector<int32_t> hp, hc;
+  std::vector<char> hd;
+  if (src_on_device) {
+    const int nbr = nrows / bm;
+    hp.resize(nbr + 1);
+    hc.resize(bnnz);
+    hd.resize(esz * (size_t)bnnz * bm * bn);
+    FH_HIP(hipMemcpy(hp.data(), browptr, sizeof(int) * (nbr + 1), hipMemcpyDeviceToHost));
+    if (bnnz) {
+      FH_HIP(hipMemcpy(hc.data(), bcolinds, sizeof(int) * bnnz, hipMemcpyDeviceToHost));
+      FH_HIP(hipMemcpy(hd.data(), bdata, hd.size(), hipMemcpyDeviceToHost));
+    }
+    browptr = hp.data();
+    bcolinds = hc.data();
+    bdata = hd.data();
+  }
+  if (bm == 32 && bn == 32)
+    *out = new fh_mat_s{make_bsr(dtype, nrows, ncols, bnnz, browptr, bcolinds, bdata, nullptr)};
+  else
+    *out = new fh_mat_s{bsr_as_csr(dtype, nrows, ncols, bm, bn, bnnz, browptr, bcolinds, bdata, nullptr)};
+  FH_API_END
+}

@@ -55,7 +55,7 @@ class Factor:
             return
         _require_torch_cuda()
         L = lib()
-        dt = _DT[np.dtype(dtype)] if not (isinstance(dtype, str) and dtype == "bf16") else _lib.FH_BF16
+        dt = _lib.FH_BF16 if (isinstance(dtype, str) and dtype == "bf16") else _DT[np.dtype(dtype)]
         if sp.issparse(M) and M.format == "bsr":
             from .bsr import bsr_handle
             self.h = bsr_handle(M, dt)
@@ -87,7 +87,9 @@ class Factor:
 
     def toarray(self):
         kind, dt, r, c, _ = self.info()
-        tdt = {_lib.FH_F64: torch.float64, _lib.FH_F32: torch.float32, _lib.FH_BF16: torch.bfloat16}[dt]
+        # fh_mat_to_dense writes BSR factors (BF16 or FP32 values) as FP32
+        tdt = torch.float32 if kind == _lib.FH_MAT_BSR else \
+            {_lib.FH_F64: torch.float64, _lib.FH_F32: torch.float32, _lib.FH_BF16: torch.bfloat16}[dt]
         out = torch.empty((c, r), dtype=tdt, device="cuda")
         check(lib().fh_mat_to_dense(self.h, _ptr(out), _stream()))
         torch.cuda.current_stream().synchronize()
@@ -127,8 +129,18 @@ class Faust:
             _facs = [f if isinstance(f, Factor) else Factor(f, dtype) for f in factors]
         self._facs = _facs
         self._transposed = _transposed
-        self.dtype = np.dtype(np.float64) if dtype == "bf16" else np.dtype(dtype)
         self._dt = dtype
+        kinds = [f.info()[:2] for f in self._facs]
+        # activation (X, Y) dtype of the chain: FP64 chains -> float64, BSR chains
+        # (BF16 or FP32 values) -> bfloat16, FP32 CSR chains -> float32
+        if any(k == _lib.FH_MAT_BSR for k, _ in kinds):
+            self.act_dtype = torch.bfloat16
+        elif all(d == _lib.FH_F64 for _, d in kinds):
+            self.act_dtype = torch.float64
+        else:
+            self.act_dtype = torch.float32
+        self.dtype = {torch.float64: np.dtype(np.float64), torch.float32: np.dtype(np.float32),
+                      torch.bfloat16: np.dtype(np.float32)}[self.act_dtype]
         self.h = C.c_void_p()
         check(L.fh_faust_create(C.byref(self.h)))
         for f in self._facs:
@@ -199,13 +211,12 @@ class Faust:
         if not is_torch and sp.issparse(A):
             A = A.toarray()
         vec = (not is_torch and np.ndim(A) == 1) or (is_torch and A.dim() == 1)
-        tdt = torch.float64 if self.dtype == np.float64 else torch.float32
-        Xd = colmajor_to_device(A, tdt)
+        Xd = colmajor_to_device(A, self.act_dtype)
         Y = self.matmul_device(Xd)
         if is_torch:
             return Y.t()
         torch.cuda.current_stream().synchronize()
-        y = Y.cpu().numpy().T
+        y = Y.float().cpu().numpy().T if self.act_dtype == torch.bfloat16 else Y.cpu().numpy().T
         return y.reshape(-1) if vec else np.asfortranarray(y)
 
     def dot(self, A):

@@ -66,7 +66,9 @@ int fh_mat_info(fh_mat_t M, int *kind, int *dtype, int *nrows, int *ncols, int64
 /* Copy back to host buffers (dense: `data` nrows*ncols col-major; CSR:
  * rowptr[nrows+1], colind[nnz], vals[nnz]; BSR: browptr, bcolinds, bdata). */
 int fh_mat_tocpu(fh_mat_t M, void *data_or_vals, int32_t *rowptr, int32_t *colind);
-/* Densify into a device buffer (col-major, ld = nrows). */
+/* Densify into a device buffer (col-major, ld = nrows): FP64 for FP64
+ * matrices, FP32 for FP32 matrices and for BSR matrices of either value type,
+ * BF16 for BF16 dense matrices. */
 int fh_mat_to_dense(fh_mat_t M, void *dense_dev, void *stream);
 int fh_mat_free(fh_mat_t M);
 
@@ -95,6 +97,14 @@ int fh_faust_free(fh_faust_t F);
  * (faust_Transform.hpp:1714-1812): right-to-left for 'N', left-to-right with
  * per-factor op for 'T'/'H', two ping-pong intermediates. */
 int fh_faust_multiply(fh_faust_t F, char op, const void *X, int p, void *Y, void *stream);
+/* Per-factor device timers: HIP events recorded on the launch stream around
+ * each factor's kernel of every fh_faust_multiply while enabled (the
+ * reference's per-factor Transform::multiply timers, faust_Transform.hpp:
+ * 1499-1536, behind __COMPILE_TIMERS__).  fh_faust_timers fills ms[i] and
+ * counts[i] for factor i (i < nfactors) and slot nfactors for layout
+ * conversions, then resets them; arrays hold nfactors+1 entries.  Synchronizes. */
+int fh_faust_enable_timers(fh_faust_t F, int on);
+int fh_faust_timers(fh_faust_t F, double *ms, int64_t *counts);
 /* Dense product of the chain into a device buffer (Transform::get_product,
  * faust_Transform.hpp:599-652). */
 int fh_faust_get_product(fh_faust_t F, void *out_dev, void *stream);

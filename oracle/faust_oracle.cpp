@@ -185,9 +185,14 @@ CSC csc_of_op(const Mat& A, char op) {
     for (int i = 0; i < A.m; ++i)
       for (int p = A.rp[i]; p < A.rp[i + 1]; ++p) push(i, A.ci[p], A.v[p]);
   } else {
-    Mat D = dense_of(A);
-    for (int j = 0; j < D.n; ++j)
-      for (int i = 0; i < D.m; ++i) push(i, j, D.d[(size_t)j * D.m + i]);
+    // BSR: entries straight from the column-major blocks
+    int nbr = A.m / A.bm;
+    for (int br = 0; br < nbr; ++br)
+      for (int b = A.brp[br]; b < A.brp[br + 1]; ++b) {
+        const double* blk = &A.bd[(size_t)b * A.bm * A.bn];
+        for (int c = 0; c < A.bn; ++c)
+          for (int r = 0; r < A.bm; ++r) push(br * A.bm + r, A.bci[b] * A.bn + c, blk[(size_t)c * A.bm + r]);
+      }
   }
   for (size_t t = 0; t < tc.size(); ++t) C.cp[tc[t] + 1]++;
   for (int j = 0; j < on; ++j) C.cp[j + 1] += C.cp[j];

@@ -1,0 +1,50 @@
+"""The C ABI library loads and exports every entry point include/faust_hip.h declares (no GPU calls)."""
+import ctypes as C
+import os
+import re
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    txt = open(os.path.join(ROOT, "include", "faust_hip.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(fh_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_header_symbols_exported():
+    from faust_amd import _lib
+    lib = C.CDLL(_lib.LIB_PATH)
+    names = _declared()
+    assert len(names) >= 30
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_python_binding_covers_header():
+    from faust_amd import _lib
+    assert set(_declared()) == set(_lib.exported_names())
+
+
+def test_no_cpu_fallback_without_library(tmp_path, monkeypatch):
+    from faust_amd import _lib
+    monkeypatch.setattr(_lib, "LIB_PATH", str(tmp_path / "missing.so"))
+    monkeypatch.setattr(_lib, "_lib", None)
+    import pytest
+    with pytest.raises(ImportError):
+        _lib.lib()
+
+
+def test_version_and_error_string():
+    from faust_amd import _lib
+    L = _lib.lib()
+    assert b"gfx950" in L.fh_version()
+    assert L.fh_last_error() is not None
+
+
+def test_argument_errors_without_gpu():
+    # argument validation happens before any device call
+    from faust_amd import _lib
+    L = _lib.lib()
+    assert L.fh_faust_size(None, None) == _lib.FH_ERR_ARG
+    assert b"null" in L.fh_last_error()

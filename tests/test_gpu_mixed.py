@@ -1,0 +1,108 @@
+"""FP32 / BF16 Faust x dense (BSR on MFMA, FP32 CSR) vs the FP64 oracle.
+
+Tolerances: BF16 activations are rounded to nearest-even after every factor
+(relative 2^-9 per entry); a single BSR product has max |err| <= 2^-8 * |y|
++ 1e-3 * ||y||_inf (FP32 accumulation over 128 terms, then one BF16
+rounding).  FP32 block values enter as a BF16 hi/lo pair (2^-16 relative).
+FP32 CSR chains are compared with relative Frobenius error <= 1e-5."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import torch
+
+import oracle
+import faust_amd
+from faust_amd.bsr import bf16_bits_to_f32, f32_to_bf16_bits, rand_bsr
+
+pytestmark = pytest.mark.gpu
+
+
+def bf16_round(a):
+    return bf16_bits_to_f32(f32_to_bf16_bits(np.asarray(a, dtype=np.float32))).astype(np.float64)
+
+
+def _bsr_rounded(B):
+    B = sp.bsr_matrix(B)
+    return sp.bsr_matrix((bf16_round(B.data), B.indices, B.indptr), shape=B.shape)
+
+
+@pytest.mark.parametrize("op", ["N", "T"])
+@pytest.mark.parametrize("p", [512, 96, 8])
+def test_bsr_bf16_single_product(gpu, op, p):
+    rng = np.random.default_rng(1)
+    B = _bsr_rounded(rand_bsr(1024, 768, 32, 4, rng, scale=1 / np.sqrt(128)))
+    rows_in = B.shape[1] if op == "N" else B.shape[0]
+    X = bf16_round(rng.standard_normal((rows_in, p)))
+    F = faust_amd.Faust([B], dtype="bf16")
+    Y = (F @ X) if op == "N" else (F.T @ X)
+    ref = oracle.chain_multiply([B], X, op)
+    tol = 2.0 ** -8 * np.abs(ref) + 1e-3 * np.abs(ref).max()
+    assert np.all(np.abs(Y - ref) <= tol)
+
+
+def test_bsr_fp32_split_single_product(gpu):
+    rng = np.random.default_rng(2)
+    B = rand_bsr(512, 512, 32, 4, rng, scale=1 / np.sqrt(128))
+    B32 = sp.bsr_matrix((B.data.astype(np.float32).astype(np.float64), B.indices, B.indptr), shape=B.shape)
+    X = bf16_round(rng.standard_normal((512, 64)))
+    Y = faust_amd.Faust([B32], dtype=np.float32) @ X
+    ref = oracle.chain_multiply([B32], X, "N")
+    tol = 2.0 ** -8 * np.abs(ref) + 1e-4 * np.abs(ref).max()
+    assert np.all(np.abs(Y - ref) <= tol)
+    # the split keeps FP32 accuracy of the block values: finer than a BF16 factor
+    Yb = faust_amd.Faust([B32], dtype="bf16") @ X
+    assert np.linalg.norm(Y - ref) < np.linalg.norm(Yb - ref)
+
+
+def test_bsr_chain_cfg3a_shape_error(gpu):
+    # 3 factors 2048^2 of the cfg3a kind; error vs FP64 oracle (same BF16 inputs) stated
+    rng = np.random.default_rng(3)
+    facs = [_bsr_rounded(rand_bsr(2048, 2048, 32, 4, rng, scale=1 / np.sqrt(128))) for _ in range(3)]
+    X = bf16_round(rng.standard_normal((2048, 512)))
+    Y = faust_amd.Faust(facs, dtype="bf16") @ X
+    ref = oracle.chain_multiply(facs, X, "N")
+    rel = np.linalg.norm(Y - ref) / np.linalg.norm(ref)
+    assert rel < 8e-3, rel
+
+
+def test_bsr_other_block_size_goes_csr(gpu):
+    rng = np.random.default_rng(4)
+    B = rand_bsr(96, 64, 16, 2, rng)
+    B32 = sp.bsr_matrix((B.data.astype(np.float32).astype(np.float64), B.indices, B.indptr), shape=B.shape)
+    X = rng.standard_normal((64, 12)).astype(np.float32).astype(np.float64)
+    Y = faust_amd.Faust([B32], dtype=np.float32) @ X
+    ref = oracle.chain_multiply([B32], X, "N")
+    assert np.linalg.norm(Y - ref) / np.linalg.norm(ref) < 1e-6
+
+
+@pytest.mark.parametrize("op", ["N", "T"])
+@pytest.mark.parametrize("p", [256, 13, 1])
+def test_csr_fp32_chain(gpu, op, p):
+    rng = np.random.default_rng(5)
+    facs = [faust_amd.rand_csr(2048, 2048, 12, rng).astype(np.float32).astype(np.float64) for _ in range(4)]
+    X = rng.standard_normal((2048, p)).astype(np.float32).astype(np.float64)
+    F = faust_amd.Faust(facs, dtype=np.float32)
+    Y = (F @ X) if op == "N" else (F.T @ X)
+    ref = oracle.chain_multiply(facs, X, op)
+    assert np.linalg.norm(Y - ref) / np.linalg.norm(ref) < 1e-5
+
+
+def test_csr_fp32_rectangular_chain(gpu):
+    rng = np.random.default_rng(6)
+    shapes = [(100, 64), (64, 200), (200, 36)]
+    facs = [sp.csr_matrix(rng.standard_normal(s) * (rng.random(s) < 0.2)).astype(np.float32).astype(np.float64)
+            for s in shapes]
+    X = rng.standard_normal((36, 7)).astype(np.float32).astype(np.float64)
+    F = faust_amd.Faust(facs, dtype=np.float32)
+    ref = oracle.chain_multiply(facs, X, "N")
+    assert np.linalg.norm(F @ X - ref) / np.linalg.norm(ref) < 1e-5
+    X2 = rng.standard_normal((100, 5)).astype(np.float32).astype(np.float64)
+    ref2 = oracle.chain_multiply(facs, X2, "T")
+    assert np.linalg.norm(F.T @ X2 - ref2) / np.linalg.norm(ref2) < 1e-5
+
+
+def test_bsr_to_dense(gpu):
+    rng = np.random.default_rng(7)
+    B = _bsr_rounded(rand_bsr(128, 96, 32, 2, rng))
+    f = faust_amd.Factor(B, "bf16")
+    assert np.array_equal(f.toarray(), B.toarray())
