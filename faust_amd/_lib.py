@@ -80,6 +80,12 @@ _SIGS = {
     "fh_palm_result_factor": (C.c_int, [VP, C.c_int, C.POINTER(VP)]),
     "fh_palm_result_traj": (C.c_int, [VP, VP, C.POINTER(C.c_int)]),
     "fh_palm_result_free": (C.c_int, [VP]),
+    "fh_palm4msa_batched": (C.c_int, [C.c_int, C.c_int, VP, VP, VP, VP, VP, VP, VP, C.POINTER(VP)]),
+    "fh_batch_result_lambda": (C.c_int, [VP, VP]),
+    "fh_batch_result_relerr": (C.c_int, [VP, VP]),
+    "fh_batch_result_factor": (C.c_int, [VP, C.c_int, C.c_int, VP]),
+    "fh_batch_result_times": (C.c_int, [VP, VP]),
+    "fh_batch_result_free": (C.c_int, [VP]),
 }
 
 _lib = None

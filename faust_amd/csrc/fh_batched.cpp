@@ -1,0 +1,595 @@
+// fh_batched.cpp -- host driver of the batched PALM4MSA engine.
+//
+// Builds the per-iteration schedule of batched kernels once (left-to-right
+// updates with packed partial products, as palm4msa2 with the
+// ParamsPalm4MSA defaults: faust_palm4msa2020.hpp:47-300), captures it into
+// a HIP graph on a private stream, and replays the graph for every
+// iteration.  Tile sizes of the grouped GEMM are picked per target size by a
+// timing autotuner at set-up.  The mixed-precision scheme runs the main
+// iterations in FP32, then refinement sweeps in FP64 with every factor
+// projected on its frozen FP32 support (prox supp), and verifies the final
+// relative error in FP64.
+#include <algorithm>
+#include <chrono>
+#include <map>
+#include <numeric>
+
+#include "fh_api_common.hpp"
+#include "fh_batched.hpp"
+
+namespace fh {
+
+struct BatchSpec {
+  int nt, nf;
+  std::vector<int> n;
+};
+
+template <class T>
+struct Batch {
+  BatchSpec sp;
+  int slots;                       // matrices per target
+  std::vector<int64_t> off;        // element offset of each target's block
+  DBuf arena, lam, nR, nL, relerr, powv;
+  std::vector<int64_t> powoff;     // per target start of its 2*nf power vectors
+  // schedule
+  struct Stage {
+    int kind;  // 0 gemm, 1 power(R->nR), 2 power(L->nL), 3 prox, 4 lambda
+    DBuf jobs;
+    int njobs = 0;
+  };
+  std::vector<Stage> stages;
+  DBuf tiles32, tiles64;
+  int nt32 = 0, nt64 = 0;
+  std::vector<int> tile_of;  // per target: 32 or 64
+  int prox_mode = 0;
+  int maxit = 100;
+  double thr = 1e-6;
+  int warm = 1;
+
+  T* mat(int b, int slot) const { return arena.as<T>() + off[b] + (int64_t)slot * sp.n[b] * sp.n[b]; }
+  // slot layout: 0 A, 1..nf S, nf+1..2nf-1 PR[0..nf-2], 2nf..3nf-2 PL[1..nf-1], then DR, TMP, TR, SM, MASK[nf]
+  int sA() const { return 0; }
+  int sS(int f) const { return 1 + f; }
+  int sPR(int i) const { return sp.nf + 1 + i; }
+  int sPL(int f) const { return 2 * sp.nf + f - 1; }
+  int sDR() const { return 3 * sp.nf - 1; }
+  int sTMP() const { return 3 * sp.nf; }
+  int sTR() const { return 3 * sp.nf + 1; }
+  int sSM() const { return 3 * sp.nf + 2; }
+  int sMASK(int f) const { return 3 * sp.nf + 3 + f; }
+};
+
+template <class T>
+static void batch_alloc(Batch<T>& B, const BatchSpec& sp, bool masks) {
+  B.sp = sp;
+  B.slots = 3 * sp.nf + 3 + (masks ? sp.nf : 0);
+  B.off.resize(sp.nt);
+  int64_t tot = 0;
+  for (int b = 0; b < sp.nt; ++b) {
+    FH_CHECK(sp.n[b] > 0 && sp.n[b] <= BATCH_NMAX, "batched PALM: target size out of range (1..2048)");
+    B.off[b] = tot;
+    tot += (int64_t)B.slots * sp.n[b] * sp.n[b];
+  }
+  B.arena.alloc(sizeof(T) * std::max<int64_t>(tot, 1));
+  FH_HIP(hipMemset(B.arena.p, 0, sizeof(T) * std::max<int64_t>(tot, 1)));
+  B.lam.alloc(sizeof(double) * sp.nt);
+  B.nR.alloc(sizeof(double) * sp.nt);
+  B.nL.alloc(sizeof(double) * sp.nt);
+  B.relerr.alloc(sizeof(double) * sp.nt);
+  B.powoff.resize(sp.nt);
+  int64_t pt = 0;
+  for (int b = 0; b < sp.nt; ++b) {
+    B.powoff[b] = pt;
+    pt += (int64_t)2 * sp.nf * sp.n[b];
+  }
+  B.powv.alloc(sizeof(double) * std::max<int64_t>(pt, 1));
+  std::vector<double> hv(std::max<int64_t>(pt, 1));
+  for (int b = 0; b < sp.nt; ++b) {
+    const std::vector<double>& x0 = power_x0(sp.n[b]);
+    for (int s = 0; s < 2 * sp.nf; ++s)
+      std::copy(x0.begin(), x0.end(), hv.begin() + B.powoff[b] + (int64_t)s * sp.n[b]);
+  }
+  FH_HIP(hipMemcpy(B.powv.p, hv.data(), sizeof(double) * hv.size(), hipMemcpyHostToDevice));
+}
+
+template <class T>
+static void build_tiles(Batch<T>& B) {
+  std::vector<int3> t32, t64;
+  for (int b = 0; b < B.sp.nt; ++b) {
+    const int tile = B.tile_of[b], n = B.sp.n[b];
+    const int nt = (n + tile - 1) / tile;
+    for (int i = 0; i < nt; ++i)
+      for (int j = 0; j < nt; ++j) (tile == 32 ? t32 : t64).push_back(make_int3(b, i, j));
+  }
+  B.nt32 = (int)t32.size();
+  B.nt64 = (int)t64.size();
+  B.tiles32.alloc(sizeof(int3) * std::max<size_t>(t32.size(), 1));
+  B.tiles64.alloc(sizeof(int3) * std::max<size_t>(t64.size(), 1));
+  if (!t32.empty()) FH_HIP(hipMemcpy(B.tiles32.p, t32.data(), sizeof(int3) * t32.size(), hipMemcpyHostToDevice));
+  if (!t64.empty()) FH_HIP(hipMemcpy(B.tiles64.p, t64.data(), sizeof(int3) * t64.size(), hipMemcpyHostToDevice));
+}
+
+template <class T>
+static void add_gemm(Batch<T>& B, std::function<BGemmJob<T>(int)> mk) {
+  std::vector<BGemmJob<T>> h(B.sp.nt);
+  for (int b = 0; b < B.sp.nt; ++b) h[b] = mk(b);
+  typename Batch<T>::Stage s;
+  s.kind = 0;
+  s.njobs = B.sp.nt;
+  s.jobs.alloc(sizeof(BGemmJob<T>) * h.size());
+  FH_HIP(hipMemcpy(s.jobs.p, h.data(), sizeof(BGemmJob<T>) * h.size(), hipMemcpyHostToDevice));
+  B.stages.push_back(std::move(s));
+}
+
+template <class T>
+static void add_power(Batch<T>& B, int kind, std::function<const T*(int)> P, int slot) {
+  std::vector<BPowJob<T>> h(B.sp.nt);
+  double* out = (kind == 1 ? B.nR : B.nL).template as<double>();
+  for (int b = 0; b < B.sp.nt; ++b)
+    h[b] = BPowJob<T>{P(b), B.sp.n[b], b, out, B.powv.template as<double>() + B.powoff[b] + (int64_t)slot * B.sp.n[b]};
+  typename Batch<T>::Stage s;
+  s.kind = kind;
+  s.njobs = B.sp.nt;
+  s.jobs.alloc(sizeof(BPowJob<T>) * h.size());
+  FH_HIP(hipMemcpy(s.jobs.p, h.data(), sizeof(BPowJob<T>) * h.size(), hipMemcpyHostToDevice));
+  B.stages.push_back(std::move(s));
+}
+
+template <class T>
+static void add_prox(Batch<T>& B, int f, const std::vector<int64_t>& k) {
+  std::vector<BProxJob<T>> h(B.sp.nt);
+  for (int b = 0; b < B.sp.nt; ++b) {
+    const int64_t len = (int64_t)B.sp.n[b] * B.sp.n[b];
+    h[b] = BProxJob<T>{B.mat(b, B.sS(f)), B.prox_mode == 1 ? B.mat(b, B.sMASK(f)) : nullptr, len,
+                       k.empty() ? len : k[b]};
+  }
+  typename Batch<T>::Stage s;
+  s.kind = 3;
+  s.njobs = B.sp.nt;
+  s.jobs.alloc(sizeof(BProxJob<T>) * h.size());
+  FH_HIP(hipMemcpy(s.jobs.p, h.data(), sizeof(BProxJob<T>) * h.size(), hipMemcpyHostToDevice));
+  B.stages.push_back(std::move(s));
+}
+
+template <class T>
+static void add_lambda(Batch<T>& B) {
+  std::vector<BLamJob<T>> h(B.sp.nt);
+  for (int b = 0; b < B.sp.nt; ++b)
+    h[b] = BLamJob<T>{B.mat(b, B.sA()), B.mat(b, B.sSM()), (int64_t)B.sp.n[b] * B.sp.n[b], b};
+  typename Batch<T>::Stage s;
+  s.kind = 4;
+  s.njobs = B.sp.nt;
+  s.jobs.alloc(sizeof(BLamJob<T>) * h.size());
+  FH_HIP(hipMemcpy(s.jobs.p, h.data(), sizeof(BLamJob<T>) * h.size(), hipMemcpyHostToDevice));
+  B.stages.push_back(std::move(s));
+}
+
+// The fixed schedule of one PALM iteration (L2R, packed partial products).
+template <class T>
+static void build_schedule(Batch<T>& B, const std::vector<int64_t>& k) {
+  const int nf = B.sp.nf;
+  B.stages.clear();
+  auto nn = [&](int b) { return B.sp.n[b]; };
+  auto job = [&](int b, const T* A, int ta, const T* Bm, int tb, const T* Cin, T* Cout, int am, int bm) {
+    return BGemmJob<T>{A, Bm, Cin, Cout, nn(b), nn(b), nn(b), ta, tb, am, bm, b};
+  };
+  // packed right products of the start of the sweep: PR'[nf-2] = S[nf-1], PR[i] = S[i+1] PR'[i+1]
+  auto PRp = [&](int b, int i) -> T* { return i == nf - 2 ? B.mat(b, B.sS(nf - 1)) : B.mat(b, B.sPR(i)); };
+  auto PLp = [&](int b, int f) -> T* { return f == 1 ? B.mat(b, B.sS(0)) : B.mat(b, B.sPL(f)); };
+  for (int i = nf - 3; i >= 0; --i)
+    add_gemm<T>(B, [&, i](int b) { return job(b, B.mat(b, B.sS(i + 1)), 0, PRp(b, i + 1), 0, nullptr, B.mat(b, B.sPR(i)), 0, 0); });
+  for (int f = 0; f < nf; ++f) {
+    const bool hasR = f < nf - 1, hasL = f > 0;
+    add_power<T>(B, 1, [&, f, hasR](int b) -> const T* { return hasR ? PRp(b, f) : nullptr; }, 2 * f);
+    add_power<T>(B, 2, [&, f, hasL](int b) -> const T* { return hasL ? PLp(b, f) : nullptr; }, 2 * f + 1);
+    if (hasR && hasL) {
+      add_gemm<T>(B, [&, f](int b) { return job(b, B.mat(b, B.sS(f)), 0, PRp(b, f), 0, nullptr, B.mat(b, B.sDR()), 0, 0); });
+      add_gemm<T>(B, [&, f](int b) {
+        return job(b, PLp(b, f), 0, B.mat(b, B.sDR()), 0, B.mat(b, B.sA()), B.mat(b, B.sTMP()), 1, 1);
+      });
+      add_gemm<T>(B, [&, f](int b) { return job(b, B.mat(b, B.sTMP()), 0, PRp(b, f), 1, nullptr, B.mat(b, B.sTR()), 0, 0); });
+      add_gemm<T>(B, [&, f](int b) {
+        return job(b, PLp(b, f), 1, B.mat(b, B.sTR()), 0, B.mat(b, B.sS(f)), B.mat(b, B.sS(f)), 2, 2);
+      });
+    } else if (hasR) {
+      add_gemm<T>(B, [&, f](int b) {
+        return job(b, B.mat(b, B.sS(f)), 0, PRp(b, f), 0, B.mat(b, B.sA()), B.mat(b, B.sTMP()), 1, 1);
+      });
+      add_gemm<T>(B, [&, f](int b) {
+        return job(b, B.mat(b, B.sTMP()), 0, PRp(b, f), 1, B.mat(b, B.sS(f)), B.mat(b, B.sS(f)), 2, 2);
+      });
+    } else {
+      add_gemm<T>(B, [&, f](int b) {
+        return job(b, PLp(b, f), 0, B.mat(b, B.sS(f)), 0, B.mat(b, B.sA()), B.mat(b, B.sTMP()), 1, 1);
+      });
+      add_gemm<T>(B, [&, f](int b) {
+        return job(b, PLp(b, f), 1, B.mat(b, B.sTMP()), 0, B.mat(b, B.sS(f)), B.mat(b, B.sS(f)), 2, 2);
+      });
+    }
+    add_prox<T>(B, f, k);
+    if (f >= 1 && f < nf - 1)
+      add_gemm<T>(B, [&, f](int b) { return job(b, PLp(b, f), 0, B.mat(b, B.sS(f)), 0, nullptr, B.mat(b, B.sPL(f + 1)), 0, 0); });
+  }
+  // update_lambda with the packed left product of the sweep: S = PL'[nf-1] S[nf-1]
+  add_gemm<T>(B, [&](int b) { return job(b, PLp(b, nf - 1), 0, B.mat(b, B.sS(nf - 1)), 0, nullptr, B.mat(b, B.sSM()), 0, 0); });
+  add_lambda<T>(B);
+}
+
+template <class T>
+static void run_stage(Batch<T>& B, const typename Batch<T>::Stage& s, hipStream_t st) {
+  BScal sc{B.lam.template as<double>(), B.nR.template as<double>(), B.nL.template as<double>()};
+  switch (s.kind) {
+    case 0:
+      launch_bgemm<T>(s.jobs.template as<BGemmJob<T>>(), B.tiles32.template as<int3>(), B.nt32, 32, sc, st);
+      launch_bgemm<T>(s.jobs.template as<BGemmJob<T>>(), B.tiles64.template as<int3>(), B.nt64, 64, sc, st);
+      break;
+    case 1:
+    case 2:
+      launch_bpower<T>(s.jobs.template as<BPowJob<T>>(), s.njobs, B.maxit, B.thr, B.warm, st);
+      break;
+    case 3:
+      launch_bprox<T>(s.jobs.template as<BProxJob<T>>(), s.njobs, B.prox_mode, st);
+      break;
+    case 4:
+      launch_blambda<T>(s.jobs.template as<BLamJob<T>>(), s.njobs, B.lam.template as<double>(), st);
+      break;
+  }
+}
+
+template <class T>
+static void run_iteration(Batch<T>& B, hipStream_t st) {
+  for (auto& s : B.stages) run_stage(B, s, st);
+}
+
+// Autotuner: for every distinct target size, time one full-size GEMM stage
+// with 32- and 64-wide tiles and keep the faster.
+template <class T>
+static void autotune(Batch<T>& B, hipStream_t st) {
+  std::map<int, std::vector<int>> by_n;
+  for (int b = 0; b < B.sp.nt; ++b) by_n[B.sp.n[b]].push_back(b);
+  B.tile_of.assign(B.sp.nt, 64);
+  // a GEMM stage over the members of one size class
+  const typename Batch<T>::Stage* g = nullptr;
+  for (auto& s : B.stages)
+    if (s.kind == 0) { g = &s; break; }
+  if (!g) return;
+  hipEvent_t e0, e1;
+  FH_HIP(hipEventCreate(&e0));
+  FH_HIP(hipEventCreate(&e1));
+  BScal sc{B.lam.template as<double>(), B.nR.template as<double>(), B.nL.template as<double>()};
+  for (auto& kv : by_n) {
+    float best = 1e30f;
+    int best_tile = 64;
+    for (int tile : {32, 64}) {
+      std::vector<int3> tl;
+      for (int b : kv.second) {
+        const int nt = (kv.first + tile - 1) / tile;
+        for (int i = 0; i < nt; ++i)
+          for (int j = 0; j < nt; ++j) tl.push_back(make_int3(b, i, j));
+      }
+      DBuf dt(sizeof(int3) * tl.size());
+      FH_HIP(hipMemcpy(dt.p, tl.data(), sizeof(int3) * tl.size(), hipMemcpyHostToDevice));
+      float ms = 0.f;
+      for (int rep = 0; rep < 3; ++rep) {
+        FH_HIP(hipEventRecord(e0, st));
+        launch_bgemm<T>(g->jobs.template as<BGemmJob<T>>(), dt.as<int3>(), (int)tl.size(), tile, sc, st);
+        FH_HIP(hipEventRecord(e1, st));
+        FH_HIP(hipEventSynchronize(e1));
+        FH_HIP(hipEventElapsedTime(&ms, e0, e1));
+      }
+      if (ms < best) {
+        best = ms;
+        best_tile = tile;
+      }
+    }
+    for (int b : kv.second) B.tile_of[b] = best_tile;
+  }
+  FH_HIP(hipEventDestroy(e0));
+  FH_HIP(hipEventDestroy(e1));
+}
+
+template <class T>
+static double run_its(Batch<T>& B, int its, bool graph, hipStream_t st) {
+  auto t0 = std::chrono::steady_clock::now();
+  if (its <= 0) return 0.0;
+  if (graph) {
+    hipGraph_t g;
+    hipGraphExec_t ge;
+    FH_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    run_iteration(B, st);
+    FH_HIP(hipStreamEndCapture(st, &g));
+    FH_HIP(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+    t0 = std::chrono::steady_clock::now();
+    for (int i = 0; i < its; ++i) FH_HIP(hipGraphLaunch(ge, st));
+    FH_HIP(hipStreamSynchronize(st));
+    auto t1 = std::chrono::steady_clock::now();
+    FH_HIP(hipGraphExecDestroy(ge));
+    FH_HIP(hipGraphDestroy(g));
+    return std::chrono::duration<double, std::milli>(t1 - t0).count();
+  }
+  for (int i = 0; i < its; ++i) run_iteration(B, st);
+  FH_HIP(hipStreamSynchronize(st));
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// FP64 relative error of lambda * S_0 ... S_{nf-1} against A, per target.
+template <class T>
+static void verify_fp64(const BatchSpec& sp, const std::vector<const double*>& Ad, Batch<T>& B, double* out,
+                        hipStream_t st) {
+  // product in FP64 through a scratch FP64 batch: P <- S_{nf-1}; P <- S_f P
+  Batch<double> V;
+  BatchSpec vs = sp;
+  vs.nf = 1;
+  batch_alloc(V, vs, false);
+  V.tile_of.assign(sp.nt, 64);
+  build_tiles(V);
+  // V slots (nf = 1): 0 A, 1 S0 (current product), 2..: scratch (DR = slot 2, TMP = 3, TR = 4, SM = 5)
+  for (int b = 0; b < sp.nt; ++b) {
+    const int64_t len = (int64_t)sp.n[b] * sp.n[b];
+    FH_HIP(hipMemcpyAsync(V.mat(b, 0), Ad[b], sizeof(double) * len, hipMemcpyDeviceToDevice, st));
+  }
+  auto cast = [&](double* dst, const T* src, int64_t len) {
+    if constexpr (std::is_same<T, float>::value)
+      launch_cast_f32_f64(dst, src, len, st);
+    else
+      launch_bcopy<double>(dst, src, len, st);
+  };
+  for (int b = 0; b < sp.nt; ++b) {
+    const int64_t len = (int64_t)sp.n[b] * sp.n[b];
+    cast(V.mat(b, 1), B.mat(b, B.sS(sp.nf - 1)), len);
+  }
+  BScal sc{V.lam.as<double>(), V.nR.as<double>(), V.nL.as<double>()};
+  for (int f = sp.nf - 2; f >= 0; --f) {
+    std::vector<BGemmJob<double>> h(sp.nt);
+    for (int b = 0; b < sp.nt; ++b) {
+      const int64_t len = (int64_t)sp.n[b] * sp.n[b];
+      cast(V.mat(b, 2), B.mat(b, B.sS(f)), len);
+      h[b] = BGemmJob<double>{V.mat(b, 2), V.mat(b, 1), nullptr, V.mat(b, 3), sp.n[b], sp.n[b], sp.n[b], 0, 0, 0, 0, b};
+    }
+    DBuf dj(sizeof(BGemmJob<double>) * sp.nt);
+    FH_HIP(hipMemcpyAsync(dj.p, h.data(), sizeof(BGemmJob<double>) * sp.nt, hipMemcpyHostToDevice, st));
+    launch_bgemm<double>(dj.as<BGemmJob<double>>(), V.tiles64.as<int3>(), V.nt64, 64, sc, st);
+    for (int b = 0; b < sp.nt; ++b)
+      launch_bcopy<double>(V.mat(b, 1), V.mat(b, 3), (int64_t)sp.n[b] * sp.n[b], st);
+    FH_HIP(hipStreamSynchronize(st));
+  }
+  std::vector<BLamJob<double>> h(sp.nt);
+  for (int b = 0; b < sp.nt; ++b) h[b] = BLamJob<double>{V.mat(b, 0), V.mat(b, 1), (int64_t)sp.n[b] * sp.n[b], b};
+  DBuf dj(sizeof(BLamJob<double>) * sp.nt), dout(sizeof(double) * sp.nt);
+  FH_HIP(hipMemcpyAsync(dj.p, h.data(), sizeof(BLamJob<double>) * sp.nt, hipMemcpyHostToDevice, st));
+  launch_brelerr<double>(dj.as<BLamJob<double>>(), sp.nt, B.lam.template as<double>(), dout.as<double>(), st);
+  FH_HIP(hipMemcpyAsync(out, dout.p, sizeof(double) * sp.nt, hipMemcpyDeviceToHost, st));
+  FH_HIP(hipStreamSynchronize(st));
+}
+
+}  // namespace fh
+
+struct fh_batch_result_s {
+  int nt = 0, nf = 0;
+  std::vector<int> n;
+  std::vector<double> lambda, relerr;
+  std::vector<std::vector<double>> fac;  // [target*nf + f] dense col-major
+  double ms[4] = {0, 0, 0, 0};           // setup, main iterations, refinement, verification
+};
+
+using namespace fh;
+
+namespace {
+
+template <class T>
+void load_batch(Batch<T>& B, const std::vector<const double*>& Ad, const double* const* init,
+                const double* lambda_init, hipStream_t st) {
+  const int nf = B.sp.nf;
+  std::vector<double> lam(B.sp.nt, 1.0);
+  for (int b = 0; b < B.sp.nt; ++b) {
+    const int n = B.sp.n[b];
+    const int64_t len = (int64_t)n * n;
+    // A (already FP64 on the device in Ad)
+    if constexpr (std::is_same<T, float>::value) {
+      std::vector<double> h(len);
+      FH_HIP(hipMemcpy(h.data(), Ad[b], sizeof(double) * len, hipMemcpyDeviceToHost));
+      std::vector<float> hf(h.begin(), h.end());
+      FH_HIP(hipMemcpy(B.mat(b, B.sA()), hf.data(), sizeof(float) * len, hipMemcpyHostToDevice));
+    } else {
+      FH_HIP(hipMemcpy(B.mat(b, B.sA()), Ad[b], sizeof(double) * len, hipMemcpyDeviceToDevice));
+    }
+    for (int f = 0; f < nf; ++f) {
+      std::vector<T> h(len, T(0));
+      const double* src = init ? init[(size_t)b * nf + f] : nullptr;
+      if (src) {
+        for (int64_t i = 0; i < len; ++i) h[i] = (T)src[i];
+      } else if (f > 0) {
+        // palm4msa2 default init, left-to-right: S_0 = 0, the others identity
+        for (int i = 0; i < n; ++i) h[(int64_t)i * n + i] = T(1);
+      }
+      FH_HIP(hipMemcpy(B.mat(b, B.sS(f)), h.data(), sizeof(T) * len, hipMemcpyHostToDevice));
+    }
+    if (lambda_init) lam[b] = lambda_init[b];
+  }
+  FH_HIP(hipMemcpy(B.lam.p, lam.data(), sizeof(double) * B.sp.nt, hipMemcpyHostToDevice));
+}
+
+}  // namespace
+
+extern "C" int fh_palm4msa_batched(int ntargets, int nfacts, const int* n, const double* const* A, const int64_t* k,
+                                   const double* const* supp, const double* const* init, const double* lambda_init,
+                                   const fh_batch_params* p, fh_batch_result_t* out) {
+  FH_API_BEGIN
+  FH_CHECK(ntargets > 0 && nfacts >= 2 && n && A && p && out, "bad argument");
+  FH_CHECK(p->dtype == FH_F32 || p->dtype == FH_F64, "batched PALM runs in FP32 or FP64");
+  FH_CHECK(p->prox == FH_PROX_SP || p->prox == FH_PROX_SUPP, "batched PALM: prox must be sp or supp");
+  FH_CHECK(p->prox == FH_PROX_SUPP ? supp != nullptr : k != nullptr, "batched PALM: missing k or supports");
+  hipStream_t st;
+  FH_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  auto r = std::make_unique<fh_batch_result_s>();
+  auto t0 = std::chrono::steady_clock::now();
+  BatchSpec sp{ntargets, nfacts, std::vector<int>(n, n + ntargets)};
+  // targets in FP64 on the device (for the refinement and the verification)
+  std::vector<DBuf> Ad(ntargets);
+  std::vector<const double*> Adp(ntargets);
+  for (int b = 0; b < ntargets; ++b) {
+    const int64_t len = (int64_t)n[b] * n[b];
+    Ad[b].alloc(sizeof(double) * len);
+    FH_HIP(hipMemcpy(Ad[b].p, A[b], sizeof(double) * len, hipMemcpyHostToDevice));
+    Adp[b] = Ad[b].as<double>();
+  }
+  std::vector<int64_t> kv;
+  if (p->prox == FH_PROX_SP) kv.assign(k, k + ntargets);
+  std::vector<double> lam_out(ntargets);
+  if (p->dtype == FH_F32) {
+    Batch<float> B;
+    batch_alloc(B, sp, p->prox == FH_PROX_SUPP);
+    B.prox_mode = p->prox == FH_PROX_SUPP ? 1 : 0;
+    B.maxit = p->norm2_max_iter > 0 ? p->norm2_max_iter : 100;
+    B.thr = p->norm2_threshold > 0 ? p->norm2_threshold : 1e-6;
+    B.warm = p->warm_power;
+    load_batch(B, Adp, init, lambda_init, st);
+    if (B.prox_mode == 1)
+      for (int b = 0; b < ntargets; ++b)
+        for (int f = 0; f < nfacts; ++f) {
+          const int64_t len = (int64_t)n[b] * n[b];
+          std::vector<float> h(supp[(size_t)b * nfacts + f], supp[(size_t)b * nfacts + f] + len);
+          FH_HIP(hipMemcpy(B.mat(b, B.sMASK(f)), h.data(), sizeof(float) * len, hipMemcpyHostToDevice));
+        }
+    B.tile_of.assign(ntargets, 64);
+    build_tiles(B);
+    build_schedule(B, kv);
+    if (p->autotune) {
+      autotune(B, st);
+      build_tiles(B);
+    }
+    r->ms[0] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    r->ms[1] = run_its(B, p->num_its, p->use_graph != 0, st);
+    // frozen-support FP64 refinement
+    Batch<double> D;
+    if (p->refine_its > 0) {
+      auto t1 = std::chrono::steady_clock::now();
+      batch_alloc(D, sp, true);
+      D.prox_mode = 1;
+      D.maxit = B.maxit;
+      D.thr = B.thr;
+      D.warm = p->warm_power;
+      for (int b = 0; b < ntargets; ++b) {
+        const int64_t len = (int64_t)n[b] * n[b];
+        FH_HIP(hipMemcpyAsync(D.mat(b, D.sA()), Adp[b], sizeof(double) * len, hipMemcpyDeviceToDevice, st));
+        for (int f = 0; f < nfacts; ++f) {
+          launch_cast_f32_f64(D.mat(b, D.sS(f)), B.mat(b, B.sS(f)), len, st);
+          launch_mask_nonzero_f32(D.mat(b, D.sMASK(f)), B.mat(b, B.sS(f)), len, st);
+        }
+      }
+      FH_HIP(hipMemcpyAsync(D.lam.p, B.lam.p, sizeof(double) * ntargets, hipMemcpyDeviceToDevice, st));
+      FH_HIP(hipMemcpyAsync(D.powv.p, B.powv.p, B.powv.bytes, hipMemcpyDeviceToDevice, st));
+      D.tile_of = B.tile_of;
+      build_tiles(D);
+      build_schedule(D, {});
+      FH_HIP(hipStreamSynchronize(st));
+      const double setup_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t1).count();
+      r->ms[2] = run_its(D, p->refine_its, p->use_graph != 0, st);
+      r->ms[0] += setup_ms;
+      auto t2 = std::chrono::steady_clock::now();
+      r->relerr.resize(ntargets);
+      verify_fp64(sp, Adp, D, r->relerr.data(), st);
+      r->ms[3] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t2).count();
+      FH_HIP(hipMemcpy(lam_out.data(), D.lam.p, sizeof(double) * ntargets, hipMemcpyDeviceToHost));
+      r->fac.resize((size_t)ntargets * nfacts);
+      for (int b = 0; b < ntargets; ++b)
+        for (int f = 0; f < nfacts; ++f) {
+          const int64_t len = (int64_t)n[b] * n[b];
+          r->fac[(size_t)b * nfacts + f].resize(len);
+          FH_HIP(hipMemcpy(r->fac[(size_t)b * nfacts + f].data(), D.mat(b, D.sS(f)), sizeof(double) * len,
+                           hipMemcpyDeviceToHost));
+        }
+    } else {
+      auto t2 = std::chrono::steady_clock::now();
+      r->relerr.resize(ntargets);
+      verify_fp64(sp, Adp, B, r->relerr.data(), st);
+      r->ms[3] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t2).count();
+      FH_HIP(hipMemcpy(lam_out.data(), B.lam.p, sizeof(double) * ntargets, hipMemcpyDeviceToHost));
+      r->fac.resize((size_t)ntargets * nfacts);
+      for (int b = 0; b < ntargets; ++b)
+        for (int f = 0; f < nfacts; ++f) {
+          const int64_t len = (int64_t)n[b] * n[b];
+          std::vector<float> h(len);
+          FH_HIP(hipMemcpy(h.data(), B.mat(b, B.sS(f)), sizeof(float) * len, hipMemcpyDeviceToHost));
+          r->fac[(size_t)b * nfacts + f].assign(h.begin(), h.end());
+        }
+    }
+  } else {
+    Batch<double> B;
+    batch_alloc(B, sp, p->prox == FH_PROX_SUPP);
+    B.prox_mode = p->prox == FH_PROX_SUPP ? 1 : 0;
+    B.maxit = p->norm2_max_iter > 0 ? p->norm2_max_iter : 100;
+    B.thr = p->norm2_threshold > 0 ? p->norm2_threshold : 1e-6;
+    B.warm = p->warm_power;
+    load_batch(B, Adp, init, lambda_init, st);
+    if (B.prox_mode == 1)
+      for (int b = 0; b < ntargets; ++b)
+        for (int f = 0; f < nfacts; ++f)
+          FH_HIP(hipMemcpy(B.mat(b, B.sMASK(f)), supp[(size_t)b * nfacts + f], sizeof(double) * n[b] * n[b],
+                           hipMemcpyHostToDevice));
+    B.tile_of.assign(ntargets, 64);
+    build_tiles(B);
+    build_schedule(B, kv);
+    if (p->autotune) {
+      autotune(B, st);
+      build_tiles(B);
+    }
+    r->ms[0] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    r->ms[1] = run_its(B, p->num_its + p->refine_its, p->use_graph != 0, st);
+    auto t2 = std::chrono::steady_clock::now();
+    r->relerr.resize(ntargets);
+    verify_fp64(sp, Adp, B, r->relerr.data(), st);
+    r->ms[3] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t2).count();
+    FH_HIP(hipMemcpy(lam_out.data(), B.lam.p, sizeof(double) * ntargets, hipMemcpyDeviceToHost));
+    r->fac.resize((size_t)ntargets * nfacts);
+    for (int b = 0; b < ntargets; ++b)
+      for (int f = 0; f < nfacts; ++f) {
+        const int64_t len = (int64_t)n[b] * n[b];
+        r->fac[(size_t)b * nfacts + f].resize(len);
+        FH_HIP(hipMemcpy(r->fac[(size_t)b * nfacts + f].data(), B.mat(b, B.sS(f)), sizeof(double) * len,
+                         hipMemcpyDeviceToHost));
+      }
+  }
+  r->nt = ntargets;
+  r->nf = nfacts;
+  r->n = sp.n;
+  r->lambda = lam_out;
+  FH_HIP(hipStreamDestroy(st));
+  *out = r.release();
+  FH_API_END
+}
+
+extern "C" int fh_batch_result_lambda(fh_batch_result_t r, double* out) {
+  FH_API_BEGIN
+  FH_CHECK(r && out, "null argument");
+  std::copy(r->lambda.begin(), r->lambda.end(), out);
+  FH_API_END
+}
+
+extern "C" int fh_batch_result_relerr(fh_batch_result_t r, double* out) {
+  FH_API_BEGIN
+  FH_CHECK(r && out, "null argument");
+  std::copy(r->relerr.begin(), r->relerr.end(), out);
+  FH_API_END
+}
+
+extern "C" int fh_batch_result_factor(fh_batch_result_t r, int target, int f, double* out) {
+  FH_API_BEGIN
+  FH_CHECK(r && out && target >= 0 && target < r->nt && f >= 0 && f < r->nf, "bad index");
+  const auto& v = r->fac[(size_t)target * r->nf + f];
+  std::copy(v.begin(), v.end(), out);
+  FH_API_END
+}
+
+extern "C" int fh_batch_result_times(fh_batch_result_t r, double* ms4) {
+  FH_API_BEGIN
+  FH_CHECK(r && ms4, "null argument");
+  for (int i = 0; i < 4; ++i) ms4[i] = r->ms[i];
+  FH_API_END
+}
+
+extern "C" int fh_batch_result_free(fh_batch_result_t r) {
+  FH_API_BEGIN
+  delete r;
+  FH_API_END
+}

@@ -1,0 +1,73 @@
+// fh_batched.hpp -- job descriptors shared by the batched PALM4MSA kernels
+// (k_batched.hip) and their host driver (fh_batched.cpp).
+#pragma once
+#include "fh_internal.hpp"
+
+namespace fh {
+
+constexpr int BATCH_NMAX = 2048;  // largest target edge (power-iteration vectors live in LDS)
+
+// C_out = alpha_b * op(A) op(B) + beta * C_in on n x n column-major blocks;
+// alpha mode 0 -> 1, 1 -> lambda_b, 2 -> -lambda_b / c_b with
+// c_b = 1.001 lambda_b^2 nR_b^2 nL_b^2; beta mode 0 -> 0, 1 -> -1, 2 -> +1.
+template <class T>
+struct BGemmJob {
+  const T* A;
+  const T* B;
+  const T* Cin;
+  T* Cout;
+  int m, n, k;
+  int transA, transB;
+  int alpha_mode, beta_mode;
+  int target;
+};
+
+struct BScal {
+  double* lambda;
+  double* nR;
+  double* nL;
+};
+
+// sqrt(lambda_max(P P^T)) of an n x n block -> out[target]; P == nullptr
+// writes 1 (no factor on that side).  vec: this slot's start vector.
+template <class T>
+struct BPowJob {
+  const T* P;
+  int n;
+  int target;
+  double* out;
+  double* vec;
+};
+
+template <class T>
+struct BProxJob {
+  T* M;
+  const T* mask;  // SUPP only
+  int64_t len;
+  int64_t k;
+};
+
+template <class T>
+struct BLamJob {
+  const T* A;
+  const T* S;
+  int64_t len;
+  int target;
+};
+
+template <class T>
+void launch_bgemm(const BGemmJob<T>* jobs, const int3* tiles, int ntiles, int tile, const BScal& sc, hipStream_t st);
+template <class T>
+void launch_bpower(const BPowJob<T>* jobs, int njobs, int maxit, double thr, int warm, hipStream_t st);
+template <class T>
+void launch_bprox(const BProxJob<T>* jobs, int njobs, int mode, hipStream_t st);
+template <class T>
+void launch_blambda(const BLamJob<T>* jobs, int njobs, double* lambda, hipStream_t st);
+template <class T>
+void launch_brelerr(const BLamJob<T>* jobs, int njobs, const double* lambda, double* out, hipStream_t st);
+template <class T>
+void launch_bcopy(T* dst, const T* src, int64_t n, hipStream_t st);
+void launch_cast_f32_f64(double* dst, const float* src, int64_t n, hipStream_t st);
+void launch_mask_nonzero_f32(double* mask, const float* src, int64_t n, hipStream_t st);
+
+}  // namespace fh

@@ -112,6 +112,10 @@ struct DMat {
   DBuf brp, bci, bdat;
   DBuf tbrp, tbci, tbdat;
   bool is_identity = false;
+  // CSR copy of a dense matrix that turned out to be mostly zeros (built by
+  // the product dispatcher; same values, so products keep the same bits)
+  mutable std::shared_ptr<DMat> csr_twin;
+  mutable bool sparsity_checked = false;
 };
 using DMatP = std::shared_ptr<DMat>;
 

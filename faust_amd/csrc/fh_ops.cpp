@@ -38,9 +38,30 @@ int64_t nnz_of(DMat& A, hipStream_t st) {
   return A.nnz;
 }
 
+// A dense operand whose density is below 1/8 is multiplied through a CSR
+// copy (the densified products of a few butterfly-like factors are such
+// matrices).  Only matrices of at least 64K entries are examined.
+static const DMat* sparse_view(const DMat& A, hipStream_t st) {
+  if (A.kind != K_DENSE || A.dtype != FH_F64) return &A;
+  const int64_t N = (int64_t)A.m * A.n;
+  if (N < 65536) return &A;
+  if (!A.sparsity_checked) {
+    A.sparsity_checked = true;
+    int64_t nz = nnz_of(const_cast<DMat&>(A), st);
+    if (nz * 8 < N) {
+      auto S = std::make_shared<DMat>();
+      dense_to_csr_f64(A.d.as<double>(), A.m, A.n, *S, st);
+      A.csr_twin = S;
+    }
+  }
+  return A.csr_twin ? A.csr_twin.get() : &A;
+}
+
 // out = alpha*op(A)*op(B) + beta*out ; out is dense (allocated here when beta == 0)
-void matmul_f64(const DMat& A, char opA, const DMat& B, char opB, double alpha, double beta, DMat& out,
+void matmul_f64(const DMat& A0, char opA, const DMat& B0, char opB, double alpha, double beta, DMat& out,
                 hipStream_t st) {
+  const DMat& A = *sparse_view(A0, st);
+  const DMat& B = (A.kind == K_CSR) ? B0 : *sparse_view(B0, st);
   const int M = (opA == 'N') ? A.m : A.n;
   const int K = (opA == 'N') ? A.n : A.m;
   const int KB = (opB == 'N') ? B.m : B.n;
@@ -57,6 +78,8 @@ void matmul_f64(const DMat& A, char opA, const DMat& B, char opB, double alpha, 
   }
   out.dtype = FH_F64;
   out.nnz = -1;
+  out.csr_twin.reset();
+  out.sparsity_checked = false;
   double* C = out.d.as<double>();
   if (M == 0 || N == 0) return;
   if (A.kind == K_CSR) {

@@ -1,0 +1,439 @@
+// k_batched.hip -- batched PALM4MSA: many independent factorizations of
+// uneven sizes advanced in lock-step, FP32 or FP64.
+//
+// Every per-target scalar (lambda, the two spectral norms of the current
+// factor update) lives in device memory and every kernel reads it there, so
+// the kernel sequence of one PALM iteration is fixed and is captured once as
+// a HIP graph, then replayed for all iterations: no host round trip inside
+// the loop.  Each batched kernel walks a flat work list (target, tile), so
+// targets of 64..512 rows share one launch.
+//
+// Algorithm per target: palm4msa2 with packing_RL and left-to-right updates
+// (faust_palm4msa2020.hpp:47-300 with the ParamsPalm4MSA defaults), all
+// factors square and kept dense.
+#include <algorithm>
+#include <vector>
+
+#include "fh_api_common.hpp"
+#include "fh_batched.hpp"
+
+namespace fh {
+
+// ---------------------------------------------------------------- grouped GEMM
+// C_out = alpha_b * op(A) op(B) + beta * C_in, alpha_b from the per-target
+// scalars: mode 0 -> 1, 1 -> lambda_b, 2 -> -lambda_b / c_b with
+// c_b = 1.001 lambda_b^2 nR_b^2 nL_b^2 (update_fact's Lipschitz step).
+
+template <class T>
+__device__ __forceinline__ T job_alpha(int mode, int b, const BScal& s) {
+  if (mode == 0) return T(1);
+  const double lam = s.lambda[b];
+  if (mode == 1) return (T)lam;
+  const double nr = s.nR[b], nl = s.nL[b];
+  const double c = 1.001 * lam * lam * nr * nr * nl * nl;
+  return (T)(-lam / c);
+}
+
+template <class T, int TILE>
+__global__ __launch_bounds__(256) void k_bgemm(const BGemmJob<T>* __restrict__ jobs, const int3* __restrict__ tiles,
+                                               BScal sc) {
+  constexpr int MT = TILE / 16;  // micro-tile edge per thread
+  constexpr int KS = 16;
+  __shared__ T As[KS][TILE + 1];
+  __shared__ T Bs[KS][TILE + 1];
+  const int3 tl = tiles[blockIdx.x];
+  const BGemmJob<T> J = jobs[tl.x];
+  const int t = threadIdx.x, tx = t & 15, ty = t >> 4;
+  const int i0 = tl.y * TILE, j0 = tl.z * TILE;
+  T acc[MT][MT];
+#pragma unroll
+  for (int r = 0; r < MT; ++r)
+#pragma unroll
+    for (int c = 0; c < MT; ++c) acc[r][c] = T(0);
+  for (int k0 = 0; k0 < J.k; k0 += KS) {
+    for (int e = t; e < TILE * KS; e += 256) {
+      int ii, kk;
+      if (!J.transA) { ii = e % TILE; kk = e / TILE; } else { kk = e % KS; ii = e / KS; }
+      const int gi = i0 + ii, gk = k0 + kk;
+      As[kk][ii] = (gi < J.m && gk < J.k) ? (J.transA ? J.A[(size_t)gi * J.k + gk] : J.A[(size_t)gk * J.m + gi])
+                                          : T(0);
+    }
+    for (int e = t; e < TILE * KS; e += 256) {
+      int jj, kk;
+      if (J.transB) { jj = e % TILE; kk = e / TILE; } else { kk = e % KS; jj = e / KS; }
+      const int gj = j0 + jj, gk = k0 + kk;
+      Bs[kk][jj] = (gj < J.n && gk < J.k) ? (J.transB ? J.B[(size_t)gk * J.n + gj] : J.B[(size_t)gj * J.k + gk])
+                                          : T(0);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < KS; ++kk) {
+      T a[MT], b[MT];
+#pragma unroll
+      for (int r = 0; r < MT; ++r) a[r] = As[kk][ty + 16 * r];
+#pragma unroll
+      for (int c = 0; c < MT; ++c) b[c] = Bs[kk][tx + 16 * c];
+#pragma unroll
+      for (int r = 0; r < MT; ++r)
+#pragma unroll
+        for (int c = 0; c < MT; ++c) acc[r][c] = fma(a[r], b[c], acc[r][c]);
+    }
+    __syncthreads();
+  }
+  const T alpha = job_alpha<T>(J.alpha_mode, J.target, sc);
+  const T beta = J.beta_mode == 0 ? T(0) : (J.beta_mode == 1 ? T(-1) : T(1));
+#pragma unroll
+  for (int r = 0; r < MT; ++r) {
+    const int gi = i0 + ty + 16 * r;
+    if (gi >= J.m) continue;
+#pragma unroll
+    for (int c = 0; c < MT; ++c) {
+      const int gj = j0 + tx + 16 * c;
+      if (gj >= J.n) continue;
+      const size_t o = (size_t)gj * J.m + gi;
+      T v = alpha * acc[r][c];
+      if (J.beta_mode) v = fma(beta, J.Cin[o], v);
+      J.Cout[o] = v;
+    }
+  }
+}
+
+// ---------------------------------------------------------------- power iteration
+// One workgroup per job: lambda_max(P P^T) by power iteration with the stop
+// rule of power_iteration (faust_linear_algebra.hpp:846-889); the result
+// sqrt(lambda) goes to out[target].  warm: start from (and store) the last
+// normalised vector of this job slot.
+
+__device__ double block_sum_d(double v, double* red) {
+  const int t = threadIdx.x;
+  red[t] = v;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s >= 1; s >>= 1) {
+    if (t < s) red[t] = red[t] + red[t + s];
+    __syncthreads();
+  }
+  const double r = red[0];
+  __syncthreads();
+  return r;
+}
+
+__device__ __forceinline__ double wave_sum_d(double v) {
+  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// The step size stays FP64 whatever the factor precision: vectors and sums
+// are double, matrix entries are widened on load.
+template <class T>
+__global__ __launch_bounds__(256) void k_bpower(const BPowJob<T>* __restrict__ jobs, int maxit, double thr,
+                                                int warm) {
+  __shared__ double x[BATCH_NMAX], y[BATCH_NMAX];
+  __shared__ double red[256];
+  const BPowJob<T> J = jobs[blockIdx.x];
+  const int n = J.n, t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  if (J.P == nullptr) {
+    if (t == 0) J.out[J.target] = 1.0;
+    return;
+  }
+  for (int i = t; i < n; i += 256) x[i] = J.vec[i];
+  __syncthreads();
+  double lambda_old = 1.0, lambda = 0.0;
+  int it = 0;
+  bool zero = false;
+  while ((fabs(lambda_old - lambda) > fabs(thr) || fabs(lambda) <= fabs(thr)) && it < maxit) {
+    ++it;
+    lambda_old = lambda;
+    double ss = 0.0;
+    for (int i = t; i < n; i += 256) ss = fma(x[i], x[i], ss);
+    const double nrm = sqrt(block_sum_d(ss, red));
+    if (nrm == 0.0) {
+      zero = true;
+      break;
+    }
+    const double inv = 1.0 / nrm;
+    for (int i = t; i < n; i += 256) x[i] = x[i] * inv;
+    __syncthreads();
+    // y = P^T x: one wave per column of P (contiguous), lanes split k
+    for (int i = wave; i < n; i += 4) {
+      const T* col = J.P + (size_t)i * n;
+      double a = 0.0;
+      for (int k = lane; k < n; k += 64) a = fma((double)col[k], x[k], a);
+      a = wave_sum_d(a);
+      if (lane == 0) y[i] = a;
+    }
+    __syncthreads();
+    // z = P y (threads over rows: coalesced), lambda = <x, z>
+    double dot = 0.0;
+    double z[BATCH_NMAX / 256];
+    int nz = 0;
+#pragma unroll
+    for (int q = 0; q < BATCH_NMAX / 256; ++q) {
+      const int i = t + 256 * q;
+      double a = 0.0;
+      if (i < n) {
+        for (int k = 0; k < n; ++k) a = fma((double)J.P[(size_t)k * n + i], y[k], a);
+        dot = fma(x[i], a, dot);
+        nz |= (a != 0.0);
+      }
+      z[q] = a;
+    }
+    lambda = block_sum_d(dot, red);
+    if (!__syncthreads_or(nz)) {
+      lambda = 0.0;
+      zero = true;
+      break;
+    }
+#pragma unroll
+    for (int q = 0; q < BATCH_NMAX / 256; ++q)
+      if (t + 256 * q < n) x[t + 256 * q] = z[q];
+    __syncthreads();
+  }
+  if (warm && !zero)
+    for (int i = t; i < n; i += 256) J.vec[i] = x[i];
+  if (t == 0) J.out[J.target] = sqrt(fabs(lambda));
+}
+
+// ---------------------------------------------------------------- projections
+// One 1024-thread workgroup per target.  SP: keep the k largest magnitudes
+// (radix select on the magnitude bits; among entries equal to the k-th
+// largest the lowest indices are kept), then divide by the Frobenius norm.
+// SUPP: multiply by the 0/1 mask, then normalise.
+
+template <class T> struct KeyOf;
+template <> struct KeyOf<float> {
+  using K = unsigned;
+  static constexpr int BITS = 32;
+  __device__ static K key(float v) { return __float_as_uint(v) & 0x7fffffffu; }
+};
+template <> struct KeyOf<double> {
+  using K = unsigned long long;
+  static constexpr int BITS = 64;
+  __device__ static K key(double v) { return (unsigned long long)__double_as_longlong(v) & 0x7fffffffffffffffull; }
+};
+
+template <class T, int MODE>  // MODE 0: SP, 1: SUPP
+__global__ __launch_bounds__(1024) void k_bprox(const BProxJob<T>* __restrict__ jobs) {
+  using KO = KeyOf<T>;
+  using K = typename KO::K;
+  __shared__ unsigned hist[256];
+  __shared__ T red[1024];
+  __shared__ unsigned scan[1024];
+  __shared__ K s_prefix;
+  __shared__ long long s_kk;
+  const BProxJob<T> J = jobs[blockIdx.x];
+  const int t = threadIdx.x;
+  const int64_t len = J.len;
+  if (MODE == 1) {
+    for (int64_t i = t; i < len; i += 1024) J.M[i] = J.M[i] * J.mask[i];
+  } else if (J.k < len) {
+    if (J.k <= 0) {
+      for (int64_t i = t; i < len; i += 1024) J.M[i] = T(0);
+    } else {
+      K prefix = 0, mask = 0;
+      long long kk = J.k;  // rank still to find among keys matching prefix
+      for (int shift = KO::BITS - 8; shift >= 0; shift -= 8) {
+        for (int b = t; b < 256; b += 1024) hist[b] = 0;
+        __syncthreads();
+        for (int64_t i = t; i < len; i += 1024) {
+          const K key = KO::key(J.M[i]);
+          if ((key & mask) == prefix) atomicAdd(&hist[(key >> shift) & 255], 1u);
+        }
+        __syncthreads();
+        if (t == 0) {
+          long long cum = 0;
+          int b = 255;
+          for (; b > 0; --b) {
+            if (cum + hist[b] >= kk) break;
+            cum += hist[b];
+          }
+          s_kk = kk - cum;
+          s_prefix = prefix | ((K)b << shift);
+        }
+        __syncthreads();
+        kk = s_kk;
+        prefix = s_prefix;
+        mask |= ((K)255 << shift);
+        __syncthreads();
+      }
+      // keys > prefix are kept; kk entries equal to prefix are kept, lowest index first
+      const int64_t chunk = (len + 1023) / 1024;
+      const int64_t c0 = t * chunk, c1 = min(len, c0 + chunk);
+      unsigned eq = 0;
+      for (int64_t i = c0; i < c1; ++i) eq += (KO::key(J.M[i]) == prefix);
+      scan[t] = eq;
+      __syncthreads();
+      for (int s = 1; s < 1024; s <<= 1) {
+        unsigned v = (t >= s) ? scan[t - s] : 0u;
+        __syncthreads();
+        scan[t] += v;
+        __syncthreads();
+      }
+      long long before = (long long)scan[t] - eq;
+      for (int64_t i = c0; i < c1; ++i) {
+        const K key = KO::key(J.M[i]);
+        bool keep = key > prefix;
+        if (key == prefix) {
+          keep = before < kk;
+          ++before;
+        }
+        if (!keep) J.M[i] = T(0);
+      }
+    }
+  }
+  __syncthreads();
+  // Frobenius normalisation
+  T ss = T(0);
+  for (int64_t i = t; i < len; i += 1024) ss = fma(J.M[i], J.M[i], ss);
+  red[t] = ss;
+  __syncthreads();
+  for (int s = 512; s >= 1; s >>= 1) {
+    if (t < s) red[t] = red[t] + red[t + s];
+    __syncthreads();
+  }
+  const T nrm = sqrt(red[0]);
+  if (nrm > T(0)) {
+    const T inv = T(1) / nrm;
+    for (int64_t i = t; i < len; i += 1024) J.M[i] = J.M[i] * inv;
+  }
+}
+
+// ---------------------------------------------------------------- lambda / errors
+// lambda_b = <A, S>_F / ||S||_F^2 (update_lambda, faust_palm4msa2020.hpp:442-493)
+
+template <class T>
+__global__ __launch_bounds__(256) void k_blambda(const BLamJob<T>* __restrict__ jobs, double* __restrict__ lambda) {
+  __shared__ double r1[256], r2[256];
+  const BLamJob<T> J = jobs[blockIdx.x];
+  const int t = threadIdx.x;
+  double tr = 0.0, ss = 0.0;
+  for (int64_t i = t; i < J.len; i += 256) {
+    const double s = (double)J.S[i];
+    tr = fma((double)J.A[i], s, tr);
+    ss = fma(s, s, ss);
+  }
+  r1[t] = tr;
+  r2[t] = ss;
+  __syncthreads();
+  for (int s = 128; s >= 1; s >>= 1) {
+    if (t < s) {
+      r1[t] += r1[t + s];
+      r2[t] += r2[t + s];
+    }
+    __syncthreads();
+  }
+  if (t == 0 && r2[0] > 0.0) lambda[J.target] = r1[0] / r2[0];
+}
+
+// rel_err_b = || lambda_b P - A ||_F / ||A||_F in FP64
+template <class T>
+__global__ __launch_bounds__(256) void k_brelerr(const BLamJob<T>* __restrict__ jobs, const double* __restrict__ lambda,
+                                                 double* __restrict__ out) {
+  __shared__ double r1[256], r2[256];
+  const BLamJob<T> J = jobs[blockIdx.x];
+  const int t = threadIdx.x;
+  const double lam = lambda[J.target];
+  double e = 0.0, a2 = 0.0;
+  for (int64_t i = t; i < J.len; i += 256) {
+    const double a = (double)J.A[i];
+    const double d = lam * (double)J.S[i] - a;
+    e = fma(d, d, e);
+    a2 = fma(a, a, a2);
+  }
+  r1[t] = e;
+  r2[t] = a2;
+  __syncthreads();
+  for (int s = 128; s >= 1; s >>= 1) {
+    if (t < s) {
+      r1[t] += r1[t + s];
+      r2[t] += r2[t + s];
+    }
+    __syncthreads();
+  }
+  if (t == 0) out[J.target] = sqrt(r1[0]) / sqrt(r2[0]);
+}
+
+template <class T>
+__global__ void k_bcopy(T* __restrict__ dst, const T* __restrict__ src, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[i];
+}
+
+__global__ void k_cast_f32_f64(double* __restrict__ dst, const float* __restrict__ src, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = (double)src[i];
+}
+__global__ void k_mask_nz(double* __restrict__ mask, const float* __restrict__ src, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) mask[i] = src[i] != 0.0f ? 1.0 : 0.0;
+}
+void launch_cast_f32_f64(double* dst, const float* src, int64_t n, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_cast_f32_f64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, dst, src, n);
+  FH_LAUNCH_CHECK();
+}
+void launch_mask_nonzero_f32(double* mask, const float* src, int64_t n, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_mask_nz, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, mask, src, n);
+  FH_LAUNCH_CHECK();
+}
+
+// ---------------------------------------------------------------- host launchers
+template <class T>
+void launch_bgemm(const BGemmJob<T>* jobs, const int3* tiles, int ntiles, int tile, const BScal& sc, hipStream_t st) {
+  if (ntiles == 0) return;
+  if (tile == 32)
+    hipLaunchKernelGGL((k_bgemm<T, 32>), dim3(ntiles), dim3(256), 0, st, jobs, tiles, sc);
+  else
+    hipLaunchKernelGGL((k_bgemm<T, 64>), dim3(ntiles), dim3(256), 0, st, jobs, tiles, sc);
+  FH_LAUNCH_CHECK();
+}
+
+template <class T>
+void launch_bpower(const BPowJob<T>* jobs, int njobs, int maxit, double thr, int warm, hipStream_t st) {
+  if (njobs == 0) return;
+  hipLaunchKernelGGL(k_bpower<T>, dim3(njobs), dim3(256), 0, st, jobs, maxit, thr, warm);
+  FH_LAUNCH_CHECK();
+}
+
+template <class T>
+void launch_bprox(const BProxJob<T>* jobs, int njobs, int mode, hipStream_t st) {
+  if (njobs == 0) return;
+  if (mode == 0)
+    hipLaunchKernelGGL((k_bprox<T, 0>), dim3(njobs), dim3(1024), 0, st, jobs);
+  else
+    hipLaunchKernelGGL((k_bprox<T, 1>), dim3(njobs), dim3(1024), 0, st, jobs);
+  FH_LAUNCH_CHECK();
+}
+
+template <class T>
+void launch_blambda(const BLamJob<T>* jobs, int njobs, double* lambda, hipStream_t st) {
+  if (njobs == 0) return;
+  hipLaunchKernelGGL(k_blambda<T>, dim3(njobs), dim3(256), 0, st, jobs, lambda);
+  FH_LAUNCH_CHECK();
+}
+
+template <class T>
+void launch_brelerr(const BLamJob<T>* jobs, int njobs, const double* lambda, double* out, hipStream_t st) {
+  if (njobs == 0) return;
+  hipLaunchKernelGGL(k_brelerr<T>, dim3(njobs), dim3(256), 0, st, jobs, lambda, out);
+  FH_LAUNCH_CHECK();
+}
+
+template <class T>
+void launch_bcopy(T* dst, const T* src, int64_t n, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_bcopy<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, dst, src, n);
+  FH_LAUNCH_CHECK();
+}
+
+#define FH_INST(T)                                                                                            \
+  template void launch_bgemm<T>(const BGemmJob<T>*, const int3*, int, int, const BScal&, hipStream_t);       \
+  template void launch_bpower<T>(const BPowJob<T>*, int, int, double, int, hipStream_t);                    \
+  template void launch_bprox<T>(const BProxJob<T>*, int, int, hipStream_t);                                 \
+  template void launch_blambda<T>(const BLamJob<T>*, int, double*, hipStream_t);                            \
+  template void launch_brelerr<T>(const BLamJob<T>*, int, const double*, double*, hipStream_t);             \
+  template void launch_bcopy<T>(T*, const T*, int64_t, hipStream_t);
+FH_INST(float)
+FH_INST(double)
+
+}  // namespace fh

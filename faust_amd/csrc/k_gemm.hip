@@ -7,16 +7,58 @@
 //
 // Tiling: a 256-thread workgroup computes a 64x64 tile of C; each thread a
 // 4x4 micro-tile with rows ty+16r and columns tx+16c (strided, so the LDS
-// reads of one wave hit distinct banks).  K advances in steps of 16 through
-// LDS tiles As[k][i] and Bs[k][j]; out-of-range entries load as 0, which
-// leaves every fma chain unchanged.
+// reads of a wave are broadcasts or distinct banks).  K advances in steps of
+// 32 through LDS tiles As[k][i] and Bs[k][j]; the next step's global values
+// are loaded into registers while the current step computes, so one
+// workgroup per CU still overlaps HBM/L2 latency with FMAs.  Out-of-range
+// entries load as 0, which leaves every fma chain unchanged.
 #include "fh_internal.hpp"
 
 namespace fh {
 
 constexpr int GT = 64;  // tile edge
-constexpr int GK = 16;  // k step
+constexpr int GK = 32;  // k step
+constexpr int GPT = GT * GK / 256;  // elements of each operand tile per thread (8)
 
+__device__ __forceinline__ void gemm_load(const double* __restrict__ P, int ld, int trans, int rows, int kdim,
+                                          int r0, int k0, int t, double (&reg)[GPT]) {
+  // tile element e = t + 256*q; non-transposed operand: fastest index is the
+  // row (contiguous in column-major), transposed: fastest is k
+#pragma unroll
+  for (int q = 0; q < GPT; ++q) {
+    const int e = t + 256 * q;
+    int rr, kk;
+    if (!trans) {
+      rr = e & (GT - 1);
+      kk = e >> 6;
+    } else {
+      kk = e & (GK - 1);
+      rr = e >> 5;
+    }
+    const int gr = r0 + rr, gk = k0 + kk;
+    reg[q] = (gr < rows && gk < kdim) ? (trans ? P[(size_t)gr * ld + gk] : P[(size_t)gk * ld + gr]) : 0.0;
+  }
+}
+
+__device__ __forceinline__ void gemm_store(double (*S)[GT + 1], int trans, int t, const double (&reg)[GPT]) {
+#pragma unroll
+  for (int q = 0; q < GPT; ++q) {
+    const int e = t + 256 * q;
+    int rr, kk;
+    if (!trans) {
+      rr = e & (GT - 1);
+      kk = e >> 6;
+    } else {
+      kk = e & (GK - 1);
+      rr = e >> 5;
+    }
+    S[kk][rr] = reg[q];
+  }
+}
+
+// op(A) is m x k: element (i, kk) = transA ? A[i*lda + kk] : A[kk*lda + i]
+// op(B) is k x n: element (kk, j) = transB ? B[kk*ldb + j] : B[j*ldb + kk]
+// B is loaded as the "row" operand of its transpose: row j, column kk.
 __global__ __launch_bounds__(256) void k_gemm_f64(int m, int n, int k, double alpha, const double* __restrict__ A,
                                                    int lda, int transA, const double* __restrict__ B, int ldb,
                                                    int transB, double beta, double* __restrict__ C, int ldc) {
@@ -30,44 +72,19 @@ __global__ __launch_bounds__(256) void k_gemm_f64(int m, int n, int k, double al
   for (int r = 0; r < 4; ++r)
 #pragma unroll
     for (int c = 0; c < 4; ++c) acc[r][c] = 0.0;
-
+  double ra[GPT], rb[GPT];
+  // op(B)^T (n x k) row j, col kk = op(B)(kk, j): stored "transposed" iff !transB
+  gemm_load(A, lda, transA, m, k, i0, 0, t, ra);
+  gemm_load(B, ldb, !transB, n, k, j0, 0, t, rb);
   for (int k0 = 0; k0 < k; k0 += GK) {
-    // op(A) tile: rows i0..i0+63, cols k0..k0+15 -> As[kk][ii]
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      int e = t + 256 * q;  // 0..1023
-      int ii, kk;
-      if (!transA) {
-        ii = e & 63;
-        kk = e >> 6;
-      } else {
-        kk = e & 15;
-        ii = e >> 4;
-      }
-      int gi = i0 + ii, gk = k0 + kk;
-      double val = 0.0;
-      if (gi < m && gk < k) val = transA ? A[(size_t)gi * lda + gk] : A[(size_t)gk * lda + gi];
-      As[kk][ii] = val;
-    }
-    // op(B) tile: rows k0..k0+15, cols j0..j0+63 -> Bs[kk][jj]
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      int e = t + 256 * q;
-      int jj, kk;
-      if (!transB) {
-        kk = e & 15;
-        jj = e >> 4;
-      } else {
-        jj = e & 63;
-        kk = e >> 6;
-      }
-      int gj = j0 + jj, gk = k0 + kk;
-      double val = 0.0;
-      if (gj < n && gk < k) val = transB ? B[(size_t)gk * ldb + gj] : B[(size_t)gj * ldb + gk];
-      Bs[kk][jj] = val;
-    }
+    gemm_store(As, transA, t, ra);
+    gemm_store(Bs, !transB, t, rb);
     __syncthreads();
-#pragma unroll
+    if (k0 + GK < k) {
+      gemm_load(A, lda, transA, m, k, i0, k0 + GK, t, ra);
+      gemm_load(B, ldb, !transB, n, k, j0, k0 + GK, t, rb);
+    }
+#pragma unroll 8
     for (int kk = 0; kk < GK; ++kk) {
       double a[4], b[4];
 #pragma unroll
@@ -83,13 +100,13 @@ __global__ __launch_bounds__(256) void k_gemm_f64(int m, int n, int k, double al
   }
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
-    int gi = i0 + ty + 16 * r;
+    const int gi = i0 + ty + 16 * r;
     if (gi >= m) continue;
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
-      int gj = j0 + tx + 16 * c;
+      const int gj = j0 + tx + 16 * c;
       if (gj >= n) continue;
-      double tt = alpha * acc[r][c];
+      const double tt = alpha * acc[r][c];
       double* cp = C + (size_t)gj * ldc + gi;
       *cp = (beta == 0.0) ? tt : fma(beta, *cp, tt);
     }

@@ -174,6 +174,43 @@ int fh_palm_result_factor(fh_palm_result_t r, int i, fh_mat_t *out);
 int fh_palm_result_traj(fh_palm_result_t r, double *out4, int *len);
 int fh_palm_result_free(fh_palm_result_t r);
 
+/* ---- batched PALM4MSA: many independent factorizations in lock-step.
+ * Every target b is n[b] x n[b] and is factorized into nfacts square factors
+ * with palm4msa2's left-to-right, packed schedule (ParamsPalm4MSA defaults).
+ * The main iterations run in `dtype` (FH_F32 or FH_F64); with refine_its > 0
+ * and FH_F32 they are followed by FP64 sweeps in which each factor is
+ * projected on its frozen support (prox supp, normalised).  Step sizes
+ * (power iteration), lambda and the final relative errors are FP64.  One
+ * iteration is captured once as a HIP graph and replayed when use_graph. */
+typedef struct {
+  int dtype;           /* FH_F32 or FH_F64 */
+  int num_its;         /* main iterations */
+  int refine_its;      /* FP64 frozen-support sweeps after FP32 iterations */
+  int prox;            /* FH_PROX_SP (k per target) or FH_PROX_SUPP (masks) */
+  int warm_power;      /* 1: start each power iteration from the previous vector */
+  int norm2_max_iter;  /* default 100 */
+  double norm2_threshold; /* default 1e-6 */
+  int use_graph;       /* capture one iteration in a HIP graph */
+  int autotune;        /* time 32/64 GEMM tiles per target size at set-up */
+} fh_batch_params;
+
+typedef struct fh_batch_result_s *fh_batch_result_t;
+
+/* A[b]: host FP64 n[b]*n[b] col-major; k[b]: sp parameter (FH_PROX_SP);
+ * supp[b*nfacts+f]: host 0/1 masks (FH_PROX_SUPP); init[b*nfacts+f]: initial
+ * factors or NULL for the palm4msa2 default (S_0 = 0, others identity);
+ * lambda_init: per-target initial lambda or NULL (1). */
+int fh_palm4msa_batched(int ntargets, int nfacts, const int *n, const double *const *A, const int64_t *k,
+                        const double *const *supp, const double *const *init, const double *lambda_init,
+                        const fh_batch_params *p, fh_batch_result_t *out);
+int fh_batch_result_lambda(fh_batch_result_t r, double *lambdas);
+/* FP64 relative error ||lambda S_0...S_{J-1} - A||_F / ||A||_F per target */
+int fh_batch_result_relerr(fh_batch_result_t r, double *errs);
+int fh_batch_result_factor(fh_batch_result_t r, int target, int f, double *dense_out);
+/* ms[0] set-up, ms[1] main iterations, ms[2] refinement, ms[3] verification */
+int fh_batch_result_times(fh_batch_result_t r, double *ms4);
+int fh_batch_result_free(fh_batch_result_t r);
+
 #ifdef __cplusplus
 }
 #endif
