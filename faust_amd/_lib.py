@@ -86,6 +86,13 @@ _SIGS = {
     "fh_batch_result_factor": (C.c_int, [VP, C.c_int, C.c_int, VP]),
     "fh_batch_result_times": (C.c_int, [VP, VP]),
     "fh_batch_result_free": (C.c_int, [VP]),
+    "fh_mlp_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64, C.POINTER(VP)]),
+    "fh_mlp_free": (C.c_int, [VP]),
+    "fh_mlp_forward": (C.c_int, [VP, VP, C.c_int, VP, VP]),
+    "fh_mlp_gradients": (C.c_int, [VP, VP, VP, C.c_int, C.POINTER(C.c_float), VP]),
+    "fh_mlp_train_step": (C.c_int, [VP, VP, VP, C.c_int, C.c_float, C.POINTER(C.c_float), VP]),
+    "fh_mlp_get": (C.c_int, [VP, C.c_int, C.c_int, VP, VP]),
+    "fh_mlp_set": (C.c_int, [VP, C.c_int, VP, VP]),
 }
 
 _lib = None

@@ -211,6 +211,24 @@ int fh_batch_result_factor(fh_batch_result_t r, int target, int f, double *dense
 int fh_batch_result_times(fh_batch_result_t r, double *ms4);
 int fh_batch_result_free(fh_batch_result_t r);
 
+/* ---- warm-start MLP (n_in -> n_h1 -> n_h2 -> n_out, GELU between layers):
+ * BF16 weights and activations on MFMA, FP32 accumulation, FP32 master
+ * weights and gradients, Adam.  No counterpart in the reference: it proposes
+ * initial factor values for PALM4MSA.  Matrices are row-major FP32 device
+ * arrays (X: B x n_in, Y/T: B x n_out). */
+typedef struct fh_mlp_s *fh_mlp_t;
+int fh_mlp_create(int n_in, int n_h1, int n_h2, int n_out, uint64_t seed, fh_mlp_t *out);
+int fh_mlp_free(fh_mlp_t M);
+int fh_mlp_forward(fh_mlp_t M, const float *X, int B, float *Y, void *stream);
+/* gradients of the mean squared error over one batch (loss to *loss; synchronizes) */
+int fh_mlp_gradients(fh_mlp_t M, const float *X, const float *T, int B, float *loss, void *stream);
+/* one Adam step (beta1 0.9, beta2 0.999, eps 1e-8) with learning rate lr */
+int fh_mlp_train_step(fh_mlp_t M, const float *X, const float *T, int B, float lr, float *loss, void *stream);
+/* host copies of layer l (0..2): what 0 = parameters, 1 = last gradients;
+ * W is in x out row-major */
+int fh_mlp_get(fh_mlp_t M, int layer, int what, float *W, float *b);
+int fh_mlp_set(fh_mlp_t M, int layer, const float *W, const float *b);
+
 #ifdef __cplusplus
 }
 #endif
