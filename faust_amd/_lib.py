@@ -85,6 +85,7 @@ _SIGS = {
     "fh_batch_result_relerr": (C.c_int, [VP, VP]),
     "fh_batch_result_factor": (C.c_int, [VP, C.c_int, C.c_int, VP]),
     "fh_batch_result_times": (C.c_int, [VP, VP]),
+    "fh_batch_result_iters": (C.c_int, [VP, VP]),
     "fh_batch_result_free": (C.c_int, [VP]),
     "fh_mlp_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64, C.POINTER(VP)]),
     "fh_mlp_free": (C.c_int, [VP]),

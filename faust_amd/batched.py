@@ -12,20 +12,24 @@ from ._lib import check, lib
 class fh_batch_params(C.Structure):
     _fields_ = [("dtype", C.c_int), ("num_its", C.c_int), ("refine_its", C.c_int), ("prox", C.c_int),
                 ("warm_power", C.c_int), ("norm2_max_iter", C.c_int), ("norm2_threshold", C.c_double),
-                ("use_graph", C.c_int), ("autotune", C.c_int)]
+                ("use_graph", C.c_int), ("autotune", C.c_int), ("check_every", C.c_int),
+                ("target_err", C.c_double)]
 
 
 class BatchResult:
-    def __init__(self, lambdas, relerr, factors, times_ms):
+    def __init__(self, lambdas, relerr, factors, times_ms, iters_to_target):
         self.lambdas = lambdas
         self.relerr = relerr
         self.factors = factors
-        self.times_ms = times_ms  # set-up, main iterations, refinement, verification
+        # set-up, main iterations, refinement, verification, time until all targets reached target_err
+        self.times_ms = times_ms
+        self.iters_to_target = iters_to_target
 
 
 def palm4msa_batched(targets, nfacts, k=None, supports=None, init=None, lambda_init=None, dtype="f32",
                      num_its=200, refine_its=20, warm_power=True, use_graph=True, autotune=True,
-                     norm2_max_iter=100, norm2_threshold=1e-6, return_factors=True):
+                     norm2_max_iter=100, norm2_threshold=1e-6, return_factors=True, check_every=0,
+                     target_err=0.0):
     """targets: list of square FP64 arrays.  k: sp parameter per target (list or
     int) for the sp projector; supports: per target a list of nfacts 0/1 masks
     (prox supp) instead.  init: per target a list of nfacts initial factors."""
@@ -45,6 +49,7 @@ def palm4msa_batched(targets, nfacts, k=None, supports=None, init=None, lambda_i
     p.num_its, p.refine_its = int(num_its), int(refine_its)
     p.warm_power, p.use_graph, p.autotune = int(warm_power), int(use_graph), int(autotune)
     p.norm2_max_iter, p.norm2_threshold = int(norm2_max_iter), float(norm2_threshold)
+    p.check_every, p.target_err = int(check_every), float(target_err)
     if supports is not None:
         p.prox = _lib.PROX["supp"]
         masks = [np.asfortranarray(m, dtype=np.float64) for s in supports for m in s]
@@ -67,7 +72,9 @@ def palm4msa_batched(targets, nfacts, k=None, supports=None, init=None, lambda_i
     try:
         lams = np.zeros(nt)
         errs = np.zeros(nt)
-        ms = np.zeros(4)
+        ms = np.zeros(5)
+        iters = np.zeros(nt, dtype=np.int32)
+        check(L.fh_batch_result_iters(out, iters.ctypes.data_as(C.c_void_p)))
         check(L.fh_batch_result_lambda(out, lams.ctypes.data_as(C.c_void_p)))
         check(L.fh_batch_result_relerr(out, errs.ctypes.data_as(C.c_void_p)))
         check(L.fh_batch_result_times(out, ms.ctypes.data_as(C.c_void_p)))
@@ -83,4 +90,4 @@ def palm4msa_batched(targets, nfacts, k=None, supports=None, init=None, lambda_i
                 facs.append(fb)
     finally:
         L.fh_batch_result_free(out)
-    return BatchResult(lams, errs, facs, ms)
+    return BatchResult(lams, errs, facs, ms, iters)

@@ -313,53 +313,106 @@ static double run_its(Batch<T>& B, int its, bool graph, hipStream_t st) {
 }
 
 // FP64 relative error of lambda * S_0 ... S_{nf-1} against A, per target.
+// The scratch FP64 batch and job lists are built once and reused by every check.
+struct Verifier {
+  BatchSpec sp;
+  Batch<double> V;
+  std::vector<DBuf> gemm_jobs;  // one job list per factor index
+  DBuf lam_jobs, out;
+  bool ready = false;
+
+  void init(const BatchSpec& s, const std::vector<const double*>& Ad, hipStream_t st) {
+    sp = s;
+    BatchSpec vs = s;
+    vs.nf = 1;
+    batch_alloc(V, vs, false);
+    V.tile_of.assign(s.nt, 64);
+    build_tiles(V);
+    // V slots (nf = 1): 0 A, 1 running product, 2 factor cast to FP64, 3 GEMM output
+    for (int b = 0; b < s.nt; ++b) {
+      const int64_t len = (int64_t)s.n[b] * s.n[b];
+      FH_HIP(hipMemcpyAsync(V.mat(b, 0), Ad[b], sizeof(double) * len, hipMemcpyDeviceToDevice, st));
+    }
+    std::vector<BGemmJob<double>> h(s.nt);
+    for (int b = 0; b < s.nt; ++b)
+      h[b] = BGemmJob<double>{V.mat(b, 2), V.mat(b, 1), nullptr, V.mat(b, 3), s.n[b], s.n[b], s.n[b], 0, 0, 0, 0, b};
+    gemm_jobs.resize(1);
+    gemm_jobs[0].alloc(sizeof(BGemmJob<double>) * s.nt);
+    FH_HIP(hipMemcpy(gemm_jobs[0].p, h.data(), sizeof(BGemmJob<double>) * s.nt, hipMemcpyHostToDevice));
+    std::vector<BLamJob<double>> hl(s.nt);
+    for (int b = 0; b < s.nt; ++b) hl[b] = BLamJob<double>{V.mat(b, 0), V.mat(b, 1), (int64_t)s.n[b] * s.n[b], b};
+    lam_jobs.alloc(sizeof(BLamJob<double>) * s.nt);
+    FH_HIP(hipMemcpy(lam_jobs.p, hl.data(), sizeof(BLamJob<double>) * s.nt, hipMemcpyHostToDevice));
+    out.alloc(sizeof(double) * s.nt);
+    ready = true;
+  }
+
+  template <class T>
+  void run(Batch<T>& B, double* host_out, hipStream_t st) {
+    auto cast = [&](double* dst, const T* src, int64_t len) {
+      if constexpr (std::is_same<T, float>::value)
+        launch_cast_f32_f64(dst, src, len, st);
+      else
+        launch_bcopy<double>(dst, src, len, st);
+    };
+    BScal sc{V.lam.as<double>(), V.nR.as<double>(), V.nL.as<double>()};
+    for (int b = 0; b < sp.nt; ++b) cast(V.mat(b, 1), B.mat(b, B.sS(sp.nf - 1)), (int64_t)sp.n[b] * sp.n[b]);
+    for (int f = sp.nf - 2; f >= 0; --f) {
+      for (int b = 0; b < sp.nt; ++b) cast(V.mat(b, 2), B.mat(b, B.sS(f)), (int64_t)sp.n[b] * sp.n[b]);
+      launch_bgemm<double>(gemm_jobs[0].as<BGemmJob<double>>(), V.tiles64.as<int3>(), V.nt64, 64, sc, st);
+      for (int b = 0; b < sp.nt; ++b)
+        launch_bcopy<double>(V.mat(b, 1), V.mat(b, 3), (int64_t)sp.n[b] * sp.n[b], st);
+    }
+    launch_brelerr<double>(lam_jobs.as<BLamJob<double>>(), sp.nt, B.lam.template as<double>(), out.as<double>(), st);
+    FH_HIP(hipMemcpyAsync(host_out, out.p, sizeof(double) * sp.nt, hipMemcpyDeviceToHost, st));
+    FH_HIP(hipStreamSynchronize(st));
+  }
+};
+
+// Runs `its` iterations, in chunks of check_every with an FP64 check after
+// each chunk when check_every > 0.  first_hit[b] records the iteration count
+// (it0 + iterations run) at which target b first reached target_err.
+// Returns true when every target has reached it.
+struct Progress {
+  std::vector<int> first_hit;
+  int done = 0;
+  double ms_all = -1.0;
+};
+
+template <class T>
+static bool run_checked(Batch<T>& B, int its, int it0, bool graph, int check_every, double target_err,
+                        Verifier& ver, Progress& pr, std::chrono::steady_clock::time_point t_start,
+                        double& ms_iter, hipStream_t st) {
+  if (check_every <= 0) {
+    ms_iter += run_its(B, its, graph, st);
+    return false;
+  }
+  std::vector<double> err(B.sp.nt);
+  int ran = 0;
+  while (ran < its) {
+    const int chunk = std::min(check_every, its - ran);
+    ms_iter += run_its(B, chunk, graph, st);
+    ran += chunk;
+    ver.run(B, err.data(), st);
+    for (int b = 0; b < B.sp.nt; ++b)
+      if (pr.first_hit[b] < 0 && err[b] <= target_err) {
+        pr.first_hit[b] = it0 + ran;
+        pr.done++;
+      }
+    if (pr.done == B.sp.nt) {
+      pr.ms_all = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+      return true;
+    }
+  }
+  return false;
+}
+
 template <class T>
 static void verify_fp64(const BatchSpec& sp, const std::vector<const double*>& Ad, Batch<T>& B, double* out,
                         hipStream_t st) {
-  // product in FP64 through a scratch FP64 batch: P <- S_{nf-1}; P <- S_f P
-  Batch<double> V;
-  BatchSpec vs = sp;
-  vs.nf = 1;
-  batch_alloc(V, vs, false);
-  V.tile_of.assign(sp.nt, 64);
-  build_tiles(V);
-  // V slots (nf = 1): 0 A, 1 S0 (current product), 2..: scratch (DR = slot 2, TMP = 3, TR = 4, SM = 5)
-  for (int b = 0; b < sp.nt; ++b) {
-    const int64_t len = (int64_t)sp.n[b] * sp.n[b];
-    FH_HIP(hipMemcpyAsync(V.mat(b, 0), Ad[b], sizeof(double) * len, hipMemcpyDeviceToDevice, st));
-  }
-  auto cast = [&](double* dst, const T* src, int64_t len) {
-    if constexpr (std::is_same<T, float>::value)
-      launch_cast_f32_f64(dst, src, len, st);
-    else
-      launch_bcopy<double>(dst, src, len, st);
-  };
-  for (int b = 0; b < sp.nt; ++b) {
-    const int64_t len = (int64_t)sp.n[b] * sp.n[b];
-    cast(V.mat(b, 1), B.mat(b, B.sS(sp.nf - 1)), len);
-  }
-  BScal sc{V.lam.as<double>(), V.nR.as<double>(), V.nL.as<double>()};
-  for (int f = sp.nf - 2; f >= 0; --f) {
-    std::vector<BGemmJob<double>> h(sp.nt);
-    for (int b = 0; b < sp.nt; ++b) {
-      const int64_t len = (int64_t)sp.n[b] * sp.n[b];
-      cast(V.mat(b, 2), B.mat(b, B.sS(f)), len);
-      h[b] = BGemmJob<double>{V.mat(b, 2), V.mat(b, 1), nullptr, V.mat(b, 3), sp.n[b], sp.n[b], sp.n[b], 0, 0, 0, 0, b};
-    }
-    DBuf dj(sizeof(BGemmJob<double>) * sp.nt);
-    FH_HIP(hipMemcpyAsync(dj.p, h.data(), sizeof(BGemmJob<double>) * sp.nt, hipMemcpyHostToDevice, st));
-    launch_bgemm<double>(dj.as<BGemmJob<double>>(), V.tiles64.as<int3>(), V.nt64, 64, sc, st);
-    for (int b = 0; b < sp.nt; ++b)
-      launch_bcopy<double>(V.mat(b, 1), V.mat(b, 3), (int64_t)sp.n[b] * sp.n[b], st);
-    FH_HIP(hipStreamSynchronize(st));
-  }
-  std::vector<BLamJob<double>> h(sp.nt);
-  for (int b = 0; b < sp.nt; ++b) h[b] = BLamJob<double>{V.mat(b, 0), V.mat(b, 1), (int64_t)sp.n[b] * sp.n[b], b};
-  DBuf dj(sizeof(BLamJob<double>) * sp.nt), dout(sizeof(double) * sp.nt);
-  FH_HIP(hipMemcpyAsync(dj.p, h.data(), sizeof(BLamJob<double>) * sp.nt, hipMemcpyHostToDevice, st));
-  launch_brelerr<double>(dj.as<BLamJob<double>>(), sp.nt, B.lam.template as<double>(), dout.as<double>(), st);
-  FH_HIP(hipMemcpyAsync(out, dout.p, sizeof(double) * sp.nt, hipMemcpyDeviceToHost, st));
-  FH_HIP(hipStreamSynchronize(st));
+  Verifier v;
+  v.init(sp, Ad, st);
+  v.run(B, out, st);
 }
 
 }  // namespace fh
@@ -369,7 +422,8 @@ struct fh_batch_result_s {
   std::vector<int> n;
   std::vector<double> lambda, relerr;
   std::vector<std::vector<double>> fac;  // [target*nf + f] dense col-major
-  double ms[4] = {0, 0, 0, 0};           // setup, main iterations, refinement, verification
+  std::vector<int> iters_to_target;      // -1: target error not reached (or not checked)
+  double ms[5] = {0, 0, 0, 0, -1};       // setup, main iterations, refinement, verification, time to all targets
 };
 
 using namespace fh;
@@ -421,6 +475,9 @@ extern "C" int fh_palm4msa_batched(int ntargets, int nfacts, const int* n, const
   FH_CHECK(p->prox == FH_PROX_SUPP ? supp != nullptr : k != nullptr, "batched PALM: missing k or supports");
   hipStream_t st;
   FH_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  // pool blocks freed by earlier calls are reused in the order of the caller's
+  // stream; this call runs on its own stream, so let that earlier work finish
+  FH_HIP(hipDeviceSynchronize());
   auto r = std::make_unique<fh_batch_result_s>();
   auto t0 = std::chrono::steady_clock::now();
   BatchSpec sp{ntargets, nfacts, std::vector<int>(n, n + ntargets)};
@@ -459,10 +516,16 @@ extern "C" int fh_palm4msa_batched(int ntargets, int nfacts, const int* n, const
       build_tiles(B);
     }
     r->ms[0] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    r->ms[1] = run_its(B, p->num_its, p->use_graph != 0, st);
+    Verifier ver;
+    Progress pr;
+    pr.first_hit.assign(ntargets, -1);
+    if (p->check_every > 0) ver.init(sp, Adp, st);
+    const auto t_start = std::chrono::steady_clock::now();
+    const bool all = run_checked(B, p->num_its, 0, p->use_graph != 0, p->check_every, p->target_err, ver, pr,
+                                 t_start, r->ms[1], st);
     // frozen-support FP64 refinement
     Batch<double> D;
-    if (p->refine_its > 0) {
+    if (p->refine_its > 0 && !all) {
       auto t1 = std::chrono::steady_clock::now();
       batch_alloc(D, sp, true);
       D.prox_mode = 1;
@@ -484,7 +547,8 @@ extern "C" int fh_palm4msa_batched(int ntargets, int nfacts, const int* n, const
       build_schedule(D, {});
       FH_HIP(hipStreamSynchronize(st));
       const double setup_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t1).count();
-      r->ms[2] = run_its(D, p->refine_its, p->use_graph != 0, st);
+      run_checked(D, p->refine_its, p->num_its, p->use_graph != 0, p->check_every, p->target_err, ver, pr, t_start,
+                  r->ms[2], st);
       r->ms[0] += setup_ms;
       auto t2 = std::chrono::steady_clock::now();
       r->relerr.resize(ntargets);
@@ -514,6 +578,8 @@ extern "C" int fh_palm4msa_batched(int ntargets, int nfacts, const int* n, const
           r->fac[(size_t)b * nfacts + f].assign(h.begin(), h.end());
         }
     }
+    r->iters_to_target = pr.first_hit;
+    r->ms[4] = pr.ms_all;
   } else {
     Batch<double> B;
     batch_alloc(B, sp, p->prox == FH_PROX_SUPP);
@@ -535,7 +601,15 @@ extern "C" int fh_palm4msa_batched(int ntargets, int nfacts, const int* n, const
       build_tiles(B);
     }
     r->ms[0] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    r->ms[1] = run_its(B, p->num_its + p->refine_its, p->use_graph != 0, st);
+    Verifier ver;
+    Progress pr;
+    pr.first_hit.assign(ntargets, -1);
+    if (p->check_every > 0) ver.init(sp, Adp, st);
+    const auto t_start = std::chrono::steady_clock::now();
+    run_checked(B, p->num_its + p->refine_its, 0, p->use_graph != 0, p->check_every, p->target_err, ver, pr, t_start,
+                r->ms[1], st);
+    r->iters_to_target = pr.first_hit;
+    r->ms[4] = pr.ms_all;
     auto t2 = std::chrono::steady_clock::now();
     r->relerr.resize(ntargets);
     verify_fp64(sp, Adp, B, r->relerr.data(), st);
@@ -550,6 +624,7 @@ extern "C" int fh_palm4msa_batched(int ntargets, int nfacts, const int* n, const
                          hipMemcpyDeviceToHost));
       }
   }
+  if (r->iters_to_target.empty()) r->iters_to_target.assign(ntargets, -1);
   r->nt = ntargets;
   r->nf = nfacts;
   r->n = sp.n;
@@ -581,10 +656,17 @@ extern "C" int fh_batch_result_factor(fh_batch_result_t r, int target, int f, do
   FH_API_END
 }
 
-extern "C" int fh_batch_result_times(fh_batch_result_t r, double* ms4) {
+extern "C" int fh_batch_result_times(fh_batch_result_t r, double* ms5) {
   FH_API_BEGIN
-  FH_CHECK(r && ms4, "null argument");
-  for (int i = 0; i < 4; ++i) ms4[i] = r->ms[i];
+  FH_CHECK(r && ms5, "null argument");
+  for (int i = 0; i < 5; ++i) ms5[i] = r->ms[i];
+  FH_API_END
+}
+
+extern "C" int fh_batch_result_iters(fh_batch_result_t r, int* iters) {
+  FH_API_BEGIN
+  FH_CHECK(r && iters, "null argument");
+  std::copy(r->iters_to_target.begin(), r->iters_to_target.end(), iters);
   FH_API_END
 }
 

@@ -1,16 +1,16 @@
 // k_mixed.hip -- FP32 / BF16 Faust x dense: BSR on MFMA and FP32 CSR SpMM.
 //
-// BSR (32x32 blocks, bm = bn = 32): Y = op(B) X with X, Y BF16 column-major,
-// FP32 accumulation in v_mfma_f32_32x32x16_bf16.  Each wave owns a 32-row x
-// 32-column output tile (one block-row, 32 columns) and, for every block of
-// its block-row, issues two MFMAs (K = 16 + 16).  Operand lanes:
+// BSR (32x32 blocks, bm = bn = 32): Y = op(B) X with X, Y BF16, FP32
+// accumulation in v_mfma_f32_32x32x16_bf16.  Each wave owns one block-row x
+// 64 columns (two 32x32 accumulators) and, for every block of its block-row,
+// issues four MFMAs (K = 16 + 16, two column tiles).  Operand lanes:
 //   A: lane l holds block row (l % 32), 8 consecutive k at 8*(l / 32);
 //   B: lane l holds X column (l % 32), the same 8 consecutive k;
-// so with blocks stored row-major and X column-major every operand is one
-// 16-byte load.  FP32 block values are stored as a BF16 (hi, lo) pair and
+// so with blocks stored row-major and X column-major (or tiled, below) every
+// operand is one 16-byte load.  FP32 block values are stored as a BF16 (hi, lo) pair and
 // cost one extra MFMA per step (error ~2^-16 relative per value, below the
 // BF16 rounding of the activations).
-// Work mapping: workgroup = 2 block-rows x 64 columns; consecutive workgroup
+// Work mapping: workgroup = 4 block-rows x 64 columns; consecutive workgroup
 // ids walk the 64-column panels first, so with 512 columns the 8 panels land
 // on the 8 XCDs and each XCD's L2 holds its 1 MiB X panel for all the
 // gathers of its block-rows.
@@ -53,25 +53,37 @@ static inline float host_bf2f(u16 h) {
 }
 
 // ---------------------------------------------------------------- BSR MFMA
-template <bool SPLIT>
+// Activation layouts: column-major (ld), or "tiled": for every 32-row block
+// rb a [p][32] array, element (r, j) at ((r/32)*p + j)*32 + r%32, so that the
+// 32 rows of one column inside a block are one 64-byte segment and the
+// B-operand of 32 consecutive columns is one contiguous 2 KiB tile.  Chains
+// keep their intermediates tiled; only the first factor reads and the last
+// factor writes column-major.
+// Wave work: one block-row x 64 columns (two 32x32 accumulators sharing the
+// A operand); workgroup: 4 block-rows x 64 columns.
+template <bool SPLIT, bool IN_TILED, bool OUT_TILED>
 __global__ __launch_bounds__(256) void k_bsr_mfma(int nbr, int p, const int* __restrict__ brp,
                                                   const int* __restrict__ bci, const u16* __restrict__ bdat,
                                                   const u16* __restrict__ X, int ldx, u16* __restrict__ Y, int ldy,
                                                   float alpha, float beta, int npanels) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int panel = blockIdx.x % npanels, pair = blockIdx.x / npanels;
-  const int br = pair * 2 + (wave >> 1);
+  const int panel = blockIdx.x % npanels, quad = blockIdx.x / npanels;
+  const int br = quad * 4 + wave;
   if (br >= nbr) return;
-  const int j = panel * 64 + (wave & 1) * 32 + (lane & 31);
   const int kh = lane >> 5;  // which 8-wide k group of the 16
-  const bool jok = j < p;
-  const u16* xcol = X + (size_t)(jok ? j : 0) * ldx;
-  constexpr int BLK = 1024;                      // entries per 32x32 block
+  int jc[2];
+  bool jok[2];
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    jc[c] = panel * 64 + c * 32 + (lane & 31);
+    jok[c] = jc[c] < p;
+  }
+  constexpr int BLK = 1024;  // entries per 32x32 block
   constexpr int PLANES = SPLIT ? 2 : 1;
-  f32x16 acc = {};
+  f32x16 acc[2] = {{}, {}};
   const int b0 = brp[br], b1 = brp[br + 1];
   for (int b = b0; b < b1; b += 4) {
-    bf16x8 a[4][2][PLANES], x[4][2];
+    bf16x8 a[4][2][PLANES], x[4][2][2];
     const int nb = min(4, b1 - b);
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
@@ -82,10 +94,13 @@ __global__ __launch_bounds__(256) void k_bsr_mfma(int nbr, int p, const int* __r
         for (int h = 0; h < 2; ++h) {
 #pragma unroll
           for (int s = 0; s < PLANES; ++s) a[q][h][s] = *reinterpret_cast<const bf16x8*>(blk + s * BLK + h * 16);
-          if (jok)
-            x[q][h] = *reinterpret_cast<const bf16x8*>(xcol + bc * 32 + h * 16 + kh * 8);
-          else
-            x[q][h] = bf16x8{};
+#pragma unroll
+          for (int c = 0; c < 2; ++c) {
+            const int j = jok[c] ? jc[c] : 0;
+            const u16* src = IN_TILED ? X + ((size_t)bc * p + j) * 32 + h * 16 + kh * 8
+                                      : X + (size_t)j * ldx + bc * 32 + h * 16 + kh * 8;
+            x[q][h][c] = jok[c] ? *reinterpret_cast<const bf16x8*>(src) : bf16x8{};
+          }
         }
       }
     }
@@ -93,52 +108,70 @@ __global__ __launch_bounds__(256) void k_bsr_mfma(int nbr, int p, const int* __r
     for (int q = 0; q < 4; ++q) {
       if (q < nb) {
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
-          acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[q][h][0], x[q][h], acc, 0, 0, 0);
-          if (SPLIT) acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[q][h][PLANES - 1], x[q][h], acc, 0, 0, 0);
-        }
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+          for (int c = 0; c < 2; ++c) {
+            acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[q][h][0], x[q][h][c], acc[c], 0, 0, 0);
+            if (SPLIT) acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[q][h][PLANES - 1], x[q][h][c], acc[c], 0, 0, 0);
+          }
       }
     }
   }
-  if (!jok) return;
-  u16* ycol = Y + (size_t)j * ldy + br * 32;
 #pragma unroll
-  for (int g = 0; g < 4; ++g) {
-    const int r0 = 8 * g + 4 * kh;
-    u16 o[4];
+  for (int c = 0; c < 2; ++c) {
+    if (!jok[c]) continue;
+    const int j = jc[c];
+    u16* ycol = OUT_TILED ? Y + ((size_t)br * p + j) * 32 : Y + (size_t)j * ldy + br * 32;
 #pragma unroll
-    for (int t = 0; t < 4; ++t) {
-      float v = alpha * acc[4 * g + t];
-      if (beta != 0.0f) v = fmaf(beta, bf2f(ycol[r0 + t]), v);
-      o[t] = f2bf(v);
+    for (int g = 0; g < 4; ++g) {
+      const int r0 = 8 * g + 4 * kh;
+      u16 o[4];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        float v = alpha * acc[c][4 * g + t];
+        if (beta != 0.0f) v = fmaf(beta, bf2f(ycol[r0 + t]), v);
+        o[t] = f2bf(v);
+      }
+      uint2 w;
+      w.x = (unsigned)o[0] | ((unsigned)o[1] << 16);
+      w.y = (unsigned)o[2] | ((unsigned)o[3] << 16);
+      *reinterpret_cast<uint2*>(ycol + r0) = w;
     }
-    uint2 w;
-    w.x = (unsigned)o[0] | ((unsigned)o[1] << 16);
-    w.y = (unsigned)o[2] | ((unsigned)o[3] << 16);
-    *reinterpret_cast<uint2*>(ycol + r0) = w;
   }
 }
 
+// in_tiled / out_tiled select the activation layouts (see above); ldx/ldy are
+// only used for column-major operands.
 static void bsr_mul(const DMat& B, char op, const void* X, int ldx, int p, void* Y, int ldy, float alpha,
-                    float beta, hipStream_t st) {
+                    float beta, hipStream_t st, bool in_tiled = false, bool out_tiled = false) {
   const bool tr = op != 'N';
   const int rows = tr ? B.n : B.m;
   const int nbr = rows / 32;
-  FH_CHECK(ldx % 8 == 0 && ldy % 4 == 0, "BSR MFMA path: ldx must be a multiple of 8 and ldy of 4");
+  if (!in_tiled) FH_CHECK(ldx % 8 == 0, "BSR MFMA path: ldx must be a multiple of 8");
+  if (!out_tiled) FH_CHECK(ldy % 4 == 0, "BSR MFMA path: ldy must be a multiple of 4");
   FH_CHECK(((uintptr_t)X & 15) == 0 && ((uintptr_t)Y & 7) == 0, "BSR MFMA path: X must be 16-byte aligned");
   if (nbr == 0 || p == 0) return;
   const int npanels = (p + 63) / 64;
-  const int pairs = (nbr + 1) / 2;
+  const int quads = (nbr + 3) / 4;
   const int* rp = tr ? B.tbrp.as<int>() : B.brp.as<int>();
   const int* ci = tr ? B.tbci.as<int>() : B.bci.as<int>();
   const u16* dat = tr ? B.tbdat.as<u16>() : B.bdat.as<u16>();
-  dim3 grid((unsigned)(npanels * pairs));
-  if (B.split)
-    hipLaunchKernelGGL(k_bsr_mfma<true>, grid, dim3(256), 0, st, nbr, p, rp, ci, dat, (const u16*)X, ldx, (u16*)Y,
-                       ldy, alpha, beta, npanels);
-  else
-    hipLaunchKernelGGL(k_bsr_mfma<false>, grid, dim3(256), 0, st, nbr, p, rp, ci, dat, (const u16*)X, ldx, (u16*)Y,
-                       ldy, alpha, beta, npanels);
+  dim3 grid((unsigned)(npanels * quads));
+#define FH_BSR_LAUNCH(S, I, O)                                                                                \
+  hipLaunchKernelGGL((k_bsr_mfma<S, I, O>), grid, dim3(256), 0, st, nbr, p, rp, ci, dat, (const u16*)X, ldx, \
+                     (u16*)Y, ldy, alpha, beta, npanels)
+  const int sel = (B.split ? 4 : 0) | (in_tiled ? 2 : 0) | (out_tiled ? 1 : 0);
+  switch (sel) {
+    case 0: FH_BSR_LAUNCH(false, false, false); break;
+    case 1: FH_BSR_LAUNCH(false, false, true); break;
+    case 2: FH_BSR_LAUNCH(false, true, false); break;
+    case 3: FH_BSR_LAUNCH(false, true, true); break;
+    case 4: FH_BSR_LAUNCH(true, false, false); break;
+    case 5: FH_BSR_LAUNCH(true, false, true); break;
+    case 6: FH_BSR_LAUNCH(true, true, false); break;
+    default: FH_BSR_LAUNCH(true, true, true); break;
+  }
+#undef FH_BSR_LAUNCH
   FH_LAUNCH_CHECK();
 }
 
@@ -158,13 +191,28 @@ __global__ __launch_bounds__(256) void k_csr_rm_f32(int rows, int p, const int* 
   float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
   if (r < rows && c0 < p) {
     const int e0 = rp[r], e1 = rp[r + 1];
-    for (int e = e0; e < e1; ++e) {
-      const float w = v[e];
-      const float4 x = *reinterpret_cast<const float4*>(X + (size_t)ci[e] * ldx + c0);
-      acc.x = fmaf(w, x.x, acc.x);
-      acc.y = fmaf(w, x.y, acc.y);
-      acc.z = fmaf(w, x.z, acc.z);
-      acc.w = fmaf(w, x.w, acc.w);
+    // batches of up to 16 nonzeros: all index/value loads, then all row
+    // gathers, then the FMAs, so 16 gathers are in flight per thread
+    for (int base = e0; base < e1; base += 16) {
+      const int nb = min(16, e1 - base);
+      int cc[16];
+      float ww[16];
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        cc[q] = q < nb ? ci[base + q] : 0;
+        ww[q] = q < nb ? v[base + q] : 0.0f;
+      }
+      float4 xx[16];
+#pragma unroll
+      for (int q = 0; q < 16; ++q)
+        xx[q] = q < nb ? *reinterpret_cast<const float4*>(X + (size_t)cc[q] * ldx + c0) : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        acc.x = fmaf(ww[q], xx[q].x, acc.x);
+        acc.y = fmaf(ww[q], xx[q].y, acc.y);
+        acc.z = fmaf(ww[q], xx[q].z, acc.z);
+        acc.w = fmaf(ww[q], xx[q].w, acc.w);
+      }
     }
   }
   if (!OUT_COL) {
@@ -344,7 +392,8 @@ void faust_multiply_mixed(const std::vector<DMatP>& f, char op, const void* X, i
       void* out = (i == J - 1) ? Y : ws[i & 1].p;
       const size_t slot = (op == 'N') ? (size_t)(J - 1 - i) : (size_t)i;
       if (tm) tm->begin(slot, st);
-      bsr_mul(S, op, in, in_rows, p, out, out_rows, 1.0f, 0.0f, st);
+      // intermediates stay in the tiled layout; X and Y are column-major
+      bsr_mul(S, op, in, in_rows, p, out, out_rows, 1.0f, 0.0f, st, i > 0, i < J - 1);
       if (tm) tm->end(slot, st);
       in = out;
       in_rows = out_rows;

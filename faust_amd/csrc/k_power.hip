@@ -27,6 +27,7 @@ struct PwOp {
   const int* ci;
   const double* v;  // sparse values or dense data (ld = rows of A)
   int lda;
+  int64_t nnz;  // sparse only
 };
 
 // canonical reduction of dot(x, y) over n values held in LDS; every thread
@@ -129,6 +130,158 @@ __global__ __launch_bounds__(PW_THREADS) void k_power_iteration(const PwOp* __re
   }
 }
 
+// ---------------------------------------------------------------------------
+// Wide form: one launch per operator per round, for chains whose operators are
+// too large for a single CU to stream every round.  Each row is still one fma
+// chain on one lane (k ascending), so the result is bit-identical to the
+// persistent kernel; the row data is staged through LDS by all 256 threads of
+// the workgroup in contiguous chunks, and the first R lanes (one per row) run
+// the chains.  A device state carries lambda, the round count and a done flag: the
+// host enqueues rounds ahead and launches after convergence return at once.
+struct PwState {
+  double lambda, lambda_old;
+  int it, status, done, pad;
+};
+
+constexpr int PW_WIDE_THREADS = 256;
+constexpr int PW_CH = 2048;  // entries per LDS chunk
+
+// sparse rows (row pointers rp, indices ci, values v): rows r0..r0+R-1 per workgroup
+__global__ __launch_bounds__(PW_WIDE_THREADS) void k_pw_csr(const int* __restrict__ rp, const int* __restrict__ ci,
+                                                            const double* __restrict__ v, int rows, int cols, int R,
+                                                            const double* __restrict__ cur, double* __restrict__ nxt,
+                                                            const PwState* __restrict__ s) {
+  __shared__ double xs[PW_VMAX];
+  __shared__ double sv[PW_CH];
+  __shared__ int sci[PW_CH];
+  if (s->done) return;
+  const int t = threadIdx.x;
+  const int r0 = blockIdx.x * R, r1 = min(rows, r0 + R), nr = r1 - r0;
+  for (int i = t; i < cols; i += PW_WIDE_THREADS) xs[i] = cur[i];
+  const int p0 = rp[r0], p1 = rp[r1];
+  int p = 0, pe = 0;
+  if (t < nr) {
+    p = rp[r0 + t];
+    pe = rp[r0 + t + 1];
+  }
+  double acc = 0.0;
+  for (int c0 = p0; c0 < p1; c0 += PW_CH) {
+    const int c1 = min(p1, c0 + PW_CH);
+    __syncthreads();  // previous chunk consumed; xs staged before the first
+    for (int q = c0 + t; q < c1; q += PW_WIDE_THREADS) {
+      sv[q - c0] = v[q];
+      sci[q - c0] = ci[q];
+    }
+    __syncthreads();
+    if (t < nr) {
+      const int e = min(pe, c1);
+      for (; p < e; ++p) acc = fma(sv[p - c0], xs[sci[p - c0]], acc);
+    }
+  }
+  if (t < nr) nxt[r0 + t] = acc;
+}
+
+// dense op(A) rows (A column-major, leading dimension lda); trans: op(A) = A^T
+__global__ __launch_bounds__(PW_WIDE_THREADS) void k_pw_dense(const double* __restrict__ A, int lda, int trans,
+                                                              int rows, int cols, int R,
+                                                              const double* __restrict__ cur,
+                                                              double* __restrict__ nxt,
+                                                              const PwState* __restrict__ s) {
+  __shared__ double xs[PW_VMAX];
+  __shared__ double tile[PW_CH];  // R rows x C columns, row stride C + 1 (odd: no bank conflicts)
+  if (s->done) return;
+  const int t = threadIdx.x;
+  const int r0 = blockIdx.x * R, nr = min(rows, r0 + R) - r0;
+  const int C = PW_CH / R - 2;
+  for (int i = t; i < cols; i += PW_WIDE_THREADS) xs[i] = cur[i];
+  double acc = 0.0;
+  for (int c0 = 0; c0 < cols; c0 += C) {
+    const int nc = min(C, cols - c0);
+    __syncthreads();
+    if (trans) {  // row i of A^T = column r0+i of A: contiguous in c
+      for (int q = t; q < nr * nc; q += PW_WIDE_THREADS) {
+        const int i = q / nc, c = q - i * nc;
+        tile[i * (C + 1) + c] = A[(size_t)(r0 + i) * lda + c0 + c];
+      }
+    } else {  // row i of A: element (i, c) at (c0+c)*lda + r0 + i
+      for (int q = t; q < nr * nc; q += PW_WIDE_THREADS) {
+        const int c = q / nr, i = q - c * nr;
+        tile[i * (C + 1) + c] = A[(size_t)(c0 + c) * lda + r0 + i];
+      }
+    }
+    __syncthreads();
+    if (t < nr) {
+      const double* row = tile + t * (C + 1);
+      for (int c = 0; c < nc; ++c) acc = fma(row[c], xs[c0 + c], acc);
+    }
+  }
+  if (t < nr) nxt[r0 + t] = acc;
+}
+
+// One round boundary.  mode 1: finish the previous round (xa = chain output):
+// zero test and lambda = <xn, xa>.  Then the loop test of the persistent
+// kernel, and if it continues: it++, normalise xa into xn.
+__global__ __launch_bounds__(PW_THREADS) void k_pw_step(int n, double* __restrict__ xa, double* __restrict__ xn,
+                                                         PwState* __restrict__ s, int maxit, double thr, int mode) {
+  __shared__ double red[RED_LANES];
+  __shared__ double tot;
+  const int t = threadIdx.x;
+  const PwState st = *s;
+  __syncthreads();
+  if (st.done) return;
+  double lambda = st.lambda, lambda_old = st.lambda_old;
+  int it = st.it;
+  if (mode == 1) {
+    int nz = 0;
+    for (int i = t; i < n; i += PW_THREADS) nz |= (xa[i] != 0.0);
+    if (!__syncthreads_or(nz)) {
+      if (t == 0) {
+        s->status = 2;
+        s->done = 1;
+      }
+      return;
+    }
+    lambda = pw_dot(xn, xa, n, red, &tot);
+  }
+  const double athr = fabs(thr);
+  if (!((fabs(lambda_old - lambda) > athr || fabs(lambda) <= athr) && it < maxit)) {
+    if (t == 0) {
+      s->lambda = lambda;
+      s->done = 1;
+    }
+    return;
+  }
+  it++;
+  lambda_old = lambda;
+  const double nrm = sqrt(pw_dot(xa, xa, n, red, &tot));
+  if (nrm == 0.0) {
+    if (t == 0) {
+      s->it = it;
+      s->status = 1;
+      s->done = 1;
+    }
+    return;
+  }
+  const double inv = 1 / nrm;
+  for (int i = t; i < n; i += PW_THREADS) xn[i] = xa[i] * inv;
+  if (t == 0) {
+    s->lambda = lambda;
+    s->lambda_old = lambda_old;
+    s->it = it;
+  }
+}
+
+__global__ void k_pw_init(PwState* __restrict__ s) {
+  if (threadIdx.x == 0) {
+    s->lambda = 0.0;
+    s->lambda_old = 1.0;
+    s->it = 0;
+    s->status = 0;
+    s->done = 0;
+    s->pad = 0;
+  }
+}
+
 const std::vector<double>& power_x0(int n) {
   static std::mutex mu;
   static std::map<int, std::vector<double>> cache;
@@ -140,6 +293,80 @@ const std::vector<double>& power_x0(int n) {
   std::vector<double> x(n);
   for (int i = 0; i < n; ++i) x[i] = -1.0 + (2.0 * (double)std::rand()) / (double)RAND_MAX;
   return cache.emplace(n, std::move(x)).first->second;
+}
+
+// Per-round time estimates (seconds): the persistent kernel streams every
+// operator through one CU (~100 GB/s); the wide form pays a launch per
+// operator but spreads the rows over the chip, leaving each row's fma chain
+// (about 4 ns per term) as its floor.  FH_POWER_WIDE=0/1 forces a form.
+static bool use_wide_form(const std::vector<PwOp>& h) {
+  if (const char* e = getenv("FH_POWER_WIDE")) {
+    if (e[0] == '0') return false;
+    if (e[0] == '1') return true;
+  }
+  double single = 3e-6, wide = 4e-6;
+  for (const PwOp& o : h) {
+    const double entries = o.kind == 1 ? (double)o.nnz : (double)o.rows * o.cols;
+    const double bytes = entries * (o.kind == 1 ? 12.0 : 8.0);
+    single += bytes / 100e9 + 0.4e-6;
+    wide += 3e-6 + (entries / std::max(o.rows, 1)) * 4e-9 + bytes / 1.5e12;
+  }
+  return wide < single;
+}
+
+static double power_iteration_wide(const std::vector<PwOp>& h, int n, const std::vector<double>& x0, int maxit,
+                                   double thr, int* flag, hipStream_t st) {
+  int vmax = n;
+  for (const PwOp& o : h) vmax = std::max(vmax, std::max(o.rows, o.cols));
+  DBuf xa(sizeof(double) * n), xn(sizeof(double) * n), b0(sizeof(double) * vmax), b1(sizeof(double) * vmax),
+      ds(sizeof(PwState));
+  PwState* dst = ds.as<PwState>();
+  FH_HIP(hipMemcpyAsync(xa.p, x0.data(), sizeof(double) * n, hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(k_pw_init, dim3(1), dim3(64), 0, st, dst);
+  FH_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_pw_step, dim3(1), dim3(PW_THREADS), 0, st, n, xa.as<double>(), xn.as<double>(), dst, maxit,
+                     thr, 0);
+  FH_LAUNCH_CHECK();
+  const int nops = (int)h.size();
+  PwState hs{};
+  int next_check = 4;
+  for (int round = 1; round <= maxit; ++round) {
+    const double* cur = xn.as<double>();
+    for (int s = nops - 1; s >= 0; --s) {
+      const PwOp& o = h[s];
+      double* out = (s == 0) ? xa.as<double>() : (((nops - 1 - s) & 1) ? b1.as<double>() : b0.as<double>());
+      int R = 1;
+      while (R < 8 && (o.rows + 2 * R - 1) / (2 * R) >= 256) R *= 2;
+      const dim3 grid((o.rows + R - 1) / R);
+      if (o.kind == 1)
+        hipLaunchKernelGGL(k_pw_csr, grid, dim3(PW_WIDE_THREADS), 0, st, o.rp, o.ci, o.v, o.rows, o.cols, R, cur,
+                           out, (const PwState*)dst);
+      else
+        hipLaunchKernelGGL(k_pw_dense, grid, dim3(PW_WIDE_THREADS), 0, st, o.v, o.lda, o.trans, o.rows, o.cols, R,
+                           cur, out, (const PwState*)dst);
+      FH_LAUNCH_CHECK();
+      cur = out;
+    }
+    hipLaunchKernelGGL(k_pw_step, dim3(1), dim3(PW_THREADS), 0, st, n, xa.as<double>(), xn.as<double>(), dst,
+                       maxit, thr, 1);
+    FH_LAUNCH_CHECK();
+    if (round == next_check || round == maxit) {
+      FH_HIP(hipMemcpyAsync(&hs, dst, sizeof(PwState), hipMemcpyDeviceToHost, st));
+      FH_HIP(hipStreamSynchronize(st));
+      if (hs.done) break;
+      next_check += 8;
+    }
+  }
+  if (!hs.done) {  // every round ran: one more step evaluates the loop test
+    hipLaunchKernelGGL(k_pw_step, dim3(1), dim3(PW_THREADS), 0, st, n, xa.as<double>(), xn.as<double>(), dst,
+                       maxit, thr, 1);
+    FH_LAUNCH_CHECK();
+    FH_HIP(hipMemcpyAsync(&hs, dst, sizeof(PwState), hipMemcpyDeviceToHost, st));
+    FH_HIP(hipStreamSynchronize(st));
+  }
+  if (hs.status == 1) throw Error(FH_ERR_NUMERIC, "Can't normalize a zero-norm vector.");
+  if (flag) *flag = (hs.it < maxit) ? hs.it : -1;
+  return hs.status == 2 ? 0.0 : hs.lambda;
 }
 
 double power_iteration_f64(const std::vector<ChainOp>& ops, int n, int maxit, double thr, int* flag,
@@ -157,6 +384,7 @@ double power_iteration_f64(const std::vector<ChainOp>& ops, int n, int maxit, do
     if (M.kind == K_CSR) {
       o.kind = 1;
       o.trans = 0;
+      o.nnz = M.nnz;
       if (tr) {
         FH_CHECK(M.has_csc, "power_iteration: CSC twin missing");
         o.rp = M.cp.as<int>();
@@ -174,9 +402,11 @@ double power_iteration_f64(const std::vector<ChainOp>& ops, int n, int maxit, do
       o.rp = o.ci = nullptr;
       o.v = M.d.as<double>();
       o.lda = M.m;
+      o.nnz = 0;
     }
   }
   const std::vector<double>& x0 = power_x0(n);
+  if (use_wide_form(h)) return power_iteration_wide(h, n, x0, maxit, thr, flag, st);
   DBuf dops(sizeof(PwOp) * h.size()), dx(sizeof(double) * n), dout(sizeof(double) * 3);
   FH_HIP(hipMemcpyAsync(dops.p, h.data(), sizeof(PwOp) * h.size(), hipMemcpyHostToDevice, st));
   FH_HIP(hipMemcpyAsync(dx.p, x0.data(), sizeof(double) * n, hipMemcpyHostToDevice, st));

@@ -1,11 +1,15 @@
 // k_prox.hip -- sp / spcol / splin / splincol / supp projectors (FP64).
 //
 // The selected support must equal the reference's std::partial_sort result
-// bit for bit, ties included, so every line (row or column) runs the exact
-// heap selection of heap_select.hpp on one lane, with the line staged in LDS
-// by its wave.  The selection is followed by a canonical Frobenius
-// normalisation (faust_prox.hpp:72-200; MatDense::normalize,
-// faust_MatDense.hpp:1226-1250).
+// bit for bit, ties included.  A workgroup radix select finds the k-th largest
+// magnitude T of each line (row or column, staged in LDS) or of the whole
+// matrix.  When the entries equal to T are all needed to reach k, the support
+// is {|x| >= T} whatever the order of the scan; only when ties at T straddle
+// the k boundary does the exact heap selection of heap_select.hpp run, on one
+// lane, to pick the tied entries partial_sort keeps.  Zero magnitudes never
+// need it: selecting a zero leaves a zero.  The selection is followed by a
+// canonical Frobenius normalisation (faust_prox.hpp:72-200;
+// MatDense::normalize, faust_MatDense.hpp:1226-1250).
 #include <algorithm>
 
 #include "fh_internal.hpp"
@@ -13,13 +17,98 @@
 
 namespace fh {
 
+// Magnitude key: the bits of |x| order non-negative doubles like their values.
+__device__ inline unsigned long long mag_key(double x) {
+  return (unsigned long long)__double_as_longlong(fabs(x));
+}
+
+// Workgroup radix select (8-bit digits, most significant first) of the k-th
+// largest of len keys, 1 <= k <= len.  On return *T is a threshold such that
+// the keys >= *T are exactly the k largest, unless the function returns true:
+// then *T is the k-th largest key itself and the keys equal to it straddle
+// the k boundary, so which of them partial_sort keeps depends on positions
+// (the caller runs the exact heap selection); a zero T is never reported as
+// ambiguous, since keeping a zero or not gives the same output.  hist: 256 LDS counters; sh: 3
+// LDS ints.  Each lane caches two (digit, count) pairs before its LDS atomics,
+// which keeps the passes where most keys share a digit (the exponent bits)
+// free of atomic contention.
+template <class KeyAt>
+__device__ bool wg_select_threshold(KeyAt key_at, int len, int k, unsigned* hist, int* sh,
+                                    unsigned long long* T) {
+  const int tid = threadIdx.x, nt = blockDim.x;
+  unsigned long long prefix = 0, mask = 0;
+  int kk = k;  // rank of the target among the candidates, from the top
+  for (int shift = 56; shift >= 0; shift -= 8) {
+    for (int b = tid; b < 256; b += nt) hist[b] = 0u;
+    __syncthreads();
+    unsigned d0 = 256, c0 = 0, d1 = 256, c1 = 0;
+    for (int i = tid; i < len; i += nt) {
+      const unsigned long long x = key_at(i);
+      if ((x & mask) != prefix) continue;
+      const unsigned d = (unsigned)(x >> shift) & 255u;
+      if (d == d0) {
+        ++c0;
+      } else if (d == d1) {
+        ++c1;
+      } else {
+        if (c1) atomicAdd(&hist[d1], c1);
+        d1 = d0;
+        c1 = c0;
+        d0 = d;
+        c0 = 1;
+      }
+    }
+    if (c0) atomicAdd(&hist[d0], c0);
+    if (c1) atomicAdd(&hist[d1], c1);
+    __syncthreads();
+    if (tid < 64) {
+      // lane l owns bins 255-4l .. 252-4l; inclusive scan of their sums from the top
+      const int hi = 255 - 4 * tid;
+      const int s = (int)(hist[hi] + hist[hi - 1] + hist[hi - 2] + hist[hi - 3]);
+      int incl = s;
+      for (int d = 1; d < 64; d <<= 1) {
+        const int o = __shfl_up(incl, d, 64);
+        if (tid >= d) incl += o;
+      }
+      const unsigned long long bal = __ballot(incl >= kk);
+      const int L = __ffsll((unsigned long long)bal) - 1;
+      if (tid == L) {
+        int acc = incl - s, b = hi, c = 0;
+        for (int j = 0; j < 4; ++j, --b) {
+          c = (int)hist[b];
+          if (acc + c >= kk) break;
+          acc += c;
+        }
+        sh[0] = b;
+        sh[1] = kk - acc;
+        sh[2] = c;
+      }
+    }
+    __syncthreads();
+    const int b = sh[0], cnt = sh[2];
+    kk = sh[1];
+    prefix |= (unsigned long long)b << shift;
+    mask |= 255ull << shift;
+    if (cnt == kk) {  // every candidate left in the bin is taken
+      *T = prefix;
+      return false;
+    }
+  }
+  *T = prefix;
+  return prefix != 0;  // tied zeros: every choice gives the same output
+}
+
 // One 64-lane workgroup per line.  along_rows: line i is row i (splin);
 // otherwise line j is column j (spcol).  Dynamic LDS holds the line values and
-// the heap when they fit (lds_ok), else per-line global scratch is used.
+// the heap when they fit (lds_ok), else per-line global scratch is used.  The
+// radix threshold decides the support unless ties straddle the k boundary;
+// only those lines run the heap selection on one lane.
 __global__ __launch_bounds__(64) void k_prox_lines(const double* __restrict__ M, int m, int n, int along_rows, int k,
                                                    double* __restrict__ out, int lds_ok, double* __restrict__ gvals,
                                                    int* __restrict__ gidx, double* __restrict__ gabs) {
   extern __shared__ double smem[];
+  __shared__ unsigned hist[256];
+  __shared__ int sh[3];
   const int line = blockIdx.x;
   const int len = along_rows ? n : m;
   double* vals;
@@ -35,14 +124,18 @@ __global__ __launch_bounds__(64) void k_prox_lines(const double* __restrict__ M,
     hidx = gidx + (size_t)line * k;
   }
   const int lane = threadIdx.x;
+  for (int t = lane; t < len; t += 64) vals[t] = along_rows ? M[(size_t)t * m + line] : M[(size_t)line * m + t];
+  __syncthreads();
+  unsigned long long T;
+  const bool ambiguous = wg_select_threshold([vals](int i) { return mag_key(vals[i]); }, len, k, hist, sh, &T);
   for (int t = lane; t < len; t += 64) {
-    double x = along_rows ? M[(size_t)t * m + line] : M[(size_t)line * m + t];
-    vals[t] = x;
+    const double x = (!ambiguous && mag_key(vals[t]) >= T) ? vals[t] : 0.0;
     if (along_rows)
-      out[(size_t)t * m + line] = 0.0;
+      out[(size_t)t * m + line] = x;
     else
-      out[(size_t)line * m + t] = 0.0;
+      out[(size_t)line * m + t] = x;
   }
+  if (!ambiguous) return;
   __syncthreads();
   if (lane == 0) heap_select<double>(len, k, [vals](int i) { return fabs(vals[i]); }, hidx, habs);
   __syncthreads();
@@ -55,14 +148,43 @@ __global__ __launch_bounds__(64) void k_prox_lines(const double* __restrict__ M,
   }
 }
 
-// Global top-k over all m*n entries in column-major order (prox_sp): one lane.
+// Global top-k over all m*n entries in column-major order (prox_sp).
+struct SpSel {
+  unsigned long long T;
+  int ambiguous;
+};
+
+__global__ __launch_bounds__(1024) void k_sp_threshold(const double* __restrict__ M, int N, int k,
+                                                       SpSel* __restrict__ sel) {
+  __shared__ unsigned hist[256];
+  __shared__ int sh[3];
+  unsigned long long T;
+  const bool amb = wg_select_threshold([M](int i) { return mag_key(M[i]); }, N, k, hist, sh, &T);
+  if (threadIdx.x == 0) {
+    sel->T = T;
+    sel->ambiguous = amb ? 1 : 0;
+  }
+}
+
+// exact heap selection on one lane, only when the threshold left ties open
 __global__ __launch_bounds__(64) void k_prox_global_select(const double* __restrict__ M, int64_t N, int k,
-                                                           int* __restrict__ hidx, double* __restrict__ habs) {
+                                                           const SpSel* __restrict__ sel, int* __restrict__ hidx,
+                                                           double* __restrict__ habs) {
+  if (!sel->ambiguous) return;
   if (threadIdx.x == 0) heap_select<double>((int)N, k, [M](int i) { return fabs(M[i]); }, hidx, habs);
 }
 
+__global__ void k_sp_apply(const double* __restrict__ M, int64_t N, const SpSel* __restrict__ sel,
+                           double* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const double x = M[i];
+  out[i] = (!sel->ambiguous && mag_key(x) >= sel->T) ? x : 0.0;
+}
+
 __global__ void k_scatter_sel(const double* __restrict__ M, const int* __restrict__ idx, int k,
-                              double* __restrict__ out) {
+                              const SpSel* __restrict__ sel, double* __restrict__ out) {
+  if (!sel->ambiguous) return;
   int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q < k) out[idx[q]] = M[idx[q]];
 }
@@ -105,7 +227,6 @@ static void prox_lines(const double* M, int m, int n, bool along_rows, int k, do
     hipLaunchKernelGGL(k_prox_lines, dim3(nlines), dim3(64), 0, st, M, m, n, along_rows ? 1 : 0, k, out, 0,
                        gv.as<double>(), gi.as<int>(), ga.as<double>());
     FH_LAUNCH_CHECK();
-    FH_HIP(hipStreamSynchronize(st));  // scratch is returned to the pool below
     return;
   }
   FH_LAUNCH_CHECK();
@@ -119,7 +240,6 @@ static void normalize_inplace(double* M, int64_t N, hipStream_t st) {
   FH_HIP(hipStreamSynchronize(st));
   if (h == 0.0) throw Error(FH_ERR_NUMERIC, "the norm is zero, can't normalize.");
   scale_f64(M, N, s.as<double>(), 0.0, 1, st);
-  FH_HIP(hipStreamSynchronize(st));
 }
 
 void prox_f64(int name, double* M, int m, int n, int64_t k, bool normalized, bool pos, const double* supp,
@@ -134,15 +254,18 @@ void prox_f64(int name, double* M, int m, int n, int64_t k, bool normalized, boo
         zero_all();
       else if (k < N) {
         FH_CHECK(N < (int64_t)1 << 31, "prox_sp: matrix too large");
-        DBuf hi(sizeof(int) * k), ha(sizeof(double) * k), out(sizeof(double) * N);
-        hipLaunchKernelGGL(k_prox_global_select, dim3(1), dim3(64), 0, st, M, N, (int)k, hi.as<int>(),
-                           ha.as<double>());
+        DBuf hi(sizeof(int) * k), ha(sizeof(double) * k), out(sizeof(double) * N), sel(sizeof(SpSel));
+        hipLaunchKernelGGL(k_sp_threshold, dim3(1), dim3(1024), 0, st, M, (int)N, (int)k, sel.as<SpSel>());
         FH_LAUNCH_CHECK();
-        FH_HIP(hipMemsetAsync(out.p, 0, sizeof(double) * N, st));
-        hipLaunchKernelGGL(k_scatter_sel, g1(k), dim3(256), 0, st, M, hi.as<int>(), (int)k, out.as<double>());
+        hipLaunchKernelGGL(k_prox_global_select, dim3(1), dim3(64), 0, st, M, N, (int)k, sel.as<SpSel>(),
+                           hi.as<int>(), ha.as<double>());
+        FH_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_sp_apply, g1(N), dim3(256), 0, st, M, N, sel.as<SpSel>(), out.as<double>());
+        FH_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_scatter_sel, g1(k), dim3(256), 0, st, M, hi.as<int>(), (int)k, sel.as<SpSel>(),
+                           out.as<double>());
         FH_LAUNCH_CHECK();
         FH_HIP(hipMemcpyAsync(M, out.p, sizeof(double) * N, hipMemcpyDeviceToDevice, st));
-        FH_HIP(hipStreamSynchronize(st));
       }
       break;
     }
@@ -156,7 +279,6 @@ void prox_f64(int name, double* M, int m, int n, int64_t k, bool normalized, boo
         DBuf out(sizeof(double) * N);
         prox_lines(M, m, n, rows, (int)k, out.as<double>(), st);
         FH_HIP(hipMemcpyAsync(M, out.p, sizeof(double) * N, hipMemcpyDeviceToDevice, st));
-        FH_HIP(hipStreamSynchronize(st));
       }
       break;
     }
@@ -175,7 +297,6 @@ void prox_f64(int name, double* M, int m, int n, int64_t k, bool normalized, boo
       }
       hipLaunchKernelGGL(k_union, g1(N), dim3(256), 0, st, mc.as<double>(), ml.as<double>(), M, N);
       FH_LAUNCH_CHECK();
-      FH_HIP(hipStreamSynchronize(st));
       break;
     }
     case FH_PROX_SUPP: {

@@ -144,30 +144,36 @@ void eye_f64(double* x, int m, int n, hipStream_t st) {
 }
 
 // ---------------------------------------------------------------- conversions
-// per-row and per-column nonzero counts of a dense col-major matrix
-__global__ void k_count_rows(const double* __restrict__ D, int m, int n, int* __restrict__ cnt) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+// Dense -> CSR and CSC in chunks of 32 columns (rows for CSC) so that every
+// thread scans 32 entries: counts per (row, chunk), a host prefix sum, then
+// an independent fill per (row, chunk) in ascending column order.
+constexpr int CV_CH = 32;
+
+__global__ void k_count_row_chunks(const double* __restrict__ D, int m, int n, int* __restrict__ cnt) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, c = blockIdx.y;
   if (i >= m) return;
-  int c = 0;
-  for (int j = 0; j < n; ++j) c += (D[(size_t)j * m + i] != 0.0);
-  cnt[i] = c;
+  const int j1 = min(n, (c + 1) * CV_CH);
+  int k = 0;
+  for (int j = c * CV_CH; j < j1; ++j) k += (D[(size_t)j * m + i] != 0.0);
+  cnt[(size_t)c * m + i] = k;
 }
-__global__ void k_count_cols(const double* __restrict__ D, int m, int n, int* __restrict__ cnt) {
-  int j = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void k_count_col_chunks(const double* __restrict__ D, int m, int n, int* __restrict__ cnt) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x, c = blockIdx.y;
   if (j >= n) return;
   const double* col = D + (size_t)j * m;
-  int c = 0;
-  for (int i = 0; i < m; ++i) c += (col[i] != 0.0);
-  cnt[j] = c;
+  const int i1 = min(m, (c + 1) * CV_CH);
+  int k = 0;
+  for (int i = c * CV_CH; i < i1; ++i) k += (col[i] != 0.0);
+  cnt[(size_t)c * n + j] = k;
 }
-// CSR fill: row i scans columns ascending (coalesced across rows)
-__global__ void k_fill_csr(const double* __restrict__ D, int m, int n, const int* __restrict__ rp,
-                           int* __restrict__ ci, double* __restrict__ v) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void k_fill_row_chunks(const double* __restrict__ D, int m, int n, const int* __restrict__ off,
+                                  int* __restrict__ ci, double* __restrict__ v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, c = blockIdx.y;
   if (i >= m) return;
-  int p = rp[i];
-  for (int j = 0; j < n; ++j) {
-    double x = D[(size_t)j * m + i];
+  int p = off[(size_t)c * m + i];
+  const int j1 = min(n, (c + 1) * CV_CH);
+  for (int j = c * CV_CH; j < j1; ++j) {
+    const double x = D[(size_t)j * m + i];
     if (x != 0.0) {
       ci[p] = j;
       v[p] = x;
@@ -175,15 +181,15 @@ __global__ void k_fill_csr(const double* __restrict__ D, int m, int n, const int
     }
   }
 }
-// CSC fill: column j scans rows ascending
-__global__ void k_fill_csc(const double* __restrict__ D, int m, int n, const int* __restrict__ cp,
-                           int* __restrict__ ri, double* __restrict__ v) {
-  int j = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void k_fill_col_chunks(const double* __restrict__ D, int m, int n, const int* __restrict__ off,
+                                  int* __restrict__ ri, double* __restrict__ v) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x, c = blockIdx.y;
   if (j >= n) return;
   const double* col = D + (size_t)j * m;
-  int p = cp[j];
-  for (int i = 0; i < m; ++i) {
-    double x = col[i];
+  int p = off[(size_t)c * n + j];
+  const int i1 = min(m, (c + 1) * CV_CH);
+  for (int i = c * CV_CH; i < i1; ++i) {
+    const double x = col[i];
     if (x != 0.0) {
       ri[p] = i;
       v[p] = x;
@@ -192,12 +198,21 @@ __global__ void k_fill_csc(const double* __restrict__ D, int m, int n, const int
   }
 }
 
-static std::vector<int> scan_counts(const int* dcnt, int len, hipStream_t st) {
-  std::vector<int> h(len + 1, 0);
-  FH_HIP(hipMemcpyAsync(h.data() + 1, dcnt, sizeof(int) * len, hipMemcpyDeviceToHost, st));
-  FH_HIP(hipStreamSynchronize(st));
-  for (int i = 0; i < len; ++i) h[i + 1] += h[i];
-  return h;
+// counts [nch][len] -> per-(chunk, line) write offsets and the line pointer array
+static void chunk_offsets(const std::vector<int>& cnt, int nch, int len, std::vector<int>& off,
+                          std::vector<int>& ptr) {
+  ptr.assign(len + 1, 0);
+  for (int c = 0; c < nch; ++c)
+    for (int i = 0; i < len; ++i) ptr[i + 1] += cnt[(size_t)c * len + i];
+  for (int i = 0; i < len; ++i) ptr[i + 1] += ptr[i];
+  off.resize((size_t)nch * len);
+  for (int i = 0; i < len; ++i) {
+    int p = ptr[i];
+    for (int c = 0; c < nch; ++c) {
+      off[(size_t)c * len + i] = p;
+      p += cnt[(size_t)c * len + i];
+    }
+  }
 }
 
 void dense_to_csr_f64(const double* D, int m, int n, DMat& out, hipStream_t st) {
@@ -205,30 +220,39 @@ void dense_to_csr_f64(const double* D, int m, int n, DMat& out, hipStream_t st) 
   out.dtype = FH_F64;
   out.m = m;
   out.n = n;
-  DBuf cr(sizeof(int) * std::max(m, 1)), cc(sizeof(int) * std::max(n, 1));
+  const int nchr = std::max(1, (n + CV_CH - 1) / CV_CH);  // column chunks (CSR)
+  const int nchc = std::max(1, (m + CV_CH - 1) / CV_CH);  // row chunks (CSC)
+  std::vector<int> hr((size_t)nchr * m, 0), hc((size_t)nchc * n, 0);
+  DBuf cr(sizeof(int) * std::max<size_t>(hr.size(), 1)), cc(sizeof(int) * std::max<size_t>(hc.size(), 1));
   if (m > 0 && n > 0) {
-    hipLaunchKernelGGL(k_count_rows, dim3((m + 255) / 256), dim3(256), 0, st, D, m, n, cr.as<int>());
+    hipLaunchKernelGGL(k_count_row_chunks, dim3((m + 255) / 256, nchr), dim3(256), 0, st, D, m, n, cr.as<int>());
     FH_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_count_cols, dim3((n + 255) / 256), dim3(256), 0, st, D, m, n, cc.as<int>());
+    hipLaunchKernelGGL(k_count_col_chunks, dim3((n + 255) / 256, nchc), dim3(256), 0, st, D, m, n, cc.as<int>());
     FH_LAUNCH_CHECK();
+    FH_HIP(hipMemcpyAsync(hr.data(), cr.p, sizeof(int) * hr.size(), hipMemcpyDeviceToHost, st));
+    FH_HIP(hipMemcpyAsync(hc.data(), cc.p, sizeof(int) * hc.size(), hipMemcpyDeviceToHost, st));
+    FH_HIP(hipStreamSynchronize(st));
   }
-  std::vector<int> hr = m > 0 && n > 0 ? scan_counts(cr.as<int>(), m, st) : std::vector<int>(m + 1, 0);
-  std::vector<int> hc = m > 0 && n > 0 ? scan_counts(cc.as<int>(), n, st) : std::vector<int>(n + 1, 0);
-  int64_t nnz = hr[m];
+  std::vector<int> offr, rp, offc, cp;
+  chunk_offsets(hr, nchr, m, offr, rp);
+  chunk_offsets(hc, nchc, n, offc, cp);
+  const int64_t nnz = rp[m];
   out.nnz = nnz;
   out.rp.alloc(sizeof(int) * (m + 1));
   out.cp.alloc(sizeof(int) * (n + 1));
-  FH_HIP(hipMemcpyAsync(out.rp.p, hr.data(), sizeof(int) * (m + 1), hipMemcpyHostToDevice, st));
-  FH_HIP(hipMemcpyAsync(out.cp.p, hc.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, st));
+  FH_HIP(hipMemcpyAsync(out.rp.p, rp.data(), sizeof(int) * (m + 1), hipMemcpyHostToDevice, st));
+  FH_HIP(hipMemcpyAsync(out.cp.p, cp.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, st));
   out.ci.alloc(sizeof(int) * std::max<int64_t>(nnz, 1));
   out.v.alloc(sizeof(double) * std::max<int64_t>(nnz, 1));
   out.ri.alloc(sizeof(int) * std::max<int64_t>(nnz, 1));
   out.cv.alloc(sizeof(double) * std::max<int64_t>(nnz, 1));
   if (nnz > 0) {
-    hipLaunchKernelGGL(k_fill_csr, dim3((m + 255) / 256), dim3(256), 0, st, D, m, n, out.rp.as<int>(),
+    FH_HIP(hipMemcpyAsync(cr.p, offr.data(), sizeof(int) * offr.size(), hipMemcpyHostToDevice, st));
+    FH_HIP(hipMemcpyAsync(cc.p, offc.data(), sizeof(int) * offc.size(), hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_fill_row_chunks, dim3((m + 255) / 256, nchr), dim3(256), 0, st, D, m, n, cr.as<int>(),
                        out.ci.as<int>(), out.v.as<double>());
     FH_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_fill_csc, dim3((n + 255) / 256), dim3(256), 0, st, D, m, n, out.cp.as<int>(),
+    hipLaunchKernelGGL(k_fill_col_chunks, dim3((n + 255) / 256, nchc), dim3(256), 0, st, D, m, n, cc.as<int>(),
                        out.ri.as<int>(), out.cv.as<double>());
     FH_LAUNCH_CHECK();
   }

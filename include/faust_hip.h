@@ -192,6 +192,8 @@ typedef struct {
   double norm2_threshold; /* default 1e-6 */
   int use_graph;       /* capture one iteration in a HIP graph */
   int autotune;        /* time 32/64 GEMM tiles per target size at set-up */
+  int check_every;     /* > 0: FP64 relative-error check every check_every iterations */
+  double target_err;   /* stop when every target's FP64 relative error <= target_err */
 } fh_batch_params;
 
 typedef struct fh_batch_result_s *fh_batch_result_t;
@@ -207,8 +209,11 @@ int fh_batch_result_lambda(fh_batch_result_t r, double *lambdas);
 /* FP64 relative error ||lambda S_0...S_{J-1} - A||_F / ||A||_F per target */
 int fh_batch_result_relerr(fh_batch_result_t r, double *errs);
 int fh_batch_result_factor(fh_batch_result_t r, int target, int f, double *dense_out);
-/* ms[0] set-up, ms[1] main iterations, ms[2] refinement, ms[3] verification */
-int fh_batch_result_times(fh_batch_result_t r, double *ms4);
+/* ms[0] set-up, ms[1] main iterations, ms[2] refinement, ms[3] verification,
+ * ms[4] wall time until every target reached target_err (-1: not reached) */
+int fh_batch_result_times(fh_batch_result_t r, double *ms5);
+/* per target: iterations run when its error was first seen <= target_err (-1: never) */
+int fh_batch_result_iters(fh_batch_result_t r, int *iters);
 int fh_batch_result_free(fh_batch_result_t r);
 
 /* ---- warm-start MLP (n_in -> n_h1 -> n_h2 -> n_out, GELU between layers):

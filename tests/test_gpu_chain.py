@@ -66,3 +66,28 @@ def test_wht_faust(gpu):
     H = oracle.hadamard(256)
     assert np.abs(F.toarray() - H).max() < 1e-14
     assert abs(F.norm(2) - 1.0) < 1e-6
+
+
+@pytest.fixture(params=["0", "1"], ids=["persistent", "wide"])
+def power_form(request, monkeypatch):
+    # FH_POWER_WIDE forces the one-workgroup or the one-launch-per-operator form
+    monkeypatch.setenv("FH_POWER_WIDE", request.param)
+    return request.param
+
+
+@pytest.mark.parametrize("case", ["sparse256", "dense_rect", "mixed", "long_rows1024"])
+def test_norm2_both_forms_bitexact(gpu, power_form, case):
+    rng = np.random.default_rng(17)
+    if case == "sparse256":
+        facs = _cfg1_factors(n=256, nnz=4, J=4)
+    elif case == "dense_rect":
+        facs = [np.asfortranarray(rng.standard_normal((300, 200))), np.asfortranarray(rng.standard_normal((200, 310)))]
+    elif case == "mixed":
+        facs = [sp.csr_matrix(rng.standard_normal((130, 70)) * (rng.random((130, 70)) < 0.2)),
+                np.asfortranarray(rng.standard_normal((70, 90))),
+                faust_amd.rand_csr(90, 90, 5, rng)]
+    else:  # residual-like factor: 1024 x 1024 with 600 nnz per row, plus a dense one
+        facs = [faust_amd.rand_csr(1024, 1024, 600, rng), np.asfortranarray(rng.standard_normal((1024, 1024)))]
+    F = faust_amd.Faust(facs)
+    nrm, _ = oracle.spectral_norm(facs)
+    assert F.norm(2) == nrm

@@ -89,3 +89,25 @@ def test_reductions_bitexact(gpu, n):
     out = C.c_double()
     _lib.check(gpu.fh_dot(_lib.FH_F64, C.c_void_p(dx.data_ptr()), C.c_void_p(dy.data_ptr()), n, C.byref(out), None))
     assert out.value == oracle.reduce_dot(x, y)
+
+
+@pytest.mark.parametrize("name,shape,k", [("sp", (512, 512), 2048), ("sp", (300, 200), 59999),
+                                          ("splincol", (1024, 1024), 512), ("spcol", (2000, 64), 700),
+                                          ("splin", (96, 3000), 1)])
+def test_prox_radix_threshold_paths(gpu, name, shape, k):
+    # no ties (radix threshold decides), partial ties inside the top k (still
+    # decided), tie groups straddling k (heap fallback), and zeros below k
+    rng = np.random.default_rng(31)
+    base = rng.standard_normal(shape)
+    cases = [base,
+             np.where(rng.random(shape) < 0.3, np.round(base, 1), base),
+             np.round(base, 1),
+             base * (rng.random(shape) < 0.002)]
+    for M in cases:
+        M = np.asfortranarray(M)
+        if not M.any():
+            continue
+        got = getattr(proj, name)(shape, k)(M)
+        ref = oracle.prox(name, M, k)
+        assert np.array_equal(got != 0, ref != 0), "support differs"
+        assert np.array_equal(got, ref)

@@ -49,3 +49,15 @@ def test_palm4msa_small(gpu, R2L, packing, fmt):
     ref = oracle.palm4msa(A, [("splin", 4, 32, 16), ("sp", 64, 16, 16), ("spcol", 3, 16, 24)], num_its=15,
                           record_traj=True, is_update_way_R2L=R2L, packing_RL=packing, factor_format=fmt)
     _check_same(res, ref)
+
+
+def test_hierarchical_hadamard_wide_power(gpu, monkeypatch):
+    # the one-launch-per-operator power iteration gives the same trajectory
+    monkeypatch.setenv("FH_POWER_WIDE", "1")
+    n = 32
+    H = oracle.hadamard(n)
+    p = factparams.ParamsHierarchicalWHT(5, num_its=20)
+    res = fact.hierarchical(H, p, ret_raw=True)
+    fac, resc, kw = oracle.wht_params(5, 20)
+    ref = oracle.hierarchical(H, fac, resc, record_traj=True, **kw)
+    _check_same(res, ref)
