@@ -72,6 +72,9 @@ static void batch_alloc(Batch<T>& B, const BatchSpec& sp, bool masks) {
   }
   B.arena.alloc(sizeof(T) * std::max<int64_t>(tot, 1));
   FH_HIP(hipMemset(B.arena.p, 0, sizeof(T) * std::max<int64_t>(tot, 1)));
+  // the engine works on a non-blocking stream that does not wait for the
+  // null stream: the clear must be complete before any of its work starts
+  FH_HIP(hipDeviceSynchronize());
   B.lam.alloc(sizeof(double) * sp.nt);
   B.nR.alloc(sizeof(double) * sp.nt);
   B.nL.alloc(sizeof(double) * sp.nt);

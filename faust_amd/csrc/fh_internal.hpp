@@ -111,6 +111,11 @@ struct DMat {
   bool split = false;
   DBuf brp, bci, bdat;
   DBuf tbrp, tbci, tbdat;
+  // ELL view of the block rows (and of the transpose): ell_w slots per block
+  // row, slot = (block index into bdat, block column), (-1, 0) when empty;
+  // ell_w == 0 when the widest block row is too wide for it.
+  int ell_w = 0, tell_w = 0;
+  DBuf ell, tell;
   bool is_identity = false;
   // CSR copy of a dense matrix that turned out to be mostly zeros (built by
   // the product dispatcher; same values, so products keep the same bits)

@@ -140,6 +140,107 @@ __global__ __launch_bounds__(256) void k_bsr_mfma(int nbr, int p, const int* __r
   }
 }
 
+// ELL form of the same product: a wave reads its block-row's slots
+// (block index, block column) as one wave-uniform scalar load, so the X and
+// block loads follow after a single index latency, and it covers NC 32-column
+// tiles (NC = 4: 128 columns, four accumulators) so that more bytes are in
+// flight behind that latency.  Workgroup = 4 block-rows x 32*NC columns.
+constexpr int BSR_ELL_MAX = 16;
+
+template <bool SPLIT, bool IN_TILED, bool OUT_TILED, int NC>
+__global__ __launch_bounds__(256) void k_bsr_ell(int nbr, int p, int W, const int2* __restrict__ ell,
+                                                 const u16* __restrict__ bdat, const u16* __restrict__ X, int ldx,
+                                                 u16* __restrict__ Y, int ldy, float alpha, float beta,
+                                                 int npanels) {
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int panel = blockIdx.x % npanels, quad = blockIdx.x / npanels;
+  const int br = quad * 4 + wave;
+  if (br >= nbr) return;
+  const int kh = lane >> 5;
+  int jc[NC];
+  bool jok[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    jc[c] = panel * 32 * NC + c * 32 + (lane & 31);
+    jok[c] = jc[c] < p;
+  }
+  constexpr int BLK = 1024;
+  constexpr int PLANES = SPLIT ? 2 : 1;
+  f32x16 acc[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) acc[c] = f32x16{};
+  const int2* slots = ell + (size_t)br * W;
+  for (int g = 0; g < W; g += 4) {
+    int2 s[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) s[q] = (g + q < W) ? slots[g + q] : make_int2(-1, 0);
+    bf16x8 a[4][2][PLANES], x[4][2][NC];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      if (s[q].x >= 0) {
+        const int bc = s[q].y;
+        const u16* blk = bdat + (size_t)s[q].x * BLK * PLANES + (lane & 31) * 32 + kh * 8;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+#pragma unroll
+          for (int pl = 0; pl < PLANES; ++pl)
+            a[q][h][pl] = *reinterpret_cast<const bf16x8*>(blk + pl * BLK + h * 16);
+#pragma unroll
+          for (int c = 0; c < NC; ++c) {
+            const int j = jok[c] ? jc[c] : 0;
+            const u16* src = IN_TILED ? X + ((size_t)bc * p + j) * 32 + h * 16 + kh * 8
+                                      : X + (size_t)j * ldx + bc * 32 + h * 16 + kh * 8;
+            x[q][h][c] = jok[c] ? *reinterpret_cast<const bf16x8*>(src) : bf16x8{};
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      if (s[q].x >= 0) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+          for (int c = 0; c < NC; ++c) {
+            acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[q][h][0], x[q][h][c], acc[c], 0, 0, 0);
+            if (SPLIT)
+              acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[q][h][PLANES - 1], x[q][h][c], acc[c], 0, 0, 0);
+          }
+      }
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    if (!jok[c]) continue;
+    const int j = jc[c];
+    u16* ycol = OUT_TILED ? Y + ((size_t)br * p + j) * 32 : Y + (size_t)j * ldy + br * 32;
+#pragma unroll
+    for (int gg = 0; gg < 4; ++gg) {
+      const int r0 = 8 * gg + 4 * kh;
+      u16 o[4];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        float v = alpha * acc[c][4 * gg + t];
+        if (beta != 0.0f) v = fmaf(beta, bf2f(ycol[r0 + t]), v);
+        o[t] = f2bf(v);
+      }
+      uint2 w;
+      w.x = (unsigned)o[0] | ((unsigned)o[1] << 16);
+      w.y = (unsigned)o[2] | ((unsigned)o[3] << 16);
+      *reinterpret_cast<uint2*>(ycol + r0) = w;
+    }
+  }
+}
+
+// Which BSR kernel: FH_BSR_KERNEL=0 forces the row-pointer kernel, 2 / 4 the
+// ELL kernel with that many 32-column tiles per wave; default ELL with 4.
+static int bsr_kernel_choice() {
+  const char* e = getenv("FH_BSR_KERNEL");
+  if (e && (e[0] == '0' || e[0] == '2' || e[0] == '4')) return e[0] - '0';
+  return 4;
+}
+
 // in_tiled / out_tiled select the activation layouts (see above); ldx/ldy are
 // only used for column-major operands.
 static void bsr_mul(const DMat& B, char op, const void* X, int ldx, int p, void* Y, int ldy, float alpha,
@@ -151,11 +252,44 @@ static void bsr_mul(const DMat& B, char op, const void* X, int ldx, int p, void*
   if (!out_tiled) FH_CHECK(ldy % 4 == 0, "BSR MFMA path: ldy must be a multiple of 4");
   FH_CHECK(((uintptr_t)X & 15) == 0 && ((uintptr_t)Y & 7) == 0, "BSR MFMA path: X must be 16-byte aligned");
   if (nbr == 0 || p == 0) return;
-  const int npanels = (p + 63) / 64;
   const int quads = (nbr + 3) / 4;
+  const u16* dat = tr ? B.tbdat.as<u16>() : B.bdat.as<u16>();
+  const int W = tr ? B.tell_w : B.ell_w;
+  const int choice = bsr_kernel_choice();
+  if (W > 0 && choice != 0) {
+    const int2* ell = (tr ? B.tell : B.ell).as<int2>();
+    const int nc = choice;
+    const int npanels = (p + 32 * nc - 1) / (32 * nc);
+    const dim3 grid((unsigned)(npanels * quads));
+#define FH_ELL_LAUNCH(S, I, O, NC)                                                                             \
+  hipLaunchKernelGGL((k_bsr_ell<S, I, O, NC>), grid, dim3(256), 0, st, nbr, p, W, ell, dat, (const u16*)X, ldx, \
+                     (u16*)Y, ldy, alpha, beta, npanels)
+    const int sel = (B.split ? 8 : 0) | (in_tiled ? 4 : 0) | (out_tiled ? 2 : 0) | (nc == 4 ? 1 : 0);
+    switch (sel) {
+      case 0: FH_ELL_LAUNCH(false, false, false, 2); break;
+      case 1: FH_ELL_LAUNCH(false, false, false, 4); break;
+      case 2: FH_ELL_LAUNCH(false, false, true, 2); break;
+      case 3: FH_ELL_LAUNCH(false, false, true, 4); break;
+      case 4: FH_ELL_LAUNCH(false, true, false, 2); break;
+      case 5: FH_ELL_LAUNCH(false, true, false, 4); break;
+      case 6: FH_ELL_LAUNCH(false, true, true, 2); break;
+      case 7: FH_ELL_LAUNCH(false, true, true, 4); break;
+      case 8: FH_ELL_LAUNCH(true, false, false, 2); break;
+      case 9: FH_ELL_LAUNCH(true, false, false, 4); break;
+      case 10: FH_ELL_LAUNCH(true, false, true, 2); break;
+      case 11: FH_ELL_LAUNCH(true, false, true, 4); break;
+      case 12: FH_ELL_LAUNCH(true, true, false, 2); break;
+      case 13: FH_ELL_LAUNCH(true, true, false, 4); break;
+      case 14: FH_ELL_LAUNCH(true, true, true, 2); break;
+      default: FH_ELL_LAUNCH(true, true, true, 4); break;
+    }
+#undef FH_ELL_LAUNCH
+    FH_LAUNCH_CHECK();
+    return;
+  }
+  const int npanels = (p + 63) / 64;
   const int* rp = tr ? B.tbrp.as<int>() : B.brp.as<int>();
   const int* ci = tr ? B.tbci.as<int>() : B.bci.as<int>();
-  const u16* dat = tr ? B.tbdat.as<u16>() : B.bdat.as<u16>();
   dim3 grid((unsigned)(npanels * quads));
 #define FH_BSR_LAUNCH(S, I, O)                                                                                \
   hipLaunchKernelGGL((k_bsr_mfma<S, I, O>), grid, dim3(256), 0, st, nbr, p, rp, ci, dat, (const u16*)X, ldx, \
@@ -252,6 +386,108 @@ __global__ __launch_bounds__(256) void k_csr_rm_f32(int rows, int p, const int* 
   }
 }
 
+// Two rows per thread (workgroup = 64 rows x 32 columns): the index loads
+// and the gathers of both rows are issued together, twice the bytes in flight
+// behind each latency of the row-pointer -> index -> gather chain.
+template <bool OUT_COL>
+__global__ __launch_bounds__(256) void k_csr_rm2_f32(int rows, int p, const int* __restrict__ rp,
+                                                     const int* __restrict__ ci, const float* __restrict__ v,
+                                                     const float* __restrict__ X, int ldx, float* __restrict__ Y,
+                                                     int ldy, float alpha, float beta, int npanels) {
+  __shared__ float tile[64][33];
+  constexpr int NB = 8;
+  const int t = threadIdx.x;
+  const int panel = blockIdx.x % npanels, rblk = blockIdx.x / npanels;
+  const int c0 = panel * 32 + (t & 7) * 4;
+  int r[2], e0[2], e1[2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    r[i] = rblk * 64 + 32 * i + (t >> 3);
+    const bool ok = r[i] < rows && c0 < p;
+    e0[i] = ok ? rp[r[i]] : 0;
+    e1[i] = ok ? rp[r[i] + 1] : 0;
+  }
+  float4 acc[2] = {make_float4(0.f, 0.f, 0.f, 0.f), make_float4(0.f, 0.f, 0.f, 0.f)};
+  const int len = max(e1[0] - e0[0], e1[1] - e0[1]);
+  for (int base = 0; base < len; base += NB) {
+    int cc[2][NB];
+    float ww[2][NB];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int q = 0; q < NB; ++q) {
+        const int e = e0[i] + base + q;
+        const bool ok = e < e1[i];
+        cc[i][q] = ok ? ci[e] : -1;
+        ww[i][q] = ok ? v[e] : 0.0f;
+      }
+    float4 xx[2][NB];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int q = 0; q < NB; ++q)
+        xx[i][q] = cc[i][q] >= 0 ? *reinterpret_cast<const float4*>(X + (size_t)cc[i][q] * ldx + c0)
+                                 : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int q = 0; q < NB; ++q) {
+        acc[i].x = fmaf(ww[i][q], xx[i][q].x, acc[i].x);
+        acc[i].y = fmaf(ww[i][q], xx[i][q].y, acc[i].y);
+        acc[i].z = fmaf(ww[i][q], xx[i][q].z, acc[i].z);
+        acc[i].w = fmaf(ww[i][q], xx[i][q].w, acc[i].w);
+      }
+  }
+  if (!OUT_COL) {
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      if (r[i] < rows && c0 < p) {
+        float4* y = reinterpret_cast<float4*>(Y + (size_t)r[i] * ldy + c0);
+        float4 o = make_float4(alpha * acc[i].x, alpha * acc[i].y, alpha * acc[i].z, alpha * acc[i].w);
+        if (beta != 0.0f) {
+          const float4 old = *y;
+          o.x = fmaf(beta, old.x, o.x);
+          o.y = fmaf(beta, old.y, o.y);
+          o.z = fmaf(beta, old.z, o.z);
+          o.w = fmaf(beta, old.w, o.w);
+        }
+        *y = o;
+      }
+    }
+    return;
+  }
+  const int lc = (t & 7) * 4;
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    const int lr = 32 * i + (t >> 3);
+    tile[lr][lc + 0] = acc[i].x;
+    tile[lr][lc + 1] = acc[i].y;
+    tile[lr][lc + 2] = acc[i].z;
+    tile[lr][lc + 3] = acc[i].w;
+  }
+  __syncthreads();
+  // thread -> column (t >> 3), 8 consecutive rows (t & 7) * 8
+  const int cc = t >> 3, rr = (t & 7) * 8;
+  const int gc = panel * 32 + cc;
+  if (gc >= p) return;
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int gr = rblk * 64 + rr + q;
+    if (gr < rows) {
+      float o = alpha * tile[rr + q][cc];
+      float* y = Y + (size_t)gc * ldy + gr;
+      if (beta != 0.0f) o = fmaf(beta, *y, o);
+      *y = o;
+    }
+  }
+}
+
+// FH_CSR_RPT=1 selects the one-row-per-thread kernel
+static int csr_rows_per_thread() {
+  const char* e = getenv("FH_CSR_RPT");
+  return (e && e[0] == '1') ? 1 : 2;
+}
+
 // column-major (rows x p, ld) -> row-major (rows x ldr), zero-padding columns p..ldr
 __global__ __launch_bounds__(256) void k_col2row_f32(int rows, int p, const float* __restrict__ X, int ldx,
                                                      float* __restrict__ R, int ldr) {
@@ -285,6 +521,17 @@ static void csr_f32_mul_rm(const DMat& S, char op, const float* X, int ldx, int 
   const int* ci = tr ? S.ri.as<int>() : S.ci.as<int>();
   const float* v = tr ? S.cv.as<float>() : S.v.as<float>();
   const int npanels = (p + 31) / 32;
+  if (csr_rows_per_thread() == 2) {
+    const dim3 grid2((unsigned)(npanels * ((rows + 63) / 64)));
+    if (out_col)
+      hipLaunchKernelGGL(k_csr_rm2_f32<true>, grid2, dim3(256), 0, st, rows, p, rp, ci, v, X, ldx, Y, ldy, alpha,
+                         beta, npanels);
+    else
+      hipLaunchKernelGGL(k_csr_rm2_f32<false>, grid2, dim3(256), 0, st, rows, p, rp, ci, v, X, ldx, Y, ldy, alpha,
+                         beta, npanels);
+    FH_LAUNCH_CHECK();
+    return;
+  }
   dim3 grid((unsigned)(npanels * ((rows + 31) / 32)));
   if (out_col)
     hipLaunchKernelGGL(k_csr_rm_f32<true>, grid, dim3(256), 0, st, rows, p, rp, ci, v, X, ldx, Y, ldy, alpha, beta,
@@ -444,6 +691,23 @@ static void put_block(const void* src, int dtype, size_t b, bool transpose, bool
     }
 }
 
+// ELL slots of a block-row structure: width = widest block row (0 when above
+// BSR_ELL_MAX); slot (br, q) = (rp[br] + q, ci[rp[br] + q]) or (-1, 0).
+static int build_ell(int nrows, const int32_t* rp, const int32_t* ci, std::vector<int32_t>& ell) {
+  int w = 0;
+  for (int r = 0; r < nrows; ++r) w = std::max(w, rp[r + 1] - rp[r]);
+  if (w == 0 || w > BSR_ELL_MAX) return 0;
+  ell.assign((size_t)nrows * w * 2, 0);
+  for (int r = 0; r < nrows; ++r)
+    for (int q = 0; q < w; ++q) {
+      const int b = rp[r] + q;
+      const bool on = b < rp[r + 1];
+      ell[((size_t)r * w + q) * 2] = on ? b : -1;
+      ell[((size_t)r * w + q) * 2 + 1] = on ? ci[b] : 0;
+    }
+  return w;
+}
+
 DMatP make_bsr(int dtype, int m, int n, int bnnz, const int32_t* browptr, const int32_t* bcolinds,
                const void* bdata, hipStream_t st) {
   FH_CHECK(dtype == FH_BF16 || dtype == FH_F32, "BSR factors are BF16 or FP32");
@@ -489,6 +753,11 @@ DMatP make_bsr(int dtype, int m, int n, int bnnz, const int32_t* browptr, const 
   up(M->tbrp, tbrp.data(), sizeof(int) * (nbc + 1));
   up(M->tbci, tbci.data(), sizeof(int) * bnnz);
   up(M->tbdat, tdat.data(), sizeof(u16) * (size_t)bnnz * 1024 * planes);
+  std::vector<int32_t> ell, tell;
+  M->ell_w = build_ell(nbr, browptr, bcolinds, ell);
+  M->tell_w = build_ell(nbc, tbrp.data(), tbci.data(), tell);
+  if (M->ell_w) up(M->ell, ell.data(), sizeof(int32_t) * ell.size());
+  if (M->tell_w) up(M->tell, tell.data(), sizeof(int32_t) * tell.size());
   FH_HIP(hipStreamSynchronize(st));
   return M;
 }

@@ -106,3 +106,59 @@ def test_bsr_to_dense(gpu):
     B = _bsr_rounded(rand_bsr(128, 96, 32, 2, rng))
     f = faust_amd.Factor(B, "bf16")
     assert np.array_equal(f.toarray(), B.toarray())
+
+
+def _ragged_bsr(rng, nbr, nbc, bs=32, wide_row=None):
+    # block rows with 0..6 blocks (some empty); optionally one block row with
+    # `wide_row` blocks, wider than the ELL slot limit
+    rows = []
+    for br in range(nbr):
+        k = int(rng.integers(0, 7)) if br != 3 else 0
+        if wide_row and br == 5:
+            k = wide_row
+        rows.append(np.sort(rng.choice(nbc, size=min(k, nbc), replace=False)))
+    indptr = np.concatenate([[0], np.cumsum([len(r) for r in rows])])
+    idx = np.concatenate(rows).astype(np.int64)
+    data = rng.standard_normal((len(idx), bs, bs)) / np.sqrt(128)
+    return sp.bsr_matrix((data, idx, indptr), shape=(nbr * bs, nbc * bs))
+
+
+@pytest.mark.parametrize("kernel", ["0", "2", "4"])
+@pytest.mark.parametrize("wide", [None, 20])
+@pytest.mark.parametrize("op", ["N", "T"])
+def test_bsr_kernel_variants(gpu, monkeypatch, kernel, wide, op):
+    # row-pointer kernel, ELL with 64 / 128 columns per wave; ragged block
+    # rows, empty block rows and (wide) a block row beyond the ELL limit; a
+    # 3-factor chain exercises the tiled intermediates
+    monkeypatch.setenv("FH_BSR_KERNEL", kernel)
+    rng = np.random.default_rng(11)
+    facs = [_bsr_rounded(_ragged_bsr(rng, 24, 24, wide_row=wide)) for _ in range(3)]
+    for p in (200, 8):
+        X = bf16_round(rng.standard_normal((768, p)))
+        F = faust_amd.Faust(facs, dtype="bf16")
+        Y = (F @ X) if op == "N" else (F.T @ X)
+        ref = oracle.chain_multiply(facs, X, op)
+        rel = np.linalg.norm(Y - ref) / np.linalg.norm(ref)
+        assert rel < 8e-3, rel
+        Y1 = faust_amd.Faust([facs[0]], dtype="bf16") @ X
+        r1 = oracle.chain_multiply([facs[0]], X, "N")
+        assert np.all(np.abs(Y1 - r1) <= 2.0 ** -8 * np.abs(r1) + 1e-3 * np.abs(r1).max())
+
+
+@pytest.mark.parametrize("rpt", ["1", "2"])
+@pytest.mark.parametrize("op", ["N", "T"])
+def test_csr_fp32_rows_per_thread(gpu, monkeypatch, rpt, op):
+    monkeypatch.setenv("FH_CSR_RPT", rpt)
+    rng = np.random.default_rng(12)
+    facs = []
+    for s in [(300, 250), (250, 333), (333, 300)]:
+        a = rng.standard_normal(s) * (rng.random(s) < rng.uniform(0.0, 0.15, size=(s[0], 1)))
+        a[7] = 0.0  # an empty row
+        facs.append(sp.csr_matrix(a).astype(np.float32).astype(np.float64))
+    for p in (70, 33, 1):
+        rows_in = facs[-1].shape[1] if op == "N" else facs[0].shape[0]
+        X = rng.standard_normal((rows_in, p)).astype(np.float32).astype(np.float64)
+        F = faust_amd.Faust(facs, dtype=np.float32)
+        Y = (F @ X) if op == "N" else (F.T @ X)
+        ref = oracle.chain_multiply(facs, X, op)
+        assert np.linalg.norm(Y - ref) / np.linalg.norm(ref) < 1e-5
