@@ -1,6 +1,7 @@
 """Benchmark of the Faust x dense hot path (and PALM4MSA) on one MI355X.
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME] [--no-cpu-baseline] [--palm]
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME] [--no-cpu-baseline] [--no-palm]
+               [--hadamard]
 
 Prints ONE JSON line.  `value` is the algorithmic HBM throughput (GB/s) of
 one step = one Faust x dense product (all factors) with inputs resident in
@@ -9,7 +10,10 @@ divided by its average launch duration inside the graph-timed steps (HIP
 events on the launch stream around the replay; the layout-conversion time
 per step, from the library's per-factor event timers, is subtracted).  `cpu_baseline` is the
 oracle (CPU FP64 restatement of the reference) timed on this host on a
-bounded sample of the same workload.
+bounded sample of the same workload.  `palm` compares the tuned batched
+PALM4MSA (FP32 + FP64 frozen-support refinement, HIP graphs, autotuned
+tiles) with the plain FP64 GPU form on 512 targets, at the FP64-verified
+relative error each reaches.
 """
 import argparse
 import ctypes as C
@@ -34,7 +38,10 @@ def _parse():
     ap.add_argument("--workload", default="chain_bsr_bf16")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--palm", action="store_true", help="also time hierarchical PALM4MSA (Hadamard 1024)")
+    ap.add_argument("--no-palm", action="store_true",
+                    help="skip the batched PALM4MSA comparison (512 targets, tuned vs plain FP64)")
+    ap.add_argument("--hadamard", action="store_true",
+                    help="also time hierarchical PALM4MSA on the 1024 Hadamard matrix, 100 iterations per level")
     return ap.parse_args()
 
 
@@ -101,6 +108,97 @@ def workload_chain_csr_f32():
 
 WORKLOADS = {"chain_csr_f64": workload_chain_csr_f64, "chain_bsr_bf16": workload_chain_bsr_bf16,
              "chain_bsr_f32": workload_chain_bsr_f32, "chain_csr_f32": workload_chain_csr_f32}
+
+
+# ----------------------------------------------------------------- PALM4MSA
+def palm_targets(nt=512, seed=0xC4):
+    """512 square targets, n uniform in {64,128,256,512}, each the product of
+    3 random sparse n x n factors (4 nnz per row at uniform distinct columns,
+    N(0,1)) plus i.i.d. N(0, 1e-6) noise (standard deviation 1e-3)."""
+    import scipy.sparse as sp
+    import faust_amd
+    rng = np.random.default_rng(seed)
+    sizes = rng.choice([64, 128, 256, 512], size=nt)
+    As = []
+    for n in sizes:
+        P = sp.identity(int(n), format="csr")
+        for _ in range(3):
+            P = P @ faust_amd.rand_csr(int(n), int(n), 4, rng)
+        As.append(np.asfortranarray(P.toarray() + 1e-3 * rng.standard_normal((n, n))))
+    return sizes, As
+
+
+def _relerr(lam, facs, A):
+    P = np.eye(A.shape[0])
+    for f in facs:
+        P = P @ f
+    return float(np.linalg.norm(lam * P - A) / np.linalg.norm(A))
+
+
+def palm_batched_vs_plain(cpu=True):
+    """Time to the FP64-verified relative error of 220 PALM4MSA iterations
+    (3 factors, sp k = 4n): the tuned form (FP32 iterations on the batched
+    grouped-GEMM engine, one HIP graph per iteration, autotuned tiles, then 20
+    FP64 frozen-support sweeps) against the plain FP64 GPU form (one target at
+    a time, the oracle-exact path).  The plain form runs on one target of each
+    size and its total is extrapolated by the size counts."""
+    import torch
+    from faust_amd import fact, factparams
+    from faust_amd.batched import palm4msa_batched
+    sizes, As = palm_targets()
+    ks = [4 * int(n) for n in sizes]
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    tuned = palm4msa_batched(As, 3, k=ks, dtype="f32", num_its=200, refine_its=20, return_factors=False)
+    t_tuned = time.perf_counter() - t
+    t = time.perf_counter()
+    b64 = palm4msa_batched(As, 3, k=ks, dtype="f64", num_its=220, refine_its=0, return_factors=False)
+    t_b64 = time.perf_counter() - t
+    plain_s, sample, err_ratio = 0.0, [], []
+    for n in (64, 128, 256, 512):
+        i = int(np.flatnonzero(sizes == n)[0])
+        p = factparams.ParamsPalm4MSA([factparams.ConstraintInt("sp", n, n, 4 * n)] * 3,
+                                      factparams.StoppingCriterion(num_its=220))
+        t = time.perf_counter()
+        r = fact.palm4msa(As[i], p, ret_raw=True)
+        dt = time.perf_counter() - t
+        e = _relerr(r.lam, r.dense_factors(), As[i])
+        cnt = int(np.sum(sizes == n))
+        plain_s += dt * cnt
+        err_ratio.append(float(tuned.relerr[i]) / e)
+        sample.append({"n": n, "count": cnt, "plain_s_per_target": round(dt, 4), "err_plain": e,
+                       "err_tuned": float(tuned.relerr[i]), "err_batched_fp64": float(b64.relerr[i])})
+    out = {"workload": "batched_palm4msa_512_targets_n64-512_3facs_sp4n_220its",
+           "tuned_s": round(t_tuned, 3), "tuned_phase_ms": [round(x, 1) for x in tuned.times_ms[:4]],
+           "batched_fp64_s": round(t_b64, 3), "plain_fp64_gpu_s_est": round(plain_s, 2),
+           "speedup_tuned_vs_plain": round(plain_s / t_tuned, 1),
+           "median_relerr_tuned": float(np.median(tuned.relerr)),
+           "median_relerr_batched_fp64": float(np.median(b64.relerr)),
+           "tuned_over_plain_err_ratio_max": round(max(err_ratio), 5), "sample": sample}
+    if cpu:
+        import oracle
+        i = int(np.flatnonzero(sizes == 128)[0])
+        t = time.perf_counter()
+        oracle.palm4msa(As[i], [("sp", 4 * 128, 128, 128)] * 3, num_its=220)
+        dt = time.perf_counter() - t
+        est = sum(dt * (n / 128.0) ** 3 * int(np.sum(sizes == n)) for n in (64, 128, 256, 512))
+        out["cpu_baseline"] = {"value": round(est, 1), "unit": "s (512 targets, n^3-extrapolated)",
+                               "cores": len(os.sched_getaffinity(0)), "kind": "port",
+                               "sample": f"oracle FP64 palm4msa on one n=128 target, 220 its: {dt:.3f} s"}
+    return out
+
+
+def palm_hadamard():
+    """Hierarchical PALM4MSA of the 1024 Hadamard matrix into 10 factors
+    (splincol, 100 iterations per level, plain FP64 GPU form)."""
+    import oracle
+    from faust_amd import fact, factparams
+    H = oracle.hadamard(1024)
+    t = time.perf_counter()
+    res = fact.hierarchical(H, factparams.ParamsHierarchicalWHT(10, num_its=100), ret_raw=True)
+    dt = time.perf_counter() - t
+    return {"workload": "hierarchical_hadamard_1024_10facs_splincol_100its", "seconds": round(dt, 3),
+            "relerr": _relerr(res.lam, res.dense_factors(), H)}
 
 
 # ----------------------------------------------------------------- helpers
@@ -223,6 +321,10 @@ def main():
            "roofline": roof}
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(wl, args.cpu_seconds)
+    if not args.no_palm:
+        out["palm"] = palm_batched_vs_plain(cpu=not args.no_cpu_baseline)
+    if args.hadamard:
+        out["palm_hadamard"] = palm_hadamard()
     print(json.dumps(out))
 
 

@@ -48,11 +48,27 @@ def random_butterfly_factor(n, j, rng):
 
 def butterfly_target(n, rng, noise=1e-6):
     """A = B_0 B_1 ... B_{L-1} + N(0, noise^2) entries."""
+    return butterfly_sample(n, rng, noise)[0]
+
+
+def butterfly_sample(n, rng, noise=1e-6):
+    """(A, [B_0 .. B_{L-1}]): a target and the factors it was made from."""
     L = int(np.log2(n))
-    P = np.eye(n)
+    i = np.arange(n)
+    facs = []
     for j in range(L):
-        P = P @ random_butterfly_factor(n, j, rng)
-    return np.asfortranarray(P + noise * rng.standard_normal((n, n)))
+        lo = i[(i & (1 << j)) == 0]
+        hi = lo | (1 << j)
+        t = rng.uniform(0.0, 2 * np.pi, lo.size)
+        s = rng.uniform(0.5, 2.0, lo.size)
+        c, sn = s * np.cos(t), s * np.sin(t)
+        B = np.zeros((n, n))
+        B[lo, lo], B[lo, hi], B[hi, lo], B[hi, hi] = c, -sn, sn, c
+        facs.append(B)
+    P = np.eye(n)
+    for B in facs:
+        P = P @ B
+    return np.asfortranarray(P + noise * rng.standard_normal((n, n))), facs
 
 
 def pack_values(factors, masks):
@@ -78,6 +94,31 @@ def unpack_values(v, masks):
 
 def identity_init(masks):
     return [np.asfortranarray(np.eye(M.shape[0]) * (np.diag(M) != 0)) for M in masks]
+
+
+def canonicalize(factors):
+    """Canonical representative of a butterfly factorization.
+
+    F_0 ... F_{L-1} and F_0 D_0, D_0^{-1} F_1 D_1, ... (D_j diagonal,
+    invertible) have the same product and the same supports, so raw factor
+    values are not a function of the target.  Going left to right, every
+    column i of F_j (j < L-1) is scaled to unit norm with a positive diagonal
+    entry and row i of F_{j+1} takes the inverse scale; the last factor keeps
+    what is left.  Then every factor is divided by its Frobenius norm, which
+    goes into lambda.  Returns (lambda, factors) with the same product."""
+    F = [np.array(f, dtype=np.float64) for f in factors]
+    for j in range(len(F) - 1):
+        nrm = np.linalg.norm(F[j], axis=0)
+        sgn = np.where(np.diag(F[j]) < 0, -1.0, 1.0)
+        d = np.where(nrm > 0, sgn / np.where(nrm > 0, nrm, 1.0), 1.0)
+        F[j] = F[j] * d[None, :]
+        F[j + 1] = F[j + 1] / d[:, None]
+    lam = 1.0
+    for j in range(len(F)):
+        s = np.linalg.norm(F[j])
+        lam *= s
+        F[j] = np.asfortranarray(F[j] / s)
+    return lam, F
 
 
 class MLP:
@@ -127,3 +168,111 @@ class MLP:
         W = np.ascontiguousarray(W, dtype=np.float32)
         b = np.ascontiguousarray(b, dtype=np.float32)
         check(lib().fh_mlp_set(self.h, layer, W.ctypes.data_as(C.c_void_p), b.ctypes.data_as(C.c_void_p)))
+
+
+# ---------------------------------------------------------------- pipeline
+def mlp_input(A):
+    """Network input: the target scaled to unit RMS entry, flattened row-major."""
+    A = np.asarray(A, dtype=np.float64)
+    return (A * (A.shape[0] / np.linalg.norm(A))).astype(np.float32).ravel()
+
+
+def label_scale(n):
+    """Canonical factors have unit Frobenius norm over 2n entries: scale to O(1)."""
+    return float(np.sqrt(2 * n))
+
+
+def palm_labels(As, gens, masks, num_its=20, chunk=512):
+    """Training labels made by the FP64 PALM4MSA itself: each target is
+    factorized with supp projectors on the butterfly supports, starting from
+    the canonical form of the factors it was generated from, and the result is
+    brought to canonical form.  Returns (labels [N, 2nL] float32, lambdas,
+    final FP64-verified relative errors)."""
+    from .batched import palm4msa_batched
+    n, L = As[0].shape[0], len(masks)
+    labels, lams, errs = [], [], []
+    for c0 in range(0, len(As), chunk):
+        part = range(c0, min(len(As), c0 + chunk))
+        can = [canonicalize(gens[b]) for b in part]
+        r = palm4msa_batched([As[b] for b in part], L, supports=[masks] * len(part), init=[c[1] for c in can],
+                             lambda_init=[c[0] for c in can], dtype="f64", num_its=num_its, refine_its=0)
+        for q in range(len(part)):
+            lam, F = canonicalize(r.factors[q])
+            labels.append((pack_values(F, masks) * label_scale(n)).astype(np.float32))
+            lams.append(lam * r.lambdas[q])
+        errs.extend(r.relerr.tolist())
+    return np.stack(labels), np.array(lams), np.array(errs)
+
+
+def train_mlp(mlp, X, Y, epochs=10, batch=256, lr=1e-3, seed=0, log=None):
+    """Adam on the MSE between network output and labels.  X, Y: float32
+    CUDA tensors [N, n_in], [N, n_out]."""
+    import torch
+    g = torch.Generator(device="cpu").manual_seed(seed)
+    N = X.shape[0]
+    losses = []
+    for ep in range(epochs):
+        perm = torch.randperm(N, generator=g).to(X.device)
+        tot, cnt = 0.0, 0
+        lr_ep = lr * (0.5 ** (ep // max(1, epochs // 3)))
+        for b0 in range(0, N - batch + 1, batch):
+            idx = perm[b0:b0 + batch]
+            xb, yb = X.index_select(0, idx).contiguous(), Y.index_select(0, idx).contiguous()
+            tot += mlp.train_step(xb, yb, lr_ep)
+            cnt += 1
+        losses.append(tot / max(cnt, 1))
+        if log:
+            log(f"epoch {ep}: mse {losses[-1]:.5f} lr {lr_ep:.2e}")
+    return losses
+
+
+def predicted_init(mlp, As, masks, batch=256):
+    """Initial factors and lambda for PALM4MSA from the network: unpack the
+    prediction, normalise every factor like the supp projector does, and take
+    the lambda that best fits the target (<A, P> / ||P||^2)."""
+    import torch
+    n = As[0].shape[0]
+    out_f, out_l = [], []
+    for b0 in range(0, len(As), batch):
+        part = As[b0:b0 + batch]
+        X = torch.from_numpy(np.stack([mlp_input(A) for A in part])).cuda()
+        Y = mlp.forward(X).cpu().numpy().astype(np.float64) / label_scale(n)
+        for A, y in zip(part, Y):
+            F = [f / max(np.linalg.norm(f), 1e-300) for f in unpack_values(y, masks)]
+            P = np.eye(n)
+            for f in F:
+                P = P @ f
+            pp = float(np.sum(P * P))
+            out_f.append([np.asfortranarray(f) for f in F])
+            out_l.append(float(np.sum(A * P)) / pp if pp > 0 else 1.0)
+    return out_f, out_l
+
+
+def compare_warm_start(As, masks, inits, lambdas, max_its=1000, check_every=25, target_err=1e-4, dtype="f64"):
+    """PALM4MSA from the predicted initialisation and from the reference one
+    (first factor 0, the others identity), same iteration budget, FP64-verified
+    errors checked every `check_every` iterations.  A warm-started run whose
+    final error is worse than the reference run's falls back to the latter."""
+    import time
+    from .batched import palm4msa_batched
+    L = len(masks)
+    sup = [masks] * len(As)
+    t = time.perf_counter()
+    ref = palm4msa_batched(As, L, supports=sup, dtype=dtype, num_its=max_its, refine_its=0,
+                           check_every=check_every, target_err=target_err, return_factors=False)
+    t_ref = time.perf_counter() - t
+    t = time.perf_counter()
+    warm = palm4msa_batched(As, L, supports=sup, init=inits, lambda_init=lambdas, dtype=dtype, num_its=max_its,
+                            refine_its=0, check_every=check_every, target_err=target_err, return_factors=False)
+    t_warm = time.perf_counter() - t
+    fallback = warm.relerr > ref.relerr
+    final = np.where(fallback, ref.relerr, warm.relerr)
+    it_w = np.where(fallback, ref.iters_to_target, warm.iters_to_target)
+    return {"targets": len(As), "max_its": max_its, "target_err": target_err, "dtype": dtype,
+            "ref_reached": int((ref.iters_to_target > 0).sum()), "warm_reached": int((it_w > 0).sum()),
+            "ref_median_its": float(np.median(ref.iters_to_target[ref.iters_to_target > 0]))
+            if (ref.iters_to_target > 0).any() else None,
+            "warm_median_its": float(np.median(it_w[it_w > 0])) if (it_w > 0).any() else None,
+            "ref_median_err": float(np.median(ref.relerr)), "warm_median_err": float(np.median(final)),
+            "fallbacks": int(fallback.sum()), "ref_wall_s": round(t_ref, 3), "warm_wall_s": round(t_warm, 3),
+            "ref_ms_to_all": float(ref.times_ms[4]), "warm_ms_to_all": float(warm.times_ms[4])}
