@@ -35,7 +35,7 @@ class fh_palm_params(C.Structure):
     _fields_ = [("is_update_way_R2L", C.c_int), ("factor_format", C.c_int), ("packing_RL", C.c_int),
                 ("no_normalization", C.c_int), ("no_lambda", C.c_int), ("norm2_max_iter", C.c_int),
                 ("norm2_threshold", C.c_double), ("constant_step_size", C.c_int), ("step_size", C.c_double),
-                ("is_fact_side_left", C.c_int), ("record_traj", C.c_int)]
+                ("is_fact_side_left", C.c_int), ("record_traj", C.c_int), ("use_grad1", C.c_int)]
 
 
 VP = C.c_void_p

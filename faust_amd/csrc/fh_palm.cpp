@@ -4,6 +4,7 @@
 //   palm4msa2        src/algorithm/factorization/faust_palm4msa2020.hpp:47-300
 //   update_fact      faust_palm4msa2020.hpp:528-676
 //   grad2            faust_palm4msa2020.hpp:363-439 (the CPU path: dense D)
+//   grad1            faust_palm4msa2020.hpp:304-361 (USE_GRAD1 / the GPU2 path)
 //   update_lambda    faust_palm4msa2020.hpp:442-493
 //   hierarchical     src/algorithm/factorization/faust_hierarchical.hpp:1-150
 // Factors are shared_ptr device matrices so that the left/right partial
@@ -221,6 +222,45 @@ static void grad2(const Chain& pL, const Chain& pR, const DMat& A, double lambda
   sub_f64(D.d.as<double>(), g.d.as<double>(), (int64_t)D.m * D.n, st);
 }
 
+// compute_n_apply_grad1 (faust_palm4msa2020.hpp:304-361), taken by palm4msa2
+// when USE_GRAD1 is set and always on GPU2 (:633-635):
+//   tmp = lambda * dynprog(L, S_j, R) - A;
+//   no L: D += -lambda/c tmp R^T; else tmp = tmp R^T, then D += -lambda/c L^T tmp.
+// A one-factor side is used as stored; a longer one is multiplied out with
+// dynprog (the PALM product mode).
+static void grad1(const Chain& pL, const Chain& pR, const DMatP& Sj, const DMat& A, double lambda, double c,
+                  DMat& D, const fh_stop_crit& sc, double& error, hipStream_t st) {
+  Chain lsr(pL.begin(), pL.end());
+  lsr.push_back(Sj);
+  lsr.insert(lsr.end(), pR.begin(), pR.end());
+  DMatP tmp = lsr.size() == 1 ? dense_of(*lsr[0], st) : dynprog_multiply(lsr, st);
+  const int64_t len = (int64_t)tmp->m * tmp->n;
+  scale_f64(tmp->d.as<double>(), len, nullptr, lambda, 0, st);
+  sub_f64(tmp->d.as<double>(), A.d.as<double>(), len, st);
+  if (sc.is_criterion_error) error = fro_norm(*tmp, st);
+  auto side = [&](const Chain& ch, DMatP& keep) -> const DMat* {
+    if (ch.size() == 1) return ch[0].get();
+    keep = dynprog_multiply(ch, st);
+    return keep.get();
+  };
+  const double a = -lambda / c;
+  DMatP Rk, Lk;
+  if (!pR.empty()) {
+    const DMat* R = side(pR, Rk);
+    if (pL.empty()) {
+      matmul_f64(*tmp, 'N', *R, 'T', a, 1.0, D, st);
+      return;
+    }
+    auto t2 = std::make_shared<DMat>();
+    matmul_f64(*tmp, 'N', *R, 'T', 1.0, 0.0, *t2, st);
+    tmp = t2;
+  }
+  if (!pL.empty()) {
+    const DMat* L = side(pL, Lk);
+    matmul_f64(*L, 'T', *tmp, 'N', a, 1.0, D, st);
+  }
+}
+
 // update_lambda (faust_palm4msa2020.hpp:442-493): lambda = tr(A^T S)/||S||_F^2
 static void update_lambda(const Chain& S, const std::vector<Chain>& pL, const std::vector<Chain>& pR, const DMat& A,
                           double& lambda, hipStream_t st) {
@@ -381,7 +421,10 @@ static void palm4msa2(Palm& P, Chain& S, double& lambda, hipStream_t st) {
         c = 1.001 * lambda * lambda * nR * nR * nL * nL;
       }
       DMatP D = dense_of(*S[f_id], st);
-      grad2(pL[f_id], pR[f_id], A, lambda, c, *S[f_id], *D, P.sc, error, st);
+      if (p.use_grad1)
+        grad1(pL[f_id], pR[f_id], S[f_id], A, lambda, c, *D, P.sc, error, st);
+      else
+        grad2(pL[f_id], pR[f_id], A, lambda, c, *S[f_id], *D, P.sc, error, st);
       S[f_id] = project_gen(cons[f_id], P.cons[f_id]->supp.as<double>(), D, p.factor_format, st);
       if (p.is_update_way_R2L) {
         if (f_id > 0) {

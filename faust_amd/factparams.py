@@ -1,6 +1,8 @@
 """Parameter classes of the factorization algorithms (pyfaust.factparams
 counterpart; wrapper/python/pyfaust/factparams.py).  Only the constraint
 types on the hot path are provided: sp, spcol, splin, splincol, supp."""
+import os
+
 import numpy as np
 
 from ._lib import PROX
@@ -156,6 +158,14 @@ class ParamsFact:
         p.step_size = self.step_size
         p.is_fact_side_left = int(is_fact_side_left)
         p.record_traj = int(record_traj)
+        # palm4msa2 reads USE_GRAD1 from the environment (faust_palm4msa2020.hpp:83-86)
+        g1 = getattr(self, "use_grad1", None)
+        if g1 is None:
+            try:
+                g1 = bool(int(os.environ.get("USE_GRAD1", "0")))
+            except ValueError:
+                g1 = False
+        p.use_grad1 = int(g1)
         return p
 
 

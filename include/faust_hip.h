@@ -156,6 +156,8 @@ typedef struct {
   double step_size;
   int is_fact_side_left;  /* hierarchical */
   int record_traj;        /* record (call, it, rel_err, lambda) after each iteration */
+  int use_grad1;          /* gradient as compute_n_apply_grad1 (USE_GRAD1; the reference's GPU2
+                             path), else compute_n_apply_grad2 (its CPU default) */
 } fh_palm_params;
 
 typedef struct fh_palm_result_s *fh_palm_result_t;

@@ -50,7 +50,7 @@ class or_params(C.Structure):
                 ("packing_RL", C.c_int), ("no_normalization", C.c_int),
                 ("no_lambda", C.c_int), ("norm2_max_iter", C.c_int),
                 ("norm2_threshold", C.c_double), ("constant_step_size", C.c_int),
-                ("step_size", C.c_double), ("is_fact_side_left", C.c_int)]
+                ("step_size", C.c_double), ("is_fact_side_left", C.c_int), ("use_grad1", C.c_int)]
 
 
 def build():
@@ -212,6 +212,7 @@ def _params(**kw):
     p.constant_step_size = int(kw.get("constant_step_size", False))
     p.step_size = float(kw.get("step_size", 1e-16))
     p.is_fact_side_left = int(kw.get("is_fact_side_left", False))
+    p.use_grad1 = int(kw.get("use_grad1", False))
     return p
 
 

@@ -702,6 +702,42 @@ void grad2(const Chain& pL, const Chain& pR, const Mat& A, double lambda, double
   for (size_t i = 0; i < D.d.size(); ++i) D.d[i] -= g.d[i];
 }
 
+// grad1: compute_n_apply_grad1 (faust_palm4msa2020.hpp:304-361), the form
+// palm4msa2 takes with USE_GRAD1 set and always on GPU2 (:633-635):
+// tmp = lambda * dynprog(L, S_j, R) - A; tmp = tmp R^T (or D += -lambda/c tmp R^T
+// when there is no L); D += -lambda/c L^T tmp.  A packed one-factor side is
+// used as stored, otherwise its product is formed (dynprog, the PALM prod_mod).
+void grad1(const Chain& pL, const Chain& pR, const MatP& Sj, const Mat& A, double lambda, double c, Mat& D,
+           const or_stop& sc, double& error) {
+  std::vector<MatP> lsr(pL.f.begin(), pL.f.end());
+  lsr.push_back(Sj);
+  lsr.insert(lsr.end(), pR.f.begin(), pR.f.end());
+  Mat tmp = lsr.size() == 1 ? dense_of(*lsr[0]) : dynprog_multiply(lsr);
+  for (size_t i = 0; i < tmp.d.size(); ++i) tmp.d[i] = tmp.d[i] * lambda;
+  for (size_t i = 0; i < tmp.d.size(); ++i) tmp.d[i] = tmp.d[i] - A.d[i];
+  if (sc.is_criterion_error) error = fro_norm(tmp);
+  auto side = [](const Chain& ch, Mat& keep) -> const Mat* {
+    if (ch.f.size() == 1) return ch.f[0].get();
+    keep = dynprog_multiply(ch.f);
+    return &keep;
+  };
+  const double a = -lambda / c;
+  Mat Rk, Lk;
+  if (!pR.f.empty()) {
+    const Mat* R = side(pR, Rk);
+    if (pL.f.empty()) {
+      matmul(tmp, 'N', *R, 'T', a, 1.0, D);
+      return;
+    }
+    Mat t2 = matmul_new(tmp, 'N', *R, 'T');
+    tmp = std::move(t2);
+  }
+  if (!pL.f.empty()) {
+    const Mat* L = side(pL, Lk);
+    matmul(*L, 'T', tmp, 'N', a, 1.0, D);
+  }
+}
+
 // update_lambda (faust_palm4msa2020.hpp:442-493)
 void update_lambda(const Chain& S, const std::vector<Chain>& pL, const std::vector<Chain>& pR, const Mat& A,
                    double& lambda) {
@@ -839,7 +875,10 @@ void palm4msa2(Palm& P, Chain& S, double& lambda) {
         c = 1.001 * lambda * lambda * nR * nR * nL * nL;
       }
       Mat D = dense_of(*S.f[f_id]);
-      grad2(pL[f_id], pR[f_id], A, lambda, c, D, P.sc, error);
+      if (p.use_grad1)
+        grad1(pL[f_id], pR[f_id], S.f[f_id], A, lambda, c, D, P.sc, error);
+      else
+        grad2(pL[f_id], pR[f_id], A, lambda, c, D, P.sc, error);
       const or_cons& cons = P.cons[f_id];
       if (p.factor_format == 2) {
         S.f[f_id] = project_gen(cons, std::move(D));

@@ -74,6 +74,7 @@ typedef struct {
   int constant_step_size;
   double step_size;
   int is_fact_side_left; /* hierarchical only */
+  int use_grad1;         /* compute_n_apply_grad1 (USE_GRAD1, and the GPU2 path) */
 } or_params;
 
 const char *or_last_error(void);

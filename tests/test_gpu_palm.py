@@ -61,3 +61,29 @@ def test_hierarchical_hadamard_wide_power(gpu, monkeypatch):
     fac, resc, kw = oracle.wht_params(5, 20)
     ref = oracle.hierarchical(H, fac, resc, record_traj=True, **kw)
     _check_same(res, ref)
+
+
+@pytest.mark.parametrize("R2L", [False, True])
+@pytest.mark.parametrize("fmt", ["dynamic", "dense"])
+def test_palm4msa_grad1_bitexact(gpu, R2L, fmt):
+    # the USE_GRAD1 gradient (the reference's GPU2 form) against the oracle's
+    rng = np.random.default_rng(22)
+    A = np.asfortranarray(rng.standard_normal((32, 24)))
+    cons = [factparams.ConstraintInt("splin", 32, 16, 4), factparams.ConstraintInt("sp", 16, 16, 64),
+            factparams.ConstraintInt("spcol", 16, 24, 3)]
+    p = factparams.ParamsPalm4MSA(cons, factparams.StoppingCriterion(num_its=12), is_update_way_R2L=R2L,
+                                  factor_format=fmt)
+    p.use_grad1 = True
+    res = fact.palm4msa(A, p, ret_raw=True)
+    ref = oracle.palm4msa(A, [("splin", 4, 32, 16), ("sp", 64, 16, 16), ("spcol", 3, 16, 24)], num_its=12,
+                          record_traj=True, is_update_way_R2L=R2L, factor_format=fmt, use_grad1=True)
+    _check_same(res, ref)
+
+
+def test_hierarchical_grad1_bitexact(gpu, monkeypatch):
+    monkeypatch.setenv("USE_GRAD1", "1")
+    H = oracle.hadamard(32)
+    res = fact.hierarchical(H, factparams.ParamsHierarchicalWHT(5, num_its=15), ret_raw=True)
+    fac, resc, kw = oracle.wht_params(5, 15)
+    ref = oracle.hierarchical(H, fac, resc, record_traj=True, use_grad1=True, **kw)
+    _check_same(res, ref)

@@ -211,3 +211,20 @@ def test_palm_objective_decreases_on_average():
     e = r.traj[:, 2]
     assert e[-1] < e[0]
     assert np.all(e > 0) and np.all(e < 2)
+
+
+def test_grad1_agrees_with_grad2():
+    # compute_n_apply_grad1 and compute_n_apply_grad2 are two groupings of the
+    # same update; the restatement of grad1 must follow grad2 to rounding
+    rng = np.random.default_rng(21)
+    A = np.asfortranarray(rng.standard_normal((32, 24)))
+    cons = [("splin", 4, 32, 16), ("sp", 64, 16, 16), ("spcol", 3, 16, 24)]
+    for R2L in (False, True):
+        for packing in (True, False):
+            kw = dict(num_its=15, record_traj=True, is_update_way_R2L=R2L, packing_RL=packing)
+            r2 = oracle.palm4msa(A, cons, **kw)
+            r1 = oracle.palm4msa(A, cons, use_grad1=True, **kw)
+            assert abs(r1.lam - r2.lam) <= 1e-12 * abs(r2.lam)
+            assert np.abs(np.array(r1.traj)[:, 2] - np.array(r2.traj)[:, 2]).max() < 1e-12
+            for a, b in zip(r1.factors, r2.factors):
+                assert np.abs(a - b).max() < 1e-12
