@@ -13,6 +13,7 @@
 #include <chrono>
 #include <map>
 #include <numeric>
+#include <type_traits>
 
 #include "fh_api_common.hpp"
 #include "fh_batched.hpp"
@@ -38,9 +39,9 @@ struct Batch {
     int njobs = 0;
   };
   std::vector<Stage> stages;
-  DBuf tiles32, tiles64;
-  int nt32 = 0, nt64 = 0;
-  std::vector<int> tile_of;  // per target: 32 or 64
+  DBuf tiles32, tiles64, tilesM;
+  int nt32 = 0, nt64 = 0, ntM = 0;
+  std::vector<int> tile_of;  // per target: 32 or 64 (VALU), BGEMM_MFMA (FP32 MFMA, 64 x 64)
   int prox_mode = 0;
   int maxit = 100;
   double thr = 1e-6;
@@ -97,19 +98,23 @@ static void batch_alloc(Batch<T>& B, const BatchSpec& sp, bool masks) {
 
 template <class T>
 static void build_tiles(Batch<T>& B) {
-  std::vector<int3> t32, t64;
+  std::vector<int3> t32, t64, tm;
   for (int b = 0; b < B.sp.nt; ++b) {
-    const int tile = B.tile_of[b], n = B.sp.n[b];
+    const int code = B.tile_of[b], n = B.sp.n[b];
+    const int tile = code == 32 ? 32 : 64;
     const int nt = (n + tile - 1) / tile;
+    std::vector<int3>& l = code == 32 ? t32 : (code == 64 ? t64 : tm);
     for (int i = 0; i < nt; ++i)
-      for (int j = 0; j < nt; ++j) (tile == 32 ? t32 : t64).push_back(make_int3(b, i, j));
+      for (int j = 0; j < nt; ++j) l.push_back(make_int3(b, i, j));
   }
-  B.nt32 = (int)t32.size();
-  B.nt64 = (int)t64.size();
-  B.tiles32.alloc(sizeof(int3) * std::max<size_t>(t32.size(), 1));
-  B.tiles64.alloc(sizeof(int3) * std::max<size_t>(t64.size(), 1));
-  if (!t32.empty()) FH_HIP(hipMemcpy(B.tiles32.p, t32.data(), sizeof(int3) * t32.size(), hipMemcpyHostToDevice));
-  if (!t64.empty()) FH_HIP(hipMemcpy(B.tiles64.p, t64.data(), sizeof(int3) * t64.size(), hipMemcpyHostToDevice));
+  auto up = [](DBuf& d, int& cnt, const std::vector<int3>& v) {
+    cnt = (int)v.size();
+    d.alloc(sizeof(int3) * std::max<size_t>(v.size(), 1));
+    if (!v.empty()) FH_HIP(hipMemcpy(d.p, v.data(), sizeof(int3) * v.size(), hipMemcpyHostToDevice));
+  };
+  up(B.tiles32, B.nt32, t32);
+  up(B.tiles64, B.nt64, t64);
+  up(B.tilesM, B.ntM, tm);
 }
 
 template <class T>
@@ -225,6 +230,7 @@ static void run_stage(Batch<T>& B, const typename Batch<T>::Stage& s, hipStream_
     case 0:
       launch_bgemm<T>(s.jobs.template as<BGemmJob<T>>(), B.tiles32.template as<int3>(), B.nt32, 32, sc, st);
       launch_bgemm<T>(s.jobs.template as<BGemmJob<T>>(), B.tiles64.template as<int3>(), B.nt64, 64, sc, st);
+      launch_bgemm<T>(s.jobs.template as<BGemmJob<T>>(), B.tilesM.template as<int3>(), B.ntM, BGEMM_MFMA, sc, st);
       break;
     case 1:
     case 2:
@@ -245,12 +251,21 @@ static void run_iteration(Batch<T>& B, hipStream_t st) {
 }
 
 // Autotuner: for every distinct target size, time one full-size GEMM stage
-// with 32- and 64-wide tiles and keep the faster.
+// with 32- and 64-wide VALU tiles and (FP32) the 64 x 64 MFMA tile, and keep
+// the fastest.
 template <class T>
 static void autotune(Batch<T>& B, hipStream_t st) {
   std::map<int, std::vector<int>> by_n;
   for (int b = 0; b < B.sp.nt; ++b) by_n[B.sp.n[b]].push_back(b);
   B.tile_of.assign(B.sp.nt, 64);
+  // FH_BGEMM_TILE=32 / 64 / 65 (MFMA, FP32 only) fixes the choice
+  if (const char* e = getenv("FH_BGEMM_TILE")) {
+    const int code = atoi(e);
+    if (code == 32 || code == 64 || (code == BGEMM_MFMA && std::is_same<T, float>::value)) {
+      B.tile_of.assign(B.sp.nt, code);
+      return;
+    }
+  }
   // a GEMM stage over the members of one size class
   const typename Batch<T>::Stage* g = nullptr;
   for (auto& s : B.stages)
@@ -263,7 +278,10 @@ static void autotune(Batch<T>& B, hipStream_t st) {
   for (auto& kv : by_n) {
     float best = 1e30f;
     int best_tile = 64;
-    for (int tile : {32, 64}) {
+    std::vector<int> cands{32, 64};
+    if (std::is_same<T, float>::value) cands.push_back(BGEMM_MFMA);
+    for (int code : cands) {
+      const int tile = code == 32 ? 32 : 64;
       std::vector<int3> tl;
       for (int b : kv.second) {
         const int nt = (kv.first + tile - 1) / tile;
@@ -275,14 +293,14 @@ static void autotune(Batch<T>& B, hipStream_t st) {
       float ms = 0.f;
       for (int rep = 0; rep < 3; ++rep) {
         FH_HIP(hipEventRecord(e0, st));
-        launch_bgemm<T>(g->jobs.template as<BGemmJob<T>>(), dt.as<int3>(), (int)tl.size(), tile, sc, st);
+        launch_bgemm<T>(g->jobs.template as<BGemmJob<T>>(), dt.as<int3>(), (int)tl.size(), code, sc, st);
         FH_HIP(hipEventRecord(e1, st));
         FH_HIP(hipEventSynchronize(e1));
         FH_HIP(hipEventElapsedTime(&ms, e0, e1));
       }
       if (ms < best) {
         best = ms;
-        best_tile = tile;
+        best_tile = code;
       }
     }
     for (int b : kv.second) B.tile_of[b] = best_tile;

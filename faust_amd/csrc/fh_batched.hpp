@@ -6,6 +6,7 @@
 namespace fh {
 
 constexpr int BATCH_NMAX = 2048;  // largest target edge (power-iteration vectors live in LDS)
+constexpr int BGEMM_MFMA = 65;    // tile code of the FP32 MFMA grouped GEMM (64 x 64 tiles)
 
 // C_out = alpha_b * op(A) op(B) + beta * C_in on n x n column-major blocks;
 // alpha mode 0 -> 1, 1 -> lambda_b, 2 -> -lambda_b / c_b with

@@ -12,6 +12,7 @@
 // (faust_palm4msa2020.hpp:47-300 with the ParamsPalm4MSA defaults), all
 // factors square and kept dense.
 #include <algorithm>
+#include <type_traits>
 #include <vector>
 
 #include "fh_api_common.hpp"
@@ -95,6 +96,63 @@ __global__ __launch_bounds__(256) void k_bgemm(const BGemmJob<T>* __restrict__ j
       if (J.beta_mode) v = fma(beta, J.Cin[o], v);
       J.Cout[o] = v;
     }
+  }
+}
+
+// FP32 on the matrix cores: v_mfma_f32_32x32x2_f32, workgroup tile 64 x 64
+// (four waves of 32 x 32), K staged through LDS 32 at a time.  Operand
+// lanes: A lane l holds op(A)[row l % 32][k + l / 32], B lane l holds
+// op(B)[k + l / 32][col l % 32]; the accumulator holds rows
+// 8*(r/4) + 4*(l/32) + r%4 of column l % 32 (r = 0..15).
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+__global__ __launch_bounds__(256) void k_bgemm_mfma_f32(const BGemmJob<float>* __restrict__ jobs,
+                                                        const int3* __restrict__ tiles, BScal sc) {
+  constexpr int TILE = 64, KS = 32;
+  __shared__ float As[KS][TILE + 4];
+  __shared__ float Bs[KS][TILE + 4];
+  const int3 tl = tiles[blockIdx.x];
+  const BGemmJob<float> J = jobs[tl.x];
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int wr = wave & 1, wc = wave >> 1;
+  const int i0 = tl.y * TILE, j0 = tl.z * TILE;
+  f32x16 acc = {};
+  for (int k0 = 0; k0 < J.k; k0 += KS) {
+    for (int e = t; e < TILE * KS; e += 256) {
+      int ii, kk;
+      if (!J.transA) { ii = e % TILE; kk = e / TILE; } else { kk = e % KS; ii = e / KS; }
+      const int gi = i0 + ii, gk = k0 + kk;
+      As[kk][ii] = (gi < J.m && gk < J.k) ? (J.transA ? J.A[(size_t)gi * J.k + gk] : J.A[(size_t)gk * J.m + gi])
+                                          : 0.0f;
+    }
+    for (int e = t; e < TILE * KS; e += 256) {
+      int jj, kk;
+      if (J.transB) { jj = e % TILE; kk = e / TILE; } else { kk = e % KS; jj = e / KS; }
+      const int gj = j0 + jj, gk = k0 + kk;
+      Bs[kk][jj] = (gj < J.n && gk < J.k) ? (J.transB ? J.B[(size_t)gk * J.n + gj] : J.B[(size_t)gj * J.k + gk])
+                                          : 0.0f;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < KS; kk += 2) {
+      const float a = As[kk + (lane >> 5)][wr * 32 + (lane & 31)];
+      const float b = Bs[kk + (lane >> 5)][wc * 32 + (lane & 31)];
+      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc, 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  const float alpha = job_alpha<float>(J.alpha_mode, J.target, sc);
+  const float beta = J.beta_mode == 0 ? 0.0f : (J.beta_mode == 1 ? -1.0f : 1.0f);
+  const int gj = j0 + wc * 32 + (lane & 31);
+  if (gj >= J.n) return;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int gi = i0 + wr * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+    if (gi >= J.m) continue;
+    const size_t o = (size_t)gj * J.m + gi;
+    float v = alpha * acc[r];
+    if (J.beta_mode) v = fmaf(beta, J.Cin[o], v);
+    J.Cout[o] = v;
   }
 }
 
@@ -381,6 +439,15 @@ void launch_mask_nonzero_f32(double* mask, const float* src, int64_t n, hipStrea
 template <class T>
 void launch_bgemm(const BGemmJob<T>* jobs, const int3* tiles, int ntiles, int tile, const BScal& sc, hipStream_t st) {
   if (ntiles == 0) return;
+  if (tile == BGEMM_MFMA) {
+    if constexpr (std::is_same<T, float>::value) {
+      hipLaunchKernelGGL(k_bgemm_mfma_f32, dim3(ntiles), dim3(256), 0, st, jobs, tiles, sc);
+      FH_LAUNCH_CHECK();
+      return;
+    } else {
+      throw Error(FH_ERR_ARG, "batched GEMM: the MFMA tile is FP32 only");
+    }
+  }
   if (tile == 32)
     hipLaunchKernelGGL((k_bgemm<T, 32>), dim3(ntiles), dim3(256), 0, st, jobs, tiles, sc);
   else

@@ -9,6 +9,7 @@ its FP64-verified relative error."""
 import numpy as np
 import pytest
 
+import faust_amd
 import oracle
 from faust_amd.batched import palm4msa_batched
 
@@ -88,3 +89,23 @@ def test_batched_supp_masks(gpu):
     ref = oracle.palm4msa(A, [("supp", 0, n, n, True, False, masks[0]), ("supp", 0, n, n, True, False, masks[1])],
                           num_its=30)
     assert abs(res.lambdas[0] - ref.lam) <= 1e-8 * abs(ref.lam)
+
+
+@pytest.mark.parametrize("tile", ["32", "64", "65"])
+def test_batched_gemm_tiles_agree(gpu, monkeypatch, tile):
+    # the 32/64 VALU tiles and the 64x64 FP32 MFMA tile run the same FP32
+    # iterations (sizes that are not tile multiples included); compared with
+    # the FP64 batched run of the same problem
+    monkeypatch.setenv("FH_BGEMM_TILE", tile)
+    rng = np.random.default_rng(9)
+    As, ks = [], []
+    for n in (64, 100, 160):
+        P = np.eye(n)
+        for _ in range(3):
+            P = P @ (faust_amd.rand_csr(n, n, 4, rng).toarray())
+        As.append(np.asfortranarray(P))
+        ks.append(4 * n)
+    r32 = palm4msa_batched(As, 3, k=ks, dtype="f32", num_its=30, refine_its=0, autotune=True)
+    r64 = palm4msa_batched(As, 3, k=ks, dtype="f64", num_its=30, refine_its=0, autotune=False)
+    assert np.all(np.isfinite(r32.relerr))
+    assert np.all(np.abs(r32.relerr - r64.relerr) <= 0.02 * r64.relerr + 1e-5)
