@@ -32,6 +32,9 @@ __device__ __forceinline__ T job_alpha(int mode, int b, const BScal& s) {
   if (mode == 1) return (T)lam;
   const double nr = s.nR[b], nl = s.nL[b];
   const double c = 1.001 * lam * lam * nr * nr * nl * nl;
+  // update_fact throws when a 2-norm is zero or NaN; a batched target whose
+  // step is not finite and positive keeps its factor instead
+  if (!(c > 0.0) || !isfinite(c) || !isfinite(lam)) return T(0);
   return (T)(-lam / c);
 }
 

@@ -30,6 +30,7 @@ namespace fh {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef unsigned short u16;
+typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
 
 __device__ __forceinline__ float bf2f(u16 h) { return __uint_as_float(((unsigned)h) << 16); }
 __device__ __forceinline__ u16 f2bf(float f) {
@@ -147,14 +148,37 @@ __global__ __launch_bounds__(256) void k_bsr_mfma(int nbr, int p, const int* __r
 // flight behind that latency.  Workgroup = 4 block-rows x 32*NC columns.
 constexpr int BSR_ELL_MAX = 16;
 
-template <bool SPLIT, bool IN_TILED, bool OUT_TILED, int NC>
+// Workgroup -> (column panel, quad of block rows).  split == 1: consecutive
+// ids walk the panels (each XCD keeps fixed panels and re-reads every block);
+// split == 2: the XCDs form 8/2 column groups x 2 halves of the block rows,
+// so every block is read by half as many XCDs while each X panel slice still
+// stays in one XCD's L2.
+__device__ __forceinline__ void bsr_map(int bid, int P, int Q, int split, int& panel, int& quad) {
+  if (split == 1) {
+    panel = bid % P;
+    quad = bid / P;
+    return;
+  }
+  const int G = 8 / split, ppg = P / G;
+  const int x = bid % 8, slot = bid / 8;
+  panel = (x % G) * ppg + slot % ppg;
+  quad = (x / G) * (Q / split) + slot / ppg;
+}
+
+template <class V>
+__device__ __forceinline__ V ld_stream(const V* p, bool nt) {
+  return nt ? __builtin_nontemporal_load(p) : *p;
+}
+
+template <bool SPLIT, bool IN_TILED, bool OUT_TILED, int NC, bool NT>
 __global__ __launch_bounds__(256) void k_bsr_ell(int nbr, int p, int W, const int2* __restrict__ ell,
                                                  const u16* __restrict__ bdat, const u16* __restrict__ X, int ldx,
                                                  u16* __restrict__ Y, int ldy, float alpha, float beta,
-                                                 int npanels) {
+                                                 int npanels, int nquads, int xsplit) {
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int panel = blockIdx.x % npanels, quad = blockIdx.x / npanels;
+  int panel, quad;
+  bsr_map(blockIdx.x, npanels, nquads, xsplit, panel, quad);
   const int br = quad * 4 + wave;
   if (br >= nbr) return;
   const int kh = lane >> 5;
@@ -185,7 +209,7 @@ __global__ __launch_bounds__(256) void k_bsr_ell(int nbr, int p, int W, const in
         for (int h = 0; h < 2; ++h) {
 #pragma unroll
           for (int pl = 0; pl < PLANES; ++pl)
-            a[q][h][pl] = *reinterpret_cast<const bf16x8*>(blk + pl * BLK + h * 16);
+            a[q][h][pl] = ld_stream(reinterpret_cast<const bf16x8*>(blk + pl * BLK + h * 16), NT);
 #pragma unroll
           for (int c = 0; c < NC; ++c) {
             const int j = jok[c] ? jc[c] : 0;
@@ -228,17 +252,63 @@ __global__ __launch_bounds__(256) void k_bsr_ell(int nbr, int p, int W, const in
       uint2 w;
       w.x = (unsigned)o[0] | ((unsigned)o[1] << 16);
       w.y = (unsigned)o[2] | ((unsigned)o[3] << 16);
-      *reinterpret_cast<uint2*>(ycol + r0) = w;
+      if (NT) {
+        u32x2 wv = {w.x, w.y};
+        __builtin_nontemporal_store(wv, reinterpret_cast<u32x2*>(ycol + r0));
+      } else {
+        *reinterpret_cast<uint2*>(ycol + r0) = w;
+      }
     }
   }
 }
 
 // Which BSR kernel: FH_BSR_KERNEL=0 forces the row-pointer kernel, 2 / 4 the
-// ELL kernel with that many 32-column tiles per wave; default ELL with 4.
-static int bsr_kernel_choice() {
+// ELL kernel with that many 32-column tiles per wave.  Default (measured on
+// 8192^2, 4 blocks per block row, 512 columns): 2 for BF16 blocks (11.3 us
+// vs 12.6 us per factor), 4 for FP32 hi/lo blocks (13.8 vs 15.2 us), whose
+// doubled block bytes favour fewer column panels re-reading them.
+static int bsr_kernel_choice(bool split) {
   const char* e = getenv("FH_BSR_KERNEL");
   if (e && (e[0] == '0' || e[0] == '2' || e[0] == '4')) return e[0] - '0';
-  return 4;
+  return split ? 4 : 2;
+}
+
+template <bool S, bool I, bool O, int NC, bool NT>
+static void bsr_ell_go(dim3 grid, hipStream_t st, int nbr, int p, int W, const int2* ell, const u16* dat,
+                       const u16* X, int ldx, u16* Y, int ldy, float alpha, float beta, int P, int Q, int xs) {
+  hipLaunchKernelGGL((k_bsr_ell<S, I, O, NC, NT>), grid, dim3(256), 0, st, nbr, p, W, ell, dat, X, ldx, Y, ldy,
+                     alpha, beta, P, Q, xs);
+}
+
+template <bool S, bool I, bool O>
+static void bsr_ell_nc(int nc, bool nt, dim3 grid, hipStream_t st, int nbr, int p, int W, const int2* ell,
+                       const u16* dat, const u16* X, int ldx, u16* Y, int ldy, float alpha, float beta, int P, int Q,
+                       int xs) {
+  if (nc == 4) {
+    if (nt) bsr_ell_go<S, I, O, 4, true>(grid, st, nbr, p, W, ell, dat, X, ldx, Y, ldy, alpha, beta, P, Q, xs);
+    else bsr_ell_go<S, I, O, 4, false>(grid, st, nbr, p, W, ell, dat, X, ldx, Y, ldy, alpha, beta, P, Q, xs);
+  } else {
+    if (nt) bsr_ell_go<S, I, O, 2, true>(grid, st, nbr, p, W, ell, dat, X, ldx, Y, ldy, alpha, beta, P, Q, xs);
+    else bsr_ell_go<S, I, O, 2, false>(grid, st, nbr, p, W, ell, dat, X, ldx, Y, ldy, alpha, beta, P, Q, xs);
+  }
+}
+
+static void bsr_ell_launch(bool split, bool in_tiled, bool out_tiled, int nc, bool nt, dim3 grid, hipStream_t st,
+                           int nbr, int p, int W, const int2* ell, const u16* dat, const u16* X, int ldx, u16* Y,
+                           int ldy, float alpha, float beta, int P, int Q, int xs) {
+  const int sel = (split ? 4 : 0) | (in_tiled ? 2 : 0) | (out_tiled ? 1 : 0);
+#define FH_ELL(S, I, O) bsr_ell_nc<S, I, O>(nc, nt, grid, st, nbr, p, W, ell, dat, X, ldx, Y, ldy, alpha, beta, P, Q, xs)
+  switch (sel) {
+    case 0: FH_ELL(false, false, false); break;
+    case 1: FH_ELL(false, false, true); break;
+    case 2: FH_ELL(false, true, false); break;
+    case 3: FH_ELL(false, true, true); break;
+    case 4: FH_ELL(true, false, false); break;
+    case 5: FH_ELL(true, false, true); break;
+    case 6: FH_ELL(true, true, false); break;
+    default: FH_ELL(true, true, true); break;
+  }
+#undef FH_ELL
 }
 
 // in_tiled / out_tiled select the activation layouts (see above); ldx/ldy are
@@ -255,35 +325,19 @@ static void bsr_mul(const DMat& B, char op, const void* X, int ldx, int p, void*
   const int quads = (nbr + 3) / 4;
   const u16* dat = tr ? B.tbdat.as<u16>() : B.bdat.as<u16>();
   const int W = tr ? B.tell_w : B.ell_w;
-  const int choice = bsr_kernel_choice();
+  const int choice = bsr_kernel_choice(B.split);
   if (W > 0 && choice != 0) {
     const int2* ell = (tr ? B.tell : B.ell).as<int2>();
     const int nc = choice;
     const int npanels = (p + 32 * nc - 1) / (32 * nc);
     const dim3 grid((unsigned)(npanels * quads));
-#define FH_ELL_LAUNCH(S, I, O, NC)                                                                             \
-  hipLaunchKernelGGL((k_bsr_ell<S, I, O, NC>), grid, dim3(256), 0, st, nbr, p, W, ell, dat, (const u16*)X, ldx, \
-                     (u16*)Y, ldy, alpha, beta, npanels)
-    const int sel = (B.split ? 8 : 0) | (in_tiled ? 4 : 0) | (out_tiled ? 2 : 0) | (nc == 4 ? 1 : 0);
-    switch (sel) {
-      case 0: FH_ELL_LAUNCH(false, false, false, 2); break;
-      case 1: FH_ELL_LAUNCH(false, false, false, 4); break;
-      case 2: FH_ELL_LAUNCH(false, false, true, 2); break;
-      case 3: FH_ELL_LAUNCH(false, false, true, 4); break;
-      case 4: FH_ELL_LAUNCH(false, true, false, 2); break;
-      case 5: FH_ELL_LAUNCH(false, true, false, 4); break;
-      case 6: FH_ELL_LAUNCH(false, true, true, 2); break;
-      case 7: FH_ELL_LAUNCH(false, true, true, 4); break;
-      case 8: FH_ELL_LAUNCH(true, false, false, 2); break;
-      case 9: FH_ELL_LAUNCH(true, false, false, 4); break;
-      case 10: FH_ELL_LAUNCH(true, false, true, 2); break;
-      case 11: FH_ELL_LAUNCH(true, false, true, 4); break;
-      case 12: FH_ELL_LAUNCH(true, true, false, 2); break;
-      case 13: FH_ELL_LAUNCH(true, true, false, 4); break;
-      case 14: FH_ELL_LAUNCH(true, true, true, 2); break;
-      default: FH_ELL_LAUNCH(true, true, true, 4); break;
-    }
-#undef FH_ELL_LAUNCH
+    const char* e_nt = getenv("FH_BSR_NT");
+    const bool nt = e_nt && e_nt[0] == '1';
+    int xsplit = 1;
+    if (const char* e = getenv("FH_BSR_XSPLIT")) xsplit = (e[0] == '2') ? 2 : 1;
+    if (xsplit == 2 && ((npanels * quads) % 8 != 0 || npanels % 4 != 0 || quads % 2 != 0)) xsplit = 1;
+    bsr_ell_launch(B.split, in_tiled, out_tiled, nc, nt, grid, st, nbr, p, W, ell, dat, (const u16*)X, ldx,
+                   (u16*)Y, ldy, alpha, beta, npanels, quads, xsplit);
     FH_LAUNCH_CHECK();
     return;
   }
@@ -482,10 +536,11 @@ __global__ __launch_bounds__(256) void k_csr_rm2_f32(int rows, int p, const int*
   }
 }
 
-// FH_CSR_RPT=1 selects the one-row-per-thread kernel
+// FH_CSR_RPT=2 selects the two-rows-per-thread kernel (measured slower on
+// 16384^2, 12 nnz/row, 256 columns: 18.9 vs 17.5 us per factor)
 static int csr_rows_per_thread() {
   const char* e = getenv("FH_CSR_RPT");
-  return (e && e[0] == '1') ? 1 : 2;
+  return (e && e[0] == '2') ? 2 : 1;
 }
 
 // column-major (rows x p, ld) -> row-major (rows x ldr), zero-padding columns p..ldr

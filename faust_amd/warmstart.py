@@ -130,8 +130,9 @@ class MLP:
         check(lib().fh_mlp_create(n_in, n_h1, n_h2, n_out, seed, C.byref(self.h)))
 
     def __del__(self):
-        if getattr(self, "h", None) and self.h.value and _lib._lib is not None:
-            _lib._lib.fh_mlp_free(self.h)
+        lib_ = getattr(_lib, "_lib", None) if _lib is not None else None
+        if getattr(self, "h", None) and self.h.value and lib_ is not None:
+            lib_.fh_mlp_free(self.h)
 
     def _stream(self):
         import torch
@@ -265,7 +266,7 @@ def compare_warm_start(As, masks, inits, lambdas, max_its=1000, check_every=25, 
     warm = palm4msa_batched(As, L, supports=sup, init=inits, lambda_init=lambdas, dtype=dtype, num_its=max_its,
                             refine_its=0, check_every=check_every, target_err=target_err, return_factors=False)
     t_warm = time.perf_counter() - t
-    fallback = warm.relerr > ref.relerr
+    fallback = ~(warm.relerr <= ref.relerr)  # a non-finite warm error falls back too
     final = np.where(fallback, ref.relerr, warm.relerr)
     it_w = np.where(fallback, ref.iters_to_target, warm.iters_to_target)
     return {"targets": len(As), "max_its": max_its, "target_err": target_err, "dtype": dtype,

@@ -126,11 +126,15 @@ def _ragged_bsr(rng, nbr, nbc, bs=32, wide_row=None):
 @pytest.mark.parametrize("kernel", ["0", "2", "4"])
 @pytest.mark.parametrize("wide", [None, 20])
 @pytest.mark.parametrize("op", ["N", "T"])
-def test_bsr_kernel_variants(gpu, monkeypatch, kernel, wide, op):
-    # row-pointer kernel, ELL with 64 / 128 columns per wave; ragged block
-    # rows, empty block rows and (wide) a block row beyond the ELL limit; a
-    # 3-factor chain exercises the tiled intermediates
+@pytest.mark.parametrize("xsplit_nt", ["1-0", "2-1"])
+def test_bsr_kernel_variants(gpu, monkeypatch, kernel, wide, op, xsplit_nt):
+    # row-pointer kernel, ELL with 64 / 128 columns per wave (with the XCD
+    # split of the block rows and streaming loads/stores); ragged block rows,
+    # empty block rows and (wide) a block row beyond the ELL limit; a 3-factor
+    # chain exercises the tiled intermediates
     monkeypatch.setenv("FH_BSR_KERNEL", kernel)
+    monkeypatch.setenv("FH_BSR_XSPLIT", xsplit_nt[0])
+    monkeypatch.setenv("FH_BSR_NT", xsplit_nt[2])
     rng = np.random.default_rng(11)
     facs = [_bsr_rounded(_ragged_bsr(rng, 24, 24, wide_row=wide)) for _ in range(3)]
     for p in (200, 8):
