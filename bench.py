@@ -42,6 +42,9 @@ def _parse():
                     help="skip the batched PALM4MSA comparison (512 targets, tuned vs plain FP64)")
     ap.add_argument("--hadamard", action="store_true",
                     help="also time hierarchical PALM4MSA on the 1024 Hadamard matrix, 100 iterations per level")
+    ap.add_argument("--warmstart", type=int, default=0, metavar="NTRAIN",
+                    help="also run the learned warm-start experiment (256x256 butterfly targets) with NTRAIN "
+                         "training targets and 1000 held-out ones")
     return ap.parse_args()
 
 
@@ -325,6 +328,9 @@ def main():
         out["palm"] = palm_batched_vs_plain(cpu=not args.no_cpu_baseline)
     if args.hadamard:
         out["palm_hadamard"] = palm_hadamard()
+    if args.warmstart:
+        from faust_amd import warmstart
+        out["warmstart"] = warmstart.experiment(ntrain=args.warmstart, log=lambda m: print(m, file=sys.stderr))
     print(json.dumps(out))
 
 

@@ -206,11 +206,17 @@ def palm_labels(As, gens, masks, num_its=20, chunk=512):
 
 
 def train_mlp(mlp, X, Y, epochs=10, batch=256, lr=1e-3, seed=0, log=None):
-    """Adam on the MSE between network output and labels.  X, Y: float32
-    CUDA tensors [N, n_in], [N, n_out]."""
+    """Adam on the MSE between network output and standardised labels (each
+    output's training mean and standard deviation are kept on the network
+    object and undone by `predicted_init`).  X, Y: float32 CUDA tensors
+    [N, n_in], [N, n_out]."""
     import torch
     g = torch.Generator(device="cpu").manual_seed(seed)
     N = X.shape[0]
+    mu = Y.mean(dim=0, keepdim=True)
+    sd = Y.std(dim=0, keepdim=True).clamp_min(1e-6)
+    mlp.label_mu, mlp.label_sd = mu, sd
+    Y = ((Y - mu) / sd).contiguous()
     losses = []
     for ep in range(epochs):
         perm = torch.randperm(N, generator=g).to(X.device)
@@ -237,7 +243,10 @@ def predicted_init(mlp, As, masks, batch=256):
     for b0 in range(0, len(As), batch):
         part = As[b0:b0 + batch]
         X = torch.from_numpy(np.stack([mlp_input(A) for A in part])).cuda()
-        Y = mlp.forward(X).cpu().numpy().astype(np.float64) / label_scale(n)
+        Y = mlp.forward(X)
+        if getattr(mlp, "label_mu", None) is not None:
+            Y = Y * mlp.label_sd + mlp.label_mu
+        Y = Y.cpu().numpy().astype(np.float64) / label_scale(n)
         for A, y in zip(part, Y):
             F = [f / max(np.linalg.norm(f), 1e-300) for f in unpack_values(y, masks)]
             P = np.eye(n)
@@ -275,5 +284,53 @@ def compare_warm_start(As, masks, inits, lambdas, max_its=1000, check_every=25, 
             if (ref.iters_to_target > 0).any() else None,
             "warm_median_its": float(np.median(it_w[it_w > 0])) if (it_w > 0).any() else None,
             "ref_median_err": float(np.median(ref.relerr)), "warm_median_err": float(np.median(final)),
+            "warm_median_err_before_fallback": float(np.median(np.where(np.isfinite(warm.relerr), warm.relerr,
+                                                                          np.inf))),
+            "warm_reached_before_fallback": int((warm.iters_to_target > 0).sum()),
             "fallbacks": int(fallback.sum()), "ref_wall_s": round(t_ref, 3), "warm_wall_s": round(t_warm, 3),
             "ref_ms_to_all": float(ref.times_ms[4]), "warm_ms_to_all": float(warm.times_ms[4])}
+
+
+def experiment(n=256, ntrain=16384, nhold=1000, epochs=40, label_its=5, max_its=1000, check_every=25,
+               target_err=1e-4, noise=1e-6, hidden=512, dtype="f32", seed=0x5EED, log=print):
+    """The learned-warm-start experiment end to end: butterfly targets, labels
+    from FP64 PALM4MSA, MLP training on the GPU, then PALM4MSA on held-out
+    targets from the predicted and from the reference initialisation."""
+    import time
+    import torch
+    rng = np.random.default_rng(seed)
+    masks = butterfly_supports(n)
+    L = len(masks)
+    t_gen = t_lab = 0.0
+    Xs, Ys, lerr = [], [], []
+    for c0 in range(0, ntrain, 512):  # training targets are streamed: only inputs and labels are kept
+        t0 = time.perf_counter()
+        part = [butterfly_sample(n, rng, noise) for _ in range(min(512, ntrain - c0))]
+        t_gen += time.perf_counter() - t0
+        t0 = time.perf_counter()
+        Yc, _, ec = palm_labels([a for a, _ in part], [g for _, g in part], masks, num_its=label_its)
+        t_lab += time.perf_counter() - t0
+        Xs.append(np.stack([mlp_input(a) for a, _ in part]))
+        Ys.append(Yc)
+        lerr.extend(ec.tolist())
+    hold = [butterfly_sample(n, rng, noise)[0] for _ in range(nhold)]
+    log(f"targets: gen {t_gen:.1f}s labels {t_lab:.1f}s (median label relerr {np.median(lerr):.2e})")
+    X = torch.from_numpy(np.concatenate(Xs)).cuda()
+    Yd = torch.from_numpy(np.concatenate(Ys)).cuda()
+    del Xs, Ys
+    mlp = MLP(n * n, hidden, hidden, 2 * n * L, seed=1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    losses = train_mlp(mlp, X, Yd, epochs=epochs, batch=256, lr=1e-3, log=log)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    del X, Yd
+    t0 = time.perf_counter()
+    inits, lam0 = predicted_init(mlp, hold, masks)
+    t_pred = time.perf_counter() - t0
+    res = compare_warm_start(hold, masks, inits, lam0, max_its=max_its, check_every=check_every,
+                             target_err=target_err, dtype=dtype)
+    res.update({"n": n, "factors": L, "noise_std": noise, "train_targets": ntrain, "label_palm_its": label_its,
+                "epochs": epochs, "final_train_mse": float(losses[-1]), "train_s": round(t_train, 2),
+                "predict_s": round(t_pred, 2), "label_s": round(t_lab, 1), "mlp": [n * n, hidden, hidden, 2 * n * L]})
+    return res
