@@ -112,7 +112,8 @@ struct DMat {
   DBuf brp, bci, bdat;
   DBuf tbrp, tbci, tbdat;
   // ELL view of the block rows (and of the transpose): ell_w slots per block
-  // row, slot = (block index into bdat, block column), (-1, 0) when empty;
+  // row (a multiple of 4), slot = (block index into bdat, block column); an
+  // empty slot points at the all-zero block bdat holds after its bnnz blocks.
   // ell_w == 0 when the widest block row is too wide for it.
   int ell_w = 0, tell_w = 0;
   DBuf ell, tell;

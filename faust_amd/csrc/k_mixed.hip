@@ -194,45 +194,48 @@ __global__ __launch_bounds__(256) void k_bsr_ell(int nbr, int p, int W, const in
   f32x16 acc[NC];
 #pragma unroll
   for (int c = 0; c < NC; ++c) acc[c] = f32x16{};
-  const int2* slots = ell + (size_t)br * W;
-  for (int g = 0; g < W; g += 4) {
-    int2 s[4];
+  // W is a multiple of 4 and empty slots point at an all-zero block, so every
+  // load below is unconditional: the whole group's loads are in flight
+  // together.  Columns beyond p read column 0 (any finite data): they only
+  // feed accumulator columns that are never stored.
+  const int4* slots = reinterpret_cast<const int4*>(ell + (size_t)br * W);
+  int j[NC];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) s[q] = (g + q < W) ? slots[g + q] : make_int2(-1, 0);
+  for (int c = 0; c < NC; ++c) j[c] = jok[c] ? jc[c] : 0;
+  for (int g = 0; g < W; g += 4) {
+    const int4 s01 = slots[g / 2], s23 = slots[g / 2 + 1];
+    const int bi[4] = {s01.x, s01.z, s23.x, s23.z};
+    const int bcol[4] = {s01.y, s01.w, s23.y, s23.w};
     bf16x8 a[4][2][PLANES], x[4][2][NC];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      if (s[q].x >= 0) {
-        const int bc = s[q].y;
-        const u16* blk = bdat + (size_t)s[q].x * BLK * PLANES + (lane & 31) * 32 + kh * 8;
+      const u16* blk = bdat + (size_t)bi[q] * BLK * PLANES + (lane & 31) * 32 + kh * 8;
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
+      for (int h = 0; h < 2; ++h) {
 #pragma unroll
-          for (int pl = 0; pl < PLANES; ++pl)
-            a[q][h][pl] = ld_stream(reinterpret_cast<const bf16x8*>(blk + pl * BLK + h * 16), NT);
+        for (int pl = 0; pl < PLANES; ++pl)
+          a[q][h][pl] = ld_stream(reinterpret_cast<const bf16x8*>(blk + pl * BLK + h * 16), NT);
 #pragma unroll
-          for (int c = 0; c < NC; ++c) {
-            const int j = jok[c] ? jc[c] : 0;
-            const u16* src = IN_TILED ? X + ((size_t)bc * p + j) * 32 + h * 16 + kh * 8
-                                      : X + (size_t)j * ldx + bc * 32 + h * 16 + kh * 8;
-            x[q][h][c] = jok[c] ? *reinterpret_cast<const bf16x8*>(src) : bf16x8{};
-          }
+        for (int c = 0; c < NC; ++c) {
+          const u16* src = IN_TILED ? X + ((size_t)bcol[q] * p + j[c]) * 32 + h * 16 + kh * 8
+                                    : X + (size_t)j[c] * ldx + bcol[q] * 32 + h * 16 + kh * 8;
+          x[q][h][c] = *reinterpret_cast<const bf16x8*>(src);
         }
       }
     }
+    // keep the scheduler from interleaving the loads with the MFMAs (it
+    // would, to save registers, and leave only a few loads in flight)
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      if (s[q].x >= 0) {
+    for (int q = 0; q < 4; ++q)
 #pragma unroll
-        for (int h = 0; h < 2; ++h)
+      for (int h = 0; h < 2; ++h)
 #pragma unroll
-          for (int c = 0; c < NC; ++c) {
-            acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[q][h][0], x[q][h][c], acc[c], 0, 0, 0);
-            if (SPLIT)
-              acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[q][h][PLANES - 1], x[q][h][c], acc[c], 0, 0, 0);
-          }
-      }
-    }
+        for (int c = 0; c < NC; ++c) {
+          acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[q][h][0], x[q][h][c], acc[c], 0, 0, 0);
+          if (SPLIT)
+            acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[q][h][PLANES - 1], x[q][h][c], acc[c], 0, 0, 0);
+        }
   }
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
@@ -380,26 +383,31 @@ __global__ __launch_bounds__(256) void k_csr_rm_f32(int rows, int p, const int* 
   if (r < rows && c0 < p) {
     const int e0 = rp[r], e1 = rp[r + 1];
     // batches of up to 16 nonzeros: all index/value loads, then all row
-    // gathers, then the FMAs, so 16 gathers are in flight per thread
+    // gathers, then the FMAs, so 16 gathers are in flight per thread.  The
+    // loads are unconditional (indices clamped to the row's last entry; the
+    // extra terms are not accumulated): a conditional load would become a
+    // branch that waits for the loads before it.
     for (int base = e0; base < e1; base += 16) {
       const int nb = min(16, e1 - base);
       int cc[16];
       float ww[16];
 #pragma unroll
       for (int q = 0; q < 16; ++q) {
-        cc[q] = q < nb ? ci[base + q] : 0;
-        ww[q] = q < nb ? v[base + q] : 0.0f;
+        const int e = base + min(q, nb - 1);
+        cc[q] = ci[e];
+        ww[q] = v[e];
       }
       float4 xx[16];
 #pragma unroll
-      for (int q = 0; q < 16; ++q)
-        xx[q] = q < nb ? *reinterpret_cast<const float4*>(X + (size_t)cc[q] * ldx + c0) : make_float4(0.f, 0.f, 0.f, 0.f);
+      for (int q = 0; q < 16; ++q) xx[q] = *reinterpret_cast<const float4*>(X + (size_t)cc[q] * ldx + c0);
+      __builtin_amdgcn_sched_barrier(0);  // all 16 gathers in flight before the first FMA
 #pragma unroll
       for (int q = 0; q < 16; ++q) {
-        acc.x = fmaf(ww[q], xx[q].x, acc.x);
-        acc.y = fmaf(ww[q], xx[q].y, acc.y);
-        acc.z = fmaf(ww[q], xx[q].z, acc.z);
-        acc.w = fmaf(ww[q], xx[q].w, acc.w);
+        const bool on = q < nb;  // terms past the row end are not accumulated
+        acc.x = on ? fmaf(ww[q], xx[q].x, acc.x) : acc.x;
+        acc.y = on ? fmaf(ww[q], xx[q].y, acc.y) : acc.y;
+        acc.z = on ? fmaf(ww[q], xx[q].z, acc.z) : acc.z;
+        acc.w = on ? fmaf(ww[q], xx[q].w, acc.w) : acc.w;
       }
     }
   }
@@ -746,18 +754,21 @@ static void put_block(const void* src, int dtype, size_t b, bool transpose, bool
     }
 }
 
-// ELL slots of a block-row structure: width = widest block row (0 when above
-// BSR_ELL_MAX); slot (br, q) = (rp[br] + q, ci[rp[br] + q]) or (-1, 0).
-static int build_ell(int nrows, const int32_t* rp, const int32_t* ci, std::vector<int32_t>& ell) {
+// ELL slots of a block-row structure: width = widest block row rounded up to
+// a multiple of 4 (0 when above BSR_ELL_MAX); slot (br, q) = (rp[br] + q,
+// ci[rp[br] + q]), or (zero_block, 0) past the end of the row, zero_block
+// being an all-zero block stored after the nnz blocks.
+static int build_ell(int nrows, const int32_t* rp, const int32_t* ci, int zero_block, std::vector<int32_t>& ell) {
   int w = 0;
   for (int r = 0; r < nrows; ++r) w = std::max(w, rp[r + 1] - rp[r]);
-  if (w == 0 || w > BSR_ELL_MAX) return 0;
+  if (w > BSR_ELL_MAX) return 0;
+  w = std::max(4, (w + 3) / 4 * 4);
   ell.assign((size_t)nrows * w * 2, 0);
   for (int r = 0; r < nrows; ++r)
     for (int q = 0; q < w; ++q) {
       const int b = rp[r] + q;
       const bool on = b < rp[r + 1];
-      ell[((size_t)r * w + q) * 2] = on ? b : -1;
+      ell[((size_t)r * w + q) * 2] = on ? b : zero_block;
       ell[((size_t)r * w + q) * 2 + 1] = on ? ci[b] : 0;
     }
   return w;
@@ -785,7 +796,8 @@ DMatP make_bsr(int dtype, int m, int n, int bnnz, const int32_t* browptr, const 
   M->bnnz = bnnz;
   M->split = split;
   M->nnz = (int64_t)bnnz * 1024;
-  std::vector<u16> dat((size_t)std::max(bnnz, 1) * 1024 * planes), tdat(dat.size());
+  // bnnz blocks plus one all-zero block (the target of empty ELL slots)
+  std::vector<u16> dat((size_t)(bnnz + 1) * 1024 * planes), tdat(dat.size());
   for (int b = 0; b < bnnz; ++b) put_block(bdata, dtype, b, false, split, dat, b);
   // transposed BSR: block (br, bc) -> (bc, br) with the block transposed
   std::vector<int32_t> tbrp(nbc + 1, 0), tbci(std::max(bnnz, 1));
@@ -804,13 +816,13 @@ DMatP make_bsr(int dtype, int m, int n, int bnnz, const int32_t* browptr, const 
   };
   up(M->brp, browptr, sizeof(int) * (nbr + 1));
   up(M->bci, bcolinds, sizeof(int) * bnnz);
-  up(M->bdat, dat.data(), sizeof(u16) * (size_t)bnnz * 1024 * planes);
+  up(M->bdat, dat.data(), sizeof(u16) * (size_t)(bnnz + 1) * 1024 * planes);
   up(M->tbrp, tbrp.data(), sizeof(int) * (nbc + 1));
   up(M->tbci, tbci.data(), sizeof(int) * bnnz);
-  up(M->tbdat, tdat.data(), sizeof(u16) * (size_t)bnnz * 1024 * planes);
+  up(M->tbdat, tdat.data(), sizeof(u16) * (size_t)(bnnz + 1) * 1024 * planes);
   std::vector<int32_t> ell, tell;
-  M->ell_w = build_ell(nbr, browptr, bcolinds, ell);
-  M->tell_w = build_ell(nbc, tbrp.data(), tbci.data(), tell);
+  M->ell_w = build_ell(nbr, browptr, bcolinds, bnnz, ell);
+  M->tell_w = build_ell(nbc, tbrp.data(), tbci.data(), bnnz, tell);
   if (M->ell_w) up(M->ell, ell.data(), sizeof(int32_t) * ell.size());
   if (M->tell_w) up(M->tell, tell.data(), sizeof(int32_t) * tell.size());
   FH_HIP(hipStreamSynchronize(st));

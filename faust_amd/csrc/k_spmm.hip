@@ -12,20 +12,38 @@ namespace fh {
 
 // Y(i,j) = alpha * sum_p v[p] * X(ci[p], j) + beta * Y(i,j)
 // transX: X(k,j) is read as X[j + k*ldx] (op(X) = X^T).
+template <bool TRANSX>
 __global__ __launch_bounds__(256) void k_csr_spmm_f64(int nrows, int ncols, const int* __restrict__ rp,
                                                        const int* __restrict__ ci, const double* __restrict__ v,
-                                                       const double* __restrict__ X, int ldx, int transX,
+                                                       const double* __restrict__ X, int ldx,
                                                        double* __restrict__ Y, int ldy, double alpha, double beta) {
   const int i = blockIdx.x * 64 + (threadIdx.x & 63);
   const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
   if (i >= nrows || j >= ncols) return;
   const int b = rp[i], e = rp[i + 1];
   double acc = 0.0;
-  if (!transX) {
-    const double* xj = X + (size_t)j * ldx;
-    for (int p = b; p < e; ++p) acc = fma(v[p], xj[ci[p]], acc);
-  } else {
-    for (int p = b; p < e; ++p) acc = fma(v[p], X[(size_t)ci[p] * ldx + j], acc);
+  // batches of 8 terms: the index/value loads and the gathers are issued
+  // unconditionally (indices clamped inside the row) before the fma chain
+  // runs in k order; terms past the row end are not accumulated
+  const double* xj = X + (size_t)j * ldx;
+  for (int base = b; base < e; base += 8) {
+    const int nb = min(8, e - base);
+    int cc[8];
+    double ww[8], xx[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int p = base + min(q, nb - 1);
+      cc[q] = ci[p];
+      ww[q] = v[p];
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) xx[q] = TRANSX ? X[(size_t)cc[q] * ldx + j] : xj[cc[q]];
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const double f = fma(ww[q], xx[q], acc);
+      acc = q < nb ? f : acc;
+    }
   }
   const double t = alpha * acc;
   double* y = Y + (size_t)j * ldy + i;
@@ -43,8 +61,12 @@ void csr_spmm_f64_x(const DMat& S, char op, const double* X, int ldx, int transX
   const int* ci = tr ? S.ri.as<int>() : S.ci.as<int>();
   const double* v = tr ? S.cv.as<double>() : S.v.as<double>();
   dim3 grid((rows + 63) / 64, (ncols + 3) / 4);
-  hipLaunchKernelGGL(k_csr_spmm_f64, grid, dim3(256), 0, st, rows, ncols, rp, ci, v, X, ldx, transX, Y, ldy, alpha,
-                     beta);
+  if (transX)
+    hipLaunchKernelGGL(k_csr_spmm_f64<true>, grid, dim3(256), 0, st, rows, ncols, rp, ci, v, X, ldx, Y, ldy, alpha,
+                       beta);
+  else
+    hipLaunchKernelGGL(k_csr_spmm_f64<false>, grid, dim3(256), 0, st, rows, ncols, rp, ci, v, X, ldx, Y, ldy, alpha,
+                       beta);
   FH_LAUNCH_CHECK();
 }
 
@@ -56,8 +78,9 @@ void csr_spmm_f64(const DMat& S, char op, const double* X, int ldx, int ncols, d
 // C(i,j) = alpha * sum_{k in col j of op(S)} op(A)(i,k) * op(S)(k,j) + beta*C(i,j)
 // (cp, ri, cv) is the column-compressed form of op(S): CSC of S for op N,
 // CSR of S for op T.
+template <bool TRANSA>
 __global__ __launch_bounds__(256) void k_dense_csr_f64(int m, int n, const double* __restrict__ A, int lda,
-                                                        int transA, const int* __restrict__ cp,
+                                                        const int* __restrict__ cp,
                                                         const int* __restrict__ ri, const double* __restrict__ cv,
                                                         double* __restrict__ C, int ldc, double alpha, double beta) {
   const int i = blockIdx.x * 64 + (threadIdx.x & 63);
@@ -65,11 +88,25 @@ __global__ __launch_bounds__(256) void k_dense_csr_f64(int m, int n, const doubl
   if (i >= m || j >= n) return;
   const int b = cp[j], e = cp[j + 1];
   double acc = 0.0;
-  if (!transA) {
-    for (int p = b; p < e; ++p) acc = fma(A[(size_t)ri[p] * lda + i], cv[p], acc);
-  } else {
-    const double* ai = A + (size_t)i * lda;
-    for (int p = b; p < e; ++p) acc = fma(ai[ri[p]], cv[p], acc);
+  const double* ai = A + (size_t)i * lda;
+  for (int base = b; base < e; base += 8) {
+    const int nb = min(8, e - base);
+    int rr[8];
+    double ww[8], aa[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int p = base + min(q, nb - 1);
+      rr[q] = ri[p];
+      ww[q] = cv[p];
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) aa[q] = TRANSA ? ai[rr[q]] : A[(size_t)rr[q] * lda + i];
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const double f = fma(aa[q], ww[q], acc);
+      acc = q < nb ? f : acc;
+    }
   }
   const double t = alpha * acc;
   double* c = C + (size_t)j * ldc + i;
@@ -87,8 +124,10 @@ void dense_csr_f64(const double* A, int lda, char opA, int m, const DMat& S, cha
   const int* ri = trS ? S.ci.as<int>() : S.ri.as<int>();
   const double* cv = trS ? S.v.as<double>() : S.cv.as<double>();
   dim3 grid((m + 63) / 64, (n + 3) / 4);
-  hipLaunchKernelGGL(k_dense_csr_f64, grid, dim3(256), 0, st, m, n, A, lda, opA != 'N' ? 1 : 0, cp, ri, cv, C, ldc,
-                     alpha, beta);
+  if (opA != 'N')
+    hipLaunchKernelGGL(k_dense_csr_f64<true>, grid, dim3(256), 0, st, m, n, A, lda, cp, ri, cv, C, ldc, alpha, beta);
+  else
+    hipLaunchKernelGGL(k_dense_csr_f64<false>, grid, dim3(256), 0, st, m, n, A, lda, cp, ri, cv, C, ldc, alpha, beta);
   FH_LAUNCH_CHECK();
 }
 
