@@ -120,20 +120,42 @@ __global__ __launch_bounds__(256) void k_bgemm_mfma_f32(const BGemmJob<float>* _
   const int wr = wave & 1, wc = wave >> 1;
   const int i0 = tl.y * TILE, j0 = tl.z * TILE;
   f32x16 acc = {};
+  const bool inside = i0 + TILE <= J.m && j0 + TILE <= J.n;
   for (int k0 = 0; k0 < J.k; k0 += KS) {
-    for (int e = t; e < TILE * KS; e += 256) {
-      int ii, kk;
-      if (!J.transA) { ii = e % TILE; kk = e / TILE; } else { kk = e % KS; ii = e / KS; }
-      const int gi = i0 + ii, gk = k0 + kk;
-      As[kk][ii] = (gi < J.m && gk < J.k) ? (J.transA ? J.A[(size_t)gi * J.k + gk] : J.A[(size_t)gk * J.m + gi])
-                                          : 0.0f;
-    }
-    for (int e = t; e < TILE * KS; e += 256) {
-      int jj, kk;
-      if (J.transB) { jj = e % TILE; kk = e / TILE; } else { kk = e % KS; jj = e / KS; }
-      const int gj = j0 + jj, gk = k0 + kk;
-      Bs[kk][jj] = (gj < J.n && gk < J.k) ? (J.transB ? J.B[(size_t)gk * J.n + gj] : J.B[(size_t)gj * J.k + gk])
-                                          : 0.0f;
+    if (inside && k0 + KS <= J.k) {
+      // whole tile inside: 8 + 8 unconditional loads per thread, all in flight
+      float av[TILE * KS / 256], bv[TILE * KS / 256];
+#pragma unroll
+      for (int q = 0; q < TILE * KS / 256; ++q) {
+        const int e = t + 256 * q;
+        const int ii = J.transA ? e / KS : e % TILE, ka = J.transA ? e % KS : e / TILE;
+        av[q] = J.transA ? J.A[(size_t)(i0 + ii) * J.k + k0 + ka] : J.A[(size_t)(k0 + ka) * J.m + i0 + ii];
+        const int jj = J.transB ? e % TILE : e / KS, kb = J.transB ? e / TILE : e % KS;
+        bv[q] = J.transB ? J.B[(size_t)(k0 + kb) * J.n + j0 + jj] : J.B[(size_t)(j0 + jj) * J.k + k0 + kb];
+      }
+#pragma unroll
+      for (int q = 0; q < TILE * KS / 256; ++q) {
+        const int e = t + 256 * q;
+        const int ii = J.transA ? e / KS : e % TILE, ka = J.transA ? e % KS : e / TILE;
+        As[ka][ii] = av[q];
+        const int jj = J.transB ? e % TILE : e / KS, kb = J.transB ? e / TILE : e % KS;
+        Bs[kb][jj] = bv[q];
+      }
+    } else {
+      for (int e = t; e < TILE * KS; e += 256) {
+        int ii, kk;
+        if (!J.transA) { ii = e % TILE; kk = e / TILE; } else { kk = e % KS; ii = e / KS; }
+        const int gi = i0 + ii, gk = k0 + kk;
+        As[kk][ii] = (gi < J.m && gk < J.k) ? (J.transA ? J.A[(size_t)gi * J.k + gk] : J.A[(size_t)gk * J.m + gi])
+                                            : 0.0f;
+      }
+      for (int e = t; e < TILE * KS; e += 256) {
+        int jj, kk;
+        if (J.transB) { jj = e % TILE; kk = e / TILE; } else { kk = e % KS; jj = e / KS; }
+        const int gj = j0 + jj, gk = k0 + kk;
+        Bs[kk][jj] = (gj < J.n && gk < J.k) ? (J.transB ? J.B[(size_t)gk * J.n + gj] : J.B[(size_t)gj * J.k + gk])
+                                            : 0.0f;
+      }
     }
     __syncthreads();
 #pragma unroll
