@@ -317,7 +317,13 @@ static double run_its(Batch<T>& B, int its, bool graph, hipStream_t st) {
     hipGraph_t g;
     hipGraphExec_t ge;
     FH_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    run_iteration(B, st);
+    try {
+      run_iteration(B, st);
+    } catch (...) {
+      // never leave the stream capturing: end (and drop) the partial graph first
+      if (hipStreamEndCapture(st, &g) == hipSuccess && g) (void)hipGraphDestroy(g);
+      throw;
+    }
     FH_HIP(hipStreamEndCapture(st, &g));
     FH_HIP(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
     t0 = std::chrono::steady_clock::now();
@@ -563,7 +569,9 @@ extern "C" int fh_palm4msa_batched(int ntargets, int nfacts, const int* n, const
       }
       FH_HIP(hipMemcpyAsync(D.lam.p, B.lam.p, sizeof(double) * ntargets, hipMemcpyDeviceToDevice, st));
       FH_HIP(hipMemcpyAsync(D.powv.p, B.powv.p, B.powv.bytes, hipMemcpyDeviceToDevice, st));
-      D.tile_of = B.tile_of;
+      D.tile_of = B.tile_of;  // the FP32 MFMA tile has no FP64 form: its targets take 64-wide VALU tiles
+      for (int& t : D.tile_of)
+        if (t == BGEMM_MFMA) t = 64;
       build_tiles(D);
       build_schedule(D, {});
       FH_HIP(hipStreamSynchronize(st));
