@@ -80,7 +80,8 @@ def _bsr_workload(values):
     return dict(name=f"faust_bsr_{values}_8192x8192_J6_b32x32_4blk_X8192x512_bf16", facs=facs, X=X,
                 dtype="bf16", fac_dtype=("bf16" if values == "bf16" else np.float32), ops=("N",),
                 bytes_per_launch=per_factor, launches_per_step=J, bytes_per_step=J * per_factor,
-                flops_per_step=J * 2 * nb * bs * bs * p, kernel="k_bsr_mfma<%s>" % ("false" if values == "bf16" else "true"),
+                flops_per_step=J * 2 * nb * bs * bs * p,
+                kernel=("k_bsr_ell<false,*,*,2>" if values == "bf16" else "k_bsr_ell<true,*,*,4>"),
                 cpu_reps_cap=1)
 
 
@@ -105,7 +106,7 @@ def workload_chain_csr_f32():
     per_factor = nnz * n * (4 + 4) + (n + 1) * 4 + n * p * 4 + n * p * 4
     return dict(name="faust_csr_fp32_16384x16384_J6_nnz12_X16384x256", facs=facs, X=X, dtype="f32",
                 fac_dtype=np.float32, ops=("N",), bytes_per_launch=per_factor, launches_per_step=J,
-                bytes_per_step=J * per_factor, flops_per_step=J * 2 * n * nnz * p, kernel="k_csr_rm_f32<false>",
+                bytes_per_step=J * per_factor, flops_per_step=J * 2 * n * nnz * p, kernel="k_csr_rm_f32",
                 cpu_reps_cap=3)
 
 
@@ -205,6 +206,23 @@ def palm_hadamard():
 
 
 # ----------------------------------------------------------------- helpers
+def measured_traffic(wl):
+    """HBM-side bytes per launch of the dominant kernel from the rocprofv3
+    FETCH_SIZE / WRITE_SIZE runs recorded in profiles/traffic.json (each
+    counter scaled by its reading on a plain copy of known size); None when
+    no record matches this workload and kernel."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            rec = json.load(f).get(wl["name"])
+    except (OSError, ValueError):
+        return None
+    if not rec or rec.get("kernel") != wl["kernel"]:
+        return None
+    return int(rec["fetch_kib"] * 1024 * rec["fetch_scale"] + rec["write_kib"] * 1024 * rec["write_scale"])
+
+
+
 def cpu_baseline(wl, budget_s):
     import oracle
     nthreads = len(os.sched_getaffinity(0))
@@ -308,8 +326,9 @@ def main():
     n_ops = len(wl["ops"])
     k_ms = (ms_step - n_ops * conv_ms_step) / wl["launches_per_step"] if graphed else k_event_ms
     achieved = wl["bytes_per_launch"] / (k_ms * 1e-3) / 1e9
+    traffic = measured_traffic(wl)
     roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": wl["kernel"],
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": wl["kernel"],
             "avg_launch_us": round(k_ms * 1e3, 3), "bytes_per_launch": wl["bytes_per_launch"],
             "event_pair_avg_launch_us": round(k_event_ms * 1e3, 3), "launches_timed": int(cnt[:J].sum()),
             "layout_conversion_us_per_step": round(conv_ms_step * 1e3, 3)}
