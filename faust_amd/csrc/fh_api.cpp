@@ -65,6 +65,10 @@ DMatP make_csr_from_host(int dtype, int m, int n, int nnz, const int32_t* rp, co
     FH_HIP(hipMemcpyAsync(M->cv.p, cv.data(), esz * nnz, hipMemcpyHostToDevice, st));
   }
   M->has_csc = true;
+  M->max_row = 0;
+  for (int i = 0; i < m; ++i) M->max_row = std::max(M->max_row, rp[i + 1] - rp[i]);
+  M->max_col = 0;
+  for (int j = 0; j < n; ++j) M->max_col = std::max(M->max_col, cp[j + 1] - cp[j]);
   FH_HIP(hipStreamSynchronize(st));
   return M;
 }

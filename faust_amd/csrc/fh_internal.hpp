@@ -102,6 +102,7 @@ struct DMat {
   DBuf d;                 // dense values
   DBuf rp, ci, v;         // CSR
   int64_t nnz = 0;        // CSR: stored entries; dense: -1 until counted
+  int max_row = -1, max_col = -1;  // CSR uploaded from the host: widest row / column (-1: unknown)
   mutable bool has_csc = false;
   mutable DBuf cp, ri, cv; // CSC
   // BSR (32x32 blocks): bdat holds each block row-major in BF16; FP32 values

@@ -369,7 +369,7 @@ static void bsr_mul(const DMat& B, char op, const void* X, int ldx, int p, void*
 // ---------------------------------------------------------------- FP32 CSR
 // Row-major gather SpMM: Y[r, c] = alpha * sum_e v[e] * X[ci[e], c] + beta*Y.
 // OUT_COL: write Y column-major (ld = ldy) through an LDS transpose.
-template <bool OUT_COL>
+template <bool OUT_COL, int NB>
 __global__ __launch_bounds__(256) void k_csr_rm_f32(int rows, int p, const int* __restrict__ rp,
                                                     const int* __restrict__ ci, const float* __restrict__ v,
                                                     const float* __restrict__ X, int ldx, float* __restrict__ Y,
@@ -382,27 +382,28 @@ __global__ __launch_bounds__(256) void k_csr_rm_f32(int rows, int p, const int* 
   float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
   if (r < rows && c0 < p) {
     const int e0 = rp[r], e1 = rp[r + 1];
-    // batches of up to 16 nonzeros: all index/value loads, then all row
-    // gathers, then the FMAs, so 16 gathers are in flight per thread.  The
-    // loads are unconditional (indices clamped to the row's last entry; the
-    // extra terms are not accumulated): a conditional load would become a
-    // branch that waits for the loads before it.
-    for (int base = e0; base < e1; base += 16) {
-      const int nb = min(16, e1 - base);
-      int cc[16];
-      float ww[16];
+    // batches of up to NB nonzeros (NB: the widest row rounded up to 4, at
+    // most 16): all index/value loads, then all row gathers, then the FMAs,
+    // so NB gathers are in flight per thread.  The loads are unconditional
+    // (indices clamped to the row's last entry; the extra terms are not
+    // accumulated): a conditional load would become a branch that waits for
+    // the loads before it.
+    for (int base = e0; base < e1; base += NB) {
+      const int nb = min(NB, e1 - base);
+      int cc[NB];
+      float ww[NB];
 #pragma unroll
-      for (int q = 0; q < 16; ++q) {
+      for (int q = 0; q < NB; ++q) {
         const int e = base + min(q, nb - 1);
         cc[q] = ci[e];
         ww[q] = v[e];
       }
-      float4 xx[16];
+      float4 xx[NB];
 #pragma unroll
-      for (int q = 0; q < 16; ++q) xx[q] = *reinterpret_cast<const float4*>(X + (size_t)cc[q] * ldx + c0);
-      __builtin_amdgcn_sched_barrier(0);  // all 16 gathers in flight before the first FMA
+      for (int q = 0; q < NB; ++q) xx[q] = *reinterpret_cast<const float4*>(X + (size_t)cc[q] * ldx + c0);
+      __builtin_amdgcn_sched_barrier(0);  // all gathers in flight before the first FMA
 #pragma unroll
-      for (int q = 0; q < 16; ++q) {
+      for (int q = 0; q < NB; ++q) {
         const bool on = q < nb;  // terms past the row end are not accumulated
         acc.x = on ? fmaf(ww[q], xx[q].x, acc.x) : acc.x;
         acc.y = on ? fmaf(ww[q], xx[q].y, acc.y) : acc.y;
@@ -596,12 +597,23 @@ static void csr_f32_mul_rm(const DMat& S, char op, const float* X, int ldx, int 
     return;
   }
   dim3 grid((unsigned)(npanels * ((rows + 31) / 32)));
-  if (out_col)
-    hipLaunchKernelGGL(k_csr_rm_f32<true>, grid, dim3(256), 0, st, rows, p, rp, ci, v, X, ldx, Y, ldy, alpha, beta,
-                       npanels);
-  else
-    hipLaunchKernelGGL(k_csr_rm_f32<false>, grid, dim3(256), 0, st, rows, p, rp, ci, v, X, ldx, Y, ldy, alpha, beta,
-                       npanels);
+  // gather batch: the widest row rounded up to 4 (16 when unknown or wider)
+  const int widest = tr ? S.max_col : S.max_row;
+  const int nb = (widest < 0 || widest > 12) ? 16 : std::max(4, (widest + 3) / 4 * 4);
+#define FH_CSR_GO(OC, NB)                                                                                  \
+  hipLaunchKernelGGL((k_csr_rm_f32<OC, NB>), grid, dim3(256), 0, st, rows, p, rp, ci, v, X, ldx, Y, ldy, alpha, \
+                     beta, npanels)
+  switch (nb * 2 + (out_col ? 1 : 0)) {
+    case 8: FH_CSR_GO(false, 4); break;
+    case 9: FH_CSR_GO(true, 4); break;
+    case 16: FH_CSR_GO(false, 8); break;
+    case 17: FH_CSR_GO(true, 8); break;
+    case 24: FH_CSR_GO(false, 12); break;
+    case 25: FH_CSR_GO(true, 12); break;
+    case 32: FH_CSR_GO(false, 16); break;
+    default: FH_CSR_GO(true, 16); break;
+  }
+#undef FH_CSR_GO
   FH_LAUNCH_CHECK();
 }
 
