@@ -258,23 +258,46 @@ def predicted_init(mlp, As, masks, batch=256):
     return out_f, out_l
 
 
-def compare_warm_start(As, masks, inits, lambdas, max_its=1000, check_every=25, target_err=1e-4, dtype="f64"):
+class _Runs:
+    """Concatenation of batched runs over consecutive chunks of targets."""
+
+    def __init__(self, parts):
+        self.relerr = np.concatenate([r.relerr for r in parts])
+        self.iters_to_target = np.concatenate([r.iters_to_target for r in parts])
+        # time until every target of a chunk reached the target error: summed
+        # over chunks when all of them got there, else -1
+        t = [r.times_ms[4] for r in parts]
+        self.times_ms = np.array([0, 0, 0, 0, sum(t) if min(t) >= 0 else -1.0])
+
+
+def compare_warm_start(As, masks, inits, lambdas, max_its=1000, check_every=25, target_err=1e-4, dtype="f64",
+                       chunk=250, log=None):
     """PALM4MSA from the predicted initialisation and from the reference one
     (first factor 0, the others identity), same iteration budget, FP64-verified
     errors checked every `check_every` iterations.  A warm-started run whose
-    final error is worse than the reference run's falls back to the latter."""
+    final error is worse than the reference run's falls back to the latter.
+    Targets go through the batched engine `chunk` at a time."""
     import time
     from .batched import palm4msa_batched
     L = len(masks)
-    sup = [masks] * len(As)
-    t = time.perf_counter()
-    ref = palm4msa_batched(As, L, supports=sup, dtype=dtype, num_its=max_its, refine_its=0,
-                           check_every=check_every, target_err=target_err, return_factors=False)
-    t_ref = time.perf_counter() - t
-    t = time.perf_counter()
-    warm = palm4msa_batched(As, L, supports=sup, init=inits, lambda_init=lambdas, dtype=dtype, num_its=max_its,
-                            refine_its=0, check_every=check_every, target_err=target_err, return_factors=False)
-    t_warm = time.perf_counter() - t
+    refs, warms = [], []
+    t_ref = t_warm = 0.0
+    for c0 in range(0, len(As), chunk):
+        sl = slice(c0, min(len(As), c0 + chunk))
+        part = As[sl]
+        sup = [masks] * len(part)
+        t = time.perf_counter()
+        refs.append(palm4msa_batched(part, L, supports=sup, dtype=dtype, num_its=max_its, refine_its=0,
+                                     check_every=check_every, target_err=target_err, return_factors=False))
+        t_ref += time.perf_counter() - t
+        t = time.perf_counter()
+        warms.append(palm4msa_batched(part, L, supports=sup, init=inits[sl], lambda_init=lambdas[sl], dtype=dtype,
+                                      num_its=max_its, refine_its=0, check_every=check_every,
+                                      target_err=target_err, return_factors=False))
+        t_warm += time.perf_counter() - t
+        if log:
+            log(f"PALM on held-out targets {sl.stop}/{len(As)}: reference {t_ref:.1f}s, warm {t_warm:.1f}s")
+    ref, warm = _Runs(refs), _Runs(warms)
     fallback = ~(warm.relerr <= ref.relerr)  # a non-finite warm error falls back too
     final = np.where(fallback, ref.relerr, warm.relerr)
     it_w = np.where(fallback, ref.iters_to_target, warm.iters_to_target)
@@ -313,6 +336,7 @@ def experiment(n=256, ntrain=16384, nhold=1000, epochs=40, label_its=5, max_its=
         Xs.append(np.stack([mlp_input(a) for a, _ in part]))
         Ys.append(Yc)
         lerr.extend(ec.tolist())
+        log(f"training targets {c0 + len(part)}/{ntrain}: gen {t_gen:.1f}s labels {t_lab:.1f}s")
     hold = [butterfly_sample(n, rng, noise)[0] for _ in range(nhold)]
     log(f"targets: gen {t_gen:.1f}s labels {t_lab:.1f}s (median label relerr {np.median(lerr):.2e})")
     X = torch.from_numpy(np.concatenate(Xs)).cuda()
@@ -328,8 +352,9 @@ def experiment(n=256, ntrain=16384, nhold=1000, epochs=40, label_its=5, max_its=
     t0 = time.perf_counter()
     inits, lam0 = predicted_init(mlp, hold, masks)
     t_pred = time.perf_counter() - t0
+    log(f"predicted inits for {nhold} held-out targets in {t_pred:.1f}s")
     res = compare_warm_start(hold, masks, inits, lam0, max_its=max_its, check_every=check_every,
-                             target_err=target_err, dtype=dtype)
+                             target_err=target_err, dtype=dtype, log=log)
     res.update({"n": n, "factors": L, "noise_std": noise, "train_targets": ntrain, "label_palm_its": label_its,
                 "epochs": epochs, "final_train_mse": float(losses[-1]), "train_s": round(t_train, 2),
                 "predict_s": round(t_pred, 2), "label_s": round(t_lab, 1), "mlp": [n * n, hidden, hidden, 2 * n * L]})
