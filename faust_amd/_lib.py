@@ -63,6 +63,7 @@ _SIGS = {
     "fh_faust_size": (C.c_int, [VP, C.POINTER(C.c_int)]),
     "fh_faust_free": (C.c_int, [VP]),
     "fh_faust_multiply": (C.c_int, [VP, C.c_char, VP, C.c_int, VP, VP]),
+    "fh_faust_last_launches": (C.c_int, [VP, C.POINTER(C.c_int)]),
     "fh_faust_get_product": (C.c_int, [VP, VP, VP]),
     "fh_faust_enable_timers": (C.c_int, [VP, C.c_int]),
     "fh_faust_timers": (C.c_int, [VP, VP, VP]),

@@ -336,10 +336,19 @@ int fh_faust_multiply(fh_faust_t F, char op, const void* X, int p, void* Y, void
   FH_CHECK(!F->f.empty(), "Faust::multiply: empty Faust");
   FH_CHECK(p >= 0, "negative column count");
   hipStream_t st = (hipStream_t)stream;
-  if (F->f.front()->dtype == FH_F64)
+  if (F->f.front()->dtype == FH_F64) {
     faust_multiply_f64(F->f, op, (const double*)X, p, (double*)Y, st, F->ws, &F->timers);
-  else
-    faust_multiply_mixed(F->f, op, X, p, Y, st, F->ws, &F->timers);
+    F->last_launches = (int)F->f.size();
+  } else {
+    F->last_launches = faust_multiply_mixed(F->f, op, X, p, Y, st, F->ws, &F->timers);
+  }
+  FH_API_END
+}
+
+int fh_faust_last_launches(fh_faust_t F, int* launches) {
+  FH_API_BEGIN
+  FH_CHECK(F && launches, "null argument");
+  *launches = F->last_launches;
   FH_API_END
 }
 

@@ -79,7 +79,8 @@ struct FaustTimers {
 // captured in a HIP graph.
 struct fh_faust_s {
   std::vector<fh::DMatP> f;
-  fh::DBuf ws[2];
+  fh::DBuf ws[3];  // two ping-pong activations, and the persistent BSR chain's counters
+  int last_launches = 0;  // kernel launches of the last product (the persistent BSR chain: 1)
   fh::FaustTimers timers;
 };
 
@@ -108,6 +109,6 @@ double faust_spectral_norm_f64(const std::vector<DMatP>& f, int maxit, double th
 void mixed_to_dense(const DMat& M, void* dense_dev, hipStream_t st);
 void mixed_mul_dense(const DMat& S, char op, const void* X, int ldx, int ncols, void* Y, int ldy, const void* alpha,
                      const void* beta, hipStream_t st);
-void faust_multiply_mixed(const std::vector<DMatP>& f, char op, const void* X, int p, void* Y, hipStream_t st,
+int faust_multiply_mixed(const std::vector<DMatP>& f, char op, const void* X, int p, void* Y, hipStream_t st,
                           DBuf* ws, FaustTimers* tm = nullptr);
 }  // namespace fh

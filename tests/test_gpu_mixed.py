@@ -166,3 +166,34 @@ def test_csr_fp32_rows_per_thread(gpu, monkeypatch, rpt, op):
         Y = (F @ X) if op == "N" else (F.T @ X)
         ref = oracle.chain_multiply(facs, X, op)
         assert np.linalg.norm(Y - ref) / np.linalg.norm(ref) < 1e-5
+
+
+@pytest.mark.parametrize("dtype", ["bf16", np.float32])
+@pytest.mark.parametrize("op", ["N", "T"])
+def test_bsr_persistent_chain_matches_per_factor(gpu, monkeypatch, dtype, op):
+    # the single-launch chain (intermediates kept per XCD) computes exactly
+    # what the per-factor ELL kernels compute
+    import ctypes as C
+    from faust_amd import _lib
+    rng = np.random.default_rng(13)
+    facs = [rand_bsr(2048, 2048, 32, 4, rng, scale=1 / np.sqrt(128)) for _ in range(3)]
+    if dtype == "bf16":
+        facs = [_bsr_rounded(f) for f in facs]
+    F = faust_amd.Faust(facs, dtype=dtype)
+    G = F if op == "N" else F.T
+    for p in (512, 200, 8):
+        X = bf16_round(rng.standard_normal((2048, p)))
+        monkeypatch.setenv("FH_BSR_PERSIST", "1")
+        Y1 = G @ X
+        n = C.c_int()
+        _lib.check(_lib.lib().fh_faust_last_launches(G.h, C.byref(n)))
+        assert n.value == 1
+        monkeypatch.setenv("FH_BSR_PERSIST", "0")
+        monkeypatch.setenv("FH_BSR_KERNEL", "2")
+        Y2 = G @ X
+        _lib.check(_lib.lib().fh_faust_last_launches(G.h, C.byref(n)))
+        assert n.value == 3
+        monkeypatch.delenv("FH_BSR_KERNEL")
+        assert np.array_equal(Y1, Y2)
+        ref = oracle.chain_multiply(facs, X, op)
+        assert np.linalg.norm(Y1 - ref) / np.linalg.norm(ref) < 8e-3
