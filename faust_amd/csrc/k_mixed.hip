@@ -895,11 +895,17 @@ static bool bsr_chain_persistent(const std::vector<DMatP>& f, char op, const voi
   hipLaunchKernelGGL(k_chain_reset, dim3(1), dim3(256), 0, st, ws[2].as<int>(), nctl);
   FH_LAUNCH_CHECK();
   const int spin_max = 1 << 22;
+  // as many workgroups as fit at once (3 per CU for BF16 blocks, 2 with the
+  // hi/lo planes): each XCD then has enough of them to take every item of a
+  // factor in one round
+  int per_cu = split ? 2 : 3;
+  if (const char* e = getenv("FH_BSR_CHAIN_WG")) per_cu = std::max(1, std::min(8, atoi(e)));
+  const dim3 grid((unsigned)(cus * per_cu));
   if (split)
-    hipLaunchKernelGGL(k_bsr_chain<true>, dim3(cus), dim3(256), 0, st, A, n, p, (const u16*)X, n, (u16*)Y, n,
+    hipLaunchKernelGGL(k_bsr_chain<true>, grid, dim3(256), 0, st, A, n, p, (const u16*)X, n, (u16*)Y, n,
                        ws[0].as<u16>(), ws[1].as<u16>(), ws[2].as<int>(), spin_max);
   else
-    hipLaunchKernelGGL(k_bsr_chain<false>, dim3(cus), dim3(256), 0, st, A, n, p, (const u16*)X, n, (u16*)Y, n,
+    hipLaunchKernelGGL(k_bsr_chain<false>, grid, dim3(256), 0, st, A, n, p, (const u16*)X, n, (u16*)Y, n,
                        ws[0].as<u16>(), ws[1].as<u16>(), ws[2].as<int>(), spin_max);
   FH_LAUNCH_CHECK();
   if (g_persist_state == 0) {
