@@ -35,7 +35,8 @@ __device__ __forceinline__ T job_alpha(int mode, int b, const BScal& s) {
   // update_fact throws when a 2-norm is zero or NaN; a batched target whose
   // step is not finite and positive keeps its factor instead
   if (!(c > 0.0) || !isfinite(c) || !isfinite(lam)) return T(0);
-  return (T)(-lam / c);
+  const T a = (T)(-lam / c);
+  return isfinite(a) ? a : T(0);  // beyond the FP32 range: no step rather than inf
 }
 
 template <class T, int TILE>
