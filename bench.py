@@ -298,22 +298,10 @@ def main():
     wall = time.perf_counter() - t0
     ms_step = e0.elapsed_time(e1) / args.steps
     value = wl["bytes_per_step"] / (ms_step * 1e-3) / 1e9
-    # product kernels per step as launched (a chain of square BSR factors runs
-    # as one persistent kernel per product)
     L = _lib.lib()
-    nl = C.c_int()
-    launches = 0
-    for op in wl["ops"]:
-        _lib.check(L.fh_faust_last_launches((F if op == "N" else FT).h, C.byref(nl)))
-        launches += nl.value
-    persistent = wl["dtype"] == "bf16" and launches == len(wl["ops"])
-    if persistent:
-        wl = dict(wl, kernel=wl["kernel"].replace("k_bsr_ell", "k_bsr_chain").split(",")[0] + ">",
-                  launches_per_step=len(wl["ops"]), bytes_per_launch=wl["bytes_per_step"] // len(wl["ops"]))
 
     # per-factor HIP events on the launch stream around each factor's kernel,
-    # over a second run of the same steps (with timers on, the chain runs one
-    # launch per factor)
+    # over a second run of the same steps
     J = F.numfactors()
     for G in {id(F): F, id(FT): FT}.values():
         _lib.check(L.fh_faust_enable_timers(G.h, 1))
@@ -336,7 +324,7 @@ def main():
     # per-factor event pairs (which add ~2 us of event overhead per launch)
     # are reported beside it
     n_ops = len(wl["ops"])
-    k_ms = (ms_step - n_ops * conv_ms_step) / wl["launches_per_step"] if (graphed or persistent) else k_event_ms
+    k_ms = (ms_step - n_ops * conv_ms_step) / wl["launches_per_step"] if graphed else k_event_ms
     achieved = wl["bytes_per_launch"] / (k_ms * 1e-3) / 1e9
     traffic = measured_traffic(wl)
     roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -79,8 +79,8 @@ struct FaustTimers {
 // captured in a HIP graph.
 struct fh_faust_s {
   std::vector<fh::DMatP> f;
-  fh::DBuf ws[3];  // two ping-pong activations, and the persistent BSR chain's counters
-  int last_launches = 0;  // kernel launches of the last product (the persistent BSR chain: 1)
+  fh::DBuf ws[2];         // ping-pong activations
+  int last_launches = 0;  // kernel launches of the last product
   fh::FaustTimers timers;
 };
 

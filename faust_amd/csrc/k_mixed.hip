@@ -314,173 +314,6 @@ static void bsr_ell_launch(bool split, bool in_tiled, bool out_tiled, int nc, bo
 #undef FH_ELL
 }
 
-// ---------------------------------------------------------------- persistent BSR chain
-// The whole chain in one launch.  Column panel k (64 columns) belongs to XCD
-// k % 8 for every factor, and the XCD is read from the hardware (XCC_ID), so
-// an XCD only ever reads activations that its own workgroups wrote: the
-// intermediates stay in that XCD's L2 instead of crossing the fabric at every
-// factor boundary, and only the blocks (and the first input / last output)
-// do.  Inside an XCD, workgroups claim (panel, quad of block rows) items from
-// an XCD-local counter, count finished items on another, and start factor
-// f+1 once all items of factor f are done (bounded wait, then L1 invalidate).
-// A late workgroup finds the items claimed and only waits, so no workgroup
-// ever depends on another one being resident.  The last workgroup to finish
-// checks that every panel-owning XCD took part (error flag otherwise).
-constexpr int BSR_CHAIN_MAX = 16;
-
-struct BsrStage {
-  const int2* ell;
-  const u16* bdat;
-  int W;
-};
-struct BsrChainArgs {
-  BsrStage st[BSR_CHAIN_MAX];
-  int J;
-};
-
-// counters: claim[8*J], done[8*J], then finish, xcd mask, error
-__global__ void k_chain_reset(int* ctl, int n) {
-  for (int i = threadIdx.x; i < n; i += blockDim.x) ctl[i] = 0;
-}
-
-__device__ __forceinline__ int xcd_add(int* p, int v) {
-  return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-}
-
-// one block row (this wave) x 64 columns of one factor
-template <bool SPLIT, bool IN_TILED, bool OUT_TILED>
-__device__ __forceinline__ void chain_item(const BsrStage& S, int br, int nbr, int panel, int p, const u16* X, int ldx,
-                                           u16* Y, int ldy, int lane) {
-  constexpr int NC = 2, BLK = 1024, PLANES = SPLIT ? 2 : 1;
-  if (br >= nbr) return;
-  const int kh = lane >> 5;
-  int jc[NC], j[NC];
-  bool jok[NC];
-#pragma unroll
-  for (int c = 0; c < NC; ++c) {
-    jc[c] = panel * 64 + c * 32 + (lane & 31);
-    jok[c] = jc[c] < p;
-    j[c] = jok[c] ? jc[c] : 0;
-  }
-  f32x16 acc[NC];
-#pragma unroll
-  for (int c = 0; c < NC; ++c) acc[c] = f32x16{};
-  const int4* slots = reinterpret_cast<const int4*>(S.ell + (size_t)br * S.W);
-  for (int g = 0; g < S.W; g += 4) {
-    const int4 s01 = slots[g / 2], s23 = slots[g / 2 + 1];
-    const int bi[4] = {s01.x, s01.z, s23.x, s23.z};
-    const int bcol[4] = {s01.y, s01.w, s23.y, s23.w};
-    bf16x8 a[4][2][PLANES], x[4][2][NC];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const u16* blk = S.bdat + (size_t)bi[q] * BLK * PLANES + (lane & 31) * 32 + kh * 8;
-#pragma unroll
-      for (int h = 0; h < 2; ++h) {
-#pragma unroll
-        for (int pl = 0; pl < PLANES; ++pl) a[q][h][pl] = *reinterpret_cast<const bf16x8*>(blk + pl * BLK + h * 16);
-#pragma unroll
-        for (int c = 0; c < NC; ++c) {
-          const u16* src = IN_TILED ? X + ((size_t)bcol[q] * p + j[c]) * 32 + h * 16 + kh * 8
-                                    : X + (size_t)j[c] * ldx + bcol[q] * 32 + h * 16 + kh * 8;
-          x[q][h][c] = *reinterpret_cast<const bf16x8*>(src);
-        }
-      }
-    }
-    __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-    for (int q = 0; q < 4; ++q)
-#pragma unroll
-      for (int h = 0; h < 2; ++h)
-#pragma unroll
-        for (int c = 0; c < NC; ++c) {
-          acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[q][h][0], x[q][h][c], acc[c], 0, 0, 0);
-          if (SPLIT) acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[q][h][PLANES - 1], x[q][h][c], acc[c], 0, 0, 0);
-        }
-  }
-#pragma unroll
-  for (int c = 0; c < NC; ++c) {
-    if (!jok[c]) continue;
-    u16* ycol = OUT_TILED ? Y + ((size_t)br * p + jc[c]) * 32 : Y + (size_t)jc[c] * ldy + br * 32;
-#pragma unroll
-    for (int gg = 0; gg < 4; ++gg) {
-      const int r0 = 8 * gg + 4 * kh;
-      u16 o[4];
-#pragma unroll
-      for (int t = 0; t < 4; ++t) o[t] = f2bf(acc[c][4 * gg + t]);
-      uint2 w;
-      w.x = (unsigned)o[0] | ((unsigned)o[1] << 16);
-      w.y = (unsigned)o[2] | ((unsigned)o[3] << 16);
-      *reinterpret_cast<uint2*>(ycol + r0) = w;
-    }
-  }
-}
-
-template <bool SPLIT>
-__global__ __launch_bounds__(256) void k_bsr_chain(BsrChainArgs A, int n, int p, const u16* __restrict__ X, int ldx,
-                                                   u16* __restrict__ Y, int ldy, u16* __restrict__ T0,
-                                                   u16* __restrict__ T1, int* __restrict__ ctl, int spin_max) {
-  __shared__ int s_item, s_fail;
-  const int lane = threadIdx.x & 63;
-  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  unsigned xreg;
-  asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID, 0, 4)" : "=s"(xreg));
-  const int xcc = (int)(xreg & 7u);
-  const int J = A.J, nbr = n / 32, quads = (nbr + 3) / 4;
-  const int P = (p + 63) / 64;
-  const int npx = P > xcc ? (P - xcc + 7) / 8 : 0;  // panels xcc, xcc + 8, ...
-  const int items = npx * quads;
-  int* claim = ctl;
-  int* done = ctl + 8 * J;
-  int* tail = ctl + 16 * J;  // finish count, XCD mask, error
-  if (threadIdx.x == 0) s_fail = 0;
-  for (int f = 0; f < J; ++f) {
-    const BsrStage S = A.st[f];
-    const u16* in = f == 0 ? X : (((f - 1) & 1) ? T1 : T0);
-    u16* out = f == J - 1 ? Y : ((f & 1) ? T1 : T0);
-    for (;;) {
-      if (threadIdx.x == 0) s_item = xcd_add(&claim[xcc * J + f], 1);
-      __syncthreads();
-      const int item = s_item;
-      __syncthreads();
-      if (item >= items) break;
-      const int panel = xcc + 8 * (item / quads), br = (item % quads) * 4 + wave;
-      const bool it_ = f > 0, ot = f < J - 1;
-      if (it_ && ot)
-        chain_item<SPLIT, true, true>(S, br, nbr, panel, p, in, ldx, out, ldy, lane);
-      else if (it_)
-        chain_item<SPLIT, true, false>(S, br, nbr, panel, p, in, ldx, out, ldy, lane);
-      else if (ot)
-        chain_item<SPLIT, false, true>(S, br, nbr, panel, p, in, ldx, out, ldy, lane);
-      else
-        chain_item<SPLIT, false, false>(S, br, nbr, panel, p, in, ldx, out, ldy, lane);
-      asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // this wave's stores are in the L2
-      __syncthreads();
-      if (threadIdx.x == 0) xcd_add(&done[xcc * J + f], 1);
-    }
-    if (f < J - 1) {
-      if (threadIdx.x == 0) {
-        int k = 0;
-        while (xcd_add(&done[xcc * J + f], 0) < items && ++k < spin_max) __builtin_amdgcn_s_sleep(2);
-        if (k >= spin_max) {
-          s_fail = 1;
-          __hip_atomic_store(&tail[2], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-      }
-      __syncthreads();
-      if (s_fail) return;
-      asm volatile("buffer_inv sc0" ::: "memory");  // drop L1 lines: read this XCD's L2
-    }
-  }
-  if (threadIdx.x == 0) {
-    __hip_atomic_fetch_or(&tail[1], 1 << xcc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (__hip_atomic_fetch_add(&tail[0], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) == (int)gridDim.x - 1) {
-      const int need = P >= 8 ? 0xff : (1 << P) - 1;
-      const int seen = __hip_atomic_load(&tail[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      if ((seen & need) != need) __hip_atomic_store(&tail[2], 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-  }
-}
-
 // in_tiled / out_tiled select the activation layouts (see above); ldx/ldy are
 // only used for column-major operands.
 static void bsr_mul(const DMat& B, char op, const void* X, int ldx, int p, void* Y, int ldy, float alpha,
@@ -849,77 +682,8 @@ void mixed_mul_dense(const DMat& S, char op, const void* X, int ldx, int ncols, 
   FH_HIP(hipStreamSynchronize(st));  // R returns to the pool
 }
 
-// The persistent chain: square factors of one size and one value format, each
-// with an ELL view, at most BSR_CHAIN_MAX of them.  FH_BSR_PERSIST=0 turns it
-// off.  The first use in a process is checked (synchronously): if the kernel
-// reports a missing XCD or a wait that ran out, the path is disabled for the
-// process and the chain is recomputed with one launch per factor.
-static int g_persist_state = 0;  // 0 unchecked, 1 verified, -1 disabled
-
-static bool bsr_chain_persistent(const std::vector<DMatP>& f, char op, const void* X, int p, void* Y, hipStream_t st,
-                                 DBuf* ws) {
-  if (g_persist_state < 0) return false;
-  if (const char* e = getenv("FH_BSR_PERSIST"))
-    if (e[0] == '0') return false;
-  const int J = (int)f.size();
-  if (J < 2 || J > BSR_CHAIN_MAX || p <= 0) return false;
-  const int n = f[0]->m;
-  const bool split = f[0]->split;
-  const bool tr = op != 'N';
-  for (auto& x : f)
-    if (x->kind != K_BSR || x->m != n || x->n != n || x->split != split || (tr ? x->tell_w : x->ell_w) == 0)
-      return false;
-  if (n % 32 != 0) return false;
-  BsrChainArgs A{};
-  A.J = J;
-  for (int i = 0; i < J; ++i) {
-    const DMat& S = (op == 'N') ? *f[J - 1 - i] : *f[i];
-    A.st[i].ell = (tr ? S.tell : S.ell).as<int2>();
-    A.st[i].bdat = (tr ? S.tbdat : S.bdat).as<u16>();
-    A.st[i].W = tr ? S.tell_w : S.ell_w;
-  }
-  const size_t act = sizeof(u16) * (size_t)n * p;
-  const int nctl = 16 * J + 3;
-  for (int b = 0; b < 2; ++b)
-    if (ws[b].bytes < act) {
-      FH_HIP(hipStreamSynchronize(st));
-      ws[b].alloc(act);
-    }
-  if (ws[2].bytes < sizeof(int) * nctl) {
-    FH_HIP(hipStreamSynchronize(st));
-    ws[2].alloc(sizeof(int) * nctl);
-  }
-  int cus = 0, dev = 0;
-  FH_HIP(hipGetDevice(&dev));
-  FH_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-  hipLaunchKernelGGL(k_chain_reset, dim3(1), dim3(256), 0, st, ws[2].as<int>(), nctl);
-  FH_LAUNCH_CHECK();
-  const int spin_max = 1 << 22;
-  // as many workgroups as fit at once (3 per CU for BF16 blocks, 2 with the
-  // hi/lo planes): each XCD then has enough of them to take every item of a
-  // factor in one round
-  int per_cu = split ? 2 : 3;
-  if (const char* e = getenv("FH_BSR_CHAIN_WG")) per_cu = std::max(1, std::min(8, atoi(e)));
-  const dim3 grid((unsigned)(cus * per_cu));
-  if (split)
-    hipLaunchKernelGGL(k_bsr_chain<true>, grid, dim3(256), 0, st, A, n, p, (const u16*)X, n, (u16*)Y, n,
-                       ws[0].as<u16>(), ws[1].as<u16>(), ws[2].as<int>(), spin_max);
-  else
-    hipLaunchKernelGGL(k_bsr_chain<false>, grid, dim3(256), 0, st, A, n, p, (const u16*)X, n, (u16*)Y, n,
-                       ws[0].as<u16>(), ws[1].as<u16>(), ws[2].as<int>(), spin_max);
-  FH_LAUNCH_CHECK();
-  if (g_persist_state == 0) {
-    int err = 0;
-    FH_HIP(hipMemcpyAsync(&err, ws[2].as<int>() + 16 * J + 2, sizeof(int), hipMemcpyDeviceToHost, st));
-    FH_HIP(hipStreamSynchronize(st));
-    g_persist_state = err ? -1 : 1;
-    if (err) return false;  // recomputed by the per-factor path
-  }
-  return true;
-}
-
 // Chain product for BSR (BF16 activations) or FP32 CSR chains.  Returns the
-// number of product kernels launched (1 for the persistent BSR chain).
+// number of product kernels launched.
 int faust_multiply_mixed(const std::vector<DMatP>& f, char op, const void* X, int p, void* Y, hipStream_t st,
                           DBuf* ws, FaustTimers* tm) {
   const int J = (int)f.size();
@@ -942,7 +706,6 @@ int faust_multiply_mixed(const std::vector<DMatP>& f, char op, const void* X, in
       ws[b].alloc(need);
     }
   int in_rows = in_rows0;
-  if (bsr && !(tm && tm->on) && bsr_chain_persistent(f, op, X, p, Y, st, ws)) return 1;
   if (bsr) {
     const void* in = X;
     for (int i = 0; i < J; ++i) {

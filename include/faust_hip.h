@@ -97,10 +97,9 @@ int fh_faust_free(fh_faust_t F);
  * (faust_Transform.hpp:1714-1812): right-to-left for 'N', left-to-right with
  * per-factor op for 'T'/'H', two ping-pong intermediates. */
 int fh_faust_multiply(fh_faust_t F, char op, const void *X, int p, void *Y, void *stream);
-/* Number of product kernels the last fh_faust_multiply on F launched: one per
- * factor, or 1 when a chain of square BSR factors ran as the single
- * persistent kernel (intermediates kept in each XCD's L2).  No counterpart in
- * the reference; for benchmarks and tests. */
+/* Number of product kernels the last fh_faust_multiply on F launched (one
+ * per factor, plus the input transpose of FP32 CSR chains).  No counterpart
+ * in the reference; for benchmarks and tests. */
 int fh_faust_last_launches(fh_faust_t F, int *launches);
 /* Per-factor device timers: HIP events recorded on the launch stream around
  * each factor's kernel of every fh_faust_multiply while enabled (the

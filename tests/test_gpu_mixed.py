@@ -169,10 +169,25 @@ def test_csr_fp32_rows_per_thread(gpu, monkeypatch, rpt, op):
 
 
 @pytest.mark.parametrize("dtype", ["bf16", np.float32])
+def test_bsr_chain_launch_count(gpu, dtype):
+    # one product kernel per factor
+    import ctypes as C
+    from faust_amd import _lib
+    rng = np.random.default_rng(13)
+    facs = [rand_bsr(512, 512, 32, 4, rng, scale=1 / np.sqrt(128)) for _ in range(3)]
+    F = faust_amd.Faust(facs, dtype=dtype)
+    F @ bf16_round(rng.standard_normal((512, 64)))
+    n = C.c_int()
+    _lib.check(_lib.lib().fh_faust_last_launches(F.h, C.byref(n)))
+    assert n.value == 3
+
+
+@pytest.mark.skip(reason="the single-launch persistent BSR chain (per-XCD work claiming, bounded waits) did not "
+                         "finish on the GPU in its first run and no cause was shown; the path was removed and "
+                         "chains run one kernel per factor")
+@pytest.mark.parametrize("dtype", ["bf16", np.float32])
 @pytest.mark.parametrize("op", ["N", "T"])
 def test_bsr_persistent_chain_matches_per_factor(gpu, monkeypatch, dtype, op):
-    # the single-launch chain (intermediates kept per XCD) computes exactly
-    # what the per-factor ELL kernels compute
     import ctypes as C
     from faust_amd import _lib
     rng = np.random.default_rng(13)
@@ -191,9 +206,5 @@ def test_bsr_persistent_chain_matches_per_factor(gpu, monkeypatch, dtype, op):
         monkeypatch.setenv("FH_BSR_PERSIST", "0")
         monkeypatch.setenv("FH_BSR_KERNEL", "2")
         Y2 = G @ X
-        _lib.check(_lib.lib().fh_faust_last_launches(G.h, C.byref(n)))
-        assert n.value == 3
         monkeypatch.delenv("FH_BSR_KERNEL")
         assert np.array_equal(Y1, Y2)
-        ref = oracle.chain_multiply(facs, X, op)
-        assert np.linalg.norm(Y1 - ref) / np.linalg.norm(ref) < 8e-3
