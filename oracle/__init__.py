@@ -167,6 +167,16 @@ def get_product(factors):
     return out
 
 
+def dynprog_tables(factors):
+    """(cost, split) n x n tables of dynprog_multiply for the chain."""
+    pk = _FacPack(factors)
+    n = len(factors)
+    cost = np.zeros((n, n), dtype=np.int32)
+    split = np.zeros((n, n), dtype=np.int32)
+    _check(lib().or_dynprog_tables(pk.arr, n, cost.ctypes.data_as(C.c_void_p), split.ctypes.data_as(C.c_void_p)))
+    return cost, split
+
+
 def spectral_norm(factors, maxit=100, thr=1e-6):
     pk = _FacPack(factors)
     nrm = C.c_double()

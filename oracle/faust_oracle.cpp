@@ -20,6 +20,26 @@
 //   stopping criterion     src/algorithm/factorization/faust_StoppingCriterion.hpp:113-150
 //
 // Arithmetic follows the canonical order documented in faust_oracle.h.
+//
+// How each part is pinned (tests/test_oracle.py unless noted):
+//   chain product, get_product  numpy products; exact integer chains F*I =
+//                               get_product and F^T X = (X^T F)^T; WHT butterflies
+//                               multiply to the Sylvester Hadamard; BSR = dense
+//   gemm alpha/beta, reductions numpy; closed sums
+//   spectral norm / power it.   diagonal and orthogonal closed forms, numpy SVD;
+//                               start vector = glibc rand through ctypes
+//   projectors                  std::partial_sort itself (tie semantics), brute force
+//                               without ties, hand traces of tied inputs
+//                               (test_heap_select.py checks the GPU heap emulation)
+//   dynprog order               brute-force enumeration of all parenthesizations
+//   palm4msa2 / hierarchical    least-squares lambda, objective decrease, exact
+//                               recovery of Hadamard 16^2..64^2; grad1 == grad2 to
+//                               rounding; golden fixtures (tests/golden/)
+// Parity unpinned (restated from the reference, no independent check):
+//   Eigen's own summation order (replaced by the canonical order);
+//   the greedy end-pair order of multiply_order_opt_all_ends and the dense/sparse
+//   format rule of prox_*_gen (their values are pinned within rounding through the
+//   products above; the exact order and the returned formats are not).
 #include "faust_oracle.h"
 
 #include <algorithm>
@@ -535,11 +555,13 @@ int32_t cost_sparse_sparse(int a_nr, int a_nc, int a_nnz, bool a_tr, int b_nnz) 
 
 // dynprog_multiply (faust_prod_opt_gen.hpp:81-200), op 'N' only (the PALM
 // packing path); BSR factors are not produced by PALM.
-Mat dynprog_multiply(const std::vector<MatP>& f) {
+// The DP tables: c[i][j] the (wrapping int32) cost of the best product of
+// f[i..j], inds[i][j] the split k of that product ((i..k)(k+1..j)).
+void dynprog_tables(const std::vector<MatP>& f, std::vector<std::vector<int32_t>>& c,
+                    std::vector<std::vector<int>>& inds) {
   int n = (int)f.size();
-  if (n == 1) return dense_of(*f[0]);
-  std::vector<std::vector<int32_t>> c(n, std::vector<int32_t>(n, 0));
-  std::vector<std::vector<int>> inds(n, std::vector<int>(n, 0));
+  c.assign(n, std::vector<int32_t>(n, 0));
+  inds.assign(n, std::vector<int>(n, 0));
   for (int p = 1; p < n; ++p)
     for (int i = 0; i < n - p; ++i) {
       int j = p + i;
@@ -564,6 +586,14 @@ Mat dynprog_multiply(const std::vector<MatP>& f) {
         }
       }
     }
+}
+
+Mat dynprog_multiply(const std::vector<MatP>& f) {
+  int n = (int)f.size();
+  if (n == 1) return dense_of(*f[0]);
+  std::vector<std::vector<int32_t>> c;
+  std::vector<std::vector<int>> inds;
+  dynprog_tables(f, c, inds);
   std::function<Mat(int, int)> rec = [&](int i, int j) -> Mat {
     if (i == j) return dense_of(*f[i]);
     if (j - i == 1) return matmul_new(*f[i], 'N', *f[j], 'N');
@@ -1026,6 +1056,20 @@ int or_get_product(const or_fac* facs, int nfacs, double* out) {
     Chain c = chain_of(facs, nfacs);
     Mat P = get_product(c);
     std::memcpy(out, P.d.data(), sizeof(double) * P.d.size());
+  })
+}
+
+int or_dynprog_tables(const or_fac* facs, int nfacs, int32_t* cost, int* split) {
+  OR_TRY({
+    Chain ch = chain_of(facs, nfacs);
+    std::vector<std::vector<int32_t>> c;
+    std::vector<std::vector<int>> inds;
+    dynprog_tables(ch.f, c, inds);
+    for (int i = 0; i < nfacs; ++i)
+      for (int j = 0; j < nfacs; ++j) {
+        cost[i * nfacs + j] = c[i][j];
+        split[i * nfacs + j] = inds[i][j];
+      }
   })
 }
 

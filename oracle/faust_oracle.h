@@ -96,6 +96,11 @@ int or_chain_multiply(const or_fac *facs, int nfacs, char op, const double *X,
 /* dense product of the chain (faust_Transform.hpp:599-652, op 'N') */
 int or_get_product(const or_fac *facs, int nfacs, double *out);
 
+/* dynprog_multiply's tables (faust_prod_opt_gen.hpp:81-200): cost[i*n+j] the
+ * wrapping int32 cost of the best product of factors i..j, split[i*n+j] its
+ * split k, i.e. (i..k)(k+1..j) */
+int or_dynprog_tables(const or_fac *facs, int nfacs, int32_t *cost, int *split);
+
 /* spectral norm by power iteration on the F F^H / F^H F chain
  * (faust_Transform.hpp:684-709, faust_linear_algebra.hpp:808-898) */
 int or_spectral_norm(const or_fac *facs, int nfacs, int maxit, double thr,

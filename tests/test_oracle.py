@@ -228,3 +228,57 @@ def test_grad1_agrees_with_grad2():
             assert np.abs(np.array(r1.traj)[:, 2] - np.array(r2.traj)[:, 2]).max() < 1e-12
             for a, b in zip(r1.factors, r2.factors):
                 assert np.abs(a - b).max() < 1e-12
+
+
+def _trees(i, j):
+    """All parenthesizations of factors i..j as (cost-free) nested tuples."""
+    if i == j:
+        yield i
+        return
+    for k in range(i, j):
+        for left in _trees(i, k):
+            for right in _trees(k + 1, j):
+                yield (left, right, i, k, j)
+
+
+def test_dynprog_order_is_the_cheapest_parenthesization():
+    # enumerate every binary tree over 2..6 mixed dense/sparse factors; the
+    # cost of joining (i..k)(k+1..j) depends on the formats of factors i and j
+    # and on the dimensions (faust_prod_opt.cpp:6-86, faust_prod_opt_gen.hpp:81-200)
+    rng = np.random.default_rng(8)
+    for trial in range(40):
+        n = int(rng.integers(2, 7))
+        dims = rng.integers(3, 12, size=n + 1)
+        facs = []
+        for t in range(n):
+            a = rng.standard_normal((dims[t], dims[t + 1])) * (rng.random((dims[t], dims[t + 1])) < 0.3)
+            facs.append(np.asfortranarray(a) if rng.random() < 0.5 else sp.csr_matrix(a))
+        dense = [isinstance(f, np.ndarray) for f in facs]
+        nnz = [int(np.count_nonzero(f)) if dense[t] else f.nnz for t, f in enumerate(facs)]
+
+        def join(i, k, j):
+            if dense[i] and dense[j]:
+                return dims[i] * dims[k + 1] * dims[j + 1]
+            if dense[i]:
+                return dims[i] * nnz[j]
+            if dense[j]:
+                return nnz[i] * dims[j + 1]
+            return nnz[i] * nnz[j] // dims[i]
+
+        def tree_cost(t):
+            if isinstance(t, (int, np.integer)):
+                return 0
+            left, right, i, k, j = t
+            return tree_cost(left) + tree_cost(right) + join(i, k, j)
+
+        best = min(tree_cost(t) for t in _trees(0, n - 1))
+        cost, split = oracle.dynprog_tables(facs)
+        assert cost[0, n - 1] == best
+
+        def chosen(i, j):
+            if i == j:
+                return 0
+            k = split[i, j]
+            return chosen(i, k) + chosen(k + 1, j) + join(i, k, j)
+
+        assert chosen(0, n - 1) == best
