@@ -82,7 +82,7 @@ def _bsr_workload(values):
                 bytes_per_launch=per_factor, launches_per_step=J, bytes_per_step=J * per_factor,
                 flops_per_step=J * 2 * nb * bs * bs * p,
                 kernel=("k_bsr_ell<false,*,*,2>" if values == "bf16" else "k_bsr_ell<true,*,*,4>"),
-                cpu_reps_cap=1)
+                cpu_reps_cap=40)
 
 
 def workload_chain_bsr_bf16():
