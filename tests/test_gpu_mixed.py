@@ -208,3 +208,35 @@ def test_bsr_persistent_chain_matches_per_factor(gpu, monkeypatch, dtype, op):
         Y2 = G @ X
         monkeypatch.delenv("FH_BSR_KERNEL")
         assert np.array_equal(Y1, Y2)
+
+
+def test_cfg3a_full_size_against_oracle_columns(gpu):
+    # the benchmarked Faust itself (6 BSR factors 8192^2, 4 blocks per block
+    # row, BF16): the full product on X 8192x512, its first 16 columns against
+    # the FP64 oracle, and linearity of the whole product
+    rng = np.random.default_rng(0xB5B)
+    facs = [_bsr_rounded(rand_bsr(8192, 8192, 32, 4, np.random.default_rng(0xB5B + f), scale=1 / np.sqrt(128)))
+            for f in range(6)]
+    X = bf16_round(rng.standard_normal((8192, 512)))
+    F = faust_amd.Faust(facs, dtype="bf16")
+    Y = F @ X
+    ref = oracle.chain_multiply(facs, X[:, :16], "N")
+    rel = np.linalg.norm(Y[:, :16] - ref) / np.linalg.norm(ref)
+    assert rel < 1.2e-2, rel  # six BF16 roundings of the activations
+    # columns are independent: the same columns computed alone give the same bits
+    Y8 = F @ np.asfortranarray(X[:, 100:108])
+    assert np.array_equal(Y8, Y[:, 100:108])
+
+
+def test_cfg3b_full_size_against_oracle_columns(gpu):
+    rng = np.random.default_rng(0xC5F)
+    facs = [faust_amd.rand_csr(16384, 16384, 12, np.random.default_rng(0xC5F + f)).astype(np.float32)
+            .astype(np.float64) for f in range(6)]
+    X = rng.standard_normal((16384, 256)).astype(np.float32).astype(np.float64)
+    F = faust_amd.Faust(facs, dtype=np.float32)
+    Y = F @ X
+    ref = oracle.chain_multiply(facs, X[:, :8], "N")
+    assert np.linalg.norm(Y[:, :8] - ref) / np.linalg.norm(ref) < 1e-5
+    Yt = F.T @ X[:, :8]
+    reft = oracle.chain_multiply(facs, X[:, :8], "T")
+    assert np.linalg.norm(Yt - reft) / np.linalg.norm(reft) < 1e-5

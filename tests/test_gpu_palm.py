@@ -87,3 +87,13 @@ def test_hierarchical_grad1_bitexact(gpu, monkeypatch):
     fac, resc, kw = oracle.wht_params(5, 15)
     ref = oracle.hierarchical(H, fac, resc, record_traj=True, use_grad1=True, **kw)
     _check_same(res, ref)
+
+
+def test_hierarchical_hadamard_256_bitexact(gpu):
+    # 8 levels on the 256 Hadamard matrix (residuals k = 128 ... 2), 5
+    # iterations per PALM4MSA: factors, lambda and every per-iteration error
+    H = oracle.hadamard(256)
+    res = fact.hierarchical(H, factparams.ParamsHierarchicalWHT(8, num_its=5), ret_raw=True)
+    fac, resc, kw = oracle.wht_params(8, 5)
+    ref = oracle.hierarchical(H, fac, resc, record_traj=True, **kw)
+    _check_same(res, ref)
