@@ -54,12 +54,23 @@ static inline float host_bf2f(u16 h) {
 }
 
 // ---------------------------------------------------------------- BSR MFMA
-// Activation layouts: column-major (ld), or "tiled": for every 32-row block
-// rb a [p][32] array, element (r, j) at ((r/32)*p + j)*32 + r%32, so that the
-// 32 rows of one column inside a block are one 64-byte segment and the
-// B-operand of 32 consecutive columns is one contiguous 2 KiB tile.  Chains
-// keep their intermediates tiled; only the first factor reads and the last
-// factor writes column-major.
+// Activation layouts: column-major (ld), or "tiled" (fragment order): the
+// columns are cut into G = ceil(p/32) groups of 32 (the last one padded), and
+// the 32 x 32 tile of block-row rb, column group g is 1024 consecutive
+// entries at (rb*G + g)*1024, entry (r, c) of the tile at
+//   ((r / 16)*64 + ((r / 8) % 2)*32 + c)*8 + r % 8.
+// That is the MFMA B-operand order: for the K half h the 64 lanes read 16
+// bytes each at (h*64 + lane)*8, one contiguous 1 KiB, and the accumulator of
+// a 32x32 output tile leaves as four contiguous 512-byte stores.  Chains keep
+// their intermediates tiled; only the first factor reads and the last factor
+// writes column-major.  Padding columns (p..32G) are never written and only
+// feed accumulator columns that are never stored.
+__device__ __forceinline__ size_t frag_tile(int rb, int G, int g) { return ((size_t)rb * G + g) * 1024; }
+// offset, inside a tile, of the 4 rows 8*gg + 4*kh .. +3 of column c (one
+// accumulator quarter of lane (c, kh))
+__device__ __forceinline__ int frag_store_off(int gg, int kh, int c) {
+  return (((gg >> 1) * 64 + (gg & 1) * 32 + c) << 3) + 4 * kh;
+}
 // Wave work: one block-row x 64 columns (two 32x32 accumulators sharing the
 // A operand); workgroup: 4 block-rows x 64 columns.
 template <bool SPLIT, bool IN_TILED, bool OUT_TILED>
@@ -72,12 +83,14 @@ __global__ __launch_bounds__(256) void k_bsr_mfma(int nbr, int p, const int* __r
   const int br = quad * 4 + wave;
   if (br >= nbr) return;
   const int kh = lane >> 5;  // which 8-wide k group of the 16
-  int jc[2];
+  const int G = (p + 31) >> 5;
+  int jc[2], gc[2];
   bool jok[2];
 #pragma unroll
   for (int c = 0; c < 2; ++c) {
     jc[c] = panel * 64 + c * 32 + (lane & 31);
     jok[c] = jc[c] < p;
+    gc[c] = (panel * 2 + c < G) ? panel * 2 + c : 0;
   }
   constexpr int BLK = 1024;  // entries per 32x32 block
   constexpr int PLANES = SPLIT ? 2 : 1;
@@ -98,7 +111,7 @@ __global__ __launch_bounds__(256) void k_bsr_mfma(int nbr, int p, const int* __r
 #pragma unroll
           for (int c = 0; c < 2; ++c) {
             const int j = jok[c] ? jc[c] : 0;
-            const u16* src = IN_TILED ? X + ((size_t)bc * p + j) * 32 + h * 16 + kh * 8
+            const u16* src = IN_TILED ? X + frag_tile(bc, G, gc[c]) + (h * 64 + lane) * 8
                                       : X + (size_t)j * ldx + bc * 32 + h * 16 + kh * 8;
             x[q][h][c] = jok[c] ? *reinterpret_cast<const bf16x8*>(src) : bf16x8{};
           }
@@ -122,21 +135,21 @@ __global__ __launch_bounds__(256) void k_bsr_mfma(int nbr, int p, const int* __r
   for (int c = 0; c < 2; ++c) {
     if (!jok[c]) continue;
     const int j = jc[c];
-    u16* ycol = OUT_TILED ? Y + ((size_t)br * p + j) * 32 : Y + (size_t)j * ldy + br * 32;
 #pragma unroll
     for (int g = 0; g < 4; ++g) {
-      const int r0 = 8 * g + 4 * kh;
+      u16* yq = OUT_TILED ? Y + frag_tile(br, G, gc[c]) + frag_store_off(g, kh, lane & 31)
+                          : Y + (size_t)j * ldy + br * 32 + 8 * g + 4 * kh;
       u16 o[4];
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
         float v = alpha * acc[c][4 * g + t];
-        if (beta != 0.0f) v = fmaf(beta, bf2f(ycol[r0 + t]), v);
+        if (beta != 0.0f) v = fmaf(beta, bf2f(yq[t]), v);
         o[t] = f2bf(v);
       }
       uint2 w;
       w.x = (unsigned)o[0] | ((unsigned)o[1] << 16);
       w.y = (unsigned)o[2] | ((unsigned)o[3] << 16);
-      *reinterpret_cast<uint2*>(ycol + r0) = w;
+      *reinterpret_cast<uint2*>(yq) = w;
     }
   }
 }
@@ -182,12 +195,14 @@ __global__ __launch_bounds__(256) void k_bsr_ell(int nbr, int p, int W, const in
   const int br = quad * 4 + wave;
   if (br >= nbr) return;
   const int kh = lane >> 5;
-  int jc[NC];
+  const int G = (p + 31) >> 5;
+  int jc[NC], gc[NC];
   bool jok[NC];
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
     jc[c] = panel * 32 * NC + c * 32 + (lane & 31);
     jok[c] = jc[c] < p;
+    gc[c] = (panel * NC + c < G) ? panel * NC + c : 0;  // groups past p read group 0
   }
   constexpr int BLK = 1024;
   constexpr int PLANES = SPLIT ? 2 : 1;
@@ -217,7 +232,7 @@ __global__ __launch_bounds__(256) void k_bsr_ell(int nbr, int p, int W, const in
           a[q][h][pl] = ld_stream(reinterpret_cast<const bf16x8*>(blk + pl * BLK + h * 16), NT);
 #pragma unroll
         for (int c = 0; c < NC; ++c) {
-          const u16* src = IN_TILED ? X + ((size_t)bcol[q] * p + j[c]) * 32 + h * 16 + kh * 8
+          const u16* src = IN_TILED ? X + frag_tile(bcol[q], G, gc[c]) + (h * 64 + lane) * 8
                                     : X + (size_t)j[c] * ldx + bcol[q] * 32 + h * 16 + kh * 8;
           x[q][h][c] = *reinterpret_cast<const bf16x8*>(src);
         }
@@ -241,15 +256,15 @@ __global__ __launch_bounds__(256) void k_bsr_ell(int nbr, int p, int W, const in
   for (int c = 0; c < NC; ++c) {
     if (!jok[c]) continue;
     const int j = jc[c];
-    u16* ycol = OUT_TILED ? Y + ((size_t)br * p + j) * 32 : Y + (size_t)j * ldy + br * 32;
 #pragma unroll
     for (int gg = 0; gg < 4; ++gg) {
-      const int r0 = 8 * gg + 4 * kh;
+      u16* yq = OUT_TILED ? Y + frag_tile(br, G, gc[c]) + frag_store_off(gg, kh, lane & 31)
+                          : Y + (size_t)j * ldy + br * 32 + 8 * gg + 4 * kh;
       u16 o[4];
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
         float v = alpha * acc[c][4 * gg + t];
-        if (beta != 0.0f) v = fmaf(beta, bf2f(ycol[r0 + t]), v);
+        if (beta != 0.0f) v = fmaf(beta, bf2f(yq[t]), v);
         o[t] = f2bf(v);
       }
       uint2 w;
@@ -257,9 +272,9 @@ __global__ __launch_bounds__(256) void k_bsr_ell(int nbr, int p, int W, const in
       w.y = (unsigned)o[2] | ((unsigned)o[3] << 16);
       if (NT) {
         u32x2 wv = {w.x, w.y};
-        __builtin_nontemporal_store(wv, reinterpret_cast<u32x2*>(ycol + r0));
+        __builtin_nontemporal_store(wv, reinterpret_cast<u32x2*>(yq));
       } else {
-        *reinterpret_cast<uint2*>(ycol + r0) = w;
+        *reinterpret_cast<uint2*>(yq) = w;
       }
     }
   }
@@ -697,7 +712,8 @@ int faust_multiply_mixed(const std::vector<DMatP>& f, char op, const void* X, in
   const int in_rows0 = (op == 'N') ? f.back()->n : f.front()->m;
   maxr = std::max(maxr, in_rows0);
   const int ldr = bsr ? 0 : (p + 3) & ~3;
-  const size_t need = bsr ? sizeof(u16) * std::max<size_t>((size_t)maxr * p, 1)
+  // BSR intermediates are tiled: p padded to whole 32-column groups
+  const size_t need = bsr ? sizeof(u16) * std::max<size_t>((size_t)maxr * (((size_t)p + 31) / 32 * 32), 1)
                           : sizeof(float) * std::max<size_t>((size_t)maxr * ldr, 1);
   const int nbuf = bsr ? std::min(2, J - 1) : std::min(2, J);
   for (int b = 0; b < nbuf; ++b)

@@ -144,6 +144,13 @@ def test_bsr_kernel_variants(gpu, monkeypatch, kernel, wide, op, xsplit_nt):
         ref = oracle.chain_multiply(facs, X, op)
         rel = np.linalg.norm(Y - ref) / np.linalg.norm(ref)
         assert rel < 8e-3, rel
+        # the tiled intermediates hold the same BF16 values as column-major
+        # ones: the chain equals the factors applied one at a time, bit for bit
+        Z = X
+        for f in (facs[::-1] if op == "N" else facs):
+            G = faust_amd.Faust([f], dtype="bf16")
+            Z = (G @ Z) if op == "N" else (G.T @ Z)
+        assert np.array_equal(Y, Z)
         Y1 = faust_amd.Faust([facs[0]], dtype="bf16") @ X
         r1 = oracle.chain_multiply([facs[0]], X, "N")
         assert np.all(np.abs(Y1 - r1) <= 2.0 ** -8 * np.abs(r1) + 1e-3 * np.abs(r1).max())
