@@ -280,15 +280,109 @@ __global__ __launch_bounds__(256) void k_bsr_ell(int nbr, int p, int W, const in
   }
 }
 
-// Which BSR kernel: FH_BSR_KERNEL=0 forces the row-pointer kernel, 2 / 4 the
-// ELL kernel with that many 32-column tiles per wave.  Default (measured on
-// 8192^2, 4 blocks per block row, 512 columns): 2 for BF16 blocks (11.3 us
-// vs 12.6 us per factor), 4 for FP32 hi/lo blocks (13.8 vs 15.2 us), whose
-// doubled block bytes favour fewer column panels re-reading them.
+// K-split ELL form: two waves share one block-row x 32 columns, each taking
+// one 16-wide K half of every block (half the loads and MFMAs per wave, twice
+// the waves in flight).  The two partial accumulators meet in LDS; each wave
+// then finishes and stores half of the output rows (K half 0 + K half 1).
+// Workgroup = 2 block-rows x 2 K halves x 32 columns.  A trailing odd block
+// row is computed twice (clamped) and stored once, so every wave reaches the
+// barrier.
+template <bool SPLIT, bool IN_TILED, bool OUT_TILED>
+__global__ __launch_bounds__(256) void k_bsr_ks(int nbr, int p, int W, const int2* __restrict__ ell,
+                                                const u16* __restrict__ bdat, const u16* __restrict__ X, int ldx,
+                                                u16* __restrict__ Y, int ldy, float alpha, float beta,
+                                                int npanels) {
+  __shared__ f32x16 part[4][64];
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int panel = blockIdx.x % npanels, pair = blockIdx.x / npanels;
+  const int hh = wave & 1;
+  const int br_raw = pair * 2 + (wave >> 1);
+  const bool row_ok = br_raw < nbr;
+  const int br = row_ok ? br_raw : nbr - 1;
+  const int kh = lane >> 5;
+  const int G = (p + 31) >> 5;  // == npanels
+  const int jc = panel * 32 + (lane & 31);
+  const bool jok = jc < p;
+  const int j = jok ? jc : 0;
+  constexpr int BLK = 1024;
+  constexpr int PLANES = SPLIT ? 2 : 1;
+  f32x16 acc = {};
+  const int4* slots = reinterpret_cast<const int4*>(ell + (size_t)br * W);
+  for (int g = 0; g < W; g += 4) {
+    const int4 s01 = slots[g / 2], s23 = slots[g / 2 + 1];
+    const int bi[4] = {s01.x, s01.z, s23.x, s23.z};
+    const int bcol[4] = {s01.y, s01.w, s23.y, s23.w};
+    bf16x8 a[4][PLANES], x[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const u16* blk = bdat + (size_t)bi[q] * BLK * PLANES + (lane & 31) * 32 + hh * 16 + kh * 8;
+#pragma unroll
+      for (int pl = 0; pl < PLANES; ++pl) a[q][pl] = *reinterpret_cast<const bf16x8*>(blk + pl * BLK);
+      const u16* src = IN_TILED ? X + frag_tile(bcol[q], G, panel) + (hh * 64 + lane) * 8
+                                : X + (size_t)j * ldx + bcol[q] * 32 + hh * 16 + kh * 8;
+      x[q] = *reinterpret_cast<const bf16x8*>(src);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[q][0], x[q], acc, 0, 0, 0);
+      if (SPLIT) acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[q][PLANES - 1], x[q], acc, 0, 0, 0);
+    }
+  }
+  part[wave][lane] = acc;
+  __syncthreads();
+  const f32x16 k0 = part[wave & ~1][lane], k1 = part[wave | 1][lane];
+  if (!row_ok || !jok) return;
+#pragma unroll
+  for (int gi = 0; gi < 2; ++gi) {
+    const int gg = 2 * hh + gi;
+    u16* yq = OUT_TILED ? Y + frag_tile(br, G, panel) + frag_store_off(gg, kh, lane & 31)
+                        : Y + (size_t)j * ldy + br * 32 + 8 * gg + 4 * kh;
+    u16 o[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      float v = alpha * (k0[4 * gg + t] + k1[4 * gg + t]);
+      if (beta != 0.0f) v = fmaf(beta, bf2f(yq[t]), v);
+      o[t] = f2bf(v);
+    }
+    uint2 w;
+    w.x = (unsigned)o[0] | ((unsigned)o[1] << 16);
+    w.y = (unsigned)o[2] | ((unsigned)o[3] << 16);
+    *reinterpret_cast<uint2*>(yq) = w;
+  }
+}
+
+static void bsr_ks_launch(bool split, bool in_tiled, bool out_tiled, dim3 grid, hipStream_t st, int nbr, int p,
+                          int W, const int2* ell, const u16* dat, const u16* X, int ldx, u16* Y, int ldy,
+                          float alpha, float beta, int P) {
+  const int sel = (split ? 4 : 0) | (in_tiled ? 2 : 0) | (out_tiled ? 1 : 0);
+#define FH_KS(S, I, O) \
+  hipLaunchKernelGGL((k_bsr_ks<S, I, O>), grid, dim3(256), 0, st, nbr, p, W, ell, dat, X, ldx, Y, ldy, alpha, beta, P)
+  switch (sel) {
+    case 0: FH_KS(false, false, false); break;
+    case 1: FH_KS(false, false, true); break;
+    case 2: FH_KS(false, true, false); break;
+    case 3: FH_KS(false, true, true); break;
+    case 4: FH_KS(true, false, false); break;
+    case 5: FH_KS(true, false, true); break;
+    case 6: FH_KS(true, true, false); break;
+    default: FH_KS(true, true, true); break;
+  }
+#undef FH_KS
+}
+
+// Which BSR kernel: FH_BSR_KERNEL=0 forces the row-pointer kernel, 1 / 2 / 4
+// the ELL kernel with that many 32-column tiles per wave, 3 the K-split ELL
+// kernel.  Defaults, measured on 8192^2, 4 blocks per block row, 512 columns,
+// per factor inside the chain: BF16 blocks 1 (8.7 us; 2: 10.0, 3: 9.1,
+// 4: 11.4), FP32 hi/lo blocks 3 (11.8 us; 1: 12.3, 2: 13.5, 4: 12.8).  More,
+// shorter waves win: each wave's time is mostly the latency of its slot load
+// and then of its gathers.
 static int bsr_kernel_choice(bool split) {
   const char* e = getenv("FH_BSR_KERNEL");
-  if (e && (e[0] == '0' || e[0] == '2' || e[0] == '4')) return e[0] - '0';
-  return split ? 4 : 2;
+  if (e && e[0] >= '0' && e[0] <= '4') return e[0] - '0';
+  return split ? 3 : 1;
 }
 
 template <bool S, bool I, bool O, int NC, bool NT>
@@ -305,6 +399,9 @@ static void bsr_ell_nc(int nc, bool nt, dim3 grid, hipStream_t st, int nbr, int 
   if (nc == 4) {
     if (nt) bsr_ell_go<S, I, O, 4, true>(grid, st, nbr, p, W, ell, dat, X, ldx, Y, ldy, alpha, beta, P, Q, xs);
     else bsr_ell_go<S, I, O, 4, false>(grid, st, nbr, p, W, ell, dat, X, ldx, Y, ldy, alpha, beta, P, Q, xs);
+  } else if (nc == 1) {
+    if (nt) bsr_ell_go<S, I, O, 1, true>(grid, st, nbr, p, W, ell, dat, X, ldx, Y, ldy, alpha, beta, P, Q, xs);
+    else bsr_ell_go<S, I, O, 1, false>(grid, st, nbr, p, W, ell, dat, X, ldx, Y, ldy, alpha, beta, P, Q, xs);
   } else {
     if (nt) bsr_ell_go<S, I, O, 2, true>(grid, st, nbr, p, W, ell, dat, X, ldx, Y, ldy, alpha, beta, P, Q, xs);
     else bsr_ell_go<S, I, O, 2, false>(grid, st, nbr, p, W, ell, dat, X, ldx, Y, ldy, alpha, beta, P, Q, xs);
@@ -344,6 +441,15 @@ static void bsr_mul(const DMat& B, char op, const void* X, int ldx, int p, void*
   const u16* dat = tr ? B.tbdat.as<u16>() : B.bdat.as<u16>();
   const int W = tr ? B.tell_w : B.ell_w;
   const int choice = bsr_kernel_choice(B.split);
+  if (W > 0 && choice == 3) {
+    const int2* ell = (tr ? B.tell : B.ell).as<int2>();
+    const int npanels = (p + 31) / 32;
+    const dim3 grid((unsigned)(npanels * ((nbr + 1) / 2)));
+    bsr_ks_launch(B.split, in_tiled, out_tiled, grid, st, nbr, p, W, ell, dat, (const u16*)X, ldx, (u16*)Y, ldy,
+                  alpha, beta, npanels);
+    FH_LAUNCH_CHECK();
+    return;
+  }
   if (W > 0 && choice != 0) {
     const int2* ell = (tr ? B.tell : B.ell).as<int2>();
     const int nc = choice;

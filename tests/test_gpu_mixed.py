@@ -123,22 +123,24 @@ def _ragged_bsr(rng, nbr, nbc, bs=32, wide_row=None):
     return sp.bsr_matrix((data, idx, indptr), shape=(nbr * bs, nbc * bs))
 
 
-@pytest.mark.parametrize("kernel", ["0", "2", "4"])
+@pytest.mark.parametrize("kernel", ["0", "1", "2", "3", "4"])
 @pytest.mark.parametrize("wide", [None, 20])
 @pytest.mark.parametrize("op", ["N", "T"])
 @pytest.mark.parametrize("xsplit_nt", ["1-0", "2-1"])
 def test_bsr_kernel_variants(gpu, monkeypatch, kernel, wide, op, xsplit_nt):
-    # row-pointer kernel, ELL with 64 / 128 columns per wave (with the XCD
-    # split of the block rows and streaming loads/stores); ragged block rows,
-    # empty block rows and (wide) a block row beyond the ELL limit; a 3-factor
-    # chain exercises the tiled intermediates
+    # row-pointer kernel, ELL with 32 / 64 / 128 columns per wave (with the XCD
+    # split of the block rows and streaming loads/stores), and the K-split ELL
+    # kernel (two block rows per workgroup: the odd block-row counts 25 leave
+    # one row alone); ragged block rows, empty block rows and (wide) a block
+    # row beyond the ELL limit; a 3-factor chain exercises the tiled
+    # intermediates
     monkeypatch.setenv("FH_BSR_KERNEL", kernel)
     monkeypatch.setenv("FH_BSR_XSPLIT", xsplit_nt[0])
     monkeypatch.setenv("FH_BSR_NT", xsplit_nt[2])
     rng = np.random.default_rng(11)
-    facs = [_bsr_rounded(_ragged_bsr(rng, 24, 24, wide_row=wide)) for _ in range(3)]
+    facs = [_bsr_rounded(_ragged_bsr(rng, r, c, wide_row=wide)) for r, c in ((25, 24), (24, 25), (25, 24))]
     for p in (200, 8):
-        X = bf16_round(rng.standard_normal((768, p)))
+        X = bf16_round(rng.standard_normal((32 * (24 if op == "N" else 25), p)))
         F = faust_amd.Faust(facs, dtype="bf16")
         Y = (F @ X) if op == "N" else (F.T @ X)
         ref = oracle.chain_multiply(facs, X, op)
@@ -151,8 +153,8 @@ def test_bsr_kernel_variants(gpu, monkeypatch, kernel, wide, op, xsplit_nt):
             G = faust_amd.Faust([f], dtype="bf16")
             Z = (G @ Z) if op == "N" else (G.T @ Z)
         assert np.array_equal(Y, Z)
-        Y1 = faust_amd.Faust([facs[0]], dtype="bf16") @ X
-        r1 = oracle.chain_multiply([facs[0]], X, "N")
+        Y1 = faust_amd.Faust([facs[-1]], dtype="bf16") @ X if op == "N" else faust_amd.Faust([facs[0]], dtype="bf16").T @ X
+        r1 = oracle.chain_multiply([facs[-1] if op == "N" else facs[0]], X, op)
         assert np.all(np.abs(Y1 - r1) <= 2.0 ** -8 * np.abs(r1) + 1e-3 * np.abs(r1).max())
 
 
