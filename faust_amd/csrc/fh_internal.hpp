@@ -105,7 +105,8 @@ struct DMat {
   int max_row = -1, max_col = -1;  // CSR uploaded from the host: widest row / column (-1: unknown)
   mutable bool has_csc = false;
   mutable DBuf cp, ri, cv; // CSC
-  // BSR (32x32 blocks): bdat holds each block row-major in BF16; FP32 values
+  // BSR (32x32 blocks): bdat holds each block in MFMA A-operand order, entry
+  // (i, k) at ((k/16)*64 + ((k/8)%2)*32 + i)*8 + k%8, in BF16; FP32 values
   // are stored as a BF16 pair per entry (hi plane then lo plane per block,
   // hi = bf16(v), lo = bf16(v - hi)).  t* is the BSR of the transpose.
   int bm = 0, bn = 0, bnnz = 0;

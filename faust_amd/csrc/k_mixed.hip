@@ -6,8 +6,10 @@
 // issues four MFMAs (K = 16 + 16, two column tiles).  Operand lanes:
 //   A: lane l holds block row (l % 32), 8 consecutive k at 8*(l / 32);
 //   B: lane l holds X column (l % 32), the same 8 consecutive k;
-// so with blocks stored row-major and X column-major (or tiled, below) every
-// operand is one 16-byte load.  FP32 block values are stored as a BF16 (hi, lo) pair and
+// blocks are stored in that operand order (entry (i, k) of a block at
+// ((k / 16)*64 + ((k / 8) % 2)*32 + i)*8 + k % 8), so the A operand of one K
+// half is one contiguous 1 KiB load, and X column-major (or tiled, below)
+// gives the B operand as one 16-byte load per lane.  FP32 block values are stored as a BF16 (hi, lo) pair and
 // cost one extra MFMA per step (error ~2^-16 relative per value, below the
 // BF16 rounding of the activations).
 // Work mapping: workgroup = 4 block-rows x 64 columns; consecutive workgroup
@@ -33,11 +35,14 @@ typedef unsigned short u16;
 typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
 
 __device__ __forceinline__ float bf2f(u16 h) { return __uint_as_float(((unsigned)h) << 16); }
-__device__ __forceinline__ u16 f2bf(float f) {
-  unsigned u = __float_as_uint(f);
-  if ((u & 0x7f800000u) == 0x7f800000u) return (u16)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));  // inf / nan
-  u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even
-  return (u16)(u >> 16);
+// Two FP32 -> packed BF16 (lo = a, hi = b), round to nearest even: one
+// v_cvt_pk_bf16_f32 on gfx950 (inf stays inf, NaN stays NaN).
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ unsigned f2bf2(float a, float b) {
+  bf16x2 r;
+  r[0] = (__bf16)a;
+  r[1] = (__bf16)b;
+  return __builtin_bit_cast(unsigned, r);
 }
 static inline u16 host_f2bf(float f) {
   unsigned u;
@@ -73,6 +78,32 @@ __device__ __forceinline__ int frag_store_off(int gg, int kh, int c) {
 }
 // Wave work: one block-row x 64 columns (two 32x32 accumulators sharing the
 // A operand); workgroup: 4 block-rows x 64 columns.
+// Epilogue of 4 consecutive rows of one column: y = bf16(alpha*acc) or, with
+// beta != 0 (a wave-uniform test), bf16(fma(beta, y_old, alpha*acc)); the
+// old values are one 8-byte load.
+template <bool NT = false>
+__device__ __forceinline__ void bsr_store4(u16* yq, float a0, float a1, float a2, float a3, float alpha,
+                                           float beta) {
+  float v0 = alpha * a0, v1 = alpha * a1, v2 = alpha * a2, v3 = alpha * a3;
+  if (beta != 0.0f) {
+    const uint2 old = *reinterpret_cast<const uint2*>(yq);
+    v0 = fmaf(beta, __uint_as_float(old.x << 16), v0);
+    v1 = fmaf(beta, __uint_as_float(old.x & 0xffff0000u), v1);
+    v2 = fmaf(beta, __uint_as_float(old.y << 16), v2);
+    v3 = fmaf(beta, __uint_as_float(old.y & 0xffff0000u), v3);
+  }
+  const unsigned wx = f2bf2(v0, v1), wy = f2bf2(v2, v3);
+  if (NT) {
+    u32x2 wv = {wx, wy};
+    __builtin_nontemporal_store(wv, reinterpret_cast<u32x2*>(yq));
+  } else {
+    uint2 w;
+    w.x = wx;
+    w.y = wy;
+    *reinterpret_cast<uint2*>(yq) = w;
+  }
+}
+
 template <bool SPLIT, bool IN_TILED, bool OUT_TILED>
 __global__ __launch_bounds__(256) void k_bsr_mfma(int nbr, int p, const int* __restrict__ brp,
                                                   const int* __restrict__ bci, const u16* __restrict__ bdat,
@@ -103,11 +134,11 @@ __global__ __launch_bounds__(256) void k_bsr_mfma(int nbr, int p, const int* __r
     for (int q = 0; q < 4; ++q) {
       if (q < nb) {
         const int bc = bci[b + q];
-        const u16* blk = bdat + (size_t)(b + q) * BLK * PLANES + (lane & 31) * 32 + kh * 8;
+        const u16* blk = bdat + (size_t)(b + q) * BLK * PLANES + lane * 8;
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
 #pragma unroll
-          for (int s = 0; s < PLANES; ++s) a[q][h][s] = *reinterpret_cast<const bf16x8*>(blk + s * BLK + h * 16);
+          for (int s = 0; s < PLANES; ++s) a[q][h][s] = *reinterpret_cast<const bf16x8*>(blk + s * BLK + h * 512);
 #pragma unroll
           for (int c = 0; c < 2; ++c) {
             const int j = jok[c] ? jc[c] : 0;
@@ -139,17 +170,7 @@ __global__ __launch_bounds__(256) void k_bsr_mfma(int nbr, int p, const int* __r
     for (int g = 0; g < 4; ++g) {
       u16* yq = OUT_TILED ? Y + frag_tile(br, G, gc[c]) + frag_store_off(g, kh, lane & 31)
                           : Y + (size_t)j * ldy + br * 32 + 8 * g + 4 * kh;
-      u16 o[4];
-#pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        float v = alpha * acc[c][4 * g + t];
-        if (beta != 0.0f) v = fmaf(beta, bf2f(yq[t]), v);
-        o[t] = f2bf(v);
-      }
-      uint2 w;
-      w.x = (unsigned)o[0] | ((unsigned)o[1] << 16);
-      w.y = (unsigned)o[2] | ((unsigned)o[3] << 16);
-      *reinterpret_cast<uint2*>(yq) = w;
+      bsr_store4(yq, acc[c][4 * g], acc[c][4 * g + 1], acc[c][4 * g + 2], acc[c][4 * g + 3], alpha, beta);
     }
   }
 }
@@ -224,12 +245,12 @@ __global__ __launch_bounds__(256) void k_bsr_ell(int nbr, int p, int W, const in
     bf16x8 a[4][2][PLANES], x[4][2][NC];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      const u16* blk = bdat + (size_t)bi[q] * BLK * PLANES + (lane & 31) * 32 + kh * 8;
+      const u16* blk = bdat + (size_t)bi[q] * BLK * PLANES + lane * 8;
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
 #pragma unroll
         for (int pl = 0; pl < PLANES; ++pl)
-          a[q][h][pl] = ld_stream(reinterpret_cast<const bf16x8*>(blk + pl * BLK + h * 16), NT);
+          a[q][h][pl] = ld_stream(reinterpret_cast<const bf16x8*>(blk + pl * BLK + h * 512), NT);
 #pragma unroll
         for (int c = 0; c < NC; ++c) {
           const u16* src = IN_TILED ? X + frag_tile(bcol[q], G, gc[c]) + (h * 64 + lane) * 8
@@ -260,22 +281,7 @@ __global__ __launch_bounds__(256) void k_bsr_ell(int nbr, int p, int W, const in
     for (int gg = 0; gg < 4; ++gg) {
       u16* yq = OUT_TILED ? Y + frag_tile(br, G, gc[c]) + frag_store_off(gg, kh, lane & 31)
                           : Y + (size_t)j * ldy + br * 32 + 8 * gg + 4 * kh;
-      u16 o[4];
-#pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        float v = alpha * acc[c][4 * gg + t];
-        if (beta != 0.0f) v = fmaf(beta, bf2f(yq[t]), v);
-        o[t] = f2bf(v);
-      }
-      uint2 w;
-      w.x = (unsigned)o[0] | ((unsigned)o[1] << 16);
-      w.y = (unsigned)o[2] | ((unsigned)o[3] << 16);
-      if (NT) {
-        u32x2 wv = {w.x, w.y};
-        __builtin_nontemporal_store(wv, reinterpret_cast<u32x2*>(yq));
-      } else {
-        *reinterpret_cast<uint2*>(yq) = w;
-      }
+      bsr_store4<NT>(yq, acc[c][4 * gg], acc[c][4 * gg + 1], acc[c][4 * gg + 2], acc[c][4 * gg + 3], alpha, beta);
     }
   }
 }
@@ -316,7 +322,7 @@ __global__ __launch_bounds__(256) void k_bsr_ks(int nbr, int p, int W, const int
     bf16x8 a[4][PLANES], x[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      const u16* blk = bdat + (size_t)bi[q] * BLK * PLANES + (lane & 31) * 32 + hh * 16 + kh * 8;
+      const u16* blk = bdat + (size_t)bi[q] * BLK * PLANES + (hh * 64 + lane) * 8;
 #pragma unroll
       for (int pl = 0; pl < PLANES; ++pl) a[q][pl] = *reinterpret_cast<const bf16x8*>(blk + pl * BLK);
       const u16* src = IN_TILED ? X + frag_tile(bcol[q], G, panel) + (hh * 64 + lane) * 8
@@ -339,17 +345,8 @@ __global__ __launch_bounds__(256) void k_bsr_ks(int nbr, int p, int W, const int
     const int gg = 2 * hh + gi;
     u16* yq = OUT_TILED ? Y + frag_tile(br, G, panel) + frag_store_off(gg, kh, lane & 31)
                         : Y + (size_t)j * ldy + br * 32 + 8 * gg + 4 * kh;
-    u16 o[4];
-#pragma unroll
-    for (int t = 0; t < 4; ++t) {
-      float v = alpha * (k0[4 * gg + t] + k1[4 * gg + t]);
-      if (beta != 0.0f) v = fmaf(beta, bf2f(yq[t]), v);
-      o[t] = f2bf(v);
-    }
-    uint2 w;
-    w.x = (unsigned)o[0] | ((unsigned)o[1] << 16);
-    w.y = (unsigned)o[2] | ((unsigned)o[3] << 16);
-    *reinterpret_cast<uint2*>(yq) = w;
+    bsr_store4(yq, k0[4 * gg] + k1[4 * gg], k0[4 * gg + 1] + k1[4 * gg + 1], k0[4 * gg + 2] + k1[4 * gg + 2],
+               k0[4 * gg + 3] + k1[4 * gg + 3], alpha, beta);
   }
 }
 
@@ -752,7 +749,8 @@ __global__ void k_bsr_to_dense(int nbr, const int* __restrict__ brp, const int* 
   for (int b = brp[br]; b < brp[br + 1]; ++b) {
     const u16* blk = bdat + (size_t)b * 1024 * (split ? 2 : 1);
     for (int e = threadIdx.x; e < 1024; e += blockDim.x) {
-      const int i = e >> 5, k = e & 31;
+      // e in operand order: (k / 16, (k / 8) % 2, i, k % 8)
+      const int k = ((e >> 9) << 4) | (((e >> 8) & 1) << 3) | (e & 7), i = (e >> 3) & 31;
       float v = bf2f(blk[e]);
       if (split) v += bf2f(blk[1024 + e]);
       D[(size_t)(bci[b] * 32 + k) * m + br * 32 + i] += v;
@@ -868,8 +866,11 @@ int faust_multiply_mixed(const std::vector<DMatP>& f, char op, const void* X, in
 
 // ---------------------------------------------------------------- BSR handle
 // Host: blocks arrive column-major (faust_MatBSR.h:120-150); the device keeps
-// them row-major per block, BF16 (or BF16 hi/lo planes for FP32), plus the
-// BSR of the transpose for op T/H.
+// them in MFMA A-operand order per block, BF16 (or BF16 hi/lo planes for
+// FP32), plus the BSR of the transpose for op T/H.
+static inline size_t block_slot(int i, int k) {
+  return ((size_t)((k >> 4) * 64 + ((k >> 3) & 1) * 32 + i) << 3) + (k & 7);
+}
 static void put_block(const void* src, int dtype, size_t b, bool transpose, bool split, std::vector<u16>& dst,
                       size_t slot) {
   const size_t planes = split ? 2 : 1;
@@ -885,8 +886,8 @@ static void put_block(const void* src, int dtype, size_t b, bool transpose, bool
       else
         v = static_cast<const float*>(src)[e];
       const u16 hi = host_f2bf(v);
-      out[i * 32 + k] = hi;
-      if (split) out[1024 + i * 32 + k] = host_f2bf(v - host_bf2f(hi));
+      out[block_slot(i, k)] = hi;
+      if (split) out[1024 + block_slot(i, k)] = host_f2bf(v - host_bf2f(hi));
     }
 }
 
