@@ -81,7 +81,7 @@ def _bsr_workload(values):
                 dtype="bf16", fac_dtype=("bf16" if values == "bf16" else np.float32), ops=("N",),
                 bytes_per_launch=per_factor, launches_per_step=J, bytes_per_step=J * per_factor,
                 flops_per_step=J * 2 * nb * bs * bs * p,
-                kernel=("k_bsr_ell<false,*,*,1>" if values == "bf16" else "k_bsr_ks<true,*,*>"),
+                kernel=("k_bsr_shared<false,*,*>" if values == "bf16" else "k_bsr_shared<true,*,*>"),
                 cpu_reps_cap=40)
 
 

@@ -1,21 +1,24 @@
 // k_mixed.hip -- FP32 / BF16 Faust x dense: BSR on MFMA and FP32 CSR SpMM.
 //
 // BSR (32x32 blocks, bm = bn = 32): Y = op(B) X with X, Y BF16, FP32
-// accumulation in v_mfma_f32_32x32x16_bf16.  Each wave owns one block-row x
-// 64 columns (two 32x32 accumulators) and, for every block of its block-row,
-// issues four MFMAs (K = 16 + 16, two column tiles).  Operand lanes:
+// accumulation in v_mfma_f32_32x32x16_bf16 (K = 16: two MFMAs per block and
+// 32x32 output tile).  Operand lanes:
 //   A: lane l holds block row (l % 32), 8 consecutive k at 8*(l / 32);
 //   B: lane l holds X column (l % 32), the same 8 consecutive k;
 // blocks are stored in that operand order (entry (i, k) of a block at
 // ((k / 16)*64 + ((k / 8) % 2)*32 + i)*8 + k % 8), so the A operand of one K
 // half is one contiguous 1 KiB load, and X column-major (or tiled, below)
-// gives the B operand as one 16-byte load per lane.  FP32 block values are stored as a BF16 (hi, lo) pair and
-// cost one extra MFMA per step (error ~2^-16 relative per value, below the
-// BF16 rounding of the activations).
-// Work mapping: workgroup = 4 block-rows x 64 columns; consecutive workgroup
-// ids walk the 64-column panels first, so with 512 columns the 8 panels land
-// on the 8 XCDs and each XCD's L2 holds its 1 MiB X panel for all the
-// gathers of its block-rows.
+// gives the B operand as one 16-byte load per lane.  FP32 block values are
+// stored as a BF16 (hi, lo) pair and cost one extra MFMA per step (error
+// ~2^-16 relative per value, below the BF16 rounding of the activations).
+// Default kernel (k_bsr_shared): a workgroup owns one block-row x 128
+// columns, one 32-column tile per wave; the row's blocks go through LDS once
+// per workgroup and the four waves gather their X tiles.  Consecutive
+// workgroup ids walk the column groups first, so with 512 columns each XCD
+// keeps one 128-column group (and half of the block rows) and its L2 serves
+// the repeated gathers of that group.  Other forms (one or more tiles per
+// wave without LDS, K split over two waves, a row-pointer form for rows too
+// wide for the ELL slots) are selectable.
 //
 // FP32 CSR: intermediates are row-major inside the chain so that the
 // gather of one X row segment is a contiguous 128-byte read.  A workgroup
@@ -350,6 +353,95 @@ __global__ __launch_bounds__(256) void k_bsr_ks(int nbr, int p, int W, const int
   }
 }
 
+// Shared-block ELL form: a workgroup owns one block-row x 4 column tiles of
+// 32 (one per wave).  The row's blocks are loaded once per workgroup into
+// LDS (wave q loads block q of each group of 4 slots) and every wave reads
+// its A operands from there, so each CU pulls a block from the L2 once
+// instead of once per column tile.  Waves whose tile lies past p still load
+// and pass the barriers; they store nothing.
+template <bool SPLIT, bool IN_TILED, bool OUT_TILED>
+__global__ __launch_bounds__(256) void k_bsr_shared(int nbr, int p, int W, const int2* __restrict__ ell,
+                                                    const u16* __restrict__ bdat, const u16* __restrict__ X,
+                                                    int ldx, u16* __restrict__ Y, int ldy, float alpha, float beta,
+                                                    int npanels, int pgroups) {
+  constexpr int BLK = 1024;
+  constexpr int PLANES = SPLIT ? 2 : 1;
+  __shared__ bf16x8 sA[4][PLANES][2][64];
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int pg = blockIdx.x % pgroups, br = blockIdx.x / pgroups;
+  const int panel_raw = pg * 4 + wave;
+  const bool pok = panel_raw < npanels;
+  const int panel = pok ? panel_raw : npanels - 1;
+  const int kh = lane >> 5;
+  const int G = (p + 31) >> 5;  // == npanels
+  const int jc = panel * 32 + (lane & 31);
+  const bool jok = pok && jc < p;
+  const int j = jc < p ? jc : 0;
+  f32x16 acc = {};
+  const int4* slots = reinterpret_cast<const int4*>(ell + (size_t)br * W);
+  for (int g = 0; g < W; g += 4) {
+    const int4 s01 = slots[g / 2], s23 = slots[g / 2 + 1];
+    const int bcol[4] = {s01.y, s01.w, s23.y, s23.w};
+    const int mine = wave == 0 ? s01.x : wave == 1 ? s01.z : wave == 2 ? s23.x : s23.z;
+    bf16x8 a_own[PLANES][2], x[4][2];
+    const u16* blk = bdat + (size_t)mine * BLK * PLANES + lane * 8;
+#pragma unroll
+    for (int pl = 0; pl < PLANES; ++pl)
+#pragma unroll
+      for (int h = 0; h < 2; ++h) a_own[pl][h] = *reinterpret_cast<const bf16x8*>(blk + pl * BLK + h * 512);
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const u16* src = IN_TILED ? X + frag_tile(bcol[q], G, panel) + (h * 64 + lane) * 8
+                                  : X + (size_t)j * ldx + bcol[q] * 32 + h * 16 + kh * 8;
+        x[q][h] = *reinterpret_cast<const bf16x8*>(src);
+      }
+    __builtin_amdgcn_sched_barrier(0);
+    if (g > 0) __syncthreads();  // every wave is done reading the previous group's blocks
+#pragma unroll
+    for (int pl = 0; pl < PLANES; ++pl)
+#pragma unroll
+      for (int h = 0; h < 2; ++h) sA[wave][pl][h][lane] = a_own[pl][h];
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(sA[q][0][h][lane], x[q][h], acc, 0, 0, 0);
+        if (SPLIT) acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(sA[q][PLANES - 1][h][lane], x[q][h], acc, 0, 0, 0);
+      }
+  }
+  if (!jok) return;
+#pragma unroll
+  for (int gg = 0; gg < 4; ++gg) {
+    u16* yq = OUT_TILED ? Y + frag_tile(br, G, panel) + frag_store_off(gg, kh, lane & 31)
+                        : Y + (size_t)j * ldy + br * 32 + 8 * gg + 4 * kh;
+    bsr_store4(yq, acc[4 * gg], acc[4 * gg + 1], acc[4 * gg + 2], acc[4 * gg + 3], alpha, beta);
+  }
+}
+
+static void bsr_shared_launch(bool split, bool in_tiled, bool out_tiled, dim3 grid, hipStream_t st, int nbr, int p,
+                              int W, const int2* ell, const u16* dat, const u16* X, int ldx, u16* Y, int ldy,
+                              float alpha, float beta, int P, int PG) {
+  const int sel = (split ? 4 : 0) | (in_tiled ? 2 : 0) | (out_tiled ? 1 : 0);
+#define FH_SH(S, I, O)                                                                                           \
+  hipLaunchKernelGGL((k_bsr_shared<S, I, O>), grid, dim3(256), 0, st, nbr, p, W, ell, dat, X, ldx, Y, ldy, alpha, \
+                     beta, P, PG)
+  switch (sel) {
+    case 0: FH_SH(false, false, false); break;
+    case 1: FH_SH(false, false, true); break;
+    case 2: FH_SH(false, true, false); break;
+    case 3: FH_SH(false, true, true); break;
+    case 4: FH_SH(true, false, false); break;
+    case 5: FH_SH(true, false, true); break;
+    case 6: FH_SH(true, true, false); break;
+    default: FH_SH(true, true, true); break;
+  }
+#undef FH_SH
+}
+
 static void bsr_ks_launch(bool split, bool in_tiled, bool out_tiled, dim3 grid, hipStream_t st, int nbr, int p,
                           int W, const int2* ell, const u16* dat, const u16* X, int ldx, u16* Y, int ldy,
                           float alpha, float beta, int P) {
@@ -371,15 +463,15 @@ static void bsr_ks_launch(bool split, bool in_tiled, bool out_tiled, dim3 grid, 
 
 // Which BSR kernel: FH_BSR_KERNEL=0 forces the row-pointer kernel, 1 / 2 / 4
 // the ELL kernel with that many 32-column tiles per wave, 3 the K-split ELL
-// kernel.  Defaults, measured on 8192^2, 4 blocks per block row, 512 columns,
-// per factor inside the chain: BF16 blocks 1 (8.7 us; 2: 10.0, 3: 9.1,
-// 4: 11.4), FP32 hi/lo blocks 3 (11.8 us; 1: 12.3, 2: 13.5, 4: 12.8).  More,
-// shorter waves win: each wave's time is mostly the latency of its slot load
-// and then of its gathers.
-static int bsr_kernel_choice(bool split) {
+// kernel, 5 the shared-block kernel (the default).  Measured on 8192^2,
+// 4 blocks per block row, 512 columns, per factor inside the chain: BF16
+// blocks 5: 6.9 us (1: 7.5, 3: 8.4); FP32 hi/lo blocks 5: 7.9 us (1: 11.0,
+// 3: 10.7).  A wave's time is mostly the latency of its slot load and then of
+// its gathers; what the CUs pull from the L2 per output tile decides the rest.
+static int bsr_kernel_choice(bool) {
   const char* e = getenv("FH_BSR_KERNEL");
-  if (e && e[0] >= '0' && e[0] <= '4') return e[0] - '0';
-  return split ? 3 : 1;
+  if (e && e[0] >= '0' && e[0] <= '5') return e[0] - '0';
+  return 5;
 }
 
 template <bool S, bool I, bool O, int NC, bool NT>
@@ -438,6 +530,15 @@ static void bsr_mul(const DMat& B, char op, const void* X, int ldx, int p, void*
   const u16* dat = tr ? B.tbdat.as<u16>() : B.bdat.as<u16>();
   const int W = tr ? B.tell_w : B.ell_w;
   const int choice = bsr_kernel_choice(B.split);
+  if (W > 0 && choice == 5) {
+    const int2* ell = (tr ? B.tell : B.ell).as<int2>();
+    const int npanels = (p + 31) / 32, pgroups = (npanels + 3) / 4;
+    const dim3 grid((unsigned)(pgroups * nbr));
+    bsr_shared_launch(B.split, in_tiled, out_tiled, grid, st, nbr, p, W, ell, dat, (const u16*)X, ldx, (u16*)Y, ldy,
+                      alpha, beta, npanels, pgroups);
+    FH_LAUNCH_CHECK();
+    return;
+  }
   if (W > 0 && choice == 3) {
     const int2* ell = (tr ? B.tell : B.ell).as<int2>();
     const int npanels = (p + 31) / 32;

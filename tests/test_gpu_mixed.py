@@ -123,15 +123,17 @@ def _ragged_bsr(rng, nbr, nbc, bs=32, wide_row=None):
     return sp.bsr_matrix((data, idx, indptr), shape=(nbr * bs, nbc * bs))
 
 
-@pytest.mark.parametrize("kernel", ["0", "1", "2", "3", "4"])
+@pytest.mark.parametrize("kernel", ["0", "1", "2", "3", "4", "5"])
 @pytest.mark.parametrize("wide", [None, 20])
 @pytest.mark.parametrize("op", ["N", "T"])
 @pytest.mark.parametrize("xsplit_nt", ["1-0", "2-1"])
 def test_bsr_kernel_variants(gpu, monkeypatch, kernel, wide, op, xsplit_nt):
     # row-pointer kernel, ELL with 32 / 64 / 128 columns per wave (with the XCD
-    # split of the block rows and streaming loads/stores), and the K-split ELL
+    # split of the block rows and streaming loads/stores), the K-split ELL
     # kernel (two block rows per workgroup: the odd block-row counts 25 leave
-    # one row alone); ragged block rows, empty block rows and (wide) a block
+    # one row alone) and the shared-block kernel (4 column tiles per
+    # workgroup: p = 200 leaves a partial group; rows of up to 6 blocks make
+    # two groups of 4 slots); ragged block rows, empty block rows and (wide) a block
     # row beyond the ELL limit; a 3-factor chain exercises the tiled
     # intermediates
     monkeypatch.setenv("FH_BSR_KERNEL", kernel)
