@@ -94,7 +94,7 @@ DMatP make_dense(int dtype, int m, int n, const void* data, int ld, bool on_devi
 // ws: two reusable intermediates, grown on demand (hipMalloc only when a
 // larger product than any before is requested).
 void faust_multiply_f64(const std::vector<DMatP>& f, char op, const double* X, int p, double* Y, hipStream_t st,
-                        DBuf* ws, FaustTimers* tm) {
+                        DBuf* ws, FaustTimers* tm, int* launches) {
   FH_CHECK(!f.empty(), "Faust::multiply: empty Faust");
   const int J = (int)f.size();
   int maxr = 0;
@@ -108,6 +108,31 @@ void faust_multiply_f64(const std::vector<DMatP>& f, char op, const double* X, i
   DBuf &t1 = ws[0], &t2 = ws[1];
   const double* in = X;
   int in_rows = (op == 'N') ? f.back()->n : f.front()->m;
+  if (launches) *launches = J;
+  // all-CSR chains whose intermediates fit a workgroup's LDS column panel run
+  // in one launch (same bits as the factor-by-factor path)
+  {
+    std::vector<ChainCsrOp> ops;
+    int r = in_rows;
+    bool ok = true;
+    for (int i = 0; i < J && ok; ++i) {
+      const DMat& S = (op == 'N') ? *f[J - 1 - i] : *f[i];
+      ok = S.kind == K_CSR && S.dtype == FH_F64 && ((op == 'N') ? S.n : S.m) == r && (op == 'N' || S.has_csc);
+      if (!ok) break;
+      const bool tr = op != 'N';
+      ops.push_back({tr ? S.cp.as<int>() : S.rp.as<int>(), tr ? S.ri.as<int>() : S.ci.as<int>(),
+                     tr ? S.cv.as<double>() : S.v.as<double>(), tr ? S.n : S.m, (int)S.nnz});
+      r = tr ? S.n : S.m;
+    }
+    const int wc = ok ? csr_chain_plan_f64(ops, in_rows, p) : 0;
+    if (wc > 0) {
+      if (tm) tm->begin(0, st);
+      csr_chain_run_f64(ops, in_rows, wc, X, in_rows, p, Y, r, 1.0, 0.0, st);
+      if (tm) tm->end(0, st);
+      if (launches) *launches = 1;
+      return;
+    }
+  }
   for (int i = 0; i < J; ++i) {
     const DMat& S = (op == 'N') ? *f[J - 1 - i] : *f[i];
     const int out_rows = (op == 'N') ? S.m : S.n;
@@ -337,8 +362,7 @@ int fh_faust_multiply(fh_faust_t F, char op, const void* X, int p, void* Y, void
   FH_CHECK(p >= 0, "negative column count");
   hipStream_t st = (hipStream_t)stream;
   if (F->f.front()->dtype == FH_F64) {
-    faust_multiply_f64(F->f, op, (const double*)X, p, (double*)Y, st, F->ws, &F->timers);
-    F->last_launches = (int)F->f.size();
+    faust_multiply_f64(F->f, op, (const double*)X, p, (double*)Y, st, F->ws, &F->timers, &F->last_launches);
   } else {
     F->last_launches = faust_multiply_mixed(F->f, op, X, p, Y, st, F->ws, &F->timers);
   }

@@ -102,7 +102,7 @@ DMatP make_csr_from_host(int dtype, int m, int n, int nnz, const int32_t* rp, co
                          hipStream_t st);
 DMatP make_dense(int dtype, int m, int n, const void* data, int ld, bool on_device, hipStream_t st);
 void faust_multiply_f64(const std::vector<DMatP>& f, char op, const double* X, int p, double* Y, hipStream_t st,
-                        DBuf* ws, FaustTimers* tm = nullptr);
+                        DBuf* ws, FaustTimers* tm = nullptr, int* launches = nullptr);
 DMatP faust_get_product_f64(const std::vector<DMatP>& f, hipStream_t st);
 double faust_spectral_norm_f64(const std::vector<DMatP>& f, int maxit, double thr, int* flag, hipStream_t st);
 // FP32 / BF16 paths (k_mixed.hip, k_bsr.hip)

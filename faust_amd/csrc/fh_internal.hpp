@@ -131,6 +131,22 @@ using DMatP = std::shared_ptr<DMat>;
 // Y = alpha*op(S)*X + beta*Y for CSR S, dense col-major X/Y.
 void csr_spmm_f64(const DMat& S, char op, const double* X, int ldx, int ncols, double* Y, int ldy, double alpha,
                   double beta, hipStream_t st);
+// Whole FP64 CSR chain in one launch when every intermediate column panel
+// fits a workgroup's LDS: Y = alpha * op(S_1) ... op(S_J) X + beta*Y, factors
+// given in application order (the first one applied to X comes first), each
+// op'd factor as (row pointer, column index, value, rows, nnz).  plan returns the
+// columns per workgroup, or 0 when the shape does not fit (then use the
+// factor-by-factor path).
+struct ChainCsrOp {
+  const int* rp;
+  const int* ci;
+  const double* v;
+  int rows;
+  int nnz;
+};
+int csr_chain_plan_f64(const std::vector<ChainCsrOp>& ops, int in_rows, int p);
+void csr_chain_run_f64(const std::vector<ChainCsrOp>& ops, int in_rows, int wc, const double* X, int ldx, int p,
+                       double* Y, int ldy, double alpha, double beta, hipStream_t st);
 // C = alpha*op(A)*op(S) + beta*C, A dense col-major, S CSR.
 void dense_csr_f64(const double* A, int lda, char opA, int m, const DMat& S, char opS, double* C, int ldc,
                    double alpha, double beta, hipStream_t st);

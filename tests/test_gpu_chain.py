@@ -91,3 +91,70 @@ def test_norm2_both_forms_bitexact(gpu, power_form, case):
     F = faust_amd.Faust(facs)
     nrm, _ = oracle.spectral_norm(facs)
     assert F.norm(2) == nrm
+
+
+def _launches(F):
+    import ctypes as C
+    from faust_amd import _lib
+    n = C.c_int()
+    _lib.check(_lib.lib().fh_faust_last_launches(F.h, C.byref(n)))
+    return n.value
+
+
+@pytest.mark.parametrize("wc", ["auto", "1", "2", "4", "8"])
+@pytest.mark.parametrize("op", ["N", "T"])
+def test_fused_chain_matches_per_factor(gpu, monkeypatch, wc, op):
+    # the one-launch FP64 chain (column panels in LDS) against the
+    # factor-by-factor launches and the oracle, bit for bit: rectangular
+    # factors, an empty row, column counts that leave partial panels
+    if wc != "auto":
+        monkeypatch.setenv("FH_CHAIN_WC", wc)
+    rng = np.random.default_rng(21)
+    shapes = [(300, 700), (700, 129), (129, 1000), (1000, 500)]
+    facs = []
+    for r, c in shapes:
+        a = sp.random(r, c, density=min(1.0, 9.0 / c), random_state=rng, format="csr")
+        a = sp.csr_matrix(a.toarray())
+        facs.append(a)
+    facs[1] = sp.csr_matrix(np.where(np.arange(700)[:, None] == 4, 0.0, facs[1].toarray()))  # empty row 4
+    F = faust_amd.Faust(facs)
+    G = F if op == "N" else F.T
+    rows_in = shapes[-1][1] if op == "N" else shapes[0][0]
+    for p in (1, 7, 64, 200):
+        X = np.asfortranarray(rng.standard_normal((rows_in, p)))
+        monkeypatch.setenv("FH_CHAIN_FUSED", "1")
+        Y1 = G @ X
+        assert _launches(G) == 1
+        monkeypatch.setenv("FH_CHAIN_FUSED", "0")
+        Y2 = G @ X
+        assert _launches(G) == len(facs)
+        assert np.array_equal(Y1, Y2)
+        assert np.array_equal(Y1, oracle.chain_multiply(facs, X, op))
+
+
+def test_fused_chain_falls_back_when_too_tall(gpu):
+    # 12000-row intermediates do not fit two LDS buffers even at one column
+    rng = np.random.default_rng(22)
+    facs = [faust_amd.rand_csr(12000, 12000, 3, np.random.default_rng(22 + f)) for f in range(3)]
+    F = faust_amd.Faust(facs)
+    X = np.asfortranarray(rng.standard_normal((12000, 4)))
+    Y = F @ X
+    assert _launches(F) == 3
+    assert np.array_equal(Y, oracle.chain_multiply(facs, X, "N"))
+
+
+def test_long_chains_fuse_by_default(gpu, monkeypatch):
+    # the 10-factor Hadamard Faust of order 1024 (2 nnz per row and column) is
+    # one launch by default; the 5-factor chain of 1024^2 factors is not
+    monkeypatch.delenv("FH_CHAIN_FUSED", raising=False)
+    monkeypatch.delenv("FH_CHAIN_WC", raising=False)
+    facs = [sp.csr_matrix(f) for f in faust_amd.wht_factors(1024)]
+    F = faust_amd.Faust(facs)
+    X = np.asfortranarray(np.random.default_rng(7).standard_normal((1024, 33)))
+    for G, op in ((F, "N"), (F.T, "T")):
+        Y = G @ X
+        assert _launches(G) == 1
+        assert np.array_equal(Y, oracle.chain_multiply(facs, X, op))
+    F5 = faust_amd.Faust(_cfg1_factors())
+    F5 @ X
+    assert _launches(F5) == 5
