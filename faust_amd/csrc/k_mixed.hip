@@ -586,6 +586,41 @@ static void bsr_mul(const DMat& B, char op, const void* X, int ldx, int p, void*
 }
 
 // ---------------------------------------------------------------- FP32 CSR
+// A row's batch of NB (<= 16) index/value pairs reaches its 8 lanes with two
+// loads per lane: lane s of the row's group loads entries s and 8 + s (entry
+// indices clamped to the row's last entry), and ds_swizzle hands entry q to
+// every lane of the group from lane q % 8.  All 8 lanes of a group must be
+// active (a group is one row).
+template <int Q, int NB>
+struct CsrBcast {
+  static __device__ __forceinline__ void go(int ia, int ib, float fa, float fb, int* cc, float* ww) {
+    constexpr int PAT = 0x18 | ((Q & 7) << 5);  // lane (l & ~7) | (Q % 8) within each 32
+    cc[Q] = __builtin_amdgcn_ds_swizzle(Q < 8 ? ia : ib, PAT);
+    ww[Q] = __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(Q < 8 ? fa : fb), PAT));
+    CsrBcast<Q + 1, NB>::go(ia, ib, fa, fb, cc, ww);
+  }
+};
+template <int NB>
+struct CsrBcast<NB, NB> {
+  static __device__ __forceinline__ void go(int, int, float, float, int*, float*) {}
+};
+template <int NB>
+__device__ __forceinline__ void csr_batch_idx(const int* __restrict__ ci, const float* __restrict__ v, int base,
+                                              int nb, int s, int (&cc)[NB], float (&ww)[NB]) {
+  static_assert(NB <= 16, "two entries per lane");
+  const int ea = base + min(s, nb - 1);
+  const int ia = ci[ea];
+  const float fa = v[ea];
+  int ib = ia;
+  float fb = fa;
+  if (NB > 8) {
+    const int eb = base + min(8 + s, nb - 1);
+    ib = ci[eb];
+    fb = v[eb];
+  }
+  CsrBcast<0, NB>::go(ia, ib, fa, fb, cc, ww);
+}
+
 // Row-major gather SpMM: Y[r, c] = alpha * sum_e v[e] * X[ci[e], c] + beta*Y.
 // OUT_COL: write Y column-major (ld = ldy) through an LDS transpose.
 template <bool OUT_COL, int NB>
@@ -598,11 +633,12 @@ __global__ __launch_bounds__(256) void k_csr_rm_f32(int rows, int p, const int* 
   const int panel = blockIdx.x % npanels, rblk = blockIdx.x / npanels;
   const int r = rblk * 32 + (t >> 3);
   const int c0 = panel * 32 + (t & 7) * 4;
+  const int cg = c0 < p ? c0 : 0;  // lanes past p gather column 0 and store nothing
   float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
-  if (r < rows && c0 < p) {
+  if (r < rows) {  // the same for the row's 8 lanes, as the broadcast needs
     const int e0 = rp[r], e1 = rp[r + 1];
     // batches of up to NB nonzeros (NB: the widest row rounded up to 4, at
-    // most 16): all index/value loads, then all row gathers, then the FMAs,
+    // most 16): the index/value pairs, then all row gathers, then the FMAs,
     // so NB gathers are in flight per thread.  The loads are unconditional
     // (indices clamped to the row's last entry; the extra terms are not
     // accumulated): a conditional load would become a branch that waits for
@@ -611,15 +647,10 @@ __global__ __launch_bounds__(256) void k_csr_rm_f32(int rows, int p, const int* 
       const int nb = min(NB, e1 - base);
       int cc[NB];
       float ww[NB];
-#pragma unroll
-      for (int q = 0; q < NB; ++q) {
-        const int e = base + min(q, nb - 1);
-        cc[q] = ci[e];
-        ww[q] = v[e];
-      }
+      csr_batch_idx<NB>(ci, v, base, nb, t & 7, cc, ww);
       float4 xx[NB];
 #pragma unroll
-      for (int q = 0; q < NB; ++q) xx[q] = *reinterpret_cast<const float4*>(X + (size_t)cc[q] * ldx + c0);
+      for (int q = 0; q < NB; ++q) xx[q] = *reinterpret_cast<const float4*>(X + (size_t)cc[q] * ldx + cg);
       __builtin_amdgcn_sched_barrier(0);  // all gathers in flight before the first FMA
 #pragma unroll
       for (int q = 0; q < NB; ++q) {
