@@ -13,7 +13,7 @@ from . import fact, factparams, proj
 from ._lib import FaustHipError
 from .faust import Factor, Faust
 
-__all__ = ["Faust", "Factor", "rand", "wht", "fact", "factparams", "proj", "FaustHipError"]
+__all__ = ["Faust", "Factor", "rand", "rand_bsr", "wht", "fact", "factparams", "proj", "FaustHipError"]
 
 
 def rand_csr(nrows, ncols, nnz_per_row, rng, values="normal", dtype=np.float64):
@@ -59,6 +59,45 @@ def rand(num_rows, num_cols, num_factors=None, dim_sizes=None, density=None, fac
             else:
                 facs.append(sp.csr_matrix(rand_csr(c, r, k, rng, values, dtype).T))
     return Faust(facs, dtype=dtype)
+
+
+def rand_bsr(num_rows, num_cols, bnrows, bncols, num_factors=None, density=.1, dev="gpu", dtype="float64", seed=0):
+    """pyfaust.rand_bsr (pyfaust.py:4484-4560; TransformHelper::randBSRFaust,
+    faust_TransformHelper.hpp:1430-1449): a square Faust of random BSR
+    factors, each with round(density * num_rows*num_cols/(bnrows*bncols))
+    nonzero blocks of U(0, 1) values.  num_factors: an int, a (min, max)
+    pair (inclusive, drawn uniformly) or None for 5.  dtype 'float32' or
+    'bf16' keeps the factors BSR on the GPU (32x32 blocks go to MFMA, other
+    sizes run as CSR); 'float64' runs them as FP64 CSR factors of the same
+    values (the GPU stores BSR blocks in BF16 or FP32 only)."""
+    from .bsr import rand_bsr_factor
+    if num_factors is None:
+        lo = hi = 5
+    elif isinstance(num_factors, (int, np.integer)):
+        lo = hi = int(num_factors)
+    elif isinstance(num_factors, tuple) and len(num_factors) == 2:
+        lo, hi = int(num_factors[0]), int(num_factors[1])
+    else:
+        raise ValueError("num_factors must be None, a int or a tuple of int")
+    if num_rows != num_cols:
+        raise ValueError("currently only random square BSR Fausts can be generated.")
+    if num_rows % bnrows != 0 or num_cols % bncols != 0:
+        raise ValueError("the size of blocks must evenly divide the size of Faust matrices")
+    if not dev.startswith("gpu"):
+        raise ValueError("this build runs Fausts on the GPU only (dev='gpu')")
+    bnnz = int(round(num_rows * num_cols / bnrows / bncols * density))
+    if bnnz == 0:
+        raise ValueError("the nonzero blocks are too large for this Faust/matrix size.")
+    rng = np.random.default_rng(seed)
+    J = int(rng.integers(lo, hi + 1))
+    facs = [rand_bsr_factor(num_rows, num_cols, bnrows, bncols, bnnz, rng) for _ in range(J)]
+    if dtype in ("float64", np.float64):
+        return Faust([sp.csr_matrix(f) for f in facs], dtype=np.float64)
+    if dtype in ("float32", np.float32):
+        return Faust(facs, dtype=np.float32)
+    if dtype == "bf16":
+        return Faust(facs, dtype="bf16")
+    raise ValueError("dtype must be 'float64', 'float32' or 'bf16'")
 
 
 def wht_factors(n):

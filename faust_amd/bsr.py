@@ -48,3 +48,20 @@ def rand_bsr(nrows, ncols, bs, blocks_per_row, rng, scale=1.0):
     data = rng.standard_normal((nbr * blocks_per_row, bs, bs)) * scale
     indptr = np.arange(0, nbr * blocks_per_row + 1, blocks_per_row)
     return sp.bsr_matrix((data, cols.ravel(), indptr), shape=(nrows, ncols))
+
+
+def rand_bsr_factor(nrows, ncols, bm, bn, bnnz, rng):
+    """BSRMat::rand (faust_MatBSR.hpp:693-760): bnnz blocks (at most all of
+    them) at distinct block positions drawn uniformly without replacement,
+    values U(0, 1); blocks sorted by (block row, block column)."""
+    if nrows % bm or ncols % bn:
+        raise ValueError("the size of blocks must evenly divide the size of the matrix")
+    nbr, nbc = nrows // bm, ncols // bn
+    bnnz = min(int(bnnz), nbr * nbc)
+    pos = np.sort(rng.choice(nbr * nbc, size=bnnz, replace=False))
+    brow, bcol = pos // nbc, pos % nbc
+    indptr = np.zeros(nbr + 1, dtype=np.int64)
+    np.add.at(indptr, brow + 1, 1)
+    indptr = np.cumsum(indptr)
+    data = rng.random((bnnz, bm, bn))
+    return sp.bsr_matrix((data, bcol, indptr), shape=(nrows, ncols))

@@ -263,7 +263,7 @@ int fh_mat_tocpu(fh_mat_t M, void* data_or_vals, int32_t* rowptr, int32_t* colin
       if (data_or_vals) FH_HIP(hipMemcpy(data_or_vals, d.v.p, esz * d.nnz, hipMemcpyDeviceToHost));
     }
   } else {
-    throw Error(FH_ERR_UNSUPPORTED, "tocpu: use fh_mat_to_dense for BSR handles");
+    bsr_to_host(d, data_or_vals, rowptr, colind);
   }
   FH_API_END
 }

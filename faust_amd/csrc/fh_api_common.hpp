@@ -107,6 +107,7 @@ DMatP faust_get_product_f64(const std::vector<DMatP>& f, hipStream_t st);
 double faust_spectral_norm_f64(const std::vector<DMatP>& f, int maxit, double thr, int* flag, hipStream_t st);
 // FP32 / BF16 paths (k_mixed.hip, k_bsr.hip)
 void mixed_to_dense(const DMat& M, void* dense_dev, hipStream_t st);
+void bsr_to_host(const DMat& M, void* bdata, int32_t* browptr, int32_t* bcolinds);
 void mixed_mul_dense(const DMat& S, char op, const void* X, int ldx, int ncols, void* Y, int ldy, const void* alpha,
                      const void* beta, hipStream_t st);
 int faust_multiply_mixed(const std::vector<DMatP>& f, char op, const void* X, int p, void* Y, hipStream_t st,

@@ -1023,6 +1023,34 @@ static void put_block(const void* src, int dtype, size_t b, bool transpose, bool
     }
 }
 
+// BSR handle back to host (fh_mat_tocpu): block row pointers, block column
+// indices and the blocks column-major one after the other (the layout
+// fh_mat_bsr_create takes, faust_MatBSR.h:120-150).  Values in the handle's
+// type: BF16 bit patterns, or for FP32 handles hi + lo of the stored pair
+// (within 2^-16 relative of the uploaded value).  Synchronous.
+void bsr_to_host(const DMat& M, void* bdata, int32_t* browptr, int32_t* bcolinds) {
+  FH_CHECK(M.kind == K_BSR, "bsr_to_host: not a BSR matrix");
+  const int nbr = M.m / 32, bnnz = M.bnnz;
+  const size_t planes = M.split ? 2 : 1;
+  if (browptr) FH_HIP(hipMemcpy(browptr, M.brp.p, sizeof(int) * (nbr + 1), hipMemcpyDeviceToHost));
+  if (bcolinds && bnnz) FH_HIP(hipMemcpy(bcolinds, M.bci.p, sizeof(int) * bnnz, hipMemcpyDeviceToHost));
+  if (!bdata || !bnnz) return;
+  std::vector<u16> dat((size_t)bnnz * 1024 * planes);
+  FH_HIP(hipMemcpy(dat.data(), M.bdat.p, sizeof(u16) * dat.size(), hipMemcpyDeviceToHost));
+  for (size_t b = 0; b < (size_t)bnnz; ++b) {
+    const u16* blk = &dat[b * 1024 * planes];
+    for (int k = 0; k < 32; ++k)
+      for (int i = 0; i < 32; ++i) {
+        const size_t e = b * 1024 + (size_t)k * 32 + i;  // column-major destination
+        const size_t s = block_slot(i, k);
+        if (M.dtype == FH_BF16)
+          static_cast<u16*>(bdata)[e] = blk[s];
+        else
+          static_cast<float*>(bdata)[e] = host_bf2f(blk[s]) + host_bf2f(blk[1024 + s]);
+      }
+  }
+}
+
 // ELL slots of a block-row structure: width = widest block row rounded up to
 // a multiple of 4 (0 when above BSR_ELL_MAX); slot (br, q) = (rp[br] + q,
 // ci[rp[br] + q]), or (zero_block, 0) past the end of the row, zero_block

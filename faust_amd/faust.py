@@ -50,6 +50,7 @@ class Factor:
 
     def __init__(self, M, dtype=np.float64, handle=None):
         self.h = C.c_void_p()
+        self._host = None  # exact host copy, kept for FP32 BSR factors (stored as a BF16 hi/lo pair)
         if handle is not None:
             self.h = handle
             return
@@ -59,6 +60,8 @@ class Factor:
         if sp.issparse(M) and M.format == "bsr":
             from .bsr import bsr_handle
             self.h = bsr_handle(M, dt)
+            if dt == _lib.FH_F32:
+                self._host = sp.bsr_matrix(M, dtype=np.float32)
         elif sp.issparse(M):
             S = sp.csr_matrix(M)
             S.sum_duplicates()
@@ -95,6 +98,42 @@ class Factor:
         torch.cuda.current_stream().synchronize()
         return out.t().to(torch.float64).cpu().numpy()
 
+    def to_host(self):
+        """The factor as a host matrix (MatGeneric::toMatIOVar's content):
+        numpy array (dense), scipy CSR (sparse) or scipy BSR, with float64
+        values for FP64 factors and float32 otherwise (BF16 values exactly)."""
+        if self._host is not None:
+            return self._host.copy()
+        kind, dt, r, c, nnz = self.info()
+        L = lib()
+        if kind == _lib.FH_MAT_DENSE:
+            a = self.toarray()
+            return np.asfortranarray(a if dt == _lib.FH_F64 else a.astype(np.float32))
+        if kind == _lib.FH_MAT_CSR:
+            rp = np.empty(r + 1, dtype=np.int32)
+            ci = np.empty(max(nnz, 1), dtype=np.int32)
+            raw = np.empty(max(nnz, 1), dtype={_lib.FH_F64: np.float64, _lib.FH_F32: np.float32,
+                                               _lib.FH_BF16: np.uint16}[dt])
+            check(L.fh_mat_tocpu(self.h, raw.ctypes.data_as(C.c_void_p), rp.ctypes.data_as(C.c_void_p),
+                                 ci.ctypes.data_as(C.c_void_p)))
+            vals = raw[:nnz]
+            if dt == _lib.FH_BF16:
+                from .bsr import bf16_bits_to_f32
+                vals = bf16_bits_to_f32(vals)
+            return sp.csr_matrix((vals, ci[:nnz], rp), shape=(r, c))
+        # BSR, 32 x 32 blocks: nnz = bnnz * 1024
+        bnnz = nnz // 1024
+        bp = np.empty(r // 32 + 1, dtype=np.int32)
+        bc = np.empty(max(bnnz, 1), dtype=np.int32)
+        raw = np.empty(max(bnnz, 1) * 1024, dtype=np.uint16 if dt == _lib.FH_BF16 else np.float32)
+        check(L.fh_mat_tocpu(self.h, raw.ctypes.data_as(C.c_void_p), bp.ctypes.data_as(C.c_void_p),
+                             bc.ctypes.data_as(C.c_void_p)))
+        if dt == _lib.FH_BF16:
+            from .bsr import bf16_bits_to_f32
+            raw = bf16_bits_to_f32(raw)
+        blocks = raw[:bnnz * 1024].reshape(bnnz, 32, 32).transpose(0, 2, 1)  # stored column-major
+        return sp.bsr_matrix((np.ascontiguousarray(blocks), bc[:bnnz], bp), shape=(r, c))
+
     def __del__(self):
         if getattr(self, "h", None) and self.h.value and _lib._lib is not None:
             _lib._lib.fh_mat_free(self.h)
@@ -113,16 +152,35 @@ def _host_values(a, dt):
     return np.ascontiguousarray(r) if a.ndim == 1 else np.asfortranarray(r)
 
 
+def _load_factors(filepath):
+    """Host factors of a Faust .mat file and the dtype to upload them in:
+    float32 when a factor holds single values, else float64.  FP64 chains take
+    CSR and dense factors, so BSR factors of an FP64 file become CSR."""
+    from .matio import read_factors
+    mats = read_factors(filepath)
+    single = any(np.asarray(m.data if sp.issparse(m) else m).dtype == np.float32 for m in mats)
+    if not single:
+        mats = [sp.csr_matrix(m) if sp.issparse(m) and m.format == "bsr" else m for m in mats]
+    return mats, (np.float32 if single else np.float64)
+
+
 class Faust:
     """F = S_1 S_2 ... S_J on the GPU (pyfaust.Faust counterpart, pyfaust.py:214).
 
     Faust(factors, dtype=np.float64): factors are numpy arrays (dense),
-    scipy CSR matrices (sparse) or scipy BSR matrices.
+    scipy CSR matrices (sparse) or scipy BSR matrices.  Faust(filepath) /
+    Faust(filepath=...) loads a Faust .mat file (pyfaust.py:102, Faust.load).
     """
 
-    def __init__(self, factors=None, dtype=np.float64, _facs=None, _transposed=False):
+    def __init__(self, factors=None, dtype=np.float64, _facs=None, _transposed=False, filepath=None):
         _require_torch_cuda()
         L = lib()
+        if isinstance(factors, str):
+            factors, filepath = None, factors
+        if filepath is not None:
+            if factors is not None or _facs is not None:
+                raise ValueError("give either factors or a filepath")
+            factors, dtype = _load_factors(filepath)
         if _facs is None:
             if factors is None or len(factors) == 0:
                 raise ValueError("a Faust needs at least one factor")
@@ -221,6 +279,26 @@ class Faust:
 
     def dot(self, A):
         return self @ A
+
+    # --- files ---------------------------------------------------------------
+    def save(self, filepath, format="Matlab"):
+        """Write the Faust to a MATLAB v5 .mat file (pyfaust Faust.save,
+        pyfaust.py:2564-2599; Transform::save_mat_file,
+        faust_Transform.hpp:434-476).  A transposed Faust is written as its
+        factors transposed, in reverse order, as the reference does."""
+        if format != "Matlab":
+            raise ValueError("Only Matlab or Matlab_core format is supported.")
+        from .matio import write_factors
+        mats = [f.to_host() for f in self._facs]
+        if self._transposed:
+            mats = [m.T if not sp.issparse(m) else (sp.bsr_matrix(m.T) if m.format == "bsr" else sp.csr_matrix(m.T))
+                    for m in reversed(mats)]
+        write_factors(filepath, mats)
+
+    @staticmethod
+    def load(filepath):
+        """Faust from a .mat file written by Faust.save (pyfaust.py:2601-2639)."""
+        return Faust(filepath=filepath)
 
     def toarray(self):
         """Dense product (Transform::get_product)."""

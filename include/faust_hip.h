@@ -64,7 +64,12 @@ int fh_mat_bsr_create(int dtype, int nrows, int ncols, int bm, int bn, int bnnz,
                       const int32_t *bcolinds, const void *bdata, int src_on_device, fh_mat_t *out);
 int fh_mat_info(fh_mat_t M, int *kind, int *dtype, int *nrows, int *ncols, int64_t *nnz);
 /* Copy back to host buffers (dense: `data` nrows*ncols col-major; CSR:
- * rowptr[nrows+1], colind[nnz], vals[nnz]; BSR: browptr, bcolinds, bdata). */
+ * rowptr[nrows+1], colind[nnz], vals[nnz]; BSR (32x32 blocks): browptr
+ * [nrows/32+1], bcolinds[bnnz], bdata[bnnz*1024] with the blocks column-major
+ * one after the other, as BF16 bit patterns for BF16 handles and as FP32 for
+ * FP32 handles, whose values come back within 2^-16 relative of the uploaded
+ * ones (they are stored as a BF16 hi/lo pair)).  For BSR handles `nnz` of
+ * fh_mat_info is bnnz*1024.  Any output pointer may be NULL to skip it. */
 int fh_mat_tocpu(fh_mat_t M, void *data_or_vals, int32_t *rowptr, int32_t *colind);
 /* Densify into a device buffer (col-major, ld = nrows): FP64 for FP64
  * matrices, FP32 for FP32 matrices and for BSR matrices of either value type,
