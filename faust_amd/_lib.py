@@ -12,6 +12,15 @@ LIB_PATH = os.path.join(_HERE, "libfaust_hip.so")
 FH_OK, FH_ERR_ARG, FH_ERR_HIP, FH_ERR_NUMERIC, FH_ERR_UNSUPPORTED = 0, 1, 2, 3, 4
 FH_F64, FH_F32, FH_BF16 = 0, 1, 2
 FH_MAT_DENSE, FH_MAT_CSR, FH_MAT_BSR = 0, 1, 2
+# fh_faust_set_mul_mode values (faust_hip.h)
+FH_MUL_AUTO, FH_MUL_PER_FACTOR, FH_MUL_FUSED = 0, 1, 2
+FH_MUL_CSR_ROWS1, FH_MUL_CSR_ROWS2 = 32, 33
+
+
+def FH_MUL_BSR_KERNEL(k):
+    return 16 + k
+
+
 PROX = {"sp": 0, "spcol": 1, "splin": 2, "splincol": 3, "supp": 4}
 
 
@@ -64,6 +73,8 @@ _SIGS = {
     "fh_faust_free": (C.c_int, [VP]),
     "fh_faust_multiply": (C.c_int, [VP, C.c_char, VP, C.c_int, VP, VP]),
     "fh_faust_last_launches": (C.c_int, [VP, C.POINTER(C.c_int)]),
+    "fh_faust_set_mul_mode": (C.c_int, [VP, C.c_int]),
+    "fh_faust_get_mul_mode": (C.c_int, [VP, VP]),
     "fh_faust_get_product": (C.c_int, [VP, VP, VP]),
     "fh_faust_enable_timers": (C.c_int, [VP, C.c_int]),
     "fh_faust_timers": (C.c_int, [VP, VP, VP]),

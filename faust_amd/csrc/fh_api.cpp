@@ -197,6 +197,13 @@ double faust_spectral_norm_f64(const std::vector<DMatP>& f, int maxit, double th
 
 }  // namespace fh
 
+namespace fh {
+static thread_local int tl_mul_mode = FH_MUL_AUTO;
+int mul_mode() { return tl_mul_mode; }
+MulModeScope::MulModeScope(int m) : prev(tl_mul_mode) { tl_mul_mode = m; }
+MulModeScope::~MulModeScope() { tl_mul_mode = prev; }
+}  // namespace fh
+
 extern "C" {
 
 int fh_mat_dense_create(int dtype, int nrows, int ncols, const void* data, int ld, int src_on_device, fh_mat_t* out) {
@@ -354,9 +361,28 @@ int fh_faust_free(fh_faust_t F) {
   FH_API_END
 }
 
+int fh_faust_set_mul_mode(fh_faust_t F, int mode) {
+  FH_API_BEGIN
+  FH_CHECK(F, "null handle");
+  const bool ok = mode == FH_MUL_AUTO || mode == FH_MUL_PER_FACTOR || mode == FH_MUL_FUSED ||
+                  (mode >= FH_MUL_BSR_KERNEL(0) && mode <= FH_MUL_BSR_KERNEL(5)) || mode == FH_MUL_CSR_ROWS1 ||
+                  mode == FH_MUL_CSR_ROWS2;
+  FH_CHECK(ok, "fh_faust_set_mul_mode: unknown mode");
+  F->mul_mode = mode;
+  FH_API_END
+}
+
+int fh_faust_get_mul_mode(fh_faust_t F, int* mode) {
+  FH_API_BEGIN
+  FH_CHECK(F && mode, "null argument");
+  *mode = F->mul_mode;
+  FH_API_END
+}
+
 int fh_faust_multiply(fh_faust_t F, char op, const void* X, int p, void* Y, void* stream) {
   FH_API_BEGIN
   FH_CHECK(F && X && Y, "null argument");
+  MulModeScope mode_scope(F->mul_mode);
   FH_CHECK(op == 'N' || op == 'T' || op == 'H', "op must be N, T or H");
   FH_CHECK(!F->f.empty(), "Faust::multiply: empty Faust");
   FH_CHECK(p >= 0, "negative column count");

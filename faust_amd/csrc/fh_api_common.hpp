@@ -81,6 +81,7 @@ struct fh_faust_s {
   std::vector<fh::DMatP> f;
   fh::DBuf ws[2];         // ping-pong activations
   int last_launches = 0;  // kernel launches of the last product
+  int mul_mode = 0;       // FH_MUL_*
   fh::FaustTimers timers;
 };
 

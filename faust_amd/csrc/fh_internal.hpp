@@ -127,6 +127,18 @@ struct DMat {
 };
 using DMatP = std::shared_ptr<DMat>;
 
+// ------------------------------------------------------------ product modes
+// Multiplication mode of the Faust product running on this thread (set from
+// the Faust handle by fh_faust_multiply, TransformHelper::set_FM_mul_mode's
+// role): FH_MUL_AUTO lets the defaults and the FH_* environment knobs decide,
+// the other FH_MUL_* values of faust_hip.h force one execution form.
+int mul_mode();
+struct MulModeScope {
+  int prev;
+  explicit MulModeScope(int m);
+  ~MulModeScope();
+};
+
 // ------------------------------------------------------------ kernels (FP64 canonical)
 // Y = alpha*op(S)*X + beta*Y for CSR S, dense col-major X/Y.
 void csr_spmm_f64(const DMat& S, char op, const double* X, int ldx, int ncols, double* Y, int ldy, double alpha,

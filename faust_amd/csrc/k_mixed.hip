@@ -469,6 +469,8 @@ static void bsr_ks_launch(bool split, bool in_tiled, bool out_tiled, dim3 grid, 
 // 3: 10.7).  A wave's time is mostly the latency of its slot load and then of
 // its gathers; what the CUs pull from the L2 per output tile decides the rest.
 static int bsr_kernel_choice(bool) {
+  const int m = mul_mode();
+  if (m >= FH_MUL_BSR_KERNEL(0) && m <= FH_MUL_BSR_KERNEL(5)) return m - FH_MUL_BSR_KERNEL(0);
   const char* e = getenv("FH_BSR_KERNEL");
   if (e && e[0] >= '0' && e[0] <= '5') return e[0] - '0';
   return 5;
@@ -798,6 +800,8 @@ __global__ __launch_bounds__(256) void k_csr_rm2_f32(int rows, int p, const int*
 // FH_CSR_RPT=2 selects the two-rows-per-thread kernel (measured slower on
 // 16384^2, 12 nnz/row, 256 columns: 18.9 vs 17.5 us per factor)
 static int csr_rows_per_thread() {
+  if (mul_mode() == FH_MUL_CSR_ROWS2) return 2;
+  if (mul_mode() == FH_MUL_CSR_ROWS1) return 1;
   const char* e = getenv("FH_CSR_RPT");
   return (e && e[0] == '2') ? 2 : 1;
 }

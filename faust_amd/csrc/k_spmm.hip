@@ -289,9 +289,10 @@ static void chain_go(const ChainArgsF64& a, const double* X, int ldx, int p, dou
 // forces either way.
 int csr_chain_plan_f64(const std::vector<ChainCsrOp>& ops, int in_rows, int p) {
   const int J = (int)ops.size();
+  if (mul_mode() == FH_MUL_PER_FACTOR) return 0;
   const char* e = getenv("FH_CHAIN_FUSED");
-  if (e && e[0] == '0') return 0;
-  const bool forced = e && e[0] == '1';
+  if (mul_mode() != FH_MUL_FUSED && e && e[0] == '0') return 0;
+  const bool forced = mul_mode() == FH_MUL_FUSED || (e && e[0] == '1');
   if (J < 2 || J > CHAIN_MAX || p <= 0 || (!forced && J < 6)) return 0;
   int R = in_rows, max_nnz = 0;
   for (const auto& o : ops) {

@@ -203,6 +203,7 @@ class Faust:
         check(L.fh_faust_create(C.byref(self.h)))
         for f in self._facs:
             check(L.fh_faust_push_back(self.h, f.h))
+        self._mul_mode = _lib.FH_MUL_AUTO
 
     def __del__(self):
         if getattr(self, "h", None) and self.h.value and _lib._lib is not None:
@@ -233,7 +234,62 @@ class Faust:
 
     @property
     def T(self):
-        return Faust(_facs=self._facs, dtype=self._dt, _transposed=not self._transposed)
+        G = Faust(_facs=self._facs, dtype=self._dt, _transposed=not self._transposed)
+        G._set_mul_mode(self._mul_mode)
+        return G
+
+    def _set_mul_mode(self, mode):
+        check(lib().fh_faust_set_mul_mode(self.h, int(mode)))
+        self._mul_mode = int(mode)
+
+    def _mul_candidates(self):
+        """Execution forms worth timing for this chain (fh_faust_set_mul_mode)."""
+        kinds = [f.info()[:2] for f in self._facs]
+        if any(k == _lib.FH_MAT_BSR for k, _ in kinds):
+            return {f"BSR_KERNEL_{k}": _lib.FH_MUL_BSR_KERNEL(k) for k in (5, 1, 3, 2, 4)}
+        if all(d == _lib.FH_F64 for _, d in kinds):
+            if all(k == _lib.FH_MAT_CSR for k, _ in kinds):
+                return {"PER_FACTOR": _lib.FH_MUL_PER_FACTOR, "FUSED": _lib.FH_MUL_FUSED}
+            return {"PER_FACTOR": _lib.FH_MUL_PER_FACTOR}
+        return {"CSR_ROWS1": _lib.FH_MUL_CSR_ROWS1, "CSR_ROWS2": _lib.FH_MUL_CSR_ROWS2}
+
+    def optimize_time(self, transp=False, inplace=False, nsamples=1, mat=None):
+        """pyfaust Faust.optimize_time (pyfaust.py:3184-3240;
+        TransformHelper::optimize_multiply, faust_TransformHelper.hpp:436-512):
+        time each execution form of F @ mat on the GPU (nsamples products
+        each, after one untimed product, HIP events on the launch stream)
+        and keep the quickest.  mat defaults to a random matrix with
+        min(ncols, 512) columns.  Returns F itself (inplace) or a new Faust
+        on the same factors; `mul_mode_name` and `mul_mode_times_ms` record
+        the choice and the measurements."""
+        G = self.T if transp else self
+        if mat is None:
+            rng = np.random.default_rng(0)
+            mat = rng.standard_normal((G.shape[1], min(G.shape[1], 512)))
+        Xd = mat if (torch is not None and isinstance(mat, torch.Tensor) and mat.is_cuda) else \
+            colmajor_to_device(mat, G.act_dtype)
+        out = torch.empty((Xd.shape[0], G.shape[0]), dtype=Xd.dtype, device="cuda")
+        times = {}
+        saved = G._mul_mode
+        try:
+            for name, mode in self._mul_candidates().items():
+                G._set_mul_mode(mode)
+                G.matmul_device(Xd, out)
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record()
+                for _ in range(max(1, int(nsamples))):
+                    G.matmul_device(Xd, out)
+                e1.record()
+                torch.cuda.synchronize()
+                times[name] = e0.elapsed_time(e1) / max(1, int(nsamples))
+        finally:
+            G._set_mul_mode(saved)
+        best = min(times, key=times.get)
+        target = self if inplace else Faust(_facs=self._facs, dtype=self._dt, _transposed=self._transposed)
+        target._set_mul_mode(self._mul_candidates()[best])
+        target.mul_mode_name = best
+        target.mul_mode_times_ms = times
+        return target
 
     @property
     def H(self):

@@ -106,6 +106,22 @@ int fh_faust_multiply(fh_faust_t F, char op, const void *X, int p, void *Y, void
  * per factor, plus the input transpose of FP32 CSR chains).  No counterpart
  * in the reference; for benchmarks and tests. */
 int fh_faust_last_launches(fh_faust_t F, int *launches);
+/* Execution form of F's products (TransformHelper::set_FM_mul_mode,
+ * faust_TransformHelper.hpp:436-512 optimize_multiply, which times the modes
+ * and keeps the quickest; faust_amd.Faust.optimize_time does the same on the
+ * GPU): FH_MUL_AUTO (defaults), FH_MUL_PER_FACTOR / FH_MUL_FUSED (FP64 CSR
+ * chains: one launch per factor / the one-launch chain when it fits LDS),
+ * FH_MUL_BSR_KERNEL(k) (BSR chains: kernel k = 0..5, as FH_BSR_KERNEL),
+ * FH_MUL_CSR_ROWS1 / FH_MUL_CSR_ROWS2 (FP32 CSR chains: rows per thread).
+ * A mode that does not apply to F's chain kind falls back to the default. */
+#define FH_MUL_AUTO 0
+#define FH_MUL_PER_FACTOR 1
+#define FH_MUL_FUSED 2
+#define FH_MUL_BSR_KERNEL(k) (16 + (k))
+#define FH_MUL_CSR_ROWS1 32
+#define FH_MUL_CSR_ROWS2 33
+int fh_faust_set_mul_mode(fh_faust_t F, int mode);
+int fh_faust_get_mul_mode(fh_faust_t F, int *mode);
 /* Per-factor device timers: HIP events recorded on the launch stream around
  * each factor's kernel of every fh_faust_multiply while enabled (the
  * reference's per-factor Transform::multiply timers, faust_Transform.hpp:

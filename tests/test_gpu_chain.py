@@ -158,3 +158,32 @@ def test_long_chains_fuse_by_default(gpu, monkeypatch):
     F5 = faust_amd.Faust(_cfg1_factors())
     F5 @ X
     assert _launches(F5) == 5
+
+
+def test_optimize_time_fp64(gpu):
+    # Faust.optimize_time: time the execution forms, keep the quickest; every
+    # form of an FP64 chain gives the same bits
+    import ctypes as C
+    from faust_amd import _lib
+    facs = _cfg1_factors()
+    F = faust_amd.Faust(facs)
+    X = np.asfortranarray(np.random.default_rng(8).standard_normal((1024, 64)))
+    ref = oracle.chain_multiply(facs, X, "N")
+    G = F.optimize_time(nsamples=3, mat=X)
+    assert G is not F and G.mul_mode_name in ("PER_FACTOR", "FUSED")
+    assert set(G.mul_mode_times_ms) == {"PER_FACTOR", "FUSED"}
+    assert np.array_equal(G @ X, ref)
+    m = C.c_int()
+    _lib.check(_lib.lib().fh_faust_get_mul_mode(F.h, C.byref(m)))
+    assert m.value == _lib.FH_MUL_AUTO  # F itself unchanged
+    for mode, n in ((_lib.FH_MUL_FUSED, 1), (_lib.FH_MUL_PER_FACTOR, 5)):
+        F._set_mul_mode(mode)
+        assert np.array_equal(F @ X, ref)
+        assert _launches(F) == n
+        Ft = F.T  # the mode follows .T
+        assert np.array_equal(Ft @ X, oracle.chain_multiply(facs, X, "T"))
+        assert _launches(Ft) == n
+    H = F.optimize_time(transp=True, inplace=True)
+    assert H is F
+    with pytest.raises(faust_amd.FaustHipError):
+        F._set_mul_mode(7)

@@ -251,3 +251,17 @@ def test_cfg3b_full_size_against_oracle_columns(gpu):
     Yt = F.T @ X[:, :8]
     reft = oracle.chain_multiply(facs, X[:, :8], "T")
     assert np.linalg.norm(Yt - reft) / np.linalg.norm(reft) < 1e-5
+
+
+def test_optimize_time_bsr(gpu):
+    rng = np.random.default_rng(14)
+    facs = [_bsr_rounded(rand_bsr(1024, 1024, 32, 4, np.random.default_rng(140 + f), scale=1 / np.sqrt(128)))
+            for f in range(3)]
+    F = faust_amd.Faust(facs, dtype="bf16")
+    X = bf16_round(rng.standard_normal((1024, 256)))
+    G = F.optimize_time(nsamples=2, mat=X)
+    assert G.mul_mode_name.startswith("BSR_KERNEL_") and len(G.mul_mode_times_ms) == 5
+    ref = oracle.chain_multiply(facs, X, "N")
+    for H in (G, F):
+        Y = H @ X
+        assert np.linalg.norm(Y - ref) / np.linalg.norm(ref) < 8e-3
