@@ -44,7 +44,8 @@ class fh_palm_params(C.Structure):
     _fields_ = [("is_update_way_R2L", C.c_int), ("factor_format", C.c_int), ("packing_RL", C.c_int),
                 ("no_normalization", C.c_int), ("no_lambda", C.c_int), ("norm2_max_iter", C.c_int),
                 ("norm2_threshold", C.c_double), ("constant_step_size", C.c_int), ("step_size", C.c_double),
-                ("is_fact_side_left", C.c_int), ("record_traj", C.c_int), ("use_grad1", C.c_int)]
+                ("is_fact_side_left", C.c_int), ("record_traj", C.c_int), ("use_grad1", C.c_int),
+                ("grad_calc_opt_mode", C.c_int)]
 
 
 VP = C.c_void_p
@@ -87,6 +88,11 @@ _SIGS = {
     "fh_hierarchical": (C.c_int, [VP, C.c_int, C.c_int, C.POINTER(fh_constraint), C.POINTER(fh_constraint), C.c_int,
                                   C.POINTER(fh_stop_crit), C.POINTER(fh_stop_crit), C.POINTER(fh_palm_params),
                                   C.POINTER(VP)]),
+    "fh_palm4msa2016": (C.c_int, [VP, C.c_int, C.c_int, C.POINTER(fh_constraint), C.c_int, C.POINTER(fh_stop_crit),
+                                  C.POINTER(fh_palm_params), C.c_double, VP, C.POINTER(VP)]),
+    "fh_hierarchical2016": (C.c_int, [VP, C.c_int, C.c_int, C.POINTER(fh_constraint), C.POINTER(fh_constraint),
+                                      C.c_int, C.POINTER(fh_stop_crit), C.POINTER(fh_stop_crit),
+                                      C.POINTER(fh_palm_params), C.c_double, C.POINTER(VP)]),
     "fh_palm_result_nfacts": (C.c_int, [VP, C.POINTER(C.c_int)]),
     "fh_palm_result_lambda": (C.c_int, [VP, C.POINTER(C.c_double)]),
     "fh_palm_result_factor": (C.c_int, [VP, C.c_int, C.POINTER(VP)]),

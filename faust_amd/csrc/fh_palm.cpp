@@ -448,6 +448,200 @@ static void palm4msa2(Palm& P, Chain& S, double& lambda, hipStream_t st) {
   }
 }
 
+// ------------------------------------------------------------ PALM4MSA, 2016 backend
+// Faust::Palm4MSA / Faust::HierarchicalFact (faust_Palm4MSA.hpp,
+// faust_HierarchicalFact.hpp).  Factors are dense and carry the MatDense
+// flags that change results: setZeros flags zero, setEyes flags a square
+// matrix identity (faust_MatDense.hpp:243-265); gemm with a flagged operand
+// copies the other operand's flags (beta = 0) or flags the result zero
+// (faust_linear_algebra.hpp:600-700); sub() and the projectors' writes clear
+// them.  With beta in {0, -1} the shortcut values equal the full product, so
+// every product here is the canonical one; the flags decide the spectral
+// norms only: 0 for a flagged-zero matrix, 1 for a flagged identity
+// (MatDense::spectralNorm, faust_MatDense.hpp:660-700).
+struct DF {
+  DMatP M;
+  bool zeros = false, ident = false;
+};
+
+static DMatP dense_new(int m, int n, hipStream_t st) {
+  return make_dense(FH_F64, m, n, nullptr, std::max(m, 1), false, st);
+}
+
+static DF df_zeros(int m, int n, hipStream_t st) {
+  DF d;
+  d.M = dense_new(m, n, st);
+  fill_f64(d.M->d.as<double>(), (int64_t)m * n, 0.0, st);
+  d.zeros = true;
+  return d;
+}
+
+static DF df_eye(int m, int n, hipStream_t st) {
+  DF d;
+  d.M = dense_new(m, n, st);
+  eye_f64(d.M->d.as<double>(), m, n, st);
+  d.ident = (m == n);
+  return d;
+}
+
+static DF df_copy(const DF& a, hipStream_t st) {
+  DF d{dense_of(*a.M, st), a.zeros, a.ident};
+  return d;
+}
+
+// C = alpha op(A) op(B) + beta C_old (gemm), with the flags of the result
+static DF gemm16(const DF& A, char opA, const DF& B, char opB, double alpha, double beta, const DMat* C_old,
+                 hipStream_t st) {
+  DF C;
+  C.M = std::make_shared<DMat>();
+  if (beta != 0.0) C.M = dense_of(*C_old, st);
+  matmul_f64(*A.M, opA, *B.M, opB, alpha, beta, *C.M, st);
+  if (beta == 0.0) {
+    if (A.zeros || B.zeros) {
+      fill_f64(C.M->d.as<double>(), (int64_t)C.M->m * C.M->n, 0.0, st);  // memset: +0 everywhere
+      C.zeros = true;
+    } else if (A.ident || B.ident) {
+      const DF& other = A.ident ? B : A;
+      FH_CHECK(!(other.ident && alpha != 1.0), "palm4msa2016: scaled copy of a flagged identity");
+      C.zeros = other.zeros;
+      C.ident = other.ident;
+    }
+  }
+  return C;
+}
+
+static DF mul16(const DF& A, const DF& B, hipStream_t st) { return gemm16(A, 'N', B, 'N', 1.0, 0.0, nullptr, st); }
+
+static double specnorm16(const DF& A, int maxit, double thr, hipStream_t st) {
+  if (A.zeros) return 0.0;
+  if (A.ident) return 1.0;
+  const bool wide = A.M->m <= A.M->n;
+  DF AtA = wide ? gemm16(A, 'N', A, 'T', 1.0, 0.0, nullptr, st) : gemm16(A, 'T', A, 'N', 1.0, 0.0, nullptr, st);
+  int flag = 0;
+  const double lam = power_iteration_f64({ChainOp{AtA.M.get(), 'N'}}, AtA.M->m, maxit, thr, &flag, st);
+  return std::sqrt(std::fabs(lam));
+}
+
+// trace(X^T Y): diagonal fma chains (k ascending), canonical sum
+static double trace_xty(const DMat& X, const DMat& Y, hipStream_t st) {
+  DBuf dg(sizeof(double) * std::max(X.n, 1)), s(sizeof(double));
+  diag_dot_f64(X.d.as<double>(), Y.d.as<double>(), X.m, X.n, dg.as<double>(), st);
+  reduce_sum_f64(dg.as<double>(), X.n, s.as<double>(), st);
+  return read_scalar(s, st);
+}
+
+struct Palm16 {
+  DMatP data;
+  std::vector<const DevCons*> cons;
+  fh_stop_crit sc;
+  fh_palm_params p;
+  double lambda = 1.0;
+  std::vector<DF> S;
+  int call_id = 0;
+  std::vector<TrajPoint>* traj = nullptr;
+
+  // init_fact (faust_Palm4MSA.hpp:628-665): the first factor updated is
+  // zero, the others identity
+  void init_fact(int nf, hipStream_t st) {
+    S.clear();
+    for (int i = 0; i < nf; ++i) {
+      const bool first = p.is_update_way_R2L ? (i == nf - 1) : (i == 0);
+      const fh_constraint& c = cons[i]->c;
+      S.push_back(first ? df_zeros(c.nrows, c.ncols, st) : df_eye(c.nrows, c.ncols, st));
+    }
+  }
+
+  // compute_grad_over_c_int_opt (faust_Palm4MSA.hpp:229-330): one of four
+  // associations by a flop estimate
+  DMatP grad_int(const DF& L, const DF& Sj, const DF& R, double c, hipStream_t st) {
+    const unsigned long long L1 = L.M->m, L2 = L.M->n, R2 = R.M->n, S2 = Sj.M->n;
+    const unsigned long long cx[4] = {L1 * L2 * S2 + L1 * S2 * R2 + L2 * L1 * R2 + L2 * R2 * R2,
+                                      L1 * L2 * S2 + L1 * S2 * R2 + L1 * R2 * S2 + L2 * L1 * S2,
+                                      L2 * S2 * R2 + L1 * L2 * R2 + L2 * L1 * R2 + L2 * R2 * S2,
+                                      L2 * S2 * R2 + L1 * L2 * R2 + L1 * R2 * S2 + L2 * L1 * S2};
+    int idx = 0;
+    for (int t = 1; t < 4; ++t)
+      if (cx[t] < cx[idx]) idx = t;
+    const bool disabled = p.grad_calc_opt_mode == 0;
+    DF err;
+    if (idx == 0 || idx == 1 || disabled) {
+      DF t1 = mul16(L, Sj, st);
+      err = gemm16(t1, 'N', R, 'N', lambda, -1.0, data.get(), st);
+    } else {
+      DF t1 = mul16(Sj, R, st);
+      err = gemm16(L, 'N', t1, 'N', lambda, -1.0, data.get(), st);
+    }
+    DF g;
+    if (idx == 0 || idx == 2 || disabled) {
+      DF t3 = gemm16(L, 'T', err, 'N', lambda, 0.0, nullptr, st);
+      g = gemm16(t3, 'N', R, 'T', 1.0 / c, 0.0, nullptr, st);
+    } else {
+      DF t3 = gemm16(err, 'N', R, 'T', lambda, 0.0, nullptr, st);
+      g = gemm16(L, 'T', t3, 'N', 1.0 / c, 0.0, nullptr, st);
+    }
+    return g.M;
+  }
+
+  // compute_grad_over_c_ext_opt (faust_Palm4MSA.hpp:205-227)
+  DMatP grad_ext(const DF& L, const DF& Sj, const DF& R, double c, hipStream_t st) {
+    DMatP err = dense_of(*data, st);
+    mul_all_ends({L.M.get(), Sj.M.get(), R.M.get()}, *err, lambda, -1.0, {'N'}, st);
+    auto g = std::make_shared<DMat>();
+    mul_all_ends({L.M.get(), err.get(), R.M.get()}, *g, lambda / c, 0.0, {'T', 'N', 'T'}, st);
+    return g;
+  }
+
+  // next_step (faust_Palm4MSA.hpp:690-740)
+  void next_step(int it, hipStream_t st) {
+    const int J = (int)S.size();
+    const bool R2L = p.is_update_way_R2L;
+    std::vector<DF> RorL(J);
+    DF LorR;
+    if (!R2L) {
+      LorR = df_eye(cons[0]->c.nrows, cons[0]->c.nrows, st);
+      RorL[J - 1] = df_eye(cons[J - 1]->c.ncols, cons[J - 1]->c.ncols, st);
+      for (int i = J - 2; i >= 0; --i) RorL[i] = mul16(S[i + 1], RorL[i + 1], st);
+    } else {
+      LorR = df_eye(cons[J - 1]->c.ncols, cons[J - 1]->c.ncols, st);
+      RorL[0] = df_eye(cons[0]->c.nrows, cons[0]->c.nrows, st);
+      for (int i = 0; i < J - 1; ++i) RorL[i + 1] = mul16(RorL[i], S[i], st);
+    }
+    for (int j = 0; j < J; ++j) {
+      const int f = R2L ? J - 1 - j : j;
+      double c = 1.0 / p.step_size;
+      if (!p.constant_step_size) {  // compute_c (faust_Palm4MSA.hpp:612-625)
+        const double nL = specnorm16(LorR, p.norm2_max_iter, p.norm2_threshold, st);
+        const double nR = specnorm16(RorL[f], p.norm2_max_iter, p.norm2_threshold, st);
+        c = 1.001 * nR * nR * nL * nL * lambda * lambda;
+      }
+      const DF& L = R2L ? RorL[f] : LorR;
+      const DF& R = R2L ? LorR : RorL[f];
+      DMatP g = p.grad_calc_opt_mode == 2 ? grad_ext(L, S[f], R, c, st) : grad_int(L, S[f], R, c, st);
+      // compute_projection: S -= grad/c, then project (the flags are cleared)
+      const fh_constraint& cc = cons[f]->c;
+      sub_f64(S[f].M->d.as<double>(), g->d.as<double>(), (int64_t)cc.nrows * cc.ncols, st);
+      S[f].zeros = S[f].ident = false;
+      prox_f64(cc.name, S[f].M->d.as<double>(), cc.nrows, cc.ncols, cc.k, cc.normalized != 0, cc.pos != 0,
+               cons[f]->supp.as<double>(), st);
+      S[f].M->nnz = -1;
+      LorR = R2L ? mul16(S[f], LorR, st) : mul16(LorR, S[f], st);
+    }
+    // compute_lambda (faust_Palm4MSA.hpp:333-358)
+    const double tr_hh = trace_xty(*LorR.M, *LorR.M, st);
+    if (tr_hh == 0.0) throw Error(FH_ERR_NUMERIC, "compute_lambda : Xhatt_Xhat_tr equal 0 so lambda is infinite");
+    lambda = std::fabs(trace_xty(*data, *LorR.M, st) / tr_hh);
+    if (std::isnan(lambda)) throw Error(FH_ERR_NUMERIC, "compute_lambda : lambda is NaN");
+    if (traj) traj->push_back({(double)call_id, (double)it, std::nan(""), lambda});
+  }
+
+  // compute_facts (faust_Palm4MSA.hpp:170-183), StoppingCriterion counting
+  void compute_facts(hipStream_t st) {
+    if (sc.is_criterion_error)
+      throw Error(FH_ERR_UNSUPPORTED, "2016 backend: only the iteration-count stopping criterion");
+    for (int it = 0; it < sc.num_its; ++it) next_step(it, st);
+  }
+};
+
 static std::vector<std::unique_ptr<DevCons>> upload_cons(const fh_constraint* c, int n, hipStream_t st) {
   std::vector<std::unique_ptr<DevCons>> out;
   for (int i = 0; i < n; ++i) {
@@ -553,6 +747,92 @@ int fh_hierarchical(const double* A, int nrows, int ncols, const fh_constraint* 
     palm4msa2(P2, S, glo_lambda, st);
   }
   r->lambda = glo_lambda;
+  FH_HIP(hipStreamSynchronize(st));
+  *out = r.release();
+  FH_API_END
+}
+
+int fh_palm4msa2016(const double* A, int nrows, int ncols, const fh_constraint* cons, int ncons,
+                    const fh_stop_crit* sc, const fh_palm_params* p, double lambda_init, fh_faust_t init_facts,
+                    fh_palm_result_t* out) {
+  FH_API_BEGIN
+  FH_CHECK(A && cons && sc && p && out, "null argument");
+  FH_CHECK(ncons > 0, "No constraint passed to palm4msa.");
+  hipStream_t st = nullptr;
+  auto r = std::make_unique<fh_palm_result_s>();
+  auto dc = upload_cons(cons, ncons, st);
+  Palm16 P;
+  P.data = make_dense(FH_F64, nrows, ncols, A, std::max(nrows, 1), false, st);
+  for (auto& c : dc) P.cons.push_back(c.get());
+  P.sc = *sc;
+  P.p = *p;
+  P.lambda = lambda_init;
+  P.traj = p->record_traj ? &r->traj : nullptr;
+  if (init_facts && !init_facts->f.empty()) {
+    FH_CHECK((int)init_facts->f.size() == ncons, "palm4msa2016: init_facts and constraints differ in number");
+    for (auto& f : init_facts->f) {
+      FH_CHECK(f->dtype == FH_F64 && f->kind != K_BSR, "init factors must be FP64 dense or CSR");
+      P.S.push_back(DF{dense_of(*f, st), false, false});
+    }
+  } else {
+    P.init_fact(ncons, st);
+  }
+  P.compute_facts(st);
+  for (auto& d : P.S) r->S.push_back(d.M);
+  r->lambda = P.lambda;
+  FH_HIP(hipStreamSynchronize(st));
+  *out = r.release();
+  FH_API_END
+}
+
+int fh_hierarchical2016(const double* A, int nrows, int ncols, const fh_constraint* fac_cons,
+                        const fh_constraint* res_cons, int nlevels, const fh_stop_crit* sc_local,
+                        const fh_stop_crit* sc_global, const fh_palm_params* p, double init_lambda,
+                        fh_palm_result_t* out) {
+  FH_API_BEGIN
+  FH_CHECK(A && fac_cons && res_cons && sc_local && sc_global && p && out, "null argument");
+  FH_CHECK(nlevels > 0, "hierarchical2016: no level");
+  hipStream_t st = nullptr;
+  auto r = std::make_unique<fh_palm_result_s>();
+  auto fc = upload_cons(fac_cons, nlevels, st);
+  auto rc = upload_cons(res_cons, nlevels, st);
+  const bool left = p->is_fact_side_left;
+  std::vector<TrajPoint>* traj = p->record_traj ? &r->traj : nullptr;
+  Palm16 P2, PG;
+  P2.data = make_dense(FH_F64, nrows, ncols, A, std::max(nrows, 1), false, st);
+  PG.data = P2.data;
+  P2.sc = *sc_local;
+  PG.sc = *sc_global;
+  P2.p = PG.p = *p;
+  P2.traj = PG.traj = traj;
+  PG.lambda = init_lambda;
+  // HierarchicalFact::init / next_step (faust_HierarchicalFact.hpp:95-150)
+  PG.cons = {left ? fc[0].get() : rc[0].get()};
+  PG.init_fact(1, st);
+  for (int i = 0; i < nlevels; ++i) {
+    P2.cons = {fc[i].get(), rc[i].get()};
+    P2.init_fact(2, st);
+    P2.lambda = init_lambda;
+    P2.call_id = 2 * i;
+    P2.compute_facts(st);
+    PG.lambda *= P2.lambda;  // update_lambda_from_palm
+    if (left) {
+      PG.cons[0] = fc[i].get();
+      PG.cons.insert(PG.cons.begin() + 1, rc[i].get());
+      PG.S.insert(PG.S.begin(), df_copy(P2.S[0], st));  // init_fact_from_palm
+      PG.S[1] = df_copy(P2.S[1], st);
+    } else {
+      PG.cons.insert(PG.cons.begin() + i, fc[i].get());
+      PG.cons[i + 1] = rc[i].get();
+      PG.S.back() = df_copy(P2.S[0], st);
+      PG.S.push_back(df_copy(P2.S[1], st));
+    }
+    PG.call_id = 2 * i + 1;
+    PG.compute_facts(st);
+    P2.data = dense_of(*(left ? PG.S[0] : PG.S[i + 1]).M, st);  // set_data(get_res(...))
+  }
+  for (auto& d : PG.S) r->S.push_back(d.M);
+  r->lambda = PG.lambda;
   FH_HIP(hipStreamSynchronize(st));
   *out = r.release();
   FH_API_END

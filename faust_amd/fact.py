@@ -67,12 +67,11 @@ def _palm_raw(M, p, record_traj=False):
     cons = (_lib.fh_constraint * len(p.constraints))(*[c._c(keep) for c in p.constraints])
     sc = p.stop_crit._c()
     pc = p._c(record_traj)
-    init = None
-    if p.init_facts is not None:
-        init = Faust(list(p.init_facts)).h
+    init_f = Faust(list(p.init_facts)) if p.init_facts is not None else None  # alive across the call
     out = C.c_void_p()
     check(lib().fh_palm4msa(M.ctypes.data_as(C.c_void_p), M.shape[0], M.shape[1], cons, len(p.constraints),
-                            C.byref(sc), C.byref(pc), p.init_lambda, init, C.byref(out)))
+                            C.byref(sc), C.byref(pc), p.init_lambda, init_f.h if init_f is not None else None,
+                            C.byref(out)))
     return PalmResult(out)
 
 
@@ -90,33 +89,91 @@ def _hier_raw(M, p, record_traj=False):
     return PalmResult(out)
 
 
+def _palm2016_raw(M, p, record_traj=False):
+    M = np.asfortranarray(M, dtype=np.float64)
+    keep = []
+    cons = (_lib.fh_constraint * len(p.constraints))(*[c._c(keep) for c in p.constraints])
+    sc = p.stop_crit._c()
+    pc = p._c(record_traj)
+    init_f = None  # alive across the call
+    if p.init_facts is not None:
+        init_f = Faust([np.asarray(f, dtype=np.float64) for f in p.init_facts])
+    out = C.c_void_p()
+    check(lib().fh_palm4msa2016(M.ctypes.data_as(C.c_void_p), M.shape[0], M.shape[1], cons, len(p.constraints),
+                                C.byref(sc), C.byref(pc), p.init_lambda, init_f.h if init_f is not None else None,
+                                C.byref(out)))
+    return PalmResult(out)
+
+
+def _hier2016_raw(M, p, record_traj=False):
+    M = np.asfortranarray(M, dtype=np.float64)
+    keep = []
+    n = len(p.fact_constraints)
+    fc = (_lib.fh_constraint * n)(*[c._c(keep) for c in p.fact_constraints])
+    rc = (_lib.fh_constraint * n)(*[c._c(keep) for c in p.res_constraints])
+    s1, s2 = p.stop_crits[0]._c(), p.stop_crits[1]._c()
+    pc = p._c(record_traj, p.is_fact_side_left)
+    out = C.c_void_p()
+    check(lib().fh_hierarchical2016(M.ctypes.data_as(C.c_void_p), M.shape[0], M.shape[1], fc, rc, n, C.byref(s1),
+                                    C.byref(s2), C.byref(pc), p.init_lambda, C.byref(out)))
+    return PalmResult(out)
+
+
+def _faust_2016_palm(res):
+    """fact_palm4MSA_gen's Faust (FaustFact.hpp:380-410): each dense factor
+    stays dense unless 2*nnz + rows + 1 < rows*cols (then sparse), and lambda
+    goes to TransformHelper(facts, lambda): the factor of fewest bytes."""
+    import scipy.sparse as sp
+    mats = []
+    for a in res.dense_factors():
+        nnz = int(np.count_nonzero(a))
+        mats.append(sp.csr_matrix(a) if 2 * nnz + a.shape[0] + 1 < a.shape[0] * a.shape[1] else a)
+    sizes = [(m.nnz * 12 + (m.shape[0] + 1) * 4) if sp.issparse(m) else m.size * 8 for m in mats]
+    i = int(np.argmin(sizes))
+    mats[i] = mats[i] * res.lam
+    return Faust(mats)
+
+
+def _faust_2016_hier(res):
+    """fact_hierarchical_gen's Faust (FaustFact.hpp:486-500): sparse factors
+    (exact zeros dropped), lambda multiplied into the first one."""
+    import scipy.sparse as sp
+    mats = [sp.csr_matrix(a) for a in res.dense_factors()]
+    mats[0] = mats[0] * res.lam
+    return Faust(mats)
+
+
 def palm4msa(M, p, ret_lambda=False, backend=2020, on_gpu=True, ret_raw=False):
-    """Factorize M with PALM4MSA (2020 backend) on the GPU."""
-    if backend != 2020:
-        raise ValueError("only backend=2020 is implemented")
+    """Factorize M with PALM4MSA on the GPU: backend=2020 (palm4msa2,
+    faust_palm4msa2020.hpp) or backend=2016 (Faust::Palm4MSA,
+    faust_Palm4MSA.hpp; pyfaust's default).  ret_raw returns the factors as
+    computed, lambda and the trajectory."""
+    if backend not in (2016, 2020):
+        raise ValueError("backend must be 2016 or 2020")
     if not isinstance(p, ParamsPalm4MSA):
         raise TypeError("p must be a ParamsPalm4MSA")
-    res = _palm_raw(M, p, record_traj=ret_raw)
+    res = (_palm2016_raw if backend == 2016 else _palm_raw)(M, p, record_traj=ret_raw)
     if ret_raw:
         return res
-    F = _faust_times_lambda(res)
+    F = _faust_2016_palm(res) if backend == 2016 else _faust_times_lambda(res)
     return (F, res.lam) if ret_lambda else F
 
 
 def hierarchical(M, p, ret_lambda=False, backend=2020, on_gpu=True, ret_raw=False):
-    """Hierarchical factorization (2020 backend) on the GPU.  p may be a
-    ParamsHierarchical or the string 'hadamard' / 'rectmat' shortcut for
-    'hadamard' only."""
-    if backend != 2020:
-        raise ValueError("only backend=2020 is implemented")
+    """Hierarchical factorization on the GPU: backend=2020 (hierarchical,
+    faust_hierarchical.hpp) or backend=2016 (Faust::HierarchicalFact,
+    faust_HierarchicalFact.hpp; pyfaust's default).  p may be a
+    ParamsHierarchical or the string 'hadamard'."""
+    if backend not in (2016, 2020):
+        raise ValueError("backend must be 2016 or 2020")
     if isinstance(p, str):
         if p.lower() != "hadamard":
             raise ValueError("only the 'hadamard' shortcut is supported")
         p = ParamsHierarchicalWHT(int(np.log2(M.shape[0])))
     if not isinstance(p, ParamsHierarchical):
         raise TypeError("p must be a ParamsHierarchical")
-    res = _hier_raw(M, p, record_traj=ret_raw)
+    res = (_hier2016_raw if backend == 2016 else _hier_raw)(M, p, record_traj=ret_raw)
     if ret_raw:
         return res
-    F = _faust_times_lambda(res)
+    F = _faust_2016_hier(res) if backend == 2016 else _faust_times_lambda(res)
     return (F, res.lam) if ret_lambda else F

@@ -125,11 +125,20 @@ def _constraints(cons):
 
 
 class ParamsFact:
-    """Common defaults, factparams.py:955-962."""
+    """Common defaults, factparams.py:955-962.  grad_calc_opt_mode (2016
+    backend only, factparams.py:952-962): DISABLED_OPT, INTERNAL_OPT or
+    EXTERNAL_OPT (pyfaust's default)."""
+
+    DISABLED_OPT = 0
+    INTERNAL_OPT = 1
+    EXTERNAL_OPT = 2
 
     def __init__(self, is_update_way_R2L=False, init_lambda=1.0, step_size=1e-16, constant_step_size=False,
                  is_verbose=False, factor_format="dynamic", packing_RL=True, no_normalization=False,
-                 no_lambda=False, norm2_max_iter=100, norm2_threshold=1e-6, **kw):
+                 no_lambda=False, norm2_max_iter=100, norm2_threshold=1e-6, grad_calc_opt_mode=2, **kw):
+        if grad_calc_opt_mode not in (0, 1, 2):
+            raise ValueError("grad_calc_opt_mode must be DISABLED_OPT, INTERNAL_OPT or EXTERNAL_OPT")
+        self.grad_calc_opt_mode = int(grad_calc_opt_mode)
         if factor_format not in ("dense", "sparse", "dynamic"):
             raise ValueError("factor_format must be either 'dense', 'sparse' or 'dynamic'")
         self.is_update_way_R2L = bool(is_update_way_R2L)
@@ -166,6 +175,7 @@ class ParamsFact:
             except ValueError:
                 g1 = False
         p.use_grad1 = int(g1)
+        p.grad_calc_opt_mode = int(getattr(self, "grad_calc_opt_mode", 2))
         return p
 
 
@@ -175,9 +185,10 @@ class ParamsPalm4MSA(ParamsFact):
     def __init__(self, constraints, stop_crit, init_facts=None, init_lambda=1.0, step_size=1e-16,
                  constant_step_size=False, is_update_way_R2L=False, is_verbose=False, norm2_max_iter=100,
                  norm2_threshold=1e-6, factor_format="dynamic", packing_RL=True, no_normalization=False,
-                 no_lambda=False, **kw):
+                 no_lambda=False, grad_calc_opt_mode=2, **kw):
         super().__init__(is_update_way_R2L, init_lambda, step_size, constant_step_size, is_verbose, factor_format,
-                         packing_RL, no_normalization, no_lambda, norm2_max_iter, norm2_threshold)
+                         packing_RL, no_normalization, no_lambda, norm2_max_iter, norm2_threshold,
+                         grad_calc_opt_mode)
         self.constraints = _constraints(constraints)
         self.stop_crit = stop_crit
         self.init_facts = init_facts
@@ -190,9 +201,10 @@ class ParamsHierarchical(ParamsFact):
     def __init__(self, fact_constraints, res_constraints, stop_crit1, stop_crit2, is_update_way_R2L=False,
                  init_lambda=1.0, step_size=1e-16, constant_step_size=False, is_fact_side_left=False,
                  is_verbose=False, factor_format="dynamic", packing_RL=True, no_normalization=False,
-                 no_lambda=False, norm2_max_iter=100, norm2_threshold=1e-6, **kw):
+                 no_lambda=False, norm2_max_iter=100, norm2_threshold=1e-6, grad_calc_opt_mode=2, **kw):
         super().__init__(is_update_way_R2L, init_lambda, step_size, constant_step_size, is_verbose, factor_format,
-                         packing_RL, no_normalization, no_lambda, norm2_max_iter, norm2_threshold)
+                         packing_RL, no_normalization, no_lambda, norm2_max_iter, norm2_threshold,
+                         grad_calc_opt_mode)
         fact_constraints = _constraints(fact_constraints)
         res_constraints = _constraints(res_constraints)
         if len(fact_constraints) != len(res_constraints):

@@ -183,6 +183,8 @@ typedef struct {
   int record_traj;        /* record (call, it, rel_err, lambda) after each iteration */
   int use_grad1;          /* gradient as compute_n_apply_grad1 (USE_GRAD1; the reference's GPU2
                              path), else compute_n_apply_grad2 (its CPU default) */
+  int grad_calc_opt_mode; /* 2016 backend only: 0 DISABLED_OPT, 1 INTERNAL_OPT, 2 EXTERNAL_OPT
+                             (factparams.py:952-962; pyfaust's default is 2) */
 } fh_palm_params;
 
 typedef struct fh_palm_result_s *fh_palm_result_t;
@@ -194,6 +196,23 @@ int fh_palm4msa(const double *A, int nrows, int ncols, const fh_constraint *cons
 int fh_hierarchical(const double *A, int nrows, int ncols, const fh_constraint *fac_cons,
                     const fh_constraint *res_cons, int nlevels, const fh_stop_crit *sc_local,
                     const fh_stop_crit *sc_global, const fh_palm_params *p, fh_palm_result_t *out);
+
+/* The 2016 backend (pyfaust's default, backend=2016): Faust::Palm4MSA
+ * (faust_Palm4MSA.hpp: dense factors, init_fact :628-665, next_step
+ * :690-740, compute_grad_over_c :205-330, compute_lambda :333-358,
+ * compute_c :612-625) and Faust::HierarchicalFact (faust_HierarchicalFact.hpp
+ * :95-175).  Factors stay dense; the result's factors are the dense S_i and
+ * its lambda is not folded in (FaustFact.hpp:380-420 and :486-500 build the
+ * returned Faust from them).  Stopping: iteration counts only (an error
+ * criterion gives FH_ERR_UNSUPPORTED).  The trajectory, when recorded, holds
+ * (call, iteration, NaN, lambda) after every iteration. */
+int fh_palm4msa2016(const double *A, int nrows, int ncols, const fh_constraint *cons, int ncons,
+                    const fh_stop_crit *sc, const fh_palm_params *p, double lambda_init,
+                    fh_faust_t init_facts, fh_palm_result_t *out);
+int fh_hierarchical2016(const double *A, int nrows, int ncols, const fh_constraint *fac_cons,
+                        const fh_constraint *res_cons, int nlevels, const fh_stop_crit *sc_local,
+                        const fh_stop_crit *sc_global, const fh_palm_params *p, double init_lambda,
+                        fh_palm_result_t *out);
 int fh_palm_result_nfacts(fh_palm_result_t r, int *n);
 int fh_palm_result_lambda(fh_palm_result_t r, double *lambda);
 /* factor i as a new matrix handle (owned by the caller) */

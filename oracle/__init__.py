@@ -50,7 +50,8 @@ class or_params(C.Structure):
                 ("packing_RL", C.c_int), ("no_normalization", C.c_int),
                 ("no_lambda", C.c_int), ("norm2_max_iter", C.c_int),
                 ("norm2_threshold", C.c_double), ("constant_step_size", C.c_int),
-                ("step_size", C.c_double), ("is_fact_side_left", C.c_int), ("use_grad1", C.c_int)]
+                ("step_size", C.c_double), ("is_fact_side_left", C.c_int), ("use_grad1", C.c_int),
+                ("grad_calc_opt_mode", C.c_int)]
 
 
 def build():
@@ -223,6 +224,7 @@ def _params(**kw):
     p.step_size = float(kw.get("step_size", 1e-16))
     p.is_fact_side_left = int(kw.get("is_fact_side_left", False))
     p.use_grad1 = int(kw.get("use_grad1", False))
+    p.grad_calc_opt_mode = int(kw.get("grad_calc_opt_mode", 2))
     return p
 
 
@@ -299,6 +301,41 @@ def hierarchical(A, fac_cons, res_cons, num_its_local=500, num_its_global=500, r
     res = C.c_void_p()
     _check(lib().or_hierarchical(_dp(A), A.shape[0], A.shape[1], fa, ra, len(fac_cons), C.byref(s1), C.byref(s2),
                                  C.byref(p), int(record_traj), C.byref(res)))
+    return Result(res)
+
+
+def palm4msa2016(A, cons, num_its=500, lambda_init=1.0, init_facts=None, record_traj=False, **params):
+    """The 2016 backend (faust_Palm4MSA.hpp): dense factors; trajectory rows
+    (call, iteration, NaN, lambda)."""
+    A = np.asfortranarray(A, dtype=np.float64)
+    ca, keep = _cons_array(cons)
+    p = _params(**params)
+    s = _stop(num_its)
+    res = C.c_void_p()
+    if init_facts:
+        pk = _FacPack(init_facts)
+        arr, n_init = pk.arr, len(init_facts)
+    else:
+        arr, n_init = None, 0
+    _check(lib().or_palm4msa2016(_dp(A), A.shape[0], A.shape[1], ca, len(cons), C.byref(s), C.byref(p),
+                                 C.c_double(lambda_init), arr, n_init, int(record_traj), C.byref(res)))
+    return Result(res)
+
+
+def hierarchical2016(A, fac_cons, res_cons, num_its_local=500, num_its_global=500, init_lambda=1.0,
+                     record_traj=False, **params):
+    """The 2016 backend (faust_HierarchicalFact.hpp): dense factors, lambda
+    not folded in."""
+    A = np.asfortranarray(A, dtype=np.float64)
+    fa, k1 = _cons_array(fac_cons)
+    ra, k2 = _cons_array(res_cons)
+    p = _params(**params)
+    s1 = _stop(num_its_local)
+    s2 = _stop(num_its_global)
+    res = C.c_void_p()
+    _check(lib().or_hierarchical2016(_dp(A), A.shape[0], A.shape[1], fa, ra, len(fac_cons), C.byref(s1),
+                                     C.byref(s2), C.byref(p), C.c_double(init_lambda), int(record_traj),
+                                     C.byref(res)))
     return Result(res)
 
 

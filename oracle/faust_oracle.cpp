@@ -982,6 +982,202 @@ Chain chain_of(const or_fac* facs, int n) {
   return c;
 }
 
+
+// ---------------------------------------------------------------- PALM4MSA, 2016 backend
+// faust_Palm4MSA.hpp / faust_HierarchicalFact.hpp.  The factors are dense
+// and carry the two MatDense flags that change results (faust_MatDense.h,
+// faust_MatDense.hpp:243-265): setZeros flags a matrix zero, setEyes flags a
+// *square* matrix identity; the copy constructor and operator= copy the
+// flags; sub() and the non-const getData() (through which every projector
+// writes) clear them.  gemm (faust_linear_algebra.hpp:537-700) copies the
+// flags of the other operand when one operand is flagged identity (beta =
+// 0), sets the zero flag when one is flagged zero (beta = 0), and clears them
+// otherwise.  With beta in {0, -1} the shortcut values equal the full
+// product, so values here are always the canonical matmul; the flags only
+// decide the spectral norms: a flagged-zero matrix has norm 0 and a
+// flagged-identity one norm 1, exactly (MatDense::spectralNorm,
+// faust_MatDense.hpp:660-700).
+struct DF {
+  Mat M;
+  bool zeros = false, ident = false;
+};
+
+DF df_zeros(int m, int n) {
+  DF d;
+  d.M.kind = Mat::DENSE;
+  d.M.m = m;
+  d.M.n = n;
+  d.M.d.assign((size_t)m * n, 0.0);
+  d.zeros = true;
+  return d;
+}
+
+DF df_eye(int m, int n) {
+  DF d = df_zeros(m, n);
+  for (int i = 0; i < std::min(m, n); ++i) d.M.d[(size_t)i * m + i] = 1.0;
+  d.zeros = false;
+  d.ident = (m == n);
+  return d;
+}
+
+// C = alpha op(A) op(B) + beta C (gemm), with the flags of the result
+DF gemm16(const DF& A, char opA, const DF& B, char opB, double alpha, double beta, const Mat* C_old = nullptr) {
+  DF C;
+  if (beta != 0.0) C.M = *C_old;
+  matmul(A.M, opA, B.M, opB, alpha, beta, C.M);
+  if (beta == 0.0) {
+    if (A.zeros || B.zeros) {
+      std::fill(C.M.d.begin(), C.M.d.end(), 0.0);  // memset: +0 everywhere
+      C.zeros = true;
+    } else if (A.ident || B.ident) {
+      const DF& other = A.ident ? B : A;
+      if (other.ident && alpha != 1.0) throw std::logic_error("gemm16: scaled copy of a flagged identity");
+      C.zeros = other.zeros;
+      C.ident = other.ident;
+    }
+  }
+  return C;
+}
+
+DF mul16(const DF& A, const DF& B) { return gemm16(A, 'N', B, 'N', 1.0, 0.0); }
+
+double specnorm16(const DF& A, int maxit, double thr) {
+  if (A.zeros) return 0.0;
+  if (A.ident) return 1.0;
+  const bool wide = A.M.m <= A.M.n;
+  DF AtA = wide ? gemm16(A, 'N', A, 'T', 1.0, 0.0) : gemm16(A, 'T', A, 'N', 1.0, 0.0);
+  int flag;
+  auto op = std::make_shared<Mat>(AtA.M);
+  double lam = power_iteration({{op, 'N'}}, AtA.M.m, maxit, thr, flag);
+  return std::sqrt(std::fabs(lam));
+}
+
+double trace_of_product_T(const Mat& X, const Mat& Y) {
+  // trace(X^T Y): the diagonal entries are fma chains over k ascending, summed
+  // by the canonical reduction
+  std::vector<double> dg(X.n);
+  for (int i = 0; i < X.n; ++i) {
+    double acc = 0.0;
+    for (int k = 0; k < X.m; ++k) acc = std::fma(X.d[(size_t)i * X.m + k], Y.d[(size_t)i * Y.m + k], acc);
+    dg[i] = acc;
+  }
+  return rsum(dg.data(), (int64_t)dg.size());
+}
+
+struct Palm16 {
+  Mat data;
+  std::vector<or_cons> cons;
+  or_stop sc;
+  or_params p;
+  double lambda = 1.0;
+  std::vector<DF> S;
+  int call_id = 0;
+  std::vector<TrajPoint>* traj = nullptr;
+
+  // init_fact (faust_Palm4MSA.hpp:628-665)
+  void init_fact(int nf) {
+    S.clear();
+    for (int i = 0; i < nf; ++i) {
+      const bool first = p.is_update_way_R2L ? (i == nf - 1) : (i == 0);
+      S.push_back(first ? df_zeros(cons[i].nrows, cons[i].ncols) : df_eye(cons[i].nrows, cons[i].ncols));
+    }
+  }
+
+  // compute_grad_over_c_int_opt (faust_Palm4MSA.hpp:229-330)
+  Mat grad_int(const DF& L, const DF& Sj, const DF& R, double c) {
+    const unsigned long long L1 = L.M.m, L2 = L.M.n, R2 = R.M.n, S2 = Sj.M.n;
+    unsigned long long cx[4] = {L1 * L2 * S2 + L1 * S2 * R2 + L2 * L1 * R2 + L2 * R2 * R2,
+                                L1 * L2 * S2 + L1 * S2 * R2 + L1 * R2 * S2 + L2 * L1 * S2,
+                                L2 * S2 * R2 + L1 * L2 * R2 + L2 * L1 * R2 + L2 * R2 * S2,
+                                L2 * S2 * R2 + L1 * L2 * R2 + L1 * R2 * S2 + L2 * L1 * S2};
+    int idx = 0;
+    for (int t = 1; t < 4; ++t)
+      if (cx[t] < cx[idx]) idx = t;
+    const bool disabled = p.grad_calc_opt_mode == 0;
+    DF err;
+    if (idx == 0 || idx == 1 || disabled) {
+      DF t1 = mul16(L, Sj);
+      err = gemm16(t1, 'N', R, 'N', lambda, -1.0, &data);
+    } else {
+      DF t1 = mul16(Sj, R);
+      err = gemm16(L, 'N', t1, 'N', lambda, -1.0, &data);
+    }
+    DF g;
+    if (idx == 0 || idx == 2 || disabled) {
+      DF t3 = gemm16(L, 'T', err, 'N', lambda, 0.0);
+      g = gemm16(t3, 'N', R, 'T', 1.0 / c, 0.0);
+    } else {
+      DF t3 = gemm16(err, 'N', R, 'T', lambda, 0.0);
+      g = gemm16(L, 'T', t3, 'N', 1.0 / c, 0.0);
+    }
+    return g.M;
+  }
+
+  // compute_grad_over_c_ext_opt (faust_Palm4MSA.hpp:205-227): the three-factor
+  // products of multiply_order_opt_all_ends
+  Mat grad_ext(const DF& L, const DF& Sj, const DF& R, double c) {
+    Mat err = data;
+    mul_all_ends({&L.M, &Sj.M, &R.M}, err, lambda, -1.0, {'N'});
+    Mat g;
+    mul_all_ends({&L.M, &err, &R.M}, g, lambda / c, 0.0, {'T', 'N', 'T'});
+    return g;
+  }
+
+  // next_step (faust_Palm4MSA.hpp:690-740)
+  void next_step(int it) {
+    const int J = (int)S.size();
+    const bool R2L = p.is_update_way_R2L;
+    std::vector<DF> RorL(J);
+    DF LorR;
+    if (!R2L) {
+      LorR = df_eye(cons[0].nrows, cons[0].nrows);
+      RorL[J - 1] = df_eye(cons[J - 1].ncols, cons[J - 1].ncols);
+      for (int i = J - 2; i >= 0; --i) RorL[i] = mul16(S[i + 1], RorL[i + 1]);
+    } else {
+      LorR = df_eye(cons[J - 1].ncols, cons[J - 1].ncols);
+      RorL[0] = df_eye(cons[0].nrows, cons[0].nrows);
+      for (int i = 0; i < J - 1; ++i) RorL[i + 1] = mul16(RorL[i], S[i]);
+    }
+    for (int j = 0; j < J; ++j) {
+      const int f = R2L ? J - 1 - j : j;
+      double c = 1.0 / p.step_size;
+      if (!p.constant_step_size) {  // compute_c (faust_Palm4MSA.hpp:612-625)
+        const double nL = specnorm16(LorR, p.norm2_max_iter, p.norm2_threshold);
+        const double nR = specnorm16(RorL[f], p.norm2_max_iter, p.norm2_threshold);
+        c = 1.001 * nR * nR * nL * nL * lambda * lambda;
+      }
+      const DF& L = R2L ? RorL[f] : LorR;
+      const DF& R = R2L ? LorR : RorL[f];
+      Mat g = p.grad_calc_opt_mode == 2 ? grad_ext(L, S[f], R, c) : grad_int(L, S[f], R, c);
+      // compute_projection: S -= grad/c; project (flags cleared)
+      for (size_t t = 0; t < g.d.size(); ++t) S[f].M.d[t] -= g.d[t];
+      S[f].zeros = S[f].ident = false;
+      apply_prox(cons[f], S[f].M.d);
+      if (!R2L)
+        LorR = mul16(LorR, S[f]);
+      else
+        LorR = mul16(S[f], LorR);
+    }
+    // compute_lambda (faust_Palm4MSA.hpp:333-358)
+    const double tr_hh = trace_of_product_T(LorR.M, LorR.M);
+    if (tr_hh == 0.0) throw std::domain_error("compute_lambda : Xhatt_Xhat_tr equal 0 so lambda is infinite");
+    lambda = std::fabs(trace_of_product_T(data, LorR.M) / tr_hh);
+    if (std::isnan(lambda)) throw std::domain_error("compute_lambda : lambda is NaN");
+    if (traj) traj->push_back({(double)call_id, (double)it, std::nan(""), lambda});
+  }
+
+  // compute_facts (faust_Palm4MSA.hpp:170-183) with StoppingCriterion::do_continue
+  // (an iteration count; the error criterion is not restated for this backend)
+  void compute_facts() {
+    if (sc.is_criterion_error) throw std::runtime_error("2016 backend: only the iteration-count stopping criterion");
+    int it = -1;
+    while (true) {
+      ++it;
+      if (!sc_continue(sc, it, -1.0)) break;
+      next_step(it);
+    }
+  }
+};
 }  // namespace
 
 struct or_result {
@@ -1175,6 +1371,79 @@ int or_hierarchical(const double* A, int m, int n, const or_cons* fac_cons, cons
       palm4msa2(P2, S, glo_lambda);
     }
     r->lambda = glo_lambda;
+    *out = r.release();
+  })
+}
+
+int or_palm4msa2016(const double* A, int m, int n, const or_cons* cons, int ncons, const or_stop* sc,
+                    const or_params* p, double lambda_in, const or_fac* init_facs, int n_init, int record_traj,
+                    or_result** out) {
+  OR_TRY({
+    auto r = std::make_unique<or_result>();
+    Palm16 P;
+    P.data = dense_from_ptr(A, m, n);
+    P.cons.assign(cons, cons + ncons);
+    P.sc = *sc;
+    P.p = *p;
+    P.lambda = lambda_in;
+    P.traj = record_traj ? &r->traj : nullptr;
+    if (init_facs && n_init > 0) {
+      if (n_init != ncons) throw std::runtime_error("palm4msa2016: init_facts and constraints differ in number");
+      for (int i = 0; i < n_init; ++i) {
+        DF d;
+        d.M = dense_of(*mat_of_fac(init_facs[i]));
+        P.S.push_back(d);
+      }
+    } else {
+      P.init_fact(ncons);
+    }
+    P.compute_facts();
+    for (auto& d : P.S) r->S.f.push_back(std::make_shared<Mat>(d.M));
+    r->lambda = P.lambda;
+    *out = r.release();
+  })
+}
+
+int or_hierarchical2016(const double* A, int m, int n, const or_cons* fac_cons, const or_cons* res_cons, int nlevels,
+                        const or_stop* sc_local, const or_stop* sc_global, const or_params* p, double init_lambda,
+                        int record_traj, or_result** out) {
+  OR_TRY({
+    // HierarchicalFact::init / next_step / compute_facts (faust_HierarchicalFact.hpp:95-175)
+    auto r = std::make_unique<or_result>();
+    const bool left = p->is_fact_side_left;
+    Palm16 P2, PG;
+    P2.data = PG.data = dense_from_ptr(A, m, n);
+    P2.sc = *sc_local;
+    PG.sc = *sc_global;
+    P2.p = PG.p = *p;
+    P2.traj = PG.traj = record_traj ? &r->traj : nullptr;
+    PG.lambda = init_lambda;
+    PG.cons = {left ? fac_cons[0] : res_cons[0]};
+    PG.init_fact(1);
+    for (int i = 0; i < nlevels; ++i) {
+      P2.cons = {fac_cons[i], res_cons[i]};
+      P2.init_fact(2);
+      P2.lambda = init_lambda;
+      P2.call_id = 2 * i;
+      P2.compute_facts();
+      PG.lambda *= P2.lambda;  // update_lambda_from_palm
+      if (left) {
+        PG.cons[0] = fac_cons[i];
+        PG.cons.insert(PG.cons.begin() + 1, res_cons[i]);
+        PG.S.insert(PG.S.begin(), P2.S[0]);  // init_fact_from_palm
+        PG.S[1] = P2.S[1];
+      } else {
+        PG.cons.insert(PG.cons.begin() + i, fac_cons[i]);
+        PG.cons[i + 1] = res_cons[i];
+        PG.S.back() = P2.S[0];
+        PG.S.push_back(P2.S[1]);
+      }
+      PG.call_id = 2 * i + 1;
+      PG.compute_facts();
+      P2.data = (left ? PG.S[0] : PG.S[i + 1]).M;  // get_res
+    }
+    for (auto& d : PG.S) r->S.f.push_back(std::make_shared<Mat>(d.M));
+    r->lambda = PG.lambda;
     *out = r.release();
   })
 }

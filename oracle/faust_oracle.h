@@ -75,6 +75,7 @@ typedef struct {
   double step_size;
   int is_fact_side_left; /* hierarchical only */
   int use_grad1;         /* compute_n_apply_grad1 (USE_GRAD1, and the GPU2 path) */
+  int grad_calc_opt_mode; /* 2016 backend: 0 disabled, 1 internal, 2 external (pyfaust's default) */
 } or_params;
 
 const char *or_last_error(void);
@@ -122,6 +123,16 @@ int or_hierarchical(const double *A, int m, int n, const or_cons *fac_cons,
                     const or_cons *res_cons, int nlevels, const or_stop *sc_local,
                     const or_stop *sc_global, const or_params *p,
                     int record_traj, or_result **out);
+/* The 2016 backend (faust_Palm4MSA.hpp, faust_HierarchicalFact.hpp): dense
+ * factors; the trajectory holds lambda after each iteration (relerr NaN). */
+int or_palm4msa2016(const double *A, int m, int n, const or_cons *cons, int ncons,
+                    const or_stop *sc, const or_params *p, double lambda_in,
+                    const or_fac *init_facs, int n_init, int record_traj,
+                    or_result **out);
+int or_hierarchical2016(const double *A, int m, int n, const or_cons *fac_cons,
+                        const or_cons *res_cons, int nlevels, const or_stop *sc_local,
+                        const or_stop *sc_global, const or_params *p,
+                        double init_lambda, int record_traj, or_result **out);
 int or_result_nfacts(const or_result *r);
 double or_result_lambda(const or_result *r);
 int or_result_fact_info(const or_result *r, int i, int *kind, int *nrows,

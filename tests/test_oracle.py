@@ -282,3 +282,67 @@ def test_dynprog_order_is_the_cheapest_parenthesization():
             return chosen(i, k) + chosen(k + 1, j) + join(i, k, j)
 
         assert chosen(0, n - 1) == best
+
+
+def _load_h32_config():
+    import json
+    import os
+    with open(os.path.join(os.path.dirname(__file__), "golden", "hadamard32_config.json")) as f:
+        return json.load(f)
+
+
+def test_hierarchical2016_reference_known_answer():
+    # the reference's own acceptance test of its 2016 hierarchical
+    # factorization on the 32x32 Hadamard matrix (config_HADAMARD_32.mat;
+    # lambda = 5930 +- 30, misc/test/CMakeLists.txt:339), on the oracle's
+    # restatement of Faust::HierarchicalFact / Faust::Palm4MSA
+    cfg = _load_h32_config()
+    H = oracle.hadamard(cfg["nrow"])
+    fac = [tuple(c) for c in cfg["fact_constraints"]]
+    res = [tuple(c) for c in cfg["res_constraints"]]
+    r = oracle.hierarchical2016(H, fac, res, num_its_local=cfg["niter1"], num_its_global=cfg["niter2"],
+                                is_update_way_R2L=bool(cfg["update_way"]))
+    assert len(r.factors) == cfg["nfacts"]
+    assert abs(r.lam - cfg["expected_lambda"]) <= cfg["lambda_tolerance"], r.lam
+    P = np.eye(cfg["nrow"])
+    for f in r.factors:
+        P = P @ f
+    assert np.linalg.norm(r.lam * P - H) / np.linalg.norm(H) < 1e-10
+    assert all(int((f != 0).sum()) == 64 for f in r.factors)  # splincol 2 on 32x32
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+@pytest.mark.parametrize("R2L", [False, True])
+def test_palm4msa2016_lambda_closed_form(mode, R2L):
+    # compute_lambda's closed form on the returned factors:
+    # lambda = |tr(A^T P)| / tr(P^T P), P the product of the final factors
+    rng = np.random.default_rng(7)
+    A = rng.standard_normal((24, 20))
+    cons = [("splin", 4, 24, 16), ("spcol", 3, 16, 16), ("sp", 60, 16, 20)]
+    r = oracle.palm4msa2016(A, cons, num_its=12, record_traj=True, is_update_way_R2L=R2L, grad_calc_opt_mode=mode)
+    P = r.factors[0] @ r.factors[1] @ r.factors[2]
+    lam = abs(np.trace(A.T @ P)) / np.trace(P.T @ P)
+    assert abs(r.lam - lam) <= 1e-12 * lam
+    assert r.traj.shape == (12, 4) and r.traj[-1, 3] == r.lam
+    # the projectors' supports: splin 4 per row, spcol 3 per column, sp 60
+    assert np.all((r.factors[0] != 0).sum(axis=1) <= 4)
+    assert np.all((r.factors[1] != 0).sum(axis=0) <= 3)
+    assert (r.factors[2] != 0).sum() <= 60
+    # the iterations decrease the error against A (a descent method)
+    err = np.linalg.norm(r.lam * P - A) / np.linalg.norm(A)
+    r1 = oracle.palm4msa2016(A, cons, num_its=1, is_update_way_R2L=R2L, grad_calc_opt_mode=mode)
+    P1 = r1.factors[0] @ r1.factors[1] @ r1.factors[2]
+    assert err < np.linalg.norm(r1.lam * P1 - A) / np.linalg.norm(A)
+
+
+def test_palm4msa2016_gradient_modes_agree():
+    # the three gradient forms associate the same product differently: the
+    # same iterates up to rounding
+    rng = np.random.default_rng(8)
+    A = rng.standard_normal((16, 16))
+    cons = [("splincol", 3, 16, 16), ("splincol", 3, 16, 16)]
+    rs = [oracle.palm4msa2016(A, cons, num_its=3, grad_calc_opt_mode=m) for m in (0, 1, 2)]
+    for r in rs[1:]:
+        assert abs(r.lam - rs[0].lam) <= 1e-9 * rs[0].lam
+        for a, b in zip(r.factors, rs[0].factors):
+            assert np.allclose(a, b, rtol=0, atol=1e-9 * np.abs(b).max())
