@@ -17,6 +17,9 @@
 //                          src/faust_linear_operator/CPU/faust_prod_opt.cpp:6-86
 //   palm4msa2              src/algorithm/factorization/faust_palm4msa2020.hpp:1-676
 //   hierarchical           src/algorithm/factorization/faust_hierarchical.hpp:1-150
+//   2016 backend           src/algorithm/factorization/faust_Palm4MSA.hpp:170-740,
+//                          src/algorithm/factorization/faust_HierarchicalFact.hpp:95-175,
+//                          MatDense flags src/faust_linear_operator/CPU/faust_MatDense.hpp:243-265,660-700
 //   stopping criterion     src/algorithm/factorization/faust_StoppingCriterion.hpp:113-150
 //
 // Arithmetic follows the canonical order documented in faust_oracle.h.
@@ -35,8 +38,15 @@
 //   palm4msa2 / hierarchical    least-squares lambda, objective decrease, exact
 //                               recovery of Hadamard 16^2..64^2; grad1 == grad2 to
 //                               rounding; golden fixtures (tests/golden/)
+//   2016 backend                the reference's own known answer: its Hadamard-32
+//                               hierarchical test (config_HADAMARD_32.mat, lambda
+//                               5930 +- 30) gives 5942.2 with an exact factorization;
+//                               lambda's closed form on the returned factors; the three
+//                               gradient forms agree to rounding; error decreases
 // Parity unpinned (restated from the reference, no independent check):
 //   Eigen's own summation order (replaced by the canonical order);
+//   which of the 2016 backend's MatDense flags are set when (the spectral norm of a
+//   flagged identity is exactly 1, of a flagged zero 0; followed from the source);
 //   the greedy end-pair order of multiply_order_opt_all_ends and the dense/sparse
 //   format rule of prox_*_gen (their values are pinned within rounding through the
 //   products above; the exact order and the returned formats are not).
