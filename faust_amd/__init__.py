@@ -11,9 +11,11 @@ import scipy.sparse as sp
 
 from . import fact, factparams, proj
 from ._lib import FaustHipError
+from .butterfly import bitrev_perm
 from .faust import Factor, Faust
 
-__all__ = ["Faust", "Factor", "rand", "rand_bsr", "wht", "fact", "factparams", "proj", "FaustHipError"]
+__all__ = ["Faust", "Factor", "rand", "rand_bsr", "rand_butterfly", "bitrev_perm", "wht", "fact", "factparams",
+           "proj", "FaustHipError"]
 
 
 def rand_csr(nrows, ncols, nnz_per_row, rng, values="normal", dtype=np.float64):
@@ -134,3 +136,16 @@ def wht(n, normed=True):
     if not normed:
         facs[0] = facs[0] * np.sqrt(n)
     return Faust(facs)
+
+
+def rand_butterfly(n, dtype="float64", dev="gpu", diag_opt=False):
+    """pyfaust.rand_butterfly (pyfaust.py:4810-4848): the product of log2(n)
+    factors on the butterfly supports with standard normal values drawn with
+    np.random.randn.  Real dtypes only; diag_opt is not provided."""
+    from .butterfly import rand_butterfly_factors
+    if diag_opt:
+        raise NotImplementedError("diag_opt (opt_butterfly_faust) is not provided")
+    if dtype not in ("float64", "float32", np.float64, np.float32):
+        raise ValueError("dtype must be 'float64' or 'float32'")
+    dt = np.dtype(dtype).type
+    return Faust(rand_butterfly_factors(n, dt), dtype=dt)

@@ -15,6 +15,7 @@ FH_MAT_DENSE, FH_MAT_CSR, FH_MAT_BSR = 0, 1, 2
 # fh_faust_set_mul_mode values (faust_hip.h)
 FH_MUL_AUTO, FH_MUL_PER_FACTOR, FH_MUL_FUSED = 0, 1, 2
 FH_MUL_CSR_ROWS1, FH_MUL_CSR_ROWS2 = 32, 33
+FH_BUTTERFLY_LEFT, FH_BUTTERFLY_RIGHT, FH_BUTTERFLY_BBTREE = 0, 1, 2
 
 
 def FH_MUL_BSR_KERNEL(k):
@@ -72,6 +73,9 @@ _SIGS = {
     "fh_faust_push_back": (C.c_int, [VP, VP]),
     "fh_faust_size": (C.c_int, [VP, C.POINTER(C.c_int)]),
     "fh_faust_free": (C.c_int, [VP]),
+    "fh_faust_factor": (C.c_int, [VP, C.c_int, C.POINTER(VP)]),
+    "fh_butterfly": (C.c_int, [C.POINTER(C.c_double), C.c_int, C.c_int, C.POINTER(C.c_int32), C.c_int,
+                               C.POINTER(VP)]),
     "fh_faust_multiply": (C.c_int, [VP, C.c_char, VP, C.c_int, VP, VP]),
     "fh_faust_last_launches": (C.c_int, [VP, C.POINTER(C.c_int)]),
     "fh_faust_set_mul_mode": (C.c_int, [VP, C.c_int]),

@@ -355,6 +355,16 @@ int fh_faust_size(fh_faust_t F, int* n) {
   FH_API_END
 }
 
+int fh_faust_factor(fh_faust_t F, int i, fh_mat_t* out) {
+  FH_API_BEGIN
+  FH_CHECK(F && out, "null argument");
+  FH_CHECK(i >= 0 && i < (int)F->f.size(), "factor index out of range");
+  auto* h = new fh_mat_s();
+  h->m = F->f[i];
+  *out = h;
+  FH_API_END
+}
+
 int fh_faust_free(fh_faust_t F) {
   FH_API_BEGIN
   delete F;
@@ -428,6 +438,9 @@ int fh_faust_timers(fh_faust_t F, double* ms, int64_t* counts) {
 int fh_faust_get_product(fh_faust_t F, void* out_dev, void* stream) {
   FH_API_BEGIN
   FH_CHECK(F && out_dev, "null argument");
+  for (auto& x : F->f)
+    if (x->dtype != FH_F64 || x->kind == K_BSR)
+      throw Error(FH_ERR_UNSUPPORTED, "get_product: FP64 dense / CSR chains only (multiply other chains by the identity)");
   hipStream_t st = (hipStream_t)stream;
   DMatP P = faust_get_product_f64(F->f, st);
   FH_HIP(hipMemcpyAsync(out_dev, P->d.p, sizeof(double) * (size_t)P->m * P->n, hipMemcpyDeviceToDevice, st));

@@ -1,6 +1,6 @@
-"""palm4msa / hierarchical on the GPU (pyfaust.fact counterpart,
-wrapper/python/pyfaust/fact.py:798-879 and :1020-1335, backend 2020,
-on_gpu=True; C++ entry FaustFact.hpp:580-675,740-860)."""
+"""palm4msa / hierarchical / butterfly on the GPU (pyfaust.fact counterpart,
+wrapper/python/pyfaust/fact.py:798-879, :1020-1335 and :1642-1907; C++
+entries FaustFact.hpp:580-675,740-860,905-926)."""
 import ctypes as C
 
 import numpy as np
@@ -177,3 +177,95 @@ def hierarchical(M, p, ret_lambda=False, backend=2020, on_gpu=True, ret_raw=Fals
         return res
     F = _faust_2016_hier(res) if backend == 2016 else _faust_times_lambda(res)
     return (F, res.lam) if ret_lambda else F
+
+
+_BFLY_DIR = {"left": _lib.FH_BUTTERFLY_LEFT, "right": _lib.FH_BUTTERFLY_RIGHT, "bbtree": _lib.FH_BUTTERFLY_BBTREE}
+
+
+def _butterfly_one(M, type, perm, mul_perm):
+    """One fh_butterfly call (the _FaustAlgoGen.pyx.in:592-630 binding)."""
+    if type not in _BFLY_DIR:
+        raise ValueError("dir argument must be 'right' or 'left'.")
+    A = np.asfortranarray(M, dtype=np.float64)
+    pp = None
+    if perm is not None:
+        if not isinstance(perm, (list, np.ndarray)) or len(perm) != M.shape[1]:
+            raise ValueError("perm must be a valid permutation list")
+        perm = np.asarray(perm)
+        if perm.ndim != 1:
+            raise ValueError("perm must be a vector")
+        if (perm >= M.shape[1]).any() or (perm < 0).any():
+            raise ValueError("perm contains an invalid index")
+        perm = np.ascontiguousarray(perm, dtype=np.int32)
+        pp = perm.ctypes.data_as(C.POINTER(C.c_int32))
+    L = lib()
+    h = C.c_void_p()
+    check(L.fh_butterfly(A.ctypes.data_as(C.POINTER(C.c_double)), A.shape[0], _BFLY_DIR[type], pp,
+                         1 if mul_perm else 0, C.byref(h)))
+    try:
+        n = C.c_int()
+        check(L.fh_faust_size(h, C.byref(n)))
+        facs = []
+        for i in range(n.value):
+            fh = C.c_void_p()
+            check(L.fh_faust_factor(h, i, C.byref(fh)))
+            facs.append(Factor(None, handle=fh))
+    finally:
+        L.fh_faust_free(h)
+    if M.dtype == np.float32:
+        # computed in FP64, handed back in the input's precision
+        import scipy.sparse as sp
+        host = [f.to_host() for f in facs]
+        return Faust([(x.astype(np.float32) if sp.issparse(x) else np.asarray(x, dtype=np.float32)) for x in host],
+                     dtype=np.float32)
+    return Faust(_facs=facs)
+
+
+def butterfly(M, type="bbtree", perm=None, diag_opt=False, mul_perm=None):
+    """Butterfly factorization of M on the GPU (pyfaust.fact.butterfly,
+    fact.py:1642-1907; Faust::butterfly_hierarchical,
+    faust_butterfly.hpp:545-860).
+
+    M: square float64 / float32 ndarray of order n = 2^L >= 4.  type: 'bbtree'
+    (balanced tree of supports, CSR factors), 'right' or 'left' (one factor
+    peeled off per step, dense factors).  perm: None, an index vector (P with
+    P[perm[j], j] = 1; M P^T is factorized and P put back on the right),
+    'bitrev', 'default_8', or a list of index vectors (the one with the
+    smallest relative error wins).  mul_perm (default: not diag_opt)
+    multiplies P into the last factor, else P is a factor of its own.
+    diag_opt (pyfaust.opt_butterfly_faust) is not provided.
+    Each block is a best rank-1 approximation computed in FP64 (float32
+    inputs get FP32 factors)."""
+    from .butterfly import bitrev_perm, default_8_perms, perm2indices
+    if mul_perm is None:
+        mul_perm = not diag_opt
+    if mul_perm and diag_opt:
+        raise ValueError("mul_perm and diag_opt option can not be both True.")
+    if diag_opt:
+        raise NotImplementedError("diag_opt (opt_butterfly_faust) is not provided")
+    if not isinstance(M, np.ndarray):
+        raise TypeError("butterfly() 1st argument must be a numpy ndarray.")
+    if M.dtype not in (np.float32, np.float64):
+        raise TypeError("butterfly(): only real float32 / float64 matrices are supported")
+    if M.ndim != 2:
+        raise ValueError("butterfly(): input matrix/array invalid number of dimensions")
+    m, n = M.shape
+    l2_m, l2_n = np.log2([m, n])
+    if l2_m != np.floor(l2_m) or l2_n != np.floor(l2_n) or m != n:
+        raise ValueError("M must be square and its dimensions must be a power of two.")
+    if isinstance(perm, str):
+        if perm == "bitrev":
+            return butterfly(M, type, perm=perm2indices(bitrev_perm(n)), diag_opt=diag_opt, mul_perm=mul_perm)
+        if perm == "default_8":
+            return butterfly(M, type, default_8_perms(n), diag_opt=diag_opt, mul_perm=mul_perm)
+        raise ValueError("perm must be None, an index list, a list of them, 'bitrev' or 'default_8'")
+    if isinstance(perm, (list, tuple)) and len(perm) and isinstance(perm[0], (list, tuple, np.ndarray)):
+        best_err, best_F = np.inf, None
+        nM = np.linalg.norm(M)
+        for p in perm:
+            F = butterfly(M, type, np.asarray(p), diag_opt=diag_opt, mul_perm=mul_perm)
+            err = np.linalg.norm(F.toarray() - M) / nM
+            if err < best_err:
+                best_err, best_F = err, F
+        return best_F
+    return _butterfly_one(M, type, perm, mul_perm)

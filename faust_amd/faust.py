@@ -357,8 +357,11 @@ class Faust:
         return Faust(filepath=filepath)
 
     def toarray(self):
-        """Dense product (Transform::get_product)."""
+        """Dense product (Transform::get_product).  FP32 / BF16 chains: the
+        product applied to the identity, in the chain's own precision."""
         m, n = self._facs[0].shape[0], self._facs[-1].shape[1]
+        if self.act_dtype != torch.float64:
+            return np.asfortranarray(np.asarray(self @ np.eye(self.shape[1]), dtype=np.float64))
         out = torch.empty((n, m), dtype=torch.float64, device="cuda")
         check(lib().fh_faust_get_product(self.h, _ptr(out), _stream()))
         a = out.t().cpu().numpy()

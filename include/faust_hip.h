@@ -95,6 +95,10 @@ int fh_gemm(int dtype, char opA, char opB, int m, int n, int k, const void *alph
 int fh_faust_create(fh_faust_t *out);
 int fh_faust_push_back(fh_faust_t F, fh_mat_t M);
 int fh_faust_size(fh_faust_t F, int *nfactors);
+/* factor i of F as a new matrix handle owned by the caller; the factor is
+ * shared, not copied (TransformHelper::get_gen_fact,
+ * faust_TransformHelperGen.h:63, the getter behind pyfaust's Faust.factors). */
+int fh_faust_factor(fh_faust_t F, int i, fh_mat_t *out);
 int fh_faust_free(fh_faust_t F);
 /* Y = op(F)*X, X (ncols(op(F)) x p) and Y device col-major with ld = rows.
  * TransformHelper<FPP,Cpu>::multiply(const FPP*, int, FPP*)
@@ -130,8 +134,9 @@ int fh_faust_get_mul_mode(fh_faust_t F, int *mode);
  * conversions, then resets them; arrays hold nfactors+1 entries.  Synchronizes. */
 int fh_faust_enable_timers(fh_faust_t F, int on);
 int fh_faust_timers(fh_faust_t F, double *ms, int64_t *counts);
-/* Dense product of the chain into a device buffer (Transform::get_product,
- * faust_Transform.hpp:599-652). */
+/* Dense FP64 product of an FP64 dense / CSR chain into a device buffer
+ * (Transform::get_product, faust_Transform.hpp:599-652); other chains give
+ * FH_ERR_UNSUPPORTED (their product is F times the identity). */
 int fh_faust_get_product(fh_faust_t F, void *out_dev, void *stream);
 /* 2-norm by power iteration (TransformHelper::spectralNorm,
  * faust_TransformHelper.hpp:1201-1218; power_iteration,
@@ -219,6 +224,24 @@ int fh_palm_result_lambda(fh_palm_result_t r, double *lambda);
 int fh_palm_result_factor(fh_palm_result_t r, int i, fh_mat_t *out);
 int fh_palm_result_traj(fh_palm_result_t r, double *out4, int *len);
 int fh_palm_result_free(fh_palm_result_t r);
+
+/* ---- butterfly factorization (Faust::butterfly_hierarchical,
+ * faust_butterfly.hpp:804-860, as bound by FaustFact.hpp:905-926 and
+ * _FaustAlgoGen.pyx.in:592-630).  Directions: ButterflyFactDir
+ * (faust_butterfly.h:5-10); 'left' -> 0, 'right' -> 1, 'bbtree' -> 2. */
+#define FH_BUTTERFLY_LEFT 0
+#define FH_BUTTERFLY_RIGHT 1
+#define FH_BUTTERFLY_BBTREE 2
+/* A: n x n FP64, column-major, host memory, n = 2^L >= 4.  perm: NULL or n
+ * indices of the permutation P with P[perm[i], i] = 1; then A P^T is
+ * factorized and P is multiplied into the last factor (mul_perm != 0) or
+ * appended as a factor of its own.  *out: a new Faust (owned by the caller)
+ * of L factors on the butterfly supports (L + 1 with an appended
+ * permutation): CSR factors for BBTREE, dense ones for LEFT / RIGHT (a
+ * permuted last factor is CSR).  Every block is a best rank-1
+ * approximation in FP64; BBTREE handles orders up to 16384 (the largest
+ * block must fit one workgroup's LDS), LEFT / RIGHT any order. */
+int fh_butterfly(const double *A, int n, int dir, const int32_t *perm, int mul_perm, fh_faust_t *out);
 
 /* ---- batched PALM4MSA: many independent factorizations in lock-step.
  * Every target b is n[b] x n[b] and is factorized into nfacts square factors

@@ -1,0 +1,180 @@
+"""The butterfly factorization on the GPU (fh_butterfly) vs the oracle's
+restatement of faust_butterfly.hpp.
+
+Singular vectors are defined up to sign, so factors are compared through
+sign-free quantities: the entries' absolute values and the product.  A best
+rank-1 approximation moves by about eps / (relative gap of the block's two
+top singular values); on seeded Gaussian inputs that leaves differences of
+1e-13 on products, and the bars below are
+  * product: relative Frobenius difference <= 1e-11;
+  * |factor|: max absolute difference <= 1e-9 * max |factor|.
+Exact butterflies (products of factors on the butterfly supports) are
+recovered to <= 1e-12 relative error at every order, and the reference's own
+small / large expectations (misc/test/src/Python/test_butterfly.py:43-95)
+hold on the GPU too.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import oracle
+from oracle import butterfly as ob
+import faust_amd
+from faust_amd import _lib, fact
+from faust_amd import butterfly as hb
+
+pytestmark = pytest.mark.gpu
+
+_DIR = {"left": ob.LEFT, "right": ob.RIGHT, "bbtree": ob.BALANCED}
+PROD_TOL = 1e-11
+FAC_TOL = 1e-9
+
+
+def _host(F):
+    return [F.factors(i) for i in range(F.numfactors())]
+
+
+def _dense(f):
+    return f.toarray() if sp.issparse(f) else np.asarray(f)
+
+
+def _compare(F, ref):
+    got = [_dense(f) for f in _host(F)]
+    exp = [_dense(f) for f in ref]
+    assert len(got) == len(exp)
+    pg, pe = ob.product(got), ob.product(exp)
+    assert np.linalg.norm(pg - pe) <= PROD_TOL * np.linalg.norm(pe)
+    for g, e in zip(got, exp):
+        assert np.abs(np.abs(g) - np.abs(e)).max() <= FAC_TOL * np.abs(e).max()
+
+
+@pytest.mark.parametrize("type", ["bbtree", "right", "left"])
+@pytest.mark.parametrize("n", [4, 8, 16, 64, 256, 1024])
+def test_random_vs_oracle(gpu, type, n):
+    print(f"butterfly {type} n={n}", flush=True)
+    rng = np.random.default_rng(100 + n)
+    M = rng.standard_normal((n, n))
+    F = fact.butterfly(M, type=type)
+    _compare(F, ob.butterfly(M, _DIR[type]))
+    L = int(np.log2(n))
+    kinds = [f.info()[0] for f in F._facs]
+    want = _lib.FH_MAT_CSR if type == "bbtree" else _lib.FH_MAT_DENSE
+    assert kinds == [want] * L
+    # every factor on its butterfly support
+    for i, f in enumerate(_host(F)):
+        d = _dense(f)
+        assert not np.any(d[hb.support_csr(n, i).toarray() == 0])
+
+
+@pytest.mark.parametrize("type", ["bbtree", "right", "left"])
+@pytest.mark.parametrize("n", [32, 512, 4096])
+def test_exact_butterfly_recovered(gpu, type, n):
+    print(f"butterfly exact {type} n={n}", flush=True)
+    rng = np.random.default_rng(n)
+    B = faust_amd.Faust(hb.rand_butterfly_factors(n, rng=rng))
+    M = B.toarray()
+    F = fact.butterfly(M, type=type)
+    assert np.linalg.norm(F.toarray() - M) <= 1e-12 * np.linalg.norm(M)
+
+
+def test_hadamard_recovered(gpu):
+    for n in (8, 1024):
+        H = oracle.hadamard(n)
+        for type in ("bbtree", "right", "left"):
+            F = fact.butterfly(H, type=type)
+            assert np.linalg.norm(F.toarray() - H) <= 1e-12 * np.linalg.norm(H), (n, type)
+
+
+@pytest.mark.parametrize("pname", ["bitrev", "rand"])
+@pytest.mark.parametrize("hname", ["rand_butterfly", "hadamard"])
+def test_reference_expectations(gpu, pname, hname):
+    """test_butterfly.py:43-95 through the GPU path (bit reversal is its own
+    transpose: cases 5 and 7 are cases 4 and 8 again)."""
+    from tests.test_butterfly import _cases, _expected, _rand_bfly_dense, _rand_perm
+    rng = np.random.default_rng(7)
+    N = 32
+    P = hb.bitrev_perm(N) if pname == "bitrev" else _rand_perm(N, rng)
+    H = _rand_bfly_dense(N, rng) if hname == "rand_butterfly" else oracle.hadamard(N)
+    for i, (mat, p) in enumerate(_cases(H, P)):
+        F = fact.butterfly(np.asarray(mat), type="bbtree", perm=p)
+        e = np.linalg.norm(F.toarray() - mat) / np.linalg.norm(mat)
+        assert ("small" if e < 1e-12 else "large") == _expected(i, P), (i, e)
+
+
+@pytest.mark.parametrize("type", ["bbtree", "right", "left"])
+@pytest.mark.parametrize("mul_perm", [True, False])
+def test_perm_vs_oracle(gpu, type, mul_perm):
+    rng = np.random.default_rng(21)
+    n = 64
+    M = rng.standard_normal((n, n))
+    p = rng.permutation(n)
+    F = fact.butterfly(M, type=type, perm=p, mul_perm=mul_perm)
+    ref = ob.butterfly(M, _DIR[type], p, mul_perm=mul_perm)
+    assert F.numfactors() == (6 if mul_perm else 7)
+    _compare(F, ref)
+    if mul_perm:
+        assert F._facs[-1].info()[0] == _lib.FH_MAT_CSR
+
+
+def test_named_perms(gpu):
+    """'bitrev' = its index vector; 'default_8' keeps the best of the eight."""
+    rng = np.random.default_rng(5)
+    n = 16
+    M = rng.standard_normal((n, n))
+    F1 = fact.butterfly(M, perm="bitrev")
+    F2 = fact.butterfly(M, perm=hb.perm2indices(hb.bitrev_perm(n)))
+    assert np.array_equal(F1.toarray(), F2.toarray())
+    F8 = fact.butterfly(M, perm="default_8")
+    errs = [ob.rel_err(ob.butterfly(M, ob.BALANCED, p), M) for p in hb.default_8_perms(n)]
+    e8 = np.linalg.norm(F8.toarray() - M) / np.linalg.norm(M)
+    assert abs(e8 - min(errs)) <= 1e-10
+
+
+def test_identity_perm_same_as_none(gpu):
+    rng = np.random.default_rng(3)
+    M = rng.random((4, 4))
+    F1 = fact.butterfly(M, type="bbtree")
+    F2 = fact.butterfly(M, type="bbtree", perm=np.arange(4))
+    assert np.array_equal(F1.toarray(), F2.toarray())
+
+
+def test_float32_input(gpu):
+    rng = np.random.default_rng(9)
+    M = rng.standard_normal((64, 64)).astype(np.float32)
+    F = fact.butterfly(M)
+    assert F.dtype == np.float32
+    ref = ob.product(ob.butterfly(M.astype(np.float64), ob.BALANCED))
+    # FP32 factors and an FP32 product: a few float32 roundings per factor
+    assert np.linalg.norm(F.toarray() - ref) <= 1e-5 * np.linalg.norm(ref)
+
+
+def test_errors(gpu):
+    rng = np.random.default_rng(0)
+    with pytest.raises(ValueError):
+        fact.butterfly(rng.standard_normal((8, 4)))
+    with pytest.raises(ValueError):
+        fact.butterfly(rng.standard_normal((12, 12)))
+    with pytest.raises(ValueError):
+        fact.butterfly(rng.standard_normal((8, 8)), type="up")
+    with pytest.raises(ValueError):
+        fact.butterfly(rng.standard_normal((8, 8)), perm=[0, 1, 2])
+    with pytest.raises(ValueError):
+        fact.butterfly(rng.standard_normal((8, 8)), perm=[0, 1, 2, 3, 4, 5, 6, 8])
+    with pytest.raises(ValueError):
+        fact.butterfly(rng.standard_normal((8, 8)), diag_opt=True, mul_perm=True)
+    with pytest.raises(NotImplementedError):
+        fact.butterfly(rng.standard_normal((8, 8)), diag_opt=True)
+    with pytest.raises(_lib.FaustHipError):
+        fact.butterfly(rng.standard_normal((2, 2)))
+    with pytest.raises(_lib.FaustHipError):  # repeated index: not a permutation
+        fact.butterfly(rng.standard_normal((8, 8)), perm=[0, 0, 2, 3, 4, 5, 6, 7])
+
+
+def test_rand_butterfly(gpu):
+    np.random.seed(4)
+    F = faust_amd.rand_butterfly(32)
+    assert F.numfactors() == 5
+    for i, f in enumerate(_host(F)):
+        assert np.array_equal(_dense(f) != 0, hb.support_csr(32, i).toarray() != 0)
+    G = fact.butterfly(F.toarray())
+    assert np.linalg.norm(G.toarray() - F.toarray()) <= 1e-12 * np.linalg.norm(F.toarray())

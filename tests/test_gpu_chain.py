@@ -187,3 +187,20 @@ def test_optimize_time_fp64(gpu):
     assert H is F
     with pytest.raises(faust_amd.FaustHipError):
         F._set_mul_mode(7)
+
+
+def test_get_product_fp32_chain(gpu):
+    """The FP64 get_product refuses FP32 chains; Faust.toarray forms their
+    product as F times the identity, in FP32."""
+    import ctypes as C
+    import torch
+    from faust_amd import _lib
+    rng = np.random.default_rng(12)
+    facs = [sp.csr_matrix(rng.standard_normal((48, 48)) * (rng.random((48, 48)) < 0.2)).astype(np.float32)
+            for _ in range(3)]
+    F = faust_amd.Faust(facs, dtype=np.float32)
+    out = torch.empty((48, 48), dtype=torch.float64, device="cuda")
+    rc = _lib.lib().fh_faust_get_product(F.h, C.c_void_p(out.data_ptr()), None)
+    assert rc == 4  # FH_ERR_UNSUPPORTED
+    ref = facs[0].toarray().astype(np.float64) @ facs[1].toarray() @ facs[2].toarray()
+    assert np.linalg.norm(F.toarray() - ref) <= 1e-5 * np.linalg.norm(ref)
