@@ -1,7 +1,7 @@
 """Benchmark of the Faust x dense hot path (and PALM4MSA) on one MI355X.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME] [--no-cpu-baseline] [--no-palm]
-               [--hadamard]
+               [--no-butterfly] [--hadamard]
 
 Prints ONE JSON line.  `value` is the algorithmic HBM throughput (GB/s) of
 one step = one Faust x dense product (all factors) with inputs resident in
@@ -13,7 +13,9 @@ oracle (CPU FP64 restatement of the reference) timed on this host on a
 bounded sample of the same workload.  `palm` compares the tuned batched
 PALM4MSA (FP32 + FP64 frozen-support refinement, HIP graphs, autotuned
 tiles) with the plain FP64 GPU form on 512 targets, at the FP64-verified
-relative error each reaches.
+relative error each reaches.  `butterfly` times the butterfly factorization
+(fact.butterfly, host matrix in, Faust out) at orders 1024 / 4096 / 16384 on
+Gaussian matrices, next to the oracle at order 1024.
 """
 import argparse
 import ctypes as C
@@ -40,6 +42,7 @@ def _parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-palm", action="store_true",
                     help="skip the batched PALM4MSA comparison (512 targets, tuned vs plain FP64)")
+    ap.add_argument("--no-butterfly", action="store_true", help="skip the butterfly factorization timings")
     ap.add_argument("--hadamard", action="store_true",
                     help="also time hierarchical PALM4MSA on the 1024 Hadamard matrix, 100 iterations per level")
     ap.add_argument("--warmstart", type=int, default=0, metavar="NTRAIN",
@@ -205,6 +208,56 @@ def palm_hadamard():
             "relerr": _relerr(res.lam, res.dense_factors(), H)}
 
 
+def butterfly_measure(cpu=True):
+    """fact.butterfly for 'bbtree' and 'right' on N(0,1) matrices: `gpu_ms`
+    with the matrix already in HBM (a CUDA tensor, fh_butterfly_device),
+    `gpu_ms_host_input` with the n x n FP64 host matrix uploaded by the call
+    (fh_butterfly); both until the Faust is assembled, each the second of two
+    calls.  Every run is checked on an exact butterfly of the same order
+    (recovered to <= 1e-12)."""
+    import torch
+    from faust_amd import Faust, fact
+    from faust_amd.butterfly import rand_butterfly_factors
+    out = {"workload": "butterfly_factorization_gaussian_fp64", "gpu_ms": {}, "gpu_ms_host_input": {},
+           "exact_relerr_max": 0.0}
+
+    def timed(X, t):
+        for _ in range(2):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            F = fact.butterfly(X, type=t)
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+            del F
+        return round(dt * 1e3, 2)
+
+    for n in (1024, 4096, 16384):
+        rng = np.random.default_rng(n)
+        M = np.asfortranarray(rng.standard_normal((n, n)))
+        Md = torch.from_numpy(np.ascontiguousarray(M)).cuda()
+        B = Faust(rand_butterfly_factors(n, rng=rng)).toarray()
+        for t in ("bbtree", "right"):
+            out["gpu_ms"][f"{t}_{n}"] = timed(Md, t)
+            out["gpu_ms_host_input"][f"{t}_{n}"] = timed(M, t)
+            G = fact.butterfly(B, type=t)
+            e = float(np.linalg.norm(G.toarray() - B) / np.linalg.norm(B))
+            out["exact_relerr_max"] = max(out["exact_relerr_max"], e)
+            del G
+        del Md
+        print(f"butterfly n={n} done", file=sys.stderr, flush=True)
+    if cpu:
+        from oracle import butterfly as ob
+        M = np.random.default_rng(1024).standard_normal((1024, 1024))
+        t0 = time.perf_counter()
+        ob.butterfly(M, ob.BALANCED)
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(dt * 1e3, 1), "unit": "ms (bbtree, order 1024)", "cores": 1,
+                               "kind": "port", "sample": "oracle bbtree factorization of one 1024 x 1024 N(0,1) "
+                               "matrix (a sequential loop of numpy LAPACK SVDs, one per block)"}
+        out["speedup_bbtree_1024"] = round(dt * 1e3 / out["gpu_ms"]["bbtree_1024"], 1)
+    return out
+
+
 # ----------------------------------------------------------------- helpers
 def measured_traffic(wl):
     """HBM-side bytes per launch of the dominant kernel from the rocprofv3
@@ -368,6 +421,8 @@ def main():
         out["cpu_baseline"] = cpu_baseline(wl, args.cpu_seconds)
     if not args.no_palm:
         out["palm"] = palm_batched_vs_plain(cpu=not args.no_cpu_baseline)
+    if not args.no_butterfly:
+        out["butterfly"] = butterfly_measure(cpu=not args.no_cpu_baseline)
     if args.hadamard:
         out["palm_hadamard"] = palm_hadamard()
     if args.warmstart:

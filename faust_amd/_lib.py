@@ -76,6 +76,8 @@ _SIGS = {
     "fh_faust_factor": (C.c_int, [VP, C.c_int, C.POINTER(VP)]),
     "fh_butterfly": (C.c_int, [C.POINTER(C.c_double), C.c_int, C.c_int, C.POINTER(C.c_int32), C.c_int,
                                C.POINTER(VP)]),
+    "fh_butterfly_device": (C.c_int, [VP, C.c_int, C.c_int64, C.c_int, C.c_int, C.POINTER(C.c_int32), C.c_int,
+                                      C.POINTER(VP), VP]),
     "fh_faust_multiply": (C.c_int, [VP, C.c_char, VP, C.c_int, VP, VP]),
     "fh_faust_last_launches": (C.c_int, [VP, C.POINTER(C.c_int)]),
     "fh_faust_set_mul_mode": (C.c_int, [VP, C.c_int]),

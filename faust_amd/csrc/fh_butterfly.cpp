@@ -25,13 +25,13 @@ namespace {
 
 struct BNode {
   int a = 0, b = 0;  // supports [a, b)
-  DBuf v;            // n x 2^(b-a) values; empty for the root (the input matrix)
+  DBuf v;            // n x 2^(b-a) values, row-major (the root: the matrix itself)
 };
 using BNodeP = std::shared_ptr<BNode>;
 
 struct Bfly {
   int n = 0, L = 0;
-  const double* A = nullptr;  // device copy of the matrix to factorize (column-major)
+  BNodeP root;  // [0, L): the matrix to factorize, row-major, columns permuted
   hipStream_t st = nullptr;
 
   // node [a, b) -> X on [a, c), Y on [c, b)
@@ -45,11 +45,9 @@ struct Bfly {
     Y->b = nd->b;
     X->v.alloc((sizeof(double) * (size_t)n) << (c - nd->a));
     Y->v.alloc((sizeof(double) * (size_t)n) << (nd->b - c));
-    const bool root = nd->v.p == nullptr;
-    FH_CHECK(!root || (nd->a == 0 && nd->b == L), "butterfly: node without values");
+    FH_CHECK(nd->v.p != nullptr, "butterfly: node without values");
     BflyStep s;
-    s.in = root ? A : (const double*)nd->v.p;
-    s.in_dense = root ? 1 : 0;
+    s.in = (const double*)nd->v.p;
     s.in_logB = nd->b - nd->a;
     s.in_logs = L - nd->b;
     s.n = n;
@@ -100,11 +98,8 @@ std::vector<std::vector<Range>> bfly_tree(int L) {
 std::vector<BNodeP> bfly_balanced(Bfly& B) {
   const auto tree = bfly_tree(B.L);
   const int d = (int)tree.size();
-  auto root = std::make_shared<BNode>();
-  root->a = 0;
-  root->b = B.L;
-  auto pr = B.split(root, tree[0][0].second);
-  root.reset();
+  auto pr = B.split(B.root, tree[0][0].second);
+  B.root.reset();
   std::vector<BNodeP> nxt{pr.first, pr.second}, inp = nxt;
   for (int i = 0; i + 1 < d; ++i) {
     nxt.assign(tree[i + 1].size(), nullptr);
@@ -125,9 +120,8 @@ std::vector<BNodeP> bfly_balanced(Bfly& B) {
 }
 
 std::vector<BNodeP> bfly_directional(Bfly& B, bool right) {
-  auto mat = std::make_shared<BNode>();
-  mat->a = 0;
-  mat->b = B.L;
+  BNodeP mat = B.root;
+  B.root.reset();
   std::vector<BNodeP> facs;
   if (right) {
     for (int i = 0; i + 1 < B.L; ++i) {
@@ -190,36 +184,19 @@ DMatP leaf_dense(const BNode& nd, int n, int L, hipStream_t st) {
   return M;
 }
 
-}  // namespace
-
-extern "C" {
-
-int fh_butterfly(const double* A, int n, int dir, const int32_t* perm, int mul_perm, fh_faust_t* out) {
-  FH_API_BEGIN
-  FH_CHECK(A && out, "null argument");
+// the factorization of a device matrix (column-major, or row-major with
+// row_major) into a new Faust: Faust::butterfly_hierarchical(A, dir, P,
+// mul_perm), faust_butterfly.hpp:804-860
+fh_faust_s* butterfly_run(const double* A, size_t lda, int row_major, int n, int dir, const int32_t* perm,
+                          int mul_perm, hipStream_t st) {
   FH_CHECK(n >= 4 && (n & (n - 1)) == 0, "The matrix to factorize must be of a size equal to a power of two (>= 4).");
   FH_CHECK(dir == FH_BUTTERFLY_LEFT || dir == FH_BUTTERFLY_RIGHT || dir == FH_BUTTERFLY_BBTREE,
            "dir argument must be 'right', 'left' or 'bbtree'.");
-  hipStream_t st = nullptr;
+  FH_CHECK(lda >= (size_t)n, "butterfly: bad leading dimension");
   Bfly B;
   B.n = n;
   while ((1 << B.L) < n) ++B.L;
   B.st = st;
-  // A P^T: column i of A becomes column perm[i]
-  std::vector<int> pinv;
-  std::vector<double> Ap;
-  const double* src = A;
-  if (perm) {
-    pinv.assign(n, -1);
-    for (int i = 0; i < n; ++i) {
-      FH_CHECK(perm[i] >= 0 && perm[i] < n && pinv[perm[i]] < 0, "butterfly: perm is not a permutation of 0..n-1");
-      pinv[perm[i]] = i;
-    }
-    Ap.resize((size_t)n * n);
-    for (int i = 0; i < n; ++i)
-      std::copy(A + (size_t)i * n, A + (size_t)(i + 1) * n, Ap.begin() + (size_t)perm[i] * n);
-    src = Ap.data();
-  }
   if (dir == FH_BUTTERFLY_BBTREE) {
     // every block of the tree must fit one workgroup: checked before any launch
     const auto tree = bfly_tree(B.L);
@@ -232,10 +209,23 @@ int fh_butterfly(const double* A, int n, int dir, const int32_t* perm, int mul_p
                  "butterfly: bbtree blocks of this order exceed one workgroup's LDS (order <= 16384 supported)");
       }
   }
-  DMatP Ad = make_dense(FH_F64, n, n, src, n, false, st);
-  Ap.clear();
-  Ap.shrink_to_fit();
-  B.A = (const double*)Ad->d.p;
+  // A P^T: column i of A becomes column perm[i], i.e. column c is A's column pinv[c]
+  std::vector<int> pinv;
+  DBuf pinv_dev;
+  if (perm) {
+    pinv.assign(n, -1);
+    for (int i = 0; i < n; ++i) {
+      FH_CHECK(perm[i] >= 0 && perm[i] < n && pinv[perm[i]] < 0, "butterfly: perm is not a permutation of 0..n-1");
+      pinv[perm[i]] = i;
+    }
+    pinv_dev.alloc(sizeof(int) * n);
+    FH_HIP(hipMemcpyAsync(pinv_dev.p, pinv.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
+  }
+  B.root = std::make_shared<BNode>();
+  B.root->a = 0;
+  B.root->b = B.L;
+  B.root->v.alloc(sizeof(double) * (size_t)n * n);
+  bfly_root(A, lda, row_major, n, (const int*)pinv_dev.p, (double*)B.root->v.p, st);
   std::vector<BNodeP> nodes = dir == FH_BUTTERFLY_BBTREE ? bfly_balanced(B) : bfly_directional(B, dir == FH_BUTTERFLY_RIGHT);
   FH_CHECK((int)nodes.size() == B.L, "butterfly: wrong number of factors");
   for (int k = 0; k < B.L; ++k)
@@ -265,7 +255,30 @@ int fh_butterfly(const double* A, int n, int dir, const int32_t* perm, int mul_p
     r->f.push_back(make_csr_from_host(FH_F64, n, n, n, rp.data(), ci.data(), one.data(), st));
   }
   FH_HIP(hipStreamSynchronize(st));
-  *out = r.release();
+  return r.release();
+}
+
+}  // namespace
+
+extern "C" {
+
+int fh_butterfly(const double* A, int n, int dir, const int32_t* perm, int mul_perm, fh_faust_t* out) {
+  FH_API_BEGIN
+  FH_CHECK(A && out, "null argument");
+  FH_CHECK(n >= 4 && (n & (n - 1)) == 0, "The matrix to factorize must be of a size equal to a power of two (>= 4).");
+  hipStream_t st = nullptr;
+  DBuf Ad(sizeof(double) * (size_t)n * n);
+  FH_HIP(hipMemcpyAsync(Ad.p, A, Ad.bytes, hipMemcpyHostToDevice, st));
+  *out = butterfly_run((const double*)Ad.p, (size_t)n, 0, n, dir, perm, mul_perm, st);
+  FH_API_END
+}
+
+int fh_butterfly_device(const double* A, int n, int64_t lda, int row_major, int dir, const int32_t* perm,
+                        int mul_perm, fh_faust_t* out, void* stream) {
+  FH_API_BEGIN
+  FH_CHECK(A && out, "null argument");
+  FH_CHECK(lda > 0, "butterfly: bad leading dimension");
+  *out = butterfly_run(A, (size_t)lda, row_major, n, dir, perm, mul_perm, (hipStream_t)stream);
   FH_API_END
 }
 

@@ -204,12 +204,11 @@ int64_t nnz_of(DMat& A, hipStream_t st);
 // ------------------------------------------------------------ butterfly
 // One split of a butterfly node (k_butterfly.hip): node [a, b) of L supports,
 // n x 2^in_logB values (row r, entry Q at column
-// (r & ~(2^(in_logB+in_logs) - 1)) + (r & (2^in_logs - 1)) + 2^in_logs Q), or
-// with in_dense the whole n x n matrix column-major; X receives the node
-// [a, c) (logBx = c - a, logsx = L - c), Y the node [c, b).
+// (r & ~(2^(in_logB+in_logs) - 1)) + (r & (2^in_logs - 1)) + 2^in_logs Q; the
+// root [0, L) is the whole matrix row-major); X receives the node [a, c)
+// (logBx = c - a, logsx = L - c), Y the node [c, b).
 struct BflyStep {
   const double* in = nullptr;
-  int in_dense = 0;
   int in_logB = 0, in_logs = 0;
   int n = 0;
   int logBx = 0, logsx = 0, logBy = 0, logsy = 0;
@@ -219,5 +218,8 @@ struct BflyStep {
 void bfly_split(const BflyStep& s, hipStream_t st);
 bool bfly_block_fits(int logBx, int logBy);
 void bfly_scatter_dense(const double* v, int n, int logs, double* out, hipStream_t st);
+// root node (row-major n x n) = A with columns permuted, root(r, c) = A(r, pinv[c]); A column-major (ld lda)
+// or row-major (row_major, ld lda), device memory; pinv_dev device or nullptr
+void bfly_root(const double* A, size_t lda, int row_major, int n, const int* pinv_dev, double* out, hipStream_t st);
 
 }  // namespace fh

@@ -16,11 +16,10 @@
 //   * blocks with a side of 2 (every step of the 'left' / 'right' drivers and
 //     the bottom of the tree): closed form from the 2 x 2 Gram matrix, one
 //     wave per block, any length;
-//   * larger blocks: one-sided (Hestenes) Jacobi on the block held in LDS,
-//     one workgroup per block, rounds of disjoint column pairs in
-//     round-robin order, until a sweep rotates nothing.
+//   * larger blocks (both sides >= 4): the top eigenvector of the block's
+//     Gram matrix by repeated squaring in LDS, one workgroup per block.
 // All arithmetic is FP64.  Singular vectors are defined up to sign; the sign
-// taken here is whatever the rotations leave.
+// taken here is whatever the computation leaves.
 #include <cfloat>
 
 #include "fh_internal.hpp"
@@ -28,10 +27,8 @@
 namespace fh {
 
 constexpr int BF_THREADS = 256;
-constexpr int BF_MAX_SWEEPS = 30;
 
 __device__ __forceinline__ double bf_in(const BflyStep& s, int r, int c) {
-  if (s.in_dense) return s.in[(size_t)r + (size_t)c * s.n];
   const int bs = 1 << (s.in_logB + s.in_logs);
   return s.in[((size_t)r << s.in_logB) + ((c & (bs - 1)) >> s.in_logs)];
 }
@@ -135,111 +132,185 @@ __global__ __launch_bounds__(BF_THREADS) void k_bfly_d2(BflyStep s) {
 }
 
 // ---------------------------------------------------------------- d >= 4
-// LDS: d columns of D doubles, column stride D + 1.
-__global__ __launch_bounds__(BF_THREADS) void k_bfly_jacobi(BflyStep s) {
-  extern __shared__ double W[];
-  __shared__ double nrm[128];
-  __shared__ int flag, top;
-  const int t = blockIdx.x, tid = threadIdx.x;
+// Top singular pair of the block W0 (d columns of length D) from its Gram
+// matrix G = W0^T W0 (d x d, in LDS): M <- G / tr G, then M <- M^2 / tr(M^2)
+// until tr(M^2) reaches 1 (M is then the projector v v^T on the top
+// eigenvector, the top right singular vector of W0), at most BF_MAX_SQ
+// times.  v is M's column of largest diagonal entry, normalised; w = W0 v
+// (= sigma u) comes from the block itself.  G and M carry relative errors of
+// a few eps, which move v by about eps / (relative gap of sigma_1^2 and
+// sigma_2^2).  Each d x d product is a 4 x 4 register tile per thread over
+// LDS; every thread sums the trace in the same order, so the stopping test
+// is uniform.
+constexpr int BF_MAX_SQ = 64;
+constexpr int BF_CHUNK = 16;  // rows of W0 staged per Gram step (fewer when d < 16)
+
+__device__ __forceinline__ int bf_block_index(int b, int nb) {
+  // consecutive blocks share rows of the node: keep them on one XCD (L2);
+  // workgroups are dealt to the 8 XCDs round-robin
+  const int per = nb >> 3;
+  return per > 0 && (nb & 7) == 0 ? (b & 7) * per + (b >> 3) : b;
+}
+
+__global__ __launch_bounds__(BF_THREADS) void k_bfly_gram(BflyStep s) {
+  extern __shared__ double lds[];
+  __shared__ double red[BF_THREADS / 64];
+  const int t = bf_block_index(blockIdx.x, gridDim.x), tid = threadIdx.x;
   const bool tr = s.logBx < s.logBy;
   const int logD = tr ? s.logBy : s.logBx, logd = tr ? s.logBx : s.logBy;
-  const int D = 1 << logD, d = 1 << logd, ld = D + 1;
-  for (int e = tid; e < (D << logd); e += BF_THREADS) {
-    const int j = e >> logD, i = e & (D - 1);
-    W[j * ld + i] = bf_w0(s, tr, t, j, i);
+  const int D = 1 << logD, d = 1 << logd;
+  double* A = lds;          // d x d
+  double* B = lds + d * d;  // d x d; staging rows of W0 during the Gram step
+  const int td = d >> 2;    // tiles per side
+  const bool tile = tid < td * td;
+  const int p0 = (tid / td) << 2, q0 = (tid % td) << 2;
+  double acc[4][4];
+  // G = W0^T W0
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[a][b] = 0.0;
+  const int ch = min(BF_CHUNK, d);  // ch x d staged values fit B's d x d
+  for (int i0 = 0; i0 < D; i0 += ch) {
+    for (int e = tid; e < ch * d; e += BF_THREADS) {
+      const int r = e >> logd, j = e & (d - 1);
+      B[r * d + j] = bf_w0(s, tr, t, j, i0 + r);
+    }
+    __syncthreads();
+    if (tile) {
+      for (int r = 0; r < ch; ++r) {
+        double x[4], y[4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+          x[a] = B[r * d + p0 + a];
+          y[a] = B[r * d + q0 + a];
+        }
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+          for (int b = 0; b < 4; ++b) acc[a][b] = fma(x[a], y[b], acc[a][b]);
+      }
+    }
+    __syncthreads();
   }
-  if (tid == 0) flag = 0;
+  if (tile) {
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) A[(p0 + a) * d + q0 + b] = acc[a][b];
+  }
   __syncthreads();
-  const int npair = d >> 1;
-  const int g = min(64, BF_THREADS / npair);  // lanes per pair, a power of two
-  const int k = tid / g, lg = tid % g;
-  const bool active = k < npair;
-  const double tol = (double)D * DBL_EPSILON;
-  for (int sweep = 0; sweep < BF_MAX_SWEEPS; ++sweep) {
-    for (int r = 0; r < d - 1; ++r) {
-      if (active) {
-        // round r of the circle schedule: player d-1 meets r, the others
-        // meet their mirror image about r (mod d-1)
-        int p, q;
-        if (k == 0) {
-          p = r;
-          q = d - 1;
-        } else {
-          const int m = d - 1;
-          const int x = (r + k) % m, y = (r - k + m) % m;
-          p = min(x, y);
-          q = max(x, y);
-        }
-        double* wp = W + p * ld;
-        double* wq = W + q * ld;
-        double a = 0.0, b = 0.0, gm = 0.0;
-        for (int i = lg; i < D; i += g) {
-          const double x0 = wp[i], x1 = wq[i];
-          a = fma(x0, x0, a);
-          b = fma(x1, x1, b);
-          gm = fma(x0, x1, gm);
-        }
-        for (int o = g >> 1; o >= 1; o >>= 1) {
-          a += __shfl_xor(a, o, 64);
-          b += __shfl_xor(b, o, 64);
-          gm += __shfl_xor(gm, o, 64);
-        }
-        if (fabs(gm) > tol * sqrt(a * b)) {
-          double c, sn, tn;
-          bf_rot(a, b, gm, c, sn, tn);
-          for (int i = lg; i < D; i += g) {
-            const double x0 = wp[i], x1 = wq[i];
-            wp[i] = c * x0 - sn * x1;
-            wq[i] = sn * x0 + c * x1;
+  double tau = 0.0;
+  for (int j = 0; j < d; ++j) tau += A[j * d + j];
+  double* cur = A;
+  double* nxt = B;
+  if (tau > 0.0) {
+    double scale = 1.0 / tau;  // cur holds M / scale
+    for (int k = 0; k < BF_MAX_SQ; ++k) {
+      const double sc2 = scale * scale;
+      if (tile) {
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+          for (int b = 0; b < 4; ++b) acc[a][b] = 0.0;
+        for (int r = 0; r < d; ++r) {
+          double x[4], y[4];
+#pragma unroll
+          for (int a = 0; a < 4; ++a) {
+            x[a] = cur[r * d + p0 + a];  // symmetric: row r = column r
+            y[a] = cur[r * d + q0 + a];
           }
-          if (lg == 0) flag = 1;
+#pragma unroll
+          for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) acc[a][b] = fma(x[a], y[b], acc[a][b]);
         }
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+          for (int b = 0; b < 4; ++b) nxt[(p0 + a) * d + q0 + b] = acc[a][b] * sc2;
       }
       __syncthreads();
-    }
-    const int any = flag;
-    __syncthreads();
-    if (tid == 0) flag = 0;
-    __syncthreads();
-    if (!any) break;
-  }
-  // column norms; the largest is sigma^2
-  if (active) {
-    for (int h = 0; h < 2; ++h) {
-      const int j = k + h * npair;
-      double a = 0.0;
-      for (int i = lg; i < D; i += g) a = fma(W[j * ld + i], W[j * ld + i], a);
-      for (int o = g >> 1; o >= 1; o >>= 1) a += __shfl_xor(a, o, 64);
-      if (lg == 0) nrm[j] = a;
+      double tr2 = 0.0;
+      for (int j = 0; j < d; ++j) tr2 += nxt[j * d + j];
+      double* sw = cur;
+      cur = nxt;
+      nxt = sw;
+      scale = 1.0 / tr2;
+      if (tr2 >= 1.0 - 16.0 * DBL_EPSILON) break;  // tr(M^2) = 1: M is rank one
+      __syncthreads();  // everyone has read the diagonal before it is overwritten
     }
   }
   __syncthreads();
-  if (tid == 0) {
-    int b = 0;
-    for (int j = 1; j < d; ++j)
-      if (nrm[j] > nrm[b]) b = j;
-    top = b;
+  // v: the column of largest diagonal entry, normalised
+  int jb = 0;
+  for (int j = 1; j < d; ++j)
+    if (cur[j * d + j] > cur[jb * d + jb]) jb = j;
+  double vn = 0.0;
+  for (int j = 0; j < d; ++j) vn = fma(cur[j * d + jb], cur[j * d + jb], vn);
+  const double vs = vn > 0.0 ? 1.0 / sqrt(vn) : 0.0;
+  // w = W0 v, sigma = |w|
+  double w = 0.0;
+  if (tid < D) {
+    for (int j = 0; j < d; ++j) w = fma(bf_w0(s, tr, t, j, tid), cur[j * d + jb] * vs, w);
   }
+  double part = w * w;
+  for (int i = tid + BF_THREADS; i < D; i += BF_THREADS) {
+    double wi = 0.0;
+    for (int j = 0; j < d; ++j) wi = fma(bf_w0(s, tr, t, j, i), cur[j * d + jb] * vs, wi);
+    part = fma(wi, wi, part);
+  }
+  part = wave_sum(part);
+  if ((tid & 63) == 0) red[tid >> 6] = part;
   __syncthreads();
-  const double lam = nrm[top];
+  double lam = 0.0;
+  for (int q = 0; q < BF_THREADS / 64; ++q) lam += red[q];
   double kl = 0.0, kc = 0.0;
-  if (lam > 0.0) {
+  if (lam > 0.0 && vn > 0.0) {
     const double sigma = sqrt(lam), rs = sqrt(sigma);
     kl = 1.0 / rs;
-    kc = 1.0 / (sigma * rs);
+    kc = rs;
   }
-  const double* ws = W + top * ld;
-  for (int i = tid; i < D; i += BF_THREADS) bf_put_len(s, tr, t, i, ws[i] * kl);
-  // short side: W0^T w*, from the block as it was
-  if (active) {
-    for (int h = 0; h < 2; ++h) {
-      const int j = k + h * npair;
-      double a = 0.0;
-      for (int i = lg; i < D; i += g) a = fma(bf_w0(s, tr, t, j, i), ws[i], a);
-      for (int o = g >> 1; o >= 1; o >>= 1) a += __shfl_xor(a, o, 64);
-      if (lg == 0) bf_put_col(s, tr, t, j, a * kc);
-    }
+  if (tid < D) bf_put_len(s, tr, t, tid, w * kl);
+  for (int i = tid + BF_THREADS; i < D; i += BF_THREADS) {
+    double wi = 0.0;
+    for (int j = 0; j < d; ++j) wi = fma(bf_w0(s, tr, t, j, i), cur[j * d + jb] * vs, wi);
+    bf_put_len(s, tr, t, i, wi * kl);
   }
+  for (int j = tid; j < d; j += BF_THREADS) bf_put_col(s, tr, t, j, cur[j * d + jb] * vs * kc);
+}
+
+// The root node (the whole matrix, row-major: value (r, c) at r n + c) from
+// the caller's matrix with its columns permuted: root(r, c) = A(r, pinv[c])
+// (pinv == nullptr: no permutation).  Column-major A goes through 32 x 32
+// LDS tiles so that reads run down columns and writes along rows.
+constexpr int BF_TT = 32;
+__global__ __launch_bounds__(256) void k_bfly_root_cm(const double* A, size_t lda, int n, const int* pinv,
+                                                        double* out) {
+  __shared__ double T[BF_TT][BF_TT + 1];
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+  const int r0 = blockIdx.x * BF_TT, c0 = blockIdx.y * BF_TT;
+  for (int k = ty; k < BF_TT; k += 8) {
+    const int c = c0 + k;
+    const int src = pinv ? pinv[c] : c;
+    T[k][tx] = A[(size_t)(r0 + tx) + (size_t)src * lda];
+  }
+  __syncthreads();
+  for (int k = ty; k < BF_TT; k += 8) out[(size_t)(r0 + k) * n + c0 + tx] = T[tx][k];
+}
+
+__global__ void k_bfly_root_small(const double* A, size_t lda, int n, const int* pinv, double* out) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n * n) return;
+  const int r = e / n, c = e % n;
+  out[e] = A[(size_t)r + (size_t)(pinv ? pinv[c] : c) * lda];
+}
+
+__global__ void k_bfly_root_rm(const double* A, size_t lda, int n, const int* pinv, double* out) {
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (size_t)n * n) return;
+  const int r = (int)(e / n), c = (int)(e % n);
+  out[e] = A[(size_t)r * lda + (pinv ? pinv[c] : c)];
 }
 
 // dense n x n column-major factor from a 2-entry-per-row node (B = 2, s = 2^logs)
@@ -256,12 +327,11 @@ static size_t bfly_lds_budget() {
   static size_t budget = 0;
   if (budget) return budget;
   int dev = 0, v = 0;
-  size_t b = 64 * 1024;
+  size_t b = 64 * 1024 - 256;  // the kernel's static LDS (wave sums) comes on top
   if (hipGetDevice(&dev) == hipSuccess &&
-      hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && v > (int)b) {
-    // the kernel's static LDS (norms and flags) comes on top of the dynamic part
-    const int want = v - 2048;
-    if (hipFuncSetAttribute((const void*)k_bfly_jacobi, hipFuncAttributeMaxDynamicSharedMemorySize, want) ==
+      hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && v > 64 * 1024) {
+    const int want = v - 1024;
+    if (hipFuncSetAttribute((const void*)k_bfly_gram, hipFuncAttributeMaxDynamicSharedMemorySize, want) ==
         hipSuccess)
       b = (size_t)want;
   }
@@ -271,13 +341,14 @@ static size_t bfly_lds_budget() {
 }
 
 size_t bfly_block_lds_bytes(int logBx, int logBy) {
-  const int logD = std::max(logBx, logBy), logd = std::min(logBx, logBy);
-  return sizeof(double) * (size_t)((1 << logD) + 1) * (size_t)(1 << logd);
+  const int d = 1 << std::min(logBx, logBy);
+  return sizeof(double) * 2 * (size_t)d * d;  // M and its square (or the staged rows)
 }
 
 bool bfly_block_fits(int logBx, int logBy) {
-  if (std::min(logBx, logBy) <= 1) return true;  // closed form, any length
-  if (std::min(logBx, logBy) > 7) return false;  // column norms: 128 slots
+  const int logd = std::min(logBx, logBy);
+  if (logd <= 1) return true;   // closed form, any length
+  if (logd > 6) return false;   // one 4 x 4 tile of d x d per thread: d <= 64
   return bfly_block_lds_bytes(logBx, logBy) <= bfly_lds_budget();
 }
 
@@ -286,15 +357,31 @@ void bfly_split(const BflyStep& s, hipStream_t st) {
   FH_CHECK(s.logBx >= 1 && s.logBy >= 1, "butterfly: empty split");
   FH_CHECK(s.logBx + s.logsx == s.in_logB + s.in_logs && s.logsy == s.in_logs && s.logBy + s.logsy == s.logsx,
            "butterfly: split does not match its node");
-  FH_CHECK(s.in_dense ? (s.in_logB + s.in_logs == (int)std::log2((double)s.n) && s.in_logs == 0) : true,
-           "butterfly: dense input must be the whole matrix");
   if (std::min(s.logBx, s.logBy) == 1) {
     const int per = BF_THREADS / 64;
     hipLaunchKernelGGL(k_bfly_d2, dim3((s.n + per - 1) / per), dim3(BF_THREADS), 0, st, s);
   } else {
     FH_CHECK(bfly_block_fits(s.logBx, s.logBy), "butterfly: block too large for one workgroup's LDS");
     const size_t lds = bfly_block_lds_bytes(s.logBx, s.logBy);
-    hipLaunchKernelGGL(k_bfly_jacobi, dim3(s.n), dim3(BF_THREADS), lds, st, s);
+    hipLaunchKernelGGL(k_bfly_gram, dim3(s.n), dim3(BF_THREADS), lds, st, s);
+  }
+  FH_LAUNCH_CHECK();
+}
+
+void bfly_root(const double* A, size_t lda, int row_major, int n, const int* pinv_dev, double* out, hipStream_t st) {
+  FH_CHECK(n >= 4 && (n & (n - 1)) == 0, "butterfly: order must be a power of two >= 4");
+  if (row_major || n < BF_TT) {
+    FH_CHECK(row_major || lda >= (size_t)n, "butterfly: bad leading dimension");
+    if (!row_major) {
+      // small column-major matrices: one thread per entry, reading A(r, c) at r + c lda
+      hipLaunchKernelGGL(k_bfly_root_small, dim3((n * n + 255) / 256), dim3(256), 0, st, A, lda, n, pinv_dev, out);
+    } else {
+      const size_t nn = (size_t)n * n;
+      hipLaunchKernelGGL(k_bfly_root_rm, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, st, A, lda, n, pinv_dev,
+                         out);
+    }
+  } else {
+    hipLaunchKernelGGL(k_bfly_root_cm, dim3(n / BF_TT, n / BF_TT), dim3(256), 0, st, A, lda, n, pinv_dev, out);
   }
   FH_LAUNCH_CHECK();
 }

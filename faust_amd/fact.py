@@ -182,11 +182,19 @@ def hierarchical(M, p, ret_lambda=False, backend=2020, on_gpu=True, ret_raw=Fals
 _BFLY_DIR = {"left": _lib.FH_BUTTERFLY_LEFT, "right": _lib.FH_BUTTERFLY_RIGHT, "bbtree": _lib.FH_BUTTERFLY_BBTREE}
 
 
+def _is_cuda_tensor(M):
+    try:
+        import torch
+    except ImportError:
+        return False
+    return isinstance(M, torch.Tensor) and M.is_cuda
+
+
 def _butterfly_one(M, type, perm, mul_perm):
     """One fh_butterfly call (the _FaustAlgoGen.pyx.in:592-630 binding)."""
     if type not in _BFLY_DIR:
         raise ValueError("dir argument must be 'right' or 'left'.")
-    A = np.asfortranarray(M, dtype=np.float64)
+    A = None if _is_cuda_tensor(M) else np.asfortranarray(M, dtype=np.float64)
     pp = None
     if perm is not None:
         if not isinstance(perm, (list, np.ndarray)) or len(perm) != M.shape[1]:
@@ -200,8 +208,21 @@ def _butterfly_one(M, type, perm, mul_perm):
         pp = perm.ctypes.data_as(C.POINTER(C.c_int32))
     L = lib()
     h = C.c_void_p()
-    check(L.fh_butterfly(A.ctypes.data_as(C.POINTER(C.c_double)), A.shape[0], _BFLY_DIR[type], pp,
-                         1 if mul_perm else 0, C.byref(h)))
+    if A is None:  # a CUDA tensor: factorized where it lies
+        import torch
+        T = M if M.dtype == torch.float64 else M.double()
+        if T.is_contiguous():
+            rm, ld = 1, T.stride(0)
+        elif T.t().is_contiguous():
+            rm, ld = 0, T.stride(1)
+        else:
+            T = T.contiguous()
+            rm, ld = 1, T.stride(0)
+        check(L.fh_butterfly_device(C.c_void_p(T.data_ptr()), T.shape[0], max(int(ld), 1), rm, _BFLY_DIR[type], pp,
+                                    1 if mul_perm else 0, C.byref(h), C.c_void_p(torch.cuda.current_stream().cuda_stream)))
+    else:
+        check(L.fh_butterfly(A.ctypes.data_as(C.POINTER(C.c_double)), A.shape[0], _BFLY_DIR[type], pp,
+                             1 if mul_perm else 0, C.byref(h)))
     try:
         n = C.c_int()
         check(L.fh_faust_size(h, C.byref(n)))
@@ -212,6 +233,8 @@ def _butterfly_one(M, type, perm, mul_perm):
             facs.append(Factor(None, handle=fh))
     finally:
         L.fh_faust_free(h)
+    if _is_cuda_tensor(M):
+        return Faust(_facs=facs)
     if M.dtype == np.float32:
         # computed in FP64, handed back in the input's precision
         import scipy.sparse as sp
@@ -226,7 +249,9 @@ def butterfly(M, type="bbtree", perm=None, diag_opt=False, mul_perm=None):
     fact.py:1642-1907; Faust::butterfly_hierarchical,
     faust_butterfly.hpp:545-860).
 
-    M: square float64 / float32 ndarray of order n = 2^L >= 4.  type: 'bbtree'
+    M: square float64 / float32 ndarray of order n = 2^L >= 4, or a float64
+    CUDA tensor (factorized in place on the GPU, fh_butterfly_device; an
+    extension of pyfaust's ndarray-only API).  type: 'bbtree'
     (balanced tree of supports, CSR factors), 'right' or 'left' (one factor
     peeled off per step, dense factors).  perm: None, an index vector (P with
     P[perm[j], j] = 1; M P^T is factorized and P put back on the right),
@@ -243,9 +268,14 @@ def butterfly(M, type="bbtree", perm=None, diag_opt=False, mul_perm=None):
         raise ValueError("mul_perm and diag_opt option can not be both True.")
     if diag_opt:
         raise NotImplementedError("diag_opt (opt_butterfly_faust) is not provided")
-    if not isinstance(M, np.ndarray):
+    on_gpu = _is_cuda_tensor(M)
+    if not on_gpu and not isinstance(M, np.ndarray):
         raise TypeError("butterfly() 1st argument must be a numpy ndarray.")
-    if M.dtype not in (np.float32, np.float64):
+    if on_gpu:
+        import torch
+        if M.dtype != torch.float64:
+            raise TypeError("butterfly(): CUDA tensors must be float64")
+    elif M.dtype not in (np.float32, np.float64):
         raise TypeError("butterfly(): only real float32 / float64 matrices are supported")
     if M.ndim != 2:
         raise ValueError("butterfly(): input matrix/array invalid number of dimensions")
@@ -261,10 +291,11 @@ def butterfly(M, type="bbtree", perm=None, diag_opt=False, mul_perm=None):
         raise ValueError("perm must be None, an index list, a list of them, 'bitrev' or 'default_8'")
     if isinstance(perm, (list, tuple)) and len(perm) and isinstance(perm[0], (list, tuple, np.ndarray)):
         best_err, best_F = np.inf, None
-        nM = np.linalg.norm(M)
+        Mh = M.cpu().numpy() if on_gpu else M
+        nM = np.linalg.norm(Mh)
         for p in perm:
             F = butterfly(M, type, np.asarray(p), diag_opt=diag_opt, mul_perm=mul_perm)
-            err = np.linalg.norm(F.toarray() - M) / nM
+            err = np.linalg.norm(F.toarray() - Mh) / nM
             if err < best_err:
                 best_err, best_F = err, F
         return best_F

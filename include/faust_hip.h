@@ -242,6 +242,14 @@ int fh_palm_result_free(fh_palm_result_t r);
  * approximation in FP64; BBTREE handles orders up to 16384 (the largest
  * block must fit one workgroup's LDS), LEFT / RIGHT any order. */
 int fh_butterfly(const double *A, int n, int dir, const int32_t *perm, int mul_perm, fh_faust_t *out);
+/* The same factorization of a matrix already in device memory: A column-major
+ * (row_major = 0) or row-major (row_major = 1) with leading dimension lda;
+ * perm (host memory) and the result as for fh_butterfly.  Work is enqueued
+ * on `stream` and finished before the call returns.  This form has no
+ * counterpart in the reference, whose butterfly_hierarchical takes host
+ * matrices; it is for callers whose data is already on the GPU. */
+int fh_butterfly_device(const double *A, int n, int64_t lda, int row_major, int dir, const int32_t *perm,
+                        int mul_perm, fh_faust_t *out, void *stream);
 
 /* ---- batched PALM4MSA: many independent factorizations in lock-step.
  * Every target b is n[b] x n[b] and is factorized into nfacts square factors

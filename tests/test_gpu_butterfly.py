@@ -178,3 +178,25 @@ def test_rand_butterfly(gpu):
         assert np.array_equal(_dense(f) != 0, hb.support_csr(32, i).toarray() != 0)
     G = fact.butterfly(F.toarray())
     assert np.linalg.norm(G.toarray() - F.toarray()) <= 1e-12 * np.linalg.norm(F.toarray())
+
+
+@pytest.mark.parametrize("type", ["bbtree", "right", "left"])
+def test_device_input_same_bits(gpu, type):
+    """A CUDA tensor (row-major, column-major, or a strided slice) gives the
+    factors of the host path bit for bit: the root kernels only copy."""
+    import torch
+    rng = np.random.default_rng(44)
+    n = 128
+    M = np.asfortranarray(rng.standard_normal((n, n)))
+    p = rng.permutation(n)
+    ref = fact.butterfly(M, type=type, perm=p)
+    want = [_dense(f) for f in _host(ref)]
+    big = torch.zeros((n + 3, n + 5), dtype=torch.float64, device="cuda")
+    big[:n, :n] = torch.from_numpy(np.ascontiguousarray(M)).cuda()
+    cm = torch.from_numpy(np.ascontiguousarray(M.T)).cuda().t()  # column-major view
+    for T in (torch.from_numpy(np.ascontiguousarray(M)).cuda(), cm, big[:n, :n]):
+        F = fact.butterfly(T, type=type, perm=p)
+        got = [_dense(f) for f in _host(F)]
+        assert len(got) == len(want)
+        for g, w in zip(got, want):
+            assert np.array_equal(g, w)
