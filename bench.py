@@ -43,6 +43,9 @@ def _parse():
     ap.add_argument("--no-palm", action="store_true",
                     help="skip the batched PALM4MSA comparison (512 targets, tuned vs plain FP64)")
     ap.add_argument("--no-butterfly", action="store_true", help="skip the butterfly factorization timings")
+    ap.add_argument("--butterfly-init", type=int, default=0, metavar="NHOLD",
+                    help="also run PALM4MSA on NHOLD 256x256 butterfly targets from the reference start and from "
+                         "the butterfly factorization")
     ap.add_argument("--hadamard", action="store_true",
                     help="also time hierarchical PALM4MSA on the 1024 Hadamard matrix, 100 iterations per level")
     ap.add_argument("--warmstart", type=int, default=0, metavar="NTRAIN",
@@ -425,6 +428,10 @@ def main():
         out["butterfly"] = butterfly_measure(cpu=not args.no_cpu_baseline)
     if args.hadamard:
         out["palm_hadamard"] = palm_hadamard()
+    if args.butterfly_init:
+        from faust_amd import warmstart
+        out["butterfly_init"] = warmstart.butterfly_init_experiment(
+            nhold=args.butterfly_init, log=lambda m: print(m, file=sys.stderr, flush=True))
     if args.warmstart:
         from faust_amd import warmstart
         out["warmstart"] = warmstart.experiment(ntrain=args.warmstart, log=lambda m: print(m, file=sys.stderr))

@@ -258,6 +258,36 @@ def predicted_init(mlp, As, masks, batch=256):
     return out_f, out_l
 
 
+def butterfly_init(As, masks):
+    """Initial factors and lambda for PALM4MSA from the butterfly
+    factorization (fact.butterfly, 'bbtree').  Factor j here couples the
+    indices that differ in bit j, which is butterfly support L-1-j of
+    faust_butterfly.hpp, so P A P (P the bit reversal, its own inverse) is a
+    product in the reference's support order: its factors F_j give
+    P F_j P on mask j, and the product is unchanged.  Each factor is
+    normalised like the supp projector does and lambda fits the target
+    (<A, P> / ||P||^2)."""
+    from .butterfly import bitrev
+    from .fact import butterfly
+    n = As[0].shape[0]
+    br = bitrev(n)
+    out_f, out_l = [], []
+    for A in As:
+        F = butterfly(np.asfortranarray(A[np.ix_(br, br)]), type="bbtree")
+        fs = []
+        for j, M in enumerate(masks):
+            f = F._facs[j].to_host().toarray()[np.ix_(br, br)]
+            f = f * (M != 0)
+            fs.append(np.asfortranarray(f / max(np.linalg.norm(f), 1e-300)))
+        P = np.eye(n)
+        for f in fs:
+            P = P @ f
+        pp = float(np.sum(P * P))
+        out_f.append(fs)
+        out_l.append(float(np.sum(A * P)) / pp if pp > 0 else 1.0)
+    return out_f, out_l
+
+
 class _Runs:
     """Concatenation of batched runs over consecutive chunks of targets."""
 
@@ -298,17 +328,24 @@ def compare_warm_start(As, masks, inits, lambdas, max_its=1000, check_every=25, 
         if log:
             log(f"PALM on held-out targets {sl.stop}/{len(As)}: reference {t_ref:.1f}s, warm {t_warm:.1f}s")
     ref, warm = _Runs(refs), _Runs(warms)
-    fallback = ~(warm.relerr <= ref.relerr)  # a non-finite warm error falls back too
+    # fall back when the warm run ended non-finite, or worse than a finite
+    # reference run (a target whose FP32 iteration broke down ends with a
+    # non-finite error instead of stopping the whole batch)
+    ref_ok, warm_ok = np.isfinite(ref.relerr), np.isfinite(warm.relerr)
+    fallback = ~warm_ok | (ref_ok & (warm.relerr > ref.relerr))
     final = np.where(fallback, ref.relerr, warm.relerr)
     it_w = np.where(fallback, ref.iters_to_target, warm.iters_to_target)
+
+    def med(e):  # median with non-finite errors counted as the worst
+        return float(np.median(np.where(np.isfinite(e), e, np.inf)))
     return {"targets": len(As), "max_its": max_its, "target_err": target_err, "dtype": dtype,
             "ref_reached": int((ref.iters_to_target > 0).sum()), "warm_reached": int((it_w > 0).sum()),
             "ref_median_its": float(np.median(ref.iters_to_target[ref.iters_to_target > 0]))
             if (ref.iters_to_target > 0).any() else None,
             "warm_median_its": float(np.median(it_w[it_w > 0])) if (it_w > 0).any() else None,
-            "ref_median_err": float(np.median(ref.relerr)), "warm_median_err": float(np.median(final)),
-            "warm_median_err_before_fallback": float(np.median(np.where(np.isfinite(warm.relerr), warm.relerr,
-                                                                          np.inf))),
+            "ref_median_err": med(ref.relerr), "warm_median_err": med(final),
+            "ref_nonfinite": int((~ref_ok).sum()), "warm_nonfinite": int((~warm_ok).sum()),
+            "warm_median_err_before_fallback": med(warm.relerr),
             "warm_reached_before_fallback": int((warm.iters_to_target > 0).sum()),
             "fallbacks": int(fallback.sum()), "ref_wall_s": round(t_ref, 3), "warm_wall_s": round(t_warm, 3),
             "ref_ms_to_all": float(ref.times_ms[4]), "warm_ms_to_all": float(warm.times_ms[4])}
@@ -318,7 +355,8 @@ def experiment(n=256, ntrain=16384, nhold=1000, epochs=40, label_its=5, max_its=
                target_err=1e-4, noise=1e-6, hidden=512, dtype="f32", seed=0x5EED, log=print):
     """The learned-warm-start experiment end to end: butterfly targets, labels
     from FP64 PALM4MSA, MLP training on the GPU, then PALM4MSA on held-out
-    targets from the predicted and from the reference initialisation."""
+    targets from the predicted and from the reference initialisation, and
+    from the butterfly factorization's factors (butterfly_init)."""
     import time
     import torch
     rng = np.random.default_rng(seed)
@@ -355,7 +393,38 @@ def experiment(n=256, ntrain=16384, nhold=1000, epochs=40, label_its=5, max_its=
     log(f"predicted inits for {nhold} held-out targets in {t_pred:.1f}s")
     res = compare_warm_start(hold, masks, inits, lam0, max_its=max_its, check_every=check_every,
                              target_err=target_err, dtype=dtype, log=log)
+    # the butterfly factorization as the initialiser
+    t0 = time.perf_counter()
+    binits, blam = butterfly_init(hold, masks)
+    t_bf = time.perf_counter() - t0
+    log(f"butterfly inits for {nhold} held-out targets in {t_bf:.1f}s")
+    bres = compare_warm_start(hold, masks, binits, blam, max_its=max_its, check_every=check_every,
+                              target_err=target_err, dtype=dtype, log=log)
+    res["butterfly_init"] = {k: bres[k] for k in ("warm_reached", "warm_median_its", "warm_median_err",
+                                                  "warm_reached_before_fallback", "fallbacks", "warm_nonfinite",
+                                                  "warm_wall_s", "warm_ms_to_all")}
+    res["butterfly_init"]["init_s"] = round(t_bf, 2)
     res.update({"n": n, "factors": L, "noise_std": noise, "train_targets": ntrain, "label_palm_its": label_its,
                 "epochs": epochs, "final_train_mse": float(losses[-1]), "train_s": round(t_train, 2),
                 "predict_s": round(t_pred, 2), "label_s": round(t_lab, 1), "mlp": [n * n, hidden, hidden, 2 * n * L]})
+    return res
+
+
+def butterfly_init_experiment(n=256, nhold=200, max_its=1000, check_every=25, target_err=1e-4, noise=1e-6,
+                              dtype="f32", seed=0x5EED, log=print):
+    """PALM4MSA on held-out butterfly targets from the reference start and
+    from the butterfly factorization (butterfly_init), same iteration budget
+    and checks as the learned-warm-start comparison."""
+    import time
+    rng = np.random.default_rng(seed)
+    masks = butterfly_supports(n)
+    hold = [butterfly_sample(n, rng, noise)[0] for _ in range(nhold)]
+    t0 = time.perf_counter()
+    inits, lam = butterfly_init(hold, masks)
+    t_init = time.perf_counter() - t0
+    log(f"butterfly inits for {nhold} targets in {t_init:.1f}s")
+    res = compare_warm_start(hold, masks, inits, lam, max_its=max_its, check_every=check_every,
+                             target_err=target_err, dtype=dtype, log=log)
+    res.update({"n": n, "factors": len(masks), "noise_std": noise, "init": "butterfly factorization (bbtree)",
+                "init_s": round(t_init, 2)})
     return res

@@ -33,3 +33,26 @@ def test_warm_start_beats_reference_init(gpu):
     assert res["warm_median_err_before_fallback"] < 0.7 * res["ref_median_err"], res
     assert res["fallbacks"] < 32, res
     assert res["warm_median_err"] <= res["ref_median_err"]
+
+
+def test_butterfly_factorization_as_initialiser(gpu):
+    """The butterfly factorization of a (noisy) butterfly target is already a
+    factorization of it on the warm-start masks: the product is within the
+    noise of the target before any PALM iteration, and PALM4MSA started
+    there reaches 1e-4 at its first check for every target."""
+    n = 16
+    rng = np.random.default_rng(0xB0F)
+    masks = ws.butterfly_supports(n)
+    hold = [ws.butterfly_sample(n, rng, 1e-6)[0] for _ in range(32)]
+    inits, lam = ws.butterfly_init(hold, masks)
+    for A, fs, l in zip(hold, inits, lam):
+        for f, M in zip(fs, masks):
+            assert not np.any(f[M == 0])
+        P = np.eye(n)
+        for f in fs:
+            P = P @ f
+        assert np.linalg.norm(l * P - A) <= 1e-5 * np.linalg.norm(A)
+    res = ws.compare_warm_start(hold, masks, inits, lam, max_its=100, check_every=25, target_err=1e-4,
+                                dtype="f64")
+    assert res["warm_reached_before_fallback"] == len(hold), res
+    assert res["warm_median_its"] <= 25, res
