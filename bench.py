@@ -112,7 +112,7 @@ def workload_chain_csr_f32():
     per_factor = nnz * n * (4 + 4) + (n + 1) * 4 + n * p * 4 + n * p * 4
     return dict(name="faust_csr_fp32_16384x16384_J6_nnz12_X16384x256", facs=facs, X=X, dtype="f32",
                 fac_dtype=np.float32, ops=("N",), bytes_per_launch=per_factor, launches_per_step=J,
-                bytes_per_step=J * per_factor, flops_per_step=J * 2 * n * nnz * p, kernel="k_csr_rm_f32<*,12>",
+                bytes_per_step=J * per_factor, flops_per_step=J * 2 * n * nnz * p, kernel="k_csr_rm_f32<*,12,true>",
                 cpu_reps_cap=3)
 
 

@@ -625,7 +625,10 @@ __device__ __forceinline__ void csr_batch_idx(const int* __restrict__ ci, const 
 
 // Row-major gather SpMM: Y[r, c] = alpha * sum_e v[e] * X[ci[e], c] + beta*Y.
 // OUT_COL: write Y column-major (ld = ldy) through an LDS transpose.
-template <bool OUT_COL, int NB>
+// ELL: ci / v are the ELL view (NB slots per row, column -1 in unused
+// slots; rp unused), so a row's pairs load without the row pointer first;
+// the terms and their order are those of the CSR form.
+template <bool OUT_COL, int NB, bool ELL = false>
 __global__ __launch_bounds__(256) void k_csr_rm_f32(int rows, int p, const int* __restrict__ rp,
                                                     const int* __restrict__ ci, const float* __restrict__ v,
                                                     const float* __restrict__ X, int ldx, float* __restrict__ Y,
@@ -637,7 +640,24 @@ __global__ __launch_bounds__(256) void k_csr_rm_f32(int rows, int p, const int* 
   const int c0 = panel * 32 + (t & 7) * 4;
   const int cg = c0 < p ? c0 : 0;  // lanes past p gather column 0 and store nothing
   float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
-  if (r < rows) {  // the same for the row's 8 lanes, as the broadcast needs
+  if (ELL && r < rows) {
+    int cc[NB];
+    float ww[NB];
+    csr_batch_idx<NB>(ci, v, r * NB, NB, t & 7, cc, ww);
+    float4 xx[NB];
+#pragma unroll
+    for (int q = 0; q < NB; ++q) xx[q] = *reinterpret_cast<const float4*>(X + (size_t)max(cc[q], 0) * ldx + cg);
+    __builtin_amdgcn_sched_barrier(0);  // all gathers in flight before the first FMA
+#pragma unroll
+    for (int q = 0; q < NB; ++q) {
+      const bool on = cc[q] >= 0;  // unused slots are not accumulated
+      acc.x = on ? fmaf(ww[q], xx[q].x, acc.x) : acc.x;
+      acc.y = on ? fmaf(ww[q], xx[q].y, acc.y) : acc.y;
+      acc.z = on ? fmaf(ww[q], xx[q].z, acc.z) : acc.z;
+      acc.w = on ? fmaf(ww[q], xx[q].w, acc.w) : acc.w;
+    }
+  }
+  if (!ELL && r < rows) {  // the same for the row's 8 lanes, as the broadcast needs
     const int e0 = rp[r], e1 = rp[r + 1];
     // batches of up to NB nonzeros (NB: the widest row rounded up to 4, at
     // most 16): the index/value pairs, then all row gathers, then the FMAs,
@@ -851,6 +871,29 @@ static void csr_f32_mul_rm(const DMat& S, char op, const float* X, int ldx, int 
     return;
   }
   dim3 grid((unsigned)(npanels * ((rows + 31) / 32)));
+  const int ew = tr ? S.csr_tell_w : S.csr_ell_w;
+  const char* e_ell = getenv("FH_CSR_ELL");
+  if (ew > 0 && !(e_ell && e_ell[0] == '0')) {
+    // ELL view: every row's pairs at r * ew, no row pointer in the chain
+    const int* eci = tr ? S.csr_tell_ci.as<int>() : S.csr_ell_ci.as<int>();
+    const float* ev = tr ? S.csr_tell_v.as<float>() : S.csr_ell_v.as<float>();
+#define FH_CSR_ELL_GO(OC, NB)                                                                                       \
+  hipLaunchKernelGGL((k_csr_rm_f32<OC, NB, true>), grid, dim3(256), 0, st, rows, p, nullptr, eci, ev, X, ldx, Y, ldy, \
+                     alpha, beta, npanels)
+    switch (ew * 2 + (out_col ? 1 : 0)) {
+      case 8: FH_CSR_ELL_GO(false, 4); break;
+      case 9: FH_CSR_ELL_GO(true, 4); break;
+      case 16: FH_CSR_ELL_GO(false, 8); break;
+      case 17: FH_CSR_ELL_GO(true, 8); break;
+      case 24: FH_CSR_ELL_GO(false, 12); break;
+      case 25: FH_CSR_ELL_GO(true, 12); break;
+      case 32: FH_CSR_ELL_GO(false, 16); break;
+      default: FH_CSR_ELL_GO(true, 16); break;
+    }
+#undef FH_CSR_ELL_GO
+    FH_LAUNCH_CHECK();
+    return;
+  }
   // gather batch: the widest row rounded up to 4 (16 when unknown or wider)
   const int widest = tr ? S.max_col : S.max_row;
   const int nb = (widest < 0 || widest > 12) ? 16 : std::max(4, (widest + 3) / 4 * 4);

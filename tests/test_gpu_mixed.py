@@ -265,3 +265,33 @@ def test_optimize_time_bsr(gpu):
     for H in (G, F):
         Y = H @ X
         assert np.linalg.norm(Y - ref) / np.linalg.norm(ref) < 8e-3
+
+
+@pytest.mark.parametrize("width", [3, 8, 11, 16])
+@pytest.mark.parametrize("op", ["N", "T"])
+def test_csr_fp32_ell_view_same_bits(gpu, monkeypatch, width, op):
+    """The ELL view of FP32 CSR factors (rows of at most 16 entries) gives the
+    CSR kernel's bits: the same terms in the same order, unused slots not
+    accumulated.  Ragged rows, empty rows, column-major and row-major outputs."""
+    rng = np.random.default_rng(40 + width)
+    facs = []
+    for s in [(300, 256), (256, 333), (333, 300)]:
+        a = np.zeros(s)
+        for r in range(s[0]):
+            k = rng.integers(0, width + 1)  # 0..width entries, some rows empty
+            a[r, rng.choice(s[1], size=k, replace=False)] = rng.standard_normal(k)
+        facs.append(sp.csr_matrix(a).astype(np.float32).astype(np.float64))
+    if op == "T":  # the transpose's rows are the columns: keep them short too
+        facs = [sp.csr_matrix(f.T.toarray()) for f in reversed(facs)]
+        facs = [sp.csr_matrix(f.T.toarray()) for f in reversed(facs)]
+    F = faust_amd.Faust(facs, dtype=np.float32)
+    for p in (70, 33, 1):
+        rows_in = facs[-1].shape[1] if op == "N" else facs[0].shape[0]
+        X = rng.standard_normal((rows_in, p)).astype(np.float32).astype(np.float64)
+        monkeypatch.setenv("FH_CSR_ELL", "1")
+        Y1 = (F @ X) if op == "N" else (F.T @ X)
+        monkeypatch.setenv("FH_CSR_ELL", "0")
+        Y0 = (F @ X) if op == "N" else (F.T @ X)
+        assert np.array_equal(Y1, Y0)
+        ref = oracle.chain_multiply(facs, X, op)
+        assert np.linalg.norm(Y1 - ref) / np.linalg.norm(ref) < 1e-5
