@@ -201,12 +201,12 @@ fh_faust_s* butterfly_run(const double* A, size_t lda, int row_major, int n, int
     // every block of the tree must fit one workgroup: checked before any launch
     const auto tree = bfly_tree(B.L);
     FH_CHECK(bfly_block_fits(tree[0][0].second, B.L - tree[0][0].second),
-             "butterfly: bbtree blocks of this order exceed one workgroup's LDS (order <= 16384 supported)");
+             "butterfly: bbtree blocks of this order exceed one workgroup's LDS (order <= 32768 supported)");
     for (size_t i = 0; i + 1 < tree.size(); ++i)
       for (size_t j = 0; j < tree[i].size() && 2 * j + 1 < tree[i + 1].size(); ++j) {
         const int a = tree[i][j].first, b = tree[i][j].second, c = tree[i + 1][2 * j].second;
         FH_CHECK(bfly_block_fits(c - a, b - c),
-                 "butterfly: bbtree blocks of this order exceed one workgroup's LDS (order <= 16384 supported)");
+                 "butterfly: bbtree blocks of this order exceed one workgroup's LDS (order <= 32768 supported)");
       }
   }
   // A P^T: column i of A becomes column perm[i], i.e. column c is A's column pinv[c]

@@ -139,9 +139,8 @@ __global__ __launch_bounds__(BF_THREADS) void k_bfly_d2(BflyStep s) {
 // times.  v is M's column of largest diagonal entry, normalised; w = W0 v
 // (= sigma u) comes from the block itself.  G and M carry relative errors of
 // a few eps, which move v by about eps / (relative gap of sigma_1^2 and
-// sigma_2^2).  Each d x d product is a 4 x 4 register tile per thread over
-// LDS; every thread sums the trace in the same order, so the stopping test
-// is uniform.
+// sigma_2^2).  Every thread sums the trace in the same order, so the
+// stopping test is uniform.
 constexpr int BF_MAX_SQ = 64;
 constexpr int BF_CHUNK = 16;  // rows of W0 staged per Gram step (fewer when d < 16)
 
@@ -152,6 +151,11 @@ __device__ __forceinline__ int bf_block_index(int b, int nb) {
   return per > 0 && (nb & 7) == 0 ? (b & 7) * per + (b >> 3) : b;
 }
 
+// TS x TS register tile of a d x d product per thread (TS = 4 up to d = 64,
+// 8 for d = 128; (d / TS)^2 <= 256 threads).  LDS: M (d x d), then ch x d
+// staged rows of W0 for the Gram step.  A square is formed in registers and
+// written over M after a barrier, so one d x d buffer suffices.
+template <int TS>
 __global__ __launch_bounds__(BF_THREADS) void k_bfly_gram(BflyStep s) {
   extern __shared__ double lds[];
   __shared__ double red[BF_THREADS / 64];
@@ -159,18 +163,18 @@ __global__ __launch_bounds__(BF_THREADS) void k_bfly_gram(BflyStep s) {
   const bool tr = s.logBx < s.logBy;
   const int logD = tr ? s.logBy : s.logBx, logd = tr ? s.logBx : s.logBy;
   const int D = 1 << logD, d = 1 << logd;
-  double* A = lds;          // d x d
-  double* B = lds + d * d;  // d x d; staging rows of W0 during the Gram step
-  const int td = d >> 2;    // tiles per side
+  double* M = lds;          // d x d
+  double* B = lds + d * d;  // ch x d staged rows of W0
+  const int td = d / TS;    // tiles per side
   const bool tile = tid < td * td;
-  const int p0 = (tid / td) << 2, q0 = (tid % td) << 2;
-  double acc[4][4];
+  const int p0 = (tid / td) * TS, q0 = (tid % td) * TS;
+  double acc[TS][TS];
   // G = W0^T W0
 #pragma unroll
-  for (int a = 0; a < 4; ++a)
+  for (int a = 0; a < TS; ++a)
 #pragma unroll
-    for (int b = 0; b < 4; ++b) acc[a][b] = 0.0;
-  const int ch = min(BF_CHUNK, d);  // ch x d staged values fit B's d x d
+    for (int b = 0; b < TS; ++b) acc[a][b] = 0.0;
+  const int ch = min(BF_CHUNK, D);
   for (int i0 = 0; i0 < D; i0 += ch) {
     for (int e = tid; e < ch * d; e += BF_THREADS) {
       const int r = e >> logd, j = e & (d - 1);
@@ -179,85 +183,82 @@ __global__ __launch_bounds__(BF_THREADS) void k_bfly_gram(BflyStep s) {
     __syncthreads();
     if (tile) {
       for (int r = 0; r < ch; ++r) {
-        double x[4], y[4];
+        double x[TS], y[TS];
 #pragma unroll
-        for (int a = 0; a < 4; ++a) {
+        for (int a = 0; a < TS; ++a) {
           x[a] = B[r * d + p0 + a];
           y[a] = B[r * d + q0 + a];
         }
 #pragma unroll
-        for (int a = 0; a < 4; ++a)
+        for (int a = 0; a < TS; ++a)
 #pragma unroll
-          for (int b = 0; b < 4; ++b) acc[a][b] = fma(x[a], y[b], acc[a][b]);
+          for (int b = 0; b < TS; ++b) acc[a][b] = fma(x[a], y[b], acc[a][b]);
       }
     }
     __syncthreads();
   }
   if (tile) {
 #pragma unroll
-    for (int a = 0; a < 4; ++a)
+    for (int a = 0; a < TS; ++a)
 #pragma unroll
-      for (int b = 0; b < 4; ++b) A[(p0 + a) * d + q0 + b] = acc[a][b];
+      for (int b = 0; b < TS; ++b) M[(p0 + a) * d + q0 + b] = acc[a][b];
   }
   __syncthreads();
   double tau = 0.0;
-  for (int j = 0; j < d; ++j) tau += A[j * d + j];
-  double* cur = A;
-  double* nxt = B;
+  for (int j = 0; j < d; ++j) tau += M[j * d + j];
   if (tau > 0.0) {
-    double scale = 1.0 / tau;  // cur holds M / scale
+    double scale = 1.0 / tau;  // M holds the current matrix / scale
     for (int k = 0; k < BF_MAX_SQ; ++k) {
       const double sc2 = scale * scale;
       if (tile) {
 #pragma unroll
-        for (int a = 0; a < 4; ++a)
+        for (int a = 0; a < TS; ++a)
 #pragma unroll
-          for (int b = 0; b < 4; ++b) acc[a][b] = 0.0;
+          for (int b = 0; b < TS; ++b) acc[a][b] = 0.0;
         for (int r = 0; r < d; ++r) {
-          double x[4], y[4];
+          double x[TS], y[TS];
 #pragma unroll
-          for (int a = 0; a < 4; ++a) {
-            x[a] = cur[r * d + p0 + a];  // symmetric: row r = column r
-            y[a] = cur[r * d + q0 + a];
+          for (int a = 0; a < TS; ++a) {
+            x[a] = M[r * d + p0 + a];  // symmetric: row r = column r
+            y[a] = M[r * d + q0 + a];
           }
 #pragma unroll
-          for (int a = 0; a < 4; ++a)
+          for (int a = 0; a < TS; ++a)
 #pragma unroll
-            for (int b = 0; b < 4; ++b) acc[a][b] = fma(x[a], y[b], acc[a][b]);
+            for (int b = 0; b < TS; ++b) acc[a][b] = fma(x[a], y[b], acc[a][b]);
         }
+      }
+      __syncthreads();  // every tile of the square is formed before M is overwritten
+      if (tile) {
 #pragma unroll
-        for (int a = 0; a < 4; ++a)
+        for (int a = 0; a < TS; ++a)
 #pragma unroll
-          for (int b = 0; b < 4; ++b) nxt[(p0 + a) * d + q0 + b] = acc[a][b] * sc2;
+          for (int b = 0; b < TS; ++b) M[(p0 + a) * d + q0 + b] = acc[a][b] * sc2;
       }
       __syncthreads();
       double tr2 = 0.0;
-      for (int j = 0; j < d; ++j) tr2 += nxt[j * d + j];
-      double* sw = cur;
-      cur = nxt;
-      nxt = sw;
+      for (int j = 0; j < d; ++j) tr2 += M[j * d + j];
       scale = 1.0 / tr2;
       if (tr2 >= 1.0 - 16.0 * DBL_EPSILON) break;  // tr(M^2) = 1: M is rank one
-      __syncthreads();  // everyone has read the diagonal before it is overwritten
     }
   }
   __syncthreads();
   // v: the column of largest diagonal entry, normalised
   int jb = 0;
   for (int j = 1; j < d; ++j)
-    if (cur[j * d + j] > cur[jb * d + jb]) jb = j;
+    if (M[j * d + j] > M[jb * d + jb]) jb = j;
   double vn = 0.0;
-  for (int j = 0; j < d; ++j) vn = fma(cur[j * d + jb], cur[j * d + jb], vn);
+  for (int j = 0; j < d; ++j) vn = fma(M[j * d + jb], M[j * d + jb], vn);
   const double vs = vn > 0.0 ? 1.0 / sqrt(vn) : 0.0;
   // w = W0 v, sigma = |w|
   double w = 0.0;
   if (tid < D) {
-    for (int j = 0; j < d; ++j) w = fma(bf_w0(s, tr, t, j, tid), cur[j * d + jb] * vs, w);
+    for (int j = 0; j < d; ++j) w = fma(bf_w0(s, tr, t, j, tid), M[j * d + jb] * vs, w);
   }
   double part = w * w;
   for (int i = tid + BF_THREADS; i < D; i += BF_THREADS) {
     double wi = 0.0;
-    for (int j = 0; j < d; ++j) wi = fma(bf_w0(s, tr, t, j, i), cur[j * d + jb] * vs, wi);
+    for (int j = 0; j < d; ++j) wi = fma(bf_w0(s, tr, t, j, i), M[j * d + jb] * vs, wi);
     part = fma(wi, wi, part);
   }
   part = wave_sum(part);
@@ -274,10 +275,10 @@ __global__ __launch_bounds__(BF_THREADS) void k_bfly_gram(BflyStep s) {
   if (tid < D) bf_put_len(s, tr, t, tid, w * kl);
   for (int i = tid + BF_THREADS; i < D; i += BF_THREADS) {
     double wi = 0.0;
-    for (int j = 0; j < d; ++j) wi = fma(bf_w0(s, tr, t, j, i), cur[j * d + jb] * vs, wi);
+    for (int j = 0; j < d; ++j) wi = fma(bf_w0(s, tr, t, j, i), M[j * d + jb] * vs, wi);
     bf_put_len(s, tr, t, i, wi * kl);
   }
-  for (int j = tid; j < d; j += BF_THREADS) bf_put_col(s, tr, t, j, cur[j * d + jb] * vs * kc);
+  for (int j = tid; j < d; j += BF_THREADS) bf_put_col(s, tr, t, j, M[j * d + jb] * vs * kc);
 }
 
 // The root node (the whole matrix, row-major: value (r, c) at r n + c) from
@@ -331,8 +332,10 @@ static size_t bfly_lds_budget() {
   if (hipGetDevice(&dev) == hipSuccess &&
       hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && v > 64 * 1024) {
     const int want = v - 1024;
-    if (hipFuncSetAttribute((const void*)k_bfly_gram, hipFuncAttributeMaxDynamicSharedMemorySize, want) ==
-        hipSuccess)
+    if (hipFuncSetAttribute((const void*)k_bfly_gram<4>, hipFuncAttributeMaxDynamicSharedMemorySize, want) ==
+            hipSuccess &&
+        hipFuncSetAttribute((const void*)k_bfly_gram<8>, hipFuncAttributeMaxDynamicSharedMemorySize, want) ==
+            hipSuccess)
       b = (size_t)want;
   }
   (void)hipGetLastError();  // a refused attribute leaves the 64 KiB budget, not an error
@@ -341,14 +344,14 @@ static size_t bfly_lds_budget() {
 }
 
 size_t bfly_block_lds_bytes(int logBx, int logBy) {
-  const int d = 1 << std::min(logBx, logBy);
-  return sizeof(double) * 2 * (size_t)d * d;  // M and its square (or the staged rows)
+  const size_t d = (size_t)1 << std::min(logBx, logBy), D = (size_t)1 << std::max(logBx, logBy);
+  return sizeof(double) * (d * d + std::min<size_t>(BF_CHUNK, D) * d);  // M and the staged rows
 }
 
 bool bfly_block_fits(int logBx, int logBy) {
   const int logd = std::min(logBx, logBy);
   if (logd <= 1) return true;   // closed form, any length
-  if (logd > 6) return false;   // one 4 x 4 tile of d x d per thread: d <= 64
+  if (logd > 7) return false;   // one 8 x 8 tile of d x d per thread: d <= 128
   return bfly_block_lds_bytes(logBx, logBy) <= bfly_lds_budget();
 }
 
@@ -363,7 +366,10 @@ void bfly_split(const BflyStep& s, hipStream_t st) {
   } else {
     FH_CHECK(bfly_block_fits(s.logBx, s.logBy), "butterfly: block too large for one workgroup's LDS");
     const size_t lds = bfly_block_lds_bytes(s.logBx, s.logBy);
-    hipLaunchKernelGGL(k_bfly_gram, dim3(s.n), dim3(BF_THREADS), lds, st, s);
+    if (std::min(s.logBx, s.logBy) <= 6)
+      hipLaunchKernelGGL(k_bfly_gram<4>, dim3(s.n), dim3(BF_THREADS), lds, st, s);
+    else
+      hipLaunchKernelGGL(k_bfly_gram<8>, dim3(s.n), dim3(BF_THREADS), lds, st, s);
   }
   FH_LAUNCH_CHECK();
 }

@@ -239,7 +239,7 @@ int fh_palm_result_free(fh_palm_result_t r);
  * of L factors on the butterfly supports (L + 1 with an appended
  * permutation): CSR factors for BBTREE, dense ones for LEFT / RIGHT (a
  * permuted last factor is CSR).  Every block is a best rank-1
- * approximation in FP64; BBTREE handles orders up to 16384 (the largest
+ * approximation in FP64; BBTREE handles orders up to 32768 (the largest
  * block must fit one workgroup's LDS), LEFT / RIGHT any order. */
 int fh_butterfly(const double *A, int n, int dir, const int32_t *perm, int mul_perm, fh_faust_t *out);
 /* The same factorization of a matrix already in device memory: A column-major

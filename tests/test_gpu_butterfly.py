@@ -200,3 +200,25 @@ def test_device_input_same_bits(gpu, type):
         assert len(got) == len(want)
         for g, w in zip(got, want):
             assert np.array_equal(g, w)
+
+
+def test_bbtree_order_32768(gpu):
+    """The largest bbtree order: 256 x 128 blocks at the root (a 128 x 128
+    Gram matrix per block).  An exact butterfly of order 32768 is formed on
+    the GPU (the Faust applied to the identity), factorized in place, and the
+    result checked on random columns."""
+    import torch
+    n = 32768
+    rng = np.random.default_rng(32768)
+    B = faust_amd.Faust(hb.rand_butterfly_factors(n, rng=rng))
+    eye = torch.eye(n, dtype=torch.float64, device="cuda")
+    out = B.matmul_device(eye)  # (n, n) row-major = B column-major
+    del eye
+    torch.cuda.empty_cache()
+    F = fact.butterfly(out.t(), type="bbtree")
+    del out
+    torch.cuda.empty_cache()
+    assert F.numfactors() == 15
+    X = rng.standard_normal((n, 4))
+    Y, R = F @ X, B @ X
+    assert np.linalg.norm(Y - R) <= 1e-12 * np.linalg.norm(R)
