@@ -267,7 +267,7 @@ int fh_mat_info(fh_mat_t M, int* kind, int* dtype, int* nrows, int* ncols, int64
   if (nrows) *nrows = d.m;
   if (ncols) *ncols = d.n;
   if (nnz) {
-    if (d.kind == K_DENSE && d.dtype == FH_F64)
+    if (d.kind == K_DENSE)
       *nnz = nnz_of(*M->m, nullptr);
     else if (d.kind == K_BSR)
       *nnz = (int64_t)d.bnnz * d.bm * d.bn;
@@ -473,6 +473,9 @@ int fh_faust_get_product(fh_faust_t F, void* out_dev, void* stream) {
 int fh_faust_spectral_norm(fh_faust_t F, int max_iter, double threshold, double* norm, int* flag, void* stream) {
   FH_API_BEGIN
   FH_CHECK(F && norm, "null argument");
+  for (auto& x : F->f)
+    if (x->dtype != FH_F64 || x->kind == K_BSR)
+      throw Error(FH_ERR_UNSUPPORTED, "spectral_norm: FP64 dense / CSR chains only");
   *norm = faust_spectral_norm_f64(F->f, max_iter, threshold, flag, (hipStream_t)stream);
   FH_API_END
 }

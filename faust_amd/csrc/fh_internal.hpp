@@ -180,6 +180,8 @@ void diag_dot_f64(const double* A, const double* S, int m, int n, double* diag, 
 void dense_to_csr_f64(const double* D, int m, int n, DMat& out, hipStream_t st);
 void csr_to_dense_f64(const DMat& S, double* D, hipStream_t st);
 int64_t count_nnz_f64(const double* D, int64_t n, hipStream_t st);
+// nonzero entries of n values of type dtype (FH_F64, FH_F32 or FH_BF16 bits)
+int64_t count_nnz(const void* D, int dtype, int64_t n, hipStream_t st);
 // elementwise helpers
 void axpby_f64(double* y, const double* x, int64_t n, double a, double b, hipStream_t st); // y = a*x + b*y canonical
 void scale_f64(double* x, int64_t n, const double* dscale, double hscale, int mode, hipStream_t st);

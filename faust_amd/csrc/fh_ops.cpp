@@ -34,7 +34,8 @@ DMatP dense_of(const DMat& A, hipStream_t st) {
 
 int64_t nnz_of(DMat& A, hipStream_t st) {
   if (A.kind == K_CSR) return A.nnz;
-  if (A.nnz < 0) A.nnz = count_nnz_f64(A.d.as<double>(), (int64_t)A.m * A.n, st);
+  if (A.kind != K_DENSE) throw Error(FH_ERR_UNSUPPORTED, "nnz count: dense or CSR matrices only");
+  if (A.nnz < 0) A.nnz = count_nnz(A.d.p, A.dtype, (int64_t)A.m * A.n, st);
   return A.nnz;
 }
 

@@ -218,6 +218,38 @@ __global__ __launch_bounds__(PW_WIDE_THREADS) void k_pw_dense(const double* __re
   if (t < nr) nxt[r0 + t] = acc;
 }
 
+// Operators with more than PW_VMAX columns: x read from global memory (the
+// L2 holds it), one row per thread, the same fma chain per row.
+__global__ __launch_bounds__(PW_WIDE_THREADS) void k_pw_csr_g(const int* __restrict__ rp, const int* __restrict__ ci,
+                                                              const double* __restrict__ v, int rows,
+                                                              const double* __restrict__ cur,
+                                                              double* __restrict__ nxt,
+                                                              const PwState* __restrict__ s) {
+  if (s->done) return;
+  const int r = blockIdx.x * PW_WIDE_THREADS + threadIdx.x;
+  if (r >= rows) return;
+  double acc = 0.0;
+  for (int p = rp[r], pe = rp[r + 1]; p < pe; ++p) acc = fma(v[p], cur[ci[p]], acc);
+  nxt[r] = acc;
+}
+
+__global__ __launch_bounds__(PW_WIDE_THREADS) void k_pw_dense_g(const double* __restrict__ A, int lda, int trans,
+                                                                int rows, int cols, const double* __restrict__ cur,
+                                                                double* __restrict__ nxt,
+                                                                const PwState* __restrict__ s) {
+  if (s->done) return;
+  const int r = blockIdx.x * PW_WIDE_THREADS + threadIdx.x;
+  if (r >= rows) return;
+  double acc = 0.0;
+  if (trans) {  // row r of A^T = column r of A
+    const double* col = A + (size_t)r * lda;
+    for (int c = 0; c < cols; ++c) acc = fma(col[c], cur[c], acc);
+  } else {
+    for (int c = 0; c < cols; ++c) acc = fma(A[(size_t)c * lda + r], cur[c], acc);
+  }
+  nxt[r] = acc;
+}
+
 // One round boundary.  mode 1: finish the previous round (xa = chain output):
 // zero test and lambda = <xn, xa>.  Then the loop test of the persistent
 // kernel, and if it continues: it++, normalise xa into xn.
@@ -338,7 +370,15 @@ static double power_iteration_wide(const std::vector<PwOp>& h, int n, const std:
       int R = 1;
       while (R < 8 && (o.rows + 2 * R - 1) / (2 * R) >= 256) R *= 2;
       const dim3 grid((o.rows + R - 1) / R);
-      if (o.kind == 1)
+      if (o.cols > PW_VMAX) {  // x does not fit LDS: read it from global memory
+        const dim3 gg((o.rows + PW_WIDE_THREADS - 1) / PW_WIDE_THREADS);
+        if (o.kind == 1)
+          hipLaunchKernelGGL(k_pw_csr_g, gg, dim3(PW_WIDE_THREADS), 0, st, o.rp, o.ci, o.v, o.rows, cur, out,
+                             (const PwState*)dst);
+        else
+          hipLaunchKernelGGL(k_pw_dense_g, gg, dim3(PW_WIDE_THREADS), 0, st, o.v, o.lda, o.trans, o.rows, o.cols,
+                             cur, out, (const PwState*)dst);
+      } else if (o.kind == 1)
         hipLaunchKernelGGL(k_pw_csr, grid, dim3(PW_WIDE_THREADS), 0, st, o.rp, o.ci, o.v, o.rows, o.cols, R, cur,
                            out, (const PwState*)dst);
       else
@@ -380,8 +420,8 @@ double power_iteration_f64(const std::vector<ChainOp>& ops, int n, int maxit, do
     PwOp& o = h[s];
     o.rows = tr ? M.n : M.m;
     o.cols = tr ? M.m : M.n;
-    FH_CHECK(o.rows <= PW_VMAX && o.cols <= PW_VMAX, "power_iteration: dimension above 2048");
     if (M.kind == K_CSR) {
+      FH_CHECK(M.dtype == FH_F64, "power_iteration: FP64 dense or CSR factors only");
       o.kind = 1;
       o.trans = 0;
       o.nnz = M.nnz;
@@ -406,7 +446,10 @@ double power_iteration_f64(const std::vector<ChainOp>& ops, int n, int maxit, do
     }
   }
   const std::vector<double>& x0 = power_x0(n);
-  if (use_wide_form(h)) return power_iteration_wide(h, n, x0, maxit, thr, flag, st);
+  // the persistent kernel keeps every vector in LDS: up to PW_VMAX entries
+  bool fits = n <= PW_VMAX;
+  for (const PwOp& o : h) fits = fits && o.rows <= PW_VMAX && o.cols <= PW_VMAX;
+  if (!fits || use_wide_form(h)) return power_iteration_wide(h, n, x0, maxit, thr, flag, st);
   DBuf dops(sizeof(PwOp) * h.size()), dx(sizeof(double) * n), dout(sizeof(double) * 3);
   FH_HIP(hipMemcpyAsync(dops.p, h.data(), sizeof(PwOp) * h.size(), hipMemcpyHostToDevice, st));
   FH_HIP(hipMemcpyAsync(dx.p, x0.data(), sizeof(double) * n, hipMemcpyHostToDevice, st));

@@ -277,20 +277,33 @@ void csr_to_dense_f64(const DMat& S, double* D, hipStream_t st) {
   }
 }
 
-__global__ void k_count_nnz(const double* __restrict__ D, int64_t n, unsigned long long* __restrict__ out) {
+__device__ __forceinline__ bool nz_val(double x) { return x != 0.0; }
+__device__ __forceinline__ bool nz_val(float x) { return x != 0.0f; }
+__device__ __forceinline__ bool nz_val(uint16_t x) { return (x & 0x7fffu) != 0; }  // BF16 bits: +-0 are zero
+
+template <class T>
+__global__ void k_count_nnz(const T* __restrict__ D, int64_t n, unsigned long long* __restrict__ out) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  int c = (i < n && D[i] != 0.0) ? 1 : 0;
+  int c = (i < n && nz_val(D[i])) ? 1 : 0;
   // wave-level ballot then one atomic per wave
   unsigned long long b = __ballot(c);
   if ((threadIdx.x & 63) == 0 && b) atomicAdd(out, (unsigned long long)__popcll(b));
 }
 
-int64_t count_nnz_f64(const double* D, int64_t n, hipStream_t st) {
+int64_t count_nnz_f64(const double* D, int64_t n, hipStream_t st) { return count_nnz(D, FH_F64, n, st); }
+
+int64_t count_nnz(const void* D, int dtype, int64_t n, hipStream_t st) {
   if (n <= 0) return 0;
   DBuf c(sizeof(unsigned long long));
   FH_HIP(hipMemsetAsync(c.p, 0, sizeof(unsigned long long), st));
-  hipLaunchKernelGGL(k_count_nnz, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, D, n,
-                     c.as<unsigned long long>());
+  const dim3 grid((unsigned)((n + 255) / 256));
+  if (dtype == FH_F64)
+    hipLaunchKernelGGL(k_count_nnz<double>, grid, dim3(256), 0, st, (const double*)D, n, c.as<unsigned long long>());
+  else if (dtype == FH_F32)
+    hipLaunchKernelGGL(k_count_nnz<float>, grid, dim3(256), 0, st, (const float*)D, n, c.as<unsigned long long>());
+  else
+    hipLaunchKernelGGL(k_count_nnz<uint16_t>, grid, dim3(256), 0, st, (const uint16_t*)D, n,
+                       c.as<unsigned long long>());
   FH_LAUNCH_CHECK();
   unsigned long long h = 0;
   FH_HIP(hipMemcpyAsync(&h, c.p, sizeof(h), hipMemcpyDeviceToHost, st));

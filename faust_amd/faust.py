@@ -371,6 +371,12 @@ class Faust:
         """norm(2): power iteration on the chain (TransformHelper::spectralNorm);
         'fro': Frobenius norm of the dense product."""
         if ord == 2:
+            if self.act_dtype != torch.float64:
+                # FP32 / BF16 chains: the FP64 power iteration on an FP64 copy of the factors
+                mats = [f.to_host() for f in self._facs]
+                mats = [sp.csr_matrix(m, dtype=np.float64) if sp.issparse(m) else np.asarray(m, dtype=np.float64)
+                        for m in mats]
+                return Faust(mats).norm(2, **kwargs)
             nrm, flag = C.c_double(), C.c_int()
             check(lib().fh_faust_spectral_norm(self.h, int(kwargs.get("max_num_its", 100)),
                                                 float(kwargs.get("threshold", 1e-6)), C.byref(nrm), C.byref(flag),

@@ -140,7 +140,8 @@ int fh_faust_timers(fh_faust_t F, double *ms, int64_t *counts);
 int fh_faust_get_product(fh_faust_t F, void *out_dev, void *stream);
 /* 2-norm by power iteration (TransformHelper::spectralNorm,
  * faust_TransformHelper.hpp:1201-1218; power_iteration,
- * faust_linear_algebra.hpp:808-898).  Synchronizes. */
+ * faust_linear_algebra.hpp:808-898) of an FP64 dense / CSR chain; other
+ * chains give FH_ERR_UNSUPPORTED.  Synchronizes. */
 int fh_faust_spectral_norm(fh_faust_t F, int max_iter, double threshold, double *norm, int *flag, void *stream);
 
 /* ---- projectors on a dense device matrix, in place

@@ -204,3 +204,48 @@ def test_get_product_fp32_chain(gpu):
     assert rc == 4  # FH_ERR_UNSUPPORTED
     ref = facs[0].toarray().astype(np.float64) @ facs[1].toarray() @ facs[2].toarray()
     assert np.linalg.norm(F.toarray() - ref) <= 1e-5 * np.linalg.norm(ref)
+
+
+def test_spectral_norm_fp32_chain(gpu):
+    """The FP64 power iteration refuses FP32 chains; Faust.norm(2) runs it on
+    an FP64 copy of their factors."""
+    import ctypes as C
+    from faust_amd import _lib
+    rng = np.random.default_rng(13)
+    facs = [sp.csr_matrix(rng.standard_normal((40, 40)) * (rng.random((40, 40)) < 0.3)).astype(np.float32)
+            for _ in range(3)]
+    F = faust_amd.Faust(facs, dtype=np.float32)
+    nrm, flag = C.c_double(), C.c_int()
+    assert _lib.lib().fh_faust_spectral_norm(F.h, 100, 1e-6, C.byref(nrm), C.byref(flag), None) == 4
+    ref = np.linalg.norm(facs[0].toarray().astype(np.float64) @ facs[1].toarray() @ facs[2].toarray(), 2)
+    assert abs(F.norm(2) - ref) <= 1e-4 * ref
+    # a dense FP32 factor too (no nonzero count is taken of it)
+    G = faust_amd.Faust([facs[0].toarray(), facs[1]], dtype=np.float32)
+    ref2 = np.linalg.norm(facs[0].toarray().astype(np.float64) @ facs[1].toarray(), 2)
+    assert abs(G.norm(2) - ref2) <= 1e-4 * ref2
+
+
+def test_nnz_of_dense_factors_all_dtypes(gpu):
+    """fh_mat_info counts the nonzeros of dense FP64, FP32 and BF16 factors."""
+    rng = np.random.default_rng(14)
+    a = rng.standard_normal((37, 29)) * (rng.random((37, 29)) < 0.4)
+    want = int(np.count_nonzero(a))
+    for dt in (np.float64, np.float32, "bf16"):
+        f = faust_amd.Factor(a, dt)
+        assert f.info()[4] == want, dt
+
+
+@pytest.mark.parametrize("dense", [False, True])
+def test_spectral_norm_beyond_2048_bitexact(gpu, dense):
+    """Operators wider than 2048 go through the wide power iteration with x
+    read from global memory; same fma chains, so the oracle's bits."""
+    rng = np.random.default_rng(15)
+    if dense:
+        n = 2304
+        facs = [_cfg1_factors(seed=0xD3, n=n, nnz=6, J=1)[0], np.asfortranarray(rng.standard_normal((n, n)) / 48.0),
+                _cfg1_factors(seed=0xD4, n=n, nnz=6, J=1)[0]]
+    else:
+        facs = _cfg1_factors(seed=0xB16, n=4096, nnz=8, J=3)
+    F = faust_amd.Faust(facs)
+    nrm, _ = oracle.spectral_norm(facs)
+    assert F.norm(2) == nrm
