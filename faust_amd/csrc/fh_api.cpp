@@ -411,6 +411,36 @@ int fh_faust_get_mul_mode(fh_faust_t F, int* mode) {
   FH_API_END
 }
 
+// A dense FP32 factor in an FP32 chain runs as its CSR twin (built once from
+// a host copy of the factor, kept with it): the FP32 chain kernels take CSR.
+static std::vector<DMatP> fp32_chain_factors(const std::vector<DMatP>& f, hipStream_t st) {
+  std::vector<DMatP> out = f;
+  for (auto& x : out) {
+    if (x->kind != K_DENSE || x->dtype != FH_F32) continue;
+    if (!x->csr_twin) {
+      const int m = x->m, n = x->n;
+      std::vector<float> h((size_t)m * n);
+      if (!h.empty()) FH_HIP(hipMemcpyAsync(h.data(), x->d.p, sizeof(float) * h.size(), hipMemcpyDeviceToHost, st));
+      FH_HIP(hipStreamSynchronize(st));
+      std::vector<int32_t> rp(m + 1, 0), ci;
+      std::vector<float> v;
+      for (int r = 0; r < m; ++r) {
+        for (int c = 0; c < n; ++c) {
+          const float a = h[(size_t)c * m + r];
+          if (a != 0.0f) {
+            ci.push_back(c);
+            v.push_back(a);
+          }
+        }
+        rp[r + 1] = (int32_t)ci.size();
+      }
+      x->csr_twin = make_csr_from_host(FH_F32, m, n, (int)ci.size(), rp.data(), ci.data(), v.data(), st);
+    }
+    x = x->csr_twin;
+  }
+  return out;
+}
+
 int fh_faust_multiply(fh_faust_t F, char op, const void* X, int p, void* Y, void* stream) {
   FH_API_BEGIN
   FH_CHECK(F && X && Y, "null argument");
@@ -422,7 +452,7 @@ int fh_faust_multiply(fh_faust_t F, char op, const void* X, int p, void* Y, void
   if (F->f.front()->dtype == FH_F64) {
     faust_multiply_f64(F->f, op, (const double*)X, p, (double*)Y, st, F->ws, &F->timers, &F->last_launches);
   } else {
-    F->last_launches = faust_multiply_mixed(F->f, op, X, p, Y, st, F->ws, &F->timers);
+    F->last_launches = faust_multiply_mixed(fp32_chain_factors(F->f, st), op, X, p, Y, st, F->ws, &F->timers);
   }
   FH_API_END
 }

@@ -138,12 +138,13 @@ def test_identity_perm_same_as_none(gpu):
     assert np.array_equal(F1.toarray(), F2.toarray())
 
 
-def test_float32_input(gpu):
+@pytest.mark.parametrize("type", ["bbtree", "right", "left"])
+def test_float32_input(gpu, type):
     rng = np.random.default_rng(9)
     M = rng.standard_normal((64, 64)).astype(np.float32)
-    F = fact.butterfly(M)
+    F = fact.butterfly(M, type=type)
     assert F.dtype == np.float32
-    ref = ob.product(ob.butterfly(M.astype(np.float64), ob.BALANCED))
+    ref = ob.product(ob.butterfly(M.astype(np.float64), _DIR[type]))
     # FP32 factors and an FP32 product: a few float32 roundings per factor
     assert np.linalg.norm(F.toarray() - ref) <= 1e-5 * np.linalg.norm(ref)
 

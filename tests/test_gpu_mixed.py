@@ -295,3 +295,20 @@ def test_csr_fp32_ell_view_same_bits(gpu, monkeypatch, width, op):
         assert np.array_equal(Y1, Y0)
         ref = oracle.chain_multiply(facs, X, op)
         assert np.linalg.norm(Y1 - ref) / np.linalg.norm(ref) < 1e-5
+
+
+@pytest.mark.parametrize("op", ["N", "T"])
+def test_fp32_chain_with_dense_factors(gpu, op):
+    """Dense FP32 factors in an FP32 chain run as CSR twins of themselves."""
+    rng = np.random.default_rng(50)
+    facs = [sp.csr_matrix(rng.standard_normal((96, 80)) * (rng.random((96, 80)) < 0.1)).astype(np.float32)
+            .astype(np.float64), np.asfortranarray(rng.standard_normal((80, 64)).astype(np.float32).astype(np.float64)),
+            sp.csr_matrix(rng.standard_normal((64, 72)) * (rng.random((64, 72)) < 0.1)).astype(np.float32)
+            .astype(np.float64)]
+    F = faust_amd.Faust(facs, dtype=np.float32)
+    for p in (33, 1):
+        rows_in = facs[-1].shape[1] if op == "N" else facs[0].shape[0]
+        X = rng.standard_normal((rows_in, p)).astype(np.float32).astype(np.float64)
+        Y = (F @ X) if op == "N" else (F.T @ X)
+        ref = oracle.chain_multiply(facs, X, op)
+        assert np.linalg.norm(Y - ref) / np.linalg.norm(ref) < 1e-5
