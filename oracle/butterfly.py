@@ -80,7 +80,6 @@ def cec_classes(S1, S2):
     # supports are 0/1, so the reference's comparisons within 1e-6 are
     # equality of patterns: group equal (column of S1, row of S2) pairs,
     # classes in order of their first index, members ascending
-    r = S1.shape[1]
     pat = np.concatenate([S1.T, S2], axis=1)
     _, first, inv = np.unique(pat, axis=0, return_index=True, return_inverse=True)
     inv = np.asarray(inv).reshape(-1)
