@@ -1,5 +1,6 @@
 // fh_api.cpp -- C ABI: matrix handles, products, Faust chains, projectors.
 #include <algorithm>
+#include <cstring>
 #include <vector>
 
 #include "fh_api_common.hpp"
@@ -69,27 +70,29 @@ DMatP make_csr_from_host(int dtype, int m, int n, int nnz, const int32_t* rp, co
   for (int i = 0; i < m; ++i) M->max_row = std::max(M->max_row, rp[i + 1] - rp[i]);
   M->max_col = 0;
   for (int j = 0; j < n; ++j) M->max_col = std::max(M->max_col, cp[j + 1] - cp[j]);
-  if (dtype == FH_F32) {
-    // ELL views for the row-major FP32 gather kernel (rows of S, rows of S^T)
-    auto ell = [&](int nr, const int* ptr, const int* idx, const float* val, int widest, int& w, DBuf& dc, DBuf& dv) {
+  if (dtype == FH_F32 || dtype == FH_F64) {
+    // ELL views for the gather kernels (rows of S, rows of S^T): the widest
+    // row rounded up to the kernel's batch (4 for FP32, 8 for FP64), at most 16
+    const int quant = dtype == FH_F32 ? 4 : 8;
+    auto ell = [&](int nr, const int* ptr, const int* idx, const char* val, int widest, int& w, DBuf& dc, DBuf& dv) {
       w = 0;
       if (nr == 0 || widest > 16) return;
-      w = std::max(4, (widest + 3) / 4 * 4);
+      w = std::max(quant, (widest + quant - 1) / quant * quant);
       std::vector<int> hc((size_t)nr * w, -1);
-      std::vector<float> hv((size_t)nr * w, 0.0f);
+      std::vector<char> hv((size_t)nr * w * esz, 0);
       for (int r = 0; r < nr; ++r)
         for (int e = ptr[r]; e < ptr[r + 1]; ++e) {
           hc[(size_t)r * w + (e - ptr[r])] = idx[e];
-          hv[(size_t)r * w + (e - ptr[r])] = val[e];
+          std::memcpy(hv.data() + ((size_t)r * w + (e - ptr[r])) * esz, val + (size_t)e * esz, esz);
         }
       dc.alloc(sizeof(int) * hc.size());
-      dv.alloc(sizeof(float) * hv.size());
+      dv.alloc(hv.size());
       FH_HIP(hipMemcpyAsync(dc.p, hc.data(), sizeof(int) * hc.size(), hipMemcpyHostToDevice, st));
-      FH_HIP(hipMemcpyAsync(dv.p, hv.data(), sizeof(float) * hv.size(), hipMemcpyHostToDevice, st));
+      FH_HIP(hipMemcpyAsync(dv.p, hv.data(), hv.size(), hipMemcpyHostToDevice, st));
       FH_HIP(hipStreamSynchronize(st));  // the host vectors go out of scope
     };
-    ell(m, rp, ci, (const float*)vals, M->max_row, M->csr_ell_w, M->csr_ell_ci, M->csr_ell_v);
-    ell(n, cp.data(), ri.data(), (const float*)cv.data(), M->max_col, M->csr_tell_w, M->csr_tell_ci, M->csr_tell_v);
+    ell(m, rp, ci, (const char*)vals, M->max_row, M->csr_ell_w, M->csr_ell_ci, M->csr_ell_v);
+    ell(n, cp.data(), ri.data(), cv.data(), M->max_col, M->csr_tell_w, M->csr_tell_ci, M->csr_tell_v);
   }
   FH_HIP(hipStreamSynchronize(st));
   return M;

@@ -15,7 +15,8 @@ namespace fh {
 
 // Y(i,j) = alpha * sum_p v[p] * X(ci[p], j) + beta * Y(i,j)
 // transX: X(k,j) is read as X[j + k*ldx] (op(X) = X^T).
-template <bool TRANSX>
+// EW > 0: ci / v are the matrix's ELL view with EW slots per row.
+template <bool TRANSX, int EW = 0>
 __global__ __launch_bounds__(256) void k_csr_spmm_f64(int nrows, int ncols, const int* __restrict__ rp,
                                                        const int* __restrict__ ci, const double* __restrict__ v,
                                                        const double* __restrict__ X, int ldx,
@@ -23,8 +24,40 @@ __global__ __launch_bounds__(256) void k_csr_spmm_f64(int nrows, int ncols, cons
   const int i = blockIdx.x * 64 + (threadIdx.x & 63);
   const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
   if (i >= nrows || j >= ncols) return;
-  const int b = rp[i], e = rp[i + 1];
   double acc = 0.0;
+  if (EW > 0) {
+    // ELL view (EW slots per row, column -1 in unused slots; rp unused): the
+    // same batches of 8 terms in the same order, without the row pointer
+    const int* ec = ci + (size_t)i * EW;
+    const double* ev = v + (size_t)i * EW;
+    const double* xj = X + (size_t)j * ldx;
+#pragma unroll
+    for (int base = 0; base < EW; base += 8) {
+      int cc[8];
+      double ww[8], xx[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        cc[q] = ec[base + q];
+        ww[q] = ev[base + q];
+      }
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int c = max(cc[q], 0);
+        xx[q] = TRANSX ? X[(size_t)c * ldx + j] : xj[c];
+      }
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const double f = fma(ww[q], xx[q], acc);
+        acc = cc[q] >= 0 ? f : acc;
+      }
+    }
+    const double t = alpha * acc;
+    double* y = Y + (size_t)j * ldy + i;
+    *y = (beta == 0.0) ? t : fma(beta, *y, t);
+    return;
+  }
+  const int b = rp[i], e = rp[i + 1];
   // batches of 8 terms: the index/value loads and the gathers are issued
   // unconditionally (indices clamped inside the row) before the fma chain
   // runs in k order; terms past the row end are not accumulated
@@ -64,6 +97,23 @@ void csr_spmm_f64_x(const DMat& S, char op, const double* X, int ldx, int transX
   const int* ci = tr ? S.ri.as<int>() : S.ci.as<int>();
   const double* v = tr ? S.cv.as<double>() : S.v.as<double>();
   dim3 grid((rows + 63) / 64, (ncols + 3) / 4);
+  const int ew = (S.dtype == FH_F64) ? (tr ? S.csr_tell_w : S.csr_ell_w) : 0;
+  const char* e_ell = getenv("FH_CSR_ELL");
+  if ((ew == 8 || ew == 16) && !(e_ell && e_ell[0] == '0')) {
+    // ELL view: every row's slots at i * ew, no row pointer in the chain
+    const int* eci = tr ? S.csr_tell_ci.as<int>() : S.csr_ell_ci.as<int>();
+    const double* ev = tr ? S.csr_tell_v.as<double>() : S.csr_ell_v.as<double>();
+#define FH_ELL64(T, W) \
+  hipLaunchKernelGGL((k_csr_spmm_f64<T, W>), grid, dim3(256), 0, st, rows, ncols, nullptr, eci, ev, X, ldx, Y, ldy, alpha, beta)
+    if (transX) {
+      if (ew == 8) FH_ELL64(true, 8); else FH_ELL64(true, 16);
+    } else {
+      if (ew == 8) FH_ELL64(false, 8); else FH_ELL64(false, 16);
+    }
+#undef FH_ELL64
+    FH_LAUNCH_CHECK();
+    return;
+  }
   if (transX)
     hipLaunchKernelGGL(k_csr_spmm_f64<true>, grid, dim3(256), 0, st, rows, ncols, rp, ci, v, X, ldx, Y, ldy, alpha,
                        beta);

@@ -249,3 +249,26 @@ def test_spectral_norm_beyond_2048_bitexact(gpu, dense):
     F = faust_amd.Faust(facs)
     nrm, _ = oracle.spectral_norm(facs)
     assert F.norm(2) == nrm
+
+
+@pytest.mark.parametrize("op", ["N", "T"])
+def test_fp64_ell_view_same_bits(gpu, monkeypatch, op):
+    """FP64 CSR factors with rows of at most 16 entries go through their ELL
+    view: the oracle's bits, with and without it (ragged and empty rows)."""
+    rng = np.random.default_rng(61)
+    facs = []
+    for s in [(200, 150), (150, 170), (170, 200)]:
+        a = np.zeros(s)
+        for r in range(s[0]):
+            k = rng.integers(0, 13)
+            a[r, rng.choice(s[1], size=k, replace=False)] = rng.standard_normal(k)
+        facs.append(sp.csr_matrix(a))
+    F = faust_amd.Faust(facs)
+    F._set_mul_mode(1)  # one launch per factor (the fused chain reads CSR)
+    rows_in = facs[-1].shape[1] if op == "N" else facs[0].shape[0]
+    X = np.asfortranarray(rng.standard_normal((rows_in, 37)))
+    ref = oracle.chain_multiply(facs, X, op)
+    for ell in ("1", "0"):
+        monkeypatch.setenv("FH_CSR_ELL", ell)
+        Y = (F @ X) if op == "N" else (F.T @ X)
+        assert np.array_equal(Y, ref), ell
