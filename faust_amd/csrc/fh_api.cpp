@@ -72,11 +72,12 @@ DMatP make_csr_from_host(int dtype, int m, int n, int nnz, const int32_t* rp, co
   for (int j = 0; j < n; ++j) M->max_col = std::max(M->max_col, cp[j + 1] - cp[j]);
   if (dtype == FH_F32 || dtype == FH_F64) {
     // ELL views for the gather kernels (rows of S, rows of S^T): the widest
-    // row rounded up to the kernel's batch (4 for FP32, 8 for FP64), at most 16
-    const int quant = dtype == FH_F32 ? 4 : 8;
+    // row rounded up to the kernel's batch, at most 16 slots for FP32 (in 4s)
+    // and 32 for FP64 (in 8s)
+    const int quant = dtype == FH_F32 ? 4 : 8, wmax = dtype == FH_F32 ? 16 : 32;
     auto ell = [&](int nr, const int* ptr, const int* idx, const char* val, int widest, int& w, DBuf& dc, DBuf& dv) {
       w = 0;
-      if (nr == 0 || widest > 16) return;
+      if (nr == 0 || widest > wmax) return;
       w = std::max(quant, (widest + quant - 1) / quant * quant);
       std::vector<int> hc((size_t)nr * w, -1);
       std::vector<char> hv((size_t)nr * w * esz, 0);

@@ -121,9 +121,9 @@ struct DMat {
   DBuf ell, tell;
   // CSR uploaded from the host (FP32 / FP64): ELL view of the rows (and of
   // the transpose's rows) for the gather kernels, csr_ell_w slots per row
-  // (the widest row rounded up to 4 for FP32, 8 for FP64, at most 16; 0:
-  // none), slot q of row r at r * w + q, unused slots with column -1 and
-  // value 0
+  // (the widest row rounded up to 4 for FP32, at most 16; to 8 for FP64, at
+  // most 32; 0: none), slot q of row r at r * w + q, unused slots with
+  // column -1 and value 0
   int csr_ell_w = 0, csr_tell_w = 0;
   DBuf csr_ell_ci, csr_ell_v, csr_tell_ci, csr_tell_v;
   bool is_identity = false;

@@ -99,16 +99,21 @@ void csr_spmm_f64_x(const DMat& S, char op, const double* X, int ldx, int transX
   dim3 grid((rows + 63) / 64, (ncols + 3) / 4);
   const int ew = (S.dtype == FH_F64) ? (tr ? S.csr_tell_w : S.csr_ell_w) : 0;
   const char* e_ell = getenv("FH_CSR_ELL");
-  if ((ew == 8 || ew == 16) && !(e_ell && e_ell[0] == '0')) {
+  if (ew > 0 && ew % 8 == 0 && ew <= 32 && !(e_ell && e_ell[0] == '0')) {
     // ELL view: every row's slots at i * ew, no row pointer in the chain
     const int* eci = tr ? S.csr_tell_ci.as<int>() : S.csr_ell_ci.as<int>();
     const double* ev = tr ? S.csr_tell_v.as<double>() : S.csr_ell_v.as<double>();
 #define FH_ELL64(T, W) \
   hipLaunchKernelGGL((k_csr_spmm_f64<T, W>), grid, dim3(256), 0, st, rows, ncols, nullptr, eci, ev, X, ldx, Y, ldy, alpha, beta)
-    if (transX) {
-      if (ew == 8) FH_ELL64(true, 8); else FH_ELL64(true, 16);
-    } else {
-      if (ew == 8) FH_ELL64(false, 8); else FH_ELL64(false, 16);
+    switch (ew * 2 + (transX ? 1 : 0)) {
+      case 16: FH_ELL64(false, 8); break;
+      case 17: FH_ELL64(true, 8); break;
+      case 32: FH_ELL64(false, 16); break;
+      case 33: FH_ELL64(true, 16); break;
+      case 48: FH_ELL64(false, 24); break;
+      case 49: FH_ELL64(true, 24); break;
+      case 64: FH_ELL64(false, 32); break;
+      default: FH_ELL64(true, 32); break;
     }
 #undef FH_ELL64
     FH_LAUNCH_CHECK();

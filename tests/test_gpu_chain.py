@@ -251,16 +251,18 @@ def test_spectral_norm_beyond_2048_bitexact(gpu, dense):
     assert F.norm(2) == nrm
 
 
+@pytest.mark.parametrize("widest", [12, 30])
 @pytest.mark.parametrize("op", ["N", "T"])
-def test_fp64_ell_view_same_bits(gpu, monkeypatch, op):
-    """FP64 CSR factors with rows of at most 16 entries go through their ELL
-    view: the oracle's bits, with and without it (ragged and empty rows)."""
-    rng = np.random.default_rng(61)
+def test_fp64_ell_view_same_bits(gpu, monkeypatch, op, widest):
+    """FP64 CSR factors with rows of at most 32 entries go through their ELL
+    view (8 to 32 slots): the oracle's bits, with and without it (ragged and
+    empty rows)."""
+    rng = np.random.default_rng(61 + widest)
     facs = []
     for s in [(200, 150), (150, 170), (170, 200)]:
         a = np.zeros(s)
         for r in range(s[0]):
-            k = rng.integers(0, 13)
+            k = rng.integers(0, widest + 1)
             a[r, rng.choice(s[1], size=k, replace=False)] = rng.standard_normal(k)
         facs.append(sp.csr_matrix(a))
     F = faust_amd.Faust(facs)
