@@ -141,13 +141,17 @@ void faust_multiply_f64(const std::vector<DMatP>& f, char op, const double* X, i
     std::vector<ChainCsrOp> ops;
     int r = in_rows;
     bool ok = true;
+    const char* e_ell = getenv("FH_CSR_ELL");
+    const bool ell_on = !(e_ell && e_ell[0] == '0');
     for (int i = 0; i < J && ok; ++i) {
       const DMat& S = (op == 'N') ? *f[J - 1 - i] : *f[i];
       ok = S.kind == K_CSR && S.dtype == FH_F64 && ((op == 'N') ? S.n : S.m) == r && (op == 'N' || S.has_csc);
       if (!ok) break;
       const bool tr = op != 'N';
+      const int ew = tr ? S.csr_tell_w : S.csr_ell_w;
       ops.push_back({tr ? S.cp.as<int>() : S.rp.as<int>(), tr ? S.ri.as<int>() : S.ci.as<int>(),
-                     tr ? S.cv.as<double>() : S.v.as<double>(), tr ? S.n : S.m, (int)S.nnz});
+                     tr ? S.cv.as<double>() : S.v.as<double>(), tr ? S.n : S.m, (int)S.nnz,
+                     ell_on && ew > 0 && ew % 8 == 0 && ew <= 32 ? 1 : 0});
       r = tr ? S.n : S.m;
     }
     const int wc = ok ? csr_chain_plan_f64(ops, in_rows, p) : 0;

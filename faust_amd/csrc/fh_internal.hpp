@@ -153,15 +153,17 @@ void csr_spmm_f64(const DMat& S, char op, const double* X, int ldx, int ncols, d
 // Whole FP64 CSR chain in one launch when every intermediate column panel
 // fits a workgroup's LDS: Y = alpha * op(S_1) ... op(S_J) X + beta*Y, factors
 // given in application order (the first one applied to X comes first), each
-// op'd factor as (row pointer, column index, value, rows, nnz).  plan returns the
-// columns per workgroup, or 0 when the shape does not fit (then use the
-// factor-by-factor path).
+// op'd factor as (row pointer, column index, value, rows, nnz, whether the
+// factor-by-factor kernel reads it through an ELL view).  plan returns the
+// columns per workgroup, or 0 when the shape does not fit or the
+// factor-by-factor path is the faster (then use that path).
 struct ChainCsrOp {
   const int* rp;
   const int* ci;
   const double* v;
   int rows;
   int nnz;
+  int ell;
 };
 int csr_chain_plan_f64(const std::vector<ChainCsrOp>& ops, int in_rows, int p);
 void csr_chain_run_f64(const std::vector<ChainCsrOp>& ops, int in_rows, int wc, const double* X, int ldx, int p,

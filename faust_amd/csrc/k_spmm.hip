@@ -349,6 +349,11 @@ int csr_chain_plan_f64(const std::vector<ChainCsrOp>& ops, int in_rows, int p) {
   if (mul_mode() != FH_MUL_FUSED && e && e[0] == '0') return 0;
   const bool forced = mul_mode() == FH_MUL_FUSED || (e && e[0] == '1');
   if (J < 2 || J > CHAIN_MAX || p <= 0 || (!forced && J < 6)) return 0;
+  // launches over ELL views (no row pointer in front of the gathers) beat the
+  // one-launch chain at every length measured (5 to 10 factors of 1024^2)
+  bool all_ell = true;
+  for (const auto& o : ops) all_ell = all_ell && o.ell;
+  if (!forced && all_ell) return 0;
   int R = in_rows, max_nnz = 0;
   for (const auto& o : ops) {
     R = std::max(R, o.rows);

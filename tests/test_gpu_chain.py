@@ -143,18 +143,22 @@ def test_fused_chain_falls_back_when_too_tall(gpu):
     assert np.array_equal(Y, oracle.chain_multiply(facs, X, "N"))
 
 
-def test_long_chains_fuse_by_default(gpu, monkeypatch):
-    # the 10-factor Hadamard Faust of order 1024 (2 nnz per row and column) is
-    # one launch by default; the 5-factor chain of 1024^2 factors is not
+def test_long_chains_fuse_without_ell_views(gpu, monkeypatch):
+    # the 10-factor Hadamard Faust of order 1024 (2 nnz per row and column):
+    # one launch per factor through the ELL views by default, one launch for
+    # the whole chain when the views are off; the 5-factor chain of 1024^2
+    # factors is never fused by default.  Same bits every way.
     monkeypatch.delenv("FH_CHAIN_FUSED", raising=False)
     monkeypatch.delenv("FH_CHAIN_WC", raising=False)
     facs = [sp.csr_matrix(f) for f in faust_amd.wht_factors(1024)]
     F = faust_amd.Faust(facs)
     X = np.asfortranarray(np.random.default_rng(7).standard_normal((1024, 33)))
-    for G, op in ((F, "N"), (F.T, "T")):
-        Y = G @ X
-        assert _launches(G) == 1
-        assert np.array_equal(Y, oracle.chain_multiply(facs, X, op))
+    for ell, launches in (("1", 10), ("0", 1)):
+        monkeypatch.setenv("FH_CSR_ELL", ell)
+        for G, op in ((F, "N"), (F.T, "T")):
+            Y = G @ X
+            assert _launches(G) == launches
+            assert np.array_equal(Y, oracle.chain_multiply(facs, X, op))
     F5 = faust_amd.Faust(_cfg1_factors())
     F5 @ X
     assert _launches(F5) == 5
