@@ -119,6 +119,10 @@ struct DMat {
   // ell_w == 0 when the widest block row is too wide for it.
   int ell_w = 0, tell_w = 0;
   DBuf ell, tell;
+  // the blocks again in ELL order: slot (br, q) at (br * ell_w + q) blocks
+  // (zero blocks in empty slots), so a kernel finds a slot's block without
+  // loading the slot first; tebdat likewise for the transpose
+  DBuf ebdat, tebdat;
   // CSR uploaded from the host (FP32 / FP64): ELL view of the rows (and of
   // the transpose's rows) for the gather kernels, csr_ell_w slots per row
   // (the widest row rounded up to 4 for FP32, at most 16; to 8 for FP64, at

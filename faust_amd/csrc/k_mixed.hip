@@ -383,9 +383,10 @@ __global__ __launch_bounds__(256) void k_bsr_shared(int nbr, int p, int W, const
   for (int g = 0; g < W; g += 4) {
     const int4 s01 = slots[g / 2], s23 = slots[g / 2 + 1];
     const int bcol[4] = {s01.y, s01.w, s23.y, s23.w};
-    const int mine = wave == 0 ? s01.x : wave == 1 ? s01.z : wave == 2 ? s23.x : s23.z;
     bf16x8 a_own[PLANES][2], x[4][2];
-    const u16* blk = bdat + (size_t)mine * BLK * PLANES + lane * 8;
+    // bdat is the ELL-ordered block copy: this wave's block of slot group g
+    // needs no slot load
+    const u16* blk = bdat + ((size_t)br * W + g + wave) * BLK * PLANES + lane * 8;
 #pragma unroll
     for (int pl = 0; pl < PLANES; ++pl)
 #pragma unroll
@@ -536,8 +537,9 @@ static void bsr_mul(const DMat& B, char op, const void* X, int ldx, int p, void*
     const int2* ell = (tr ? B.tell : B.ell).as<int2>();
     const int npanels = (p + 31) / 32, pgroups = (npanels + 3) / 4;
     const dim3 grid((unsigned)(pgroups * nbr));
-    bsr_shared_launch(B.split, in_tiled, out_tiled, grid, st, nbr, p, W, ell, dat, (const u16*)X, ldx, (u16*)Y, ldy,
-                      alpha, beta, npanels, pgroups);
+    const u16* edat = tr ? B.tebdat.as<u16>() : B.ebdat.as<u16>();
+    bsr_shared_launch(B.split, in_tiled, out_tiled, grid, st, nbr, p, W, ell, edat, (const u16*)X, ldx, (u16*)Y,
+                      ldy, alpha, beta, npanels, pgroups);
     FH_LAUNCH_CHECK();
     return;
   }
@@ -1169,6 +1171,26 @@ DMatP make_bsr(int dtype, int m, int n, int bnnz, const int32_t* browptr, const 
   M->tell_w = build_ell(nbc, tbrp.data(), tbci.data(), bnnz, tell);
   if (M->ell_w) up(M->ell, ell.data(), sizeof(int32_t) * ell.size());
   if (M->tell_w) up(M->tell, tell.data(), sizeof(int32_t) * tell.size());
+  // blocks in ELL order (a slot's block index is ell[2 * slot])
+  const size_t blk_elems = (size_t)1024 * planes;
+  auto ell_blocks = [&](int nrows, int w, const std::vector<int32_t>& e, const std::vector<u16>& src,
+                        std::vector<u16>& out) {
+    out.assign((size_t)nrows * w * blk_elems, 0);
+    for (size_t slot = 0; slot < (size_t)nrows * w; ++slot)
+      std::copy(src.begin() + (size_t)e[2 * slot] * blk_elems, src.begin() + (size_t)(e[2 * slot] + 1) * blk_elems,
+                out.begin() + slot * blk_elems);
+  };
+  std::vector<u16> edat;
+  if (M->ell_w) {
+    ell_blocks(nbr, M->ell_w, ell, dat, edat);
+    up(M->ebdat, edat.data(), sizeof(u16) * edat.size());
+  }
+  if (M->tell_w) {
+    std::vector<u16> tedat;
+    ell_blocks(nbc, M->tell_w, tell, tdat, tedat);
+    up(M->tebdat, tedat.data(), sizeof(u16) * tedat.size());
+    FH_HIP(hipStreamSynchronize(st));  // tedat goes out of scope
+  }
   FH_HIP(hipStreamSynchronize(st));
   return M;
 }
