@@ -451,11 +451,16 @@ static std::vector<DMatP> fp32_chain_factors(const std::vector<DMatP>& f, hipStr
 
 int fh_faust_multiply(fh_faust_t F, char op, const void* X, int p, void* Y, void* stream) {
   FH_API_BEGIN
-  FH_CHECK(F && X && Y, "null argument");
+  FH_CHECK(F, "null argument");
   MulModeScope mode_scope(F->mul_mode);
   FH_CHECK(op == 'N' || op == 'T' || op == 'H', "op must be N, T or H");
   FH_CHECK(!F->f.empty(), "Faust::multiply: empty Faust");
   FH_CHECK(p >= 0, "negative column count");
+  if (p == 0) {  // nothing to compute (X and Y may be null)
+    F->last_launches = 0;
+    return FH_OK;
+  }
+  FH_CHECK(X && Y, "null argument");
   hipStream_t st = (hipStream_t)stream;
   if (F->f.front()->dtype == FH_F64) {
     faust_multiply_f64(F->f, op, (const double*)X, p, (double*)Y, st, F->ws, &F->timers, &F->last_launches);

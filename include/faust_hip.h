@@ -104,7 +104,8 @@ int fh_faust_free(fh_faust_t F);
  * TransformHelper<FPP,Cpu>::multiply(const FPP*, int, FPP*)
  * (faust_TransformHelper.hpp:407-434) -> Transform::multiply
  * (faust_Transform.hpp:1714-1812): right-to-left for 'N', left-to-right with
- * per-factor op for 'T'/'H', two ping-pong intermediates. */
+ * per-factor op for 'T'/'H', two ping-pong intermediates.  p = 0 is a no-op
+ * (X and Y may then be null). */
 int fh_faust_multiply(fh_faust_t F, char op, const void *X, int p, void *Y, void *stream);
 /* Number of product kernels the last fh_faust_multiply on F launched (one
  * per factor, plus the input transpose of FP32 CSR chains).  No counterpart

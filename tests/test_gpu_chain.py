@@ -278,3 +278,36 @@ def test_fp64_ell_view_same_bits(gpu, monkeypatch, op, widest):
         monkeypatch.setenv("FH_CSR_ELL", ell)
         Y = (F @ X) if op == "N" else (F.T @ X)
         assert np.array_equal(Y, ref), ell
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_edge_shapes(gpu, dtype):
+    """A factor with no nonzeros, a single-factor Faust, a 1-row factor and
+    X with no columns, in FP64 and FP32 chains, N and T."""
+    rng = np.random.default_rng(70)
+    tol = 0.0 if dtype == np.float64 else 1e-5
+
+    def check(facs):
+        F = faust_amd.Faust(facs, dtype=dtype)
+        for op in ("N", "T"):
+            G = F if op == "N" else F.T
+            rows_in = facs[-1].shape[1] if op == "N" else facs[0].shape[0]
+            for p in (0, 1, 7):
+                X = rng.standard_normal((rows_in, p))
+                if dtype == np.float32:
+                    X = X.astype(np.float32).astype(np.float64)
+                Y = G @ X
+                ref = oracle.chain_multiply(facs, np.asfortranarray(X), op) if p else np.zeros((G.shape[0], 0))
+                assert Y.shape == ref.shape
+                if tol == 0.0:
+                    assert np.array_equal(Y, ref)
+                else:
+                    assert np.linalg.norm(Y - ref) <= tol * max(np.linalg.norm(ref), 1e-30)
+
+    a = sp.csr_matrix(rng.standard_normal((30, 20)) * (rng.random((30, 20)) < 0.2)).astype(dtype).astype(np.float64)
+    b = sp.csr_matrix(rng.standard_normal((20, 25)) * (rng.random((20, 25)) < 0.2)).astype(dtype).astype(np.float64)
+    zero = sp.csr_matrix((20, 20))
+    one_row = sp.csr_matrix(rng.standard_normal((1, 30)) * (rng.random((1, 30)) < 0.5)).astype(dtype).astype(np.float64)
+    check([a, zero, b])      # an all-zero factor: the product is zero
+    check([a])               # one factor
+    check([one_row, a, b])   # a 1 x 30 first factor

@@ -312,3 +312,20 @@ def test_fp32_chain_with_dense_factors(gpu, op):
         Y = (F @ X) if op == "N" else (F.T @ X)
         ref = oracle.chain_multiply(facs, X, op)
         assert np.linalg.norm(Y - ref) / np.linalg.norm(ref) < 1e-5
+
+
+@pytest.mark.parametrize("dtype", ["bf16", np.float32])
+def test_bsr_chain_with_empty_factor(gpu, dtype):
+    """A BSR factor with no blocks (every ELL slot a zero block) in a chain:
+    the product is zero; with it removed the chain matches the oracle."""
+    rng = np.random.default_rng(71)
+    A = rand_bsr(256, 256, 32, 3, rng)
+    Z = sp.bsr_matrix((np.zeros((0, 32, 32)), np.zeros(0, dtype=np.int32), np.zeros(9, dtype=np.int32)),
+                      shape=(256, 256))
+    X = rng.standard_normal((256, 64))
+    F = faust_amd.Faust([A, Z, A], dtype=dtype)
+    assert np.all(F @ X == 0.0)
+    G = faust_amd.Faust([A, A], dtype=dtype)
+    ref = oracle.chain_multiply([sp.csr_matrix(bf16_bits_to_f32(f32_to_bf16_bits(A.toarray())).astype(np.float64))
+                                 if dtype == "bf16" else sp.csr_matrix(A)] * 2, np.asfortranarray(X), "N")
+    assert np.linalg.norm(G @ X - ref) <= 2e-2 * np.linalg.norm(ref)  # BF16 activations
