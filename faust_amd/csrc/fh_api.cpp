@@ -332,8 +332,11 @@ int fh_mat_free(fh_mat_t M) {
 int fh_mat_mul_dense(fh_mat_t M, char op, const void* X, int ldx, int ncols, void* Y, int ldy, const void* alpha,
                      const void* beta, void* stream) {
   FH_API_BEGIN
-  FH_CHECK(M && X && Y, "null argument");
+  FH_CHECK(M, "null argument");
   FH_CHECK(op == 'N' || op == 'T' || op == 'H', "op must be N, T or H");
+  FH_CHECK(ncols >= 0, "negative column count");
+  if (ncols == 0) return FH_OK;  // nothing to compute (X and Y may be null)
+  FH_CHECK(X && Y, "null argument");
   hipStream_t st = (hipStream_t)stream;
   const DMat& S = *M->m;
   if (S.dtype == FH_F64) {
