@@ -281,13 +281,23 @@ __device__ __forceinline__ bool nz_val(double x) { return x != 0.0; }
 __device__ __forceinline__ bool nz_val(float x) { return x != 0.0f; }
 __device__ __forceinline__ bool nz_val(uint16_t x) { return (x & 0x7fffu) != 0; }  // BF16 bits: +-0 are zero
 
+// grid-stride count: per-thread counts, a block sum in LDS, one atomic per
+// workgroup (an atomic per wave on one counter serialised ~16K atomics for a
+// 1024^2 matrix)
 template <class T>
-__global__ void k_count_nnz(const T* __restrict__ D, int64_t n, unsigned long long* __restrict__ out) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  int c = (i < n && nz_val(D[i])) ? 1 : 0;
-  // wave-level ballot then one atomic per wave
-  unsigned long long b = __ballot(c);
-  if ((threadIdx.x & 63) == 0 && b) atomicAdd(out, (unsigned long long)__popcll(b));
+__global__ __launch_bounds__(256) void k_count_nnz(const T* __restrict__ D, int64_t n,
+                                                   unsigned long long* __restrict__ out) {
+  __shared__ unsigned int part[256];
+  unsigned int c = 0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    c += nz_val(D[i]) ? 1u : 0u;
+  part[threadIdx.x] = c;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) part[threadIdx.x] += part[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && part[0]) atomicAdd(out, (unsigned long long)part[0]);
 }
 
 int64_t count_nnz_f64(const double* D, int64_t n, hipStream_t st) { return count_nnz(D, FH_F64, n, st); }
@@ -296,7 +306,7 @@ int64_t count_nnz(const void* D, int dtype, int64_t n, hipStream_t st) {
   if (n <= 0) return 0;
   DBuf c(sizeof(unsigned long long));
   FH_HIP(hipMemsetAsync(c.p, 0, sizeof(unsigned long long), st));
-  const dim3 grid((unsigned)((n + 255) / 256));
+  const dim3 grid((unsigned)std::min<int64_t>(1024, (n + 255) / 256));
   if (dtype == FH_F64)
     hipLaunchKernelGGL(k_count_nnz<double>, grid, dim3(256), 0, st, (const double*)D, n, c.as<unsigned long long>());
   else if (dtype == FH_F32)
