@@ -1,7 +1,7 @@
 """Benchmark of the Faust x dense hot path (and PALM4MSA) on one MI355X.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME] [--no-cpu-baseline] [--no-palm]
-               [--no-butterfly] [--hadamard]
+               [--no-butterfly] [--no-notebook] [--hadamard]
 
 Prints ONE JSON line.  `value` is the algorithmic HBM throughput (GB/s) of
 one step = one Faust x dense product (all factors) with inputs resident in
@@ -15,7 +15,9 @@ PALM4MSA (FP32 + FP64 frozen-support refinement, HIP graphs, autotuned
 tiles) with the plain FP64 GPU form on 512 targets, at the FP64-verified
 relative error each reaches.  `butterfly` times the butterfly factorization
 (fact.butterfly, host matrix in, Faust out) at orders 1024 / 4096 / 16384 on
-Gaussian matrices, next to the oracle at order 1024.
+Gaussian matrices, next to the oracle at order 1024.  `notebook` times
+F.toarray() and F.norm(2) on the 1024x1024 x10 Fausts whose GPU timings the
+reference publishes.
 """
 import argparse
 import ctypes as C
@@ -43,6 +45,8 @@ def _parse():
     ap.add_argument("--no-palm", action="store_true",
                     help="skip the batched PALM4MSA comparison (512 targets, tuned vs plain FP64)")
     ap.add_argument("--no-butterfly", action="store_true", help="skip the butterfly factorization timings")
+    ap.add_argument("--no-notebook", action="store_true",
+                    help="skip F.toarray() / F.norm(2) on the reference notebook's 1024x1024 x10 Fausts")
     ap.add_argument("--butterfly-init", type=int, default=0, metavar="NHOLD",
                     help="also run PALM4MSA on NHOLD 256x256 butterfly targets from the reference start and from "
                          "the butterfly factorization")
@@ -261,6 +265,31 @@ def butterfly_measure(cpu=True):
     return out
 
 
+def notebook_cases():
+    """The reference's published GPU timings (gen_doc/using_gpu_pyfaust.ipynb,
+    Tesla V100) of F.toarray() and F.norm(2) on pyfaust.rand(1024, 1024,
+    num_factors=10, fac_type='dense') and fac_type='sparse', density=.2, FP64:
+    the same calls on Fausts of the same shapes, median of 20 after 3 warm-ups,
+    host array out as in pyfaust."""
+    import torch
+    import faust_amd
+    out = {}
+    for name, kw in (("dense", dict(fac_type="dense")), ("sparse_d0.2", dict(fac_type="sparse", density=0.2))):
+        F = faust_amd.rand(1024, 1024, num_factors=10, seed=0, **kw)
+        for what, fn in (("toarray_ms", F.toarray), ("norm2_ms", lambda: F.norm(2))):
+            ts = []
+            for _ in range(23):
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                fn()
+                torch.cuda.synchronize()
+                ts.append(time.perf_counter() - t0)
+            out[f"{name}_{what}"] = round(float(np.median(ts[3:])) * 1e3, 3)
+    out["reference_v100_ms"] = {"dense_toarray_ms": 6.82, "dense_norm2_ms": 6.85, "sparse_d0.2_toarray_ms": 13.8,
+                                "sparse_d0.2_norm2_ms": 9.86}
+    return out
+
+
 # ----------------------------------------------------------------- helpers
 def measured_traffic(wl):
     """HBM-side bytes per launch of the dominant kernel from the rocprofv3
@@ -428,6 +457,8 @@ def main():
         out["butterfly"] = butterfly_measure(cpu=not args.no_cpu_baseline)
     if args.hadamard:
         out["palm_hadamard"] = palm_hadamard()
+    if not args.no_notebook:
+        out["notebook"] = notebook_cases()
     if args.butterfly_init:
         from faust_amd import warmstart
         out["butterfly_init"] = warmstart.butterfly_init_experiment(
