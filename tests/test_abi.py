@@ -48,3 +48,16 @@ def test_argument_errors_without_gpu():
     L = _lib.lib()
     assert L.fh_faust_size(None, None) == _lib.FH_ERR_ARG
     assert b"null" in L.fh_last_error()
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py runs on one GPU: --gpus 2 exits with a message, before any GPU work."""
+    import json
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode != 0
+    msg = json.loads(r.stdout.strip().splitlines()[-1])
+    assert "one GPU" in msg["error"]
