@@ -13,7 +13,10 @@
 // ~2^-16 relative per value, below the BF16 rounding of the activations).
 // Default kernel (k_bsr_shared): a workgroup owns one block-row x 128
 // columns, one 32-column tile per wave; the row's blocks go through LDS once
-// per workgroup and the four waves gather their X tiles.  Consecutive
+// per workgroup and the four waves gather their X tiles.  The blocks are read
+// from a copy stored in ELL order (slot q of block row br at block
+// br * W + q), so the block loads do not wait for the slot loads that give
+// the X block columns.  Consecutive
 // workgroup ids walk the column groups first, so with 512 columns each XCD
 // keeps one 128-column group (and half of the block rows) and its L2 serves
 // the repeated gathers of that group.  Other forms (one or more tiles per
@@ -21,7 +24,9 @@
 // wide for the ELL slots) are selectable.
 //
 // FP32 CSR: intermediates are row-major inside the chain so that the
-// gather of one X row segment is a contiguous 128-byte read.  A workgroup
+// gather of one X row segment is a contiguous 128-byte read.  Factors with
+// rows of at most 16 entries are read through their ELL view (index/value
+// pairs at r * w, no row pointer in front of the gathers).  A workgroup
 // computes 32 rows x 32 columns (8 lanes x float4 per row); panels of 32
 // columns cycle over the XCDs the same way.  The chain input is transposed
 // once into row-major and the last factor writes column-major through LDS.

@@ -5,7 +5,10 @@
 // 64 consecutive output rows of one output column, so the Y stores and (for
 // dense x sparse) the A loads are coalesced; X gathers X[colind, j] stay in
 // one column.  op(S) = S^T walks the CSC twin of S, whose rows are ascending
-// inside each column, so the k order is the same as the oracle's.
+// inside each column, so the k order is the same as the oracle's.  CSR
+// factors uploaded from the host with rows of at most 32 entries (and their
+// transposes) are also kept as ELL views, read without the row pointer; the
+// terms and their order are unchanged.
 #include <algorithm>
 #include <cstdlib>
 
