@@ -141,34 +141,13 @@ std::vector<BNodeP> bfly_directional(Bfly& B, bool right) {
   return facs;
 }
 
-// host CSR of a leaf [i, i+1) (2 entries per row), its columns relabelled by
-// colmap (nullptr: kept) and sorted within each row
-DMatP leaf_csr(const BNode& nd, int n, int L, const std::vector<int>* colmap, hipStream_t st) {
-  std::vector<double> v(2 * (size_t)n);
-  FH_HIP(hipMemcpyAsync(v.data(), nd.v.p, sizeof(double) * v.size(), hipMemcpyDeviceToHost, st));
-  FH_HIP(hipStreamSynchronize(st));
-  const int s = 1 << (L - nd.b);
-  std::vector<int32_t> rp(n + 1), ci(2 * (size_t)n);
-  std::vector<double> vv(2 * (size_t)n);
-  for (int r = 0; r < n; ++r) {
-    rp[r] = 2 * r;
-    int c0 = (r & ~(2 * s - 1)) + (r & (s - 1)), c1 = c0 + s;
-    double v0 = v[2 * (size_t)r], v1 = v[2 * (size_t)r + 1];
-    if (colmap) {
-      c0 = (*colmap)[c0];
-      c1 = (*colmap)[c1];
-      if (c1 < c0) {
-        std::swap(c0, c1);
-        std::swap(v0, v1);
-      }
-    }
-    ci[2 * (size_t)r] = c0;
-    ci[2 * (size_t)r + 1] = c1;
-    vv[2 * (size_t)r] = v0;
-    vv[2 * (size_t)r + 1] = v1;
-  }
-  rp[n] = 2 * n;
-  return make_csr_from_host(FH_F64, n, n, 2 * n, rp.data(), ci.data(), vv.data(), st);
+// CSR of a leaf [i, i+1) (2 entries per row), built on the device; with
+// colmap / colinv (device) its columns are relabelled by colmap and sorted
+// within each row
+DMatP leaf_csr(const BNode& nd, int n, int L, const int* colmap, const int* colinv, hipStream_t st) {
+  auto M = std::make_shared<DMat>();
+  bfly_leaf_csr((const double*)nd.v.p, n, L - nd.b, colmap, colinv, *M, st);
+  return M;
 }
 
 DMatP leaf_dense(const BNode& nd, int n, int L, hipStream_t st) {
@@ -177,10 +156,11 @@ DMatP leaf_dense(const BNode& nd, int n, int L, hipStream_t st) {
   M->dtype = FH_F64;
   M->m = n;
   M->n = n;
-  M->nnz = -1;
   M->d.alloc(sizeof(double) * (size_t)n * n);
   FH_HIP(hipMemsetAsync(M->d.p, 0, M->d.bytes, st));
   bfly_scatter_dense((const double*)nd.v.p, n, L - nd.b, (double*)M->d.p, st);
+  // the 2n node values land in distinct entries: their nonzeros are the factor's
+  M->nnz = count_nnz(nd.v.p, FH_F64, 2 * (int64_t)n, st);
   return M;
 }
 
@@ -211,7 +191,7 @@ fh_faust_s* butterfly_run(const double* A, size_t lda, int row_major, int n, int
   }
   // A P^T: column i of A becomes column perm[i], i.e. column c is A's column pinv[c]
   std::vector<int> pinv;
-  DBuf pinv_dev;
+  DBuf pinv_dev, perm_dev;
   if (perm) {
     pinv.assign(n, -1);
     for (int i = 0; i < n; ++i) {
@@ -220,6 +200,8 @@ fh_faust_s* butterfly_run(const double* A, size_t lda, int row_major, int n, int
     }
     pinv_dev.alloc(sizeof(int) * n);
     FH_HIP(hipMemcpyAsync(pinv_dev.p, pinv.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
+    perm_dev.alloc(sizeof(int) * n);
+    FH_HIP(hipMemcpyAsync(perm_dev.p, perm, sizeof(int) * n, hipMemcpyHostToDevice, st));
   }
   B.root = std::make_shared<BNode>();
   B.root->a = 0;
@@ -235,9 +217,9 @@ fh_faust_s* butterfly_run(const double* A, size_t lda, int row_major, int n, int
     const bool last = k == B.L - 1;
     if (last && perm && mul_perm) {
       // (F P)[:, c] = F[:, perm[c]]: column j of the last factor moves to pinv[j]
-      r->f.push_back(leaf_csr(*nodes[k], n, B.L, &pinv, st));
+      r->f.push_back(leaf_csr(*nodes[k], n, B.L, (const int*)pinv_dev.p, (const int*)perm_dev.p, st));
     } else if (dir == FH_BUTTERFLY_BBTREE) {
-      r->f.push_back(leaf_csr(*nodes[k], n, B.L, nullptr, st));
+      r->f.push_back(leaf_csr(*nodes[k], n, B.L, nullptr, nullptr, st));
     } else {
       r->f.push_back(leaf_dense(*nodes[k], n, B.L, st));
     }

@@ -233,6 +233,11 @@ struct BflyStep {
 void bfly_split(const BflyStep& s, hipStream_t st);
 bool bfly_block_fits(int logBx, int logBy);
 void bfly_scatter_dense(const double* v, int n, int logs, double* out, hipStream_t st);
+// CSR leaf factor (CSC twin and ELL views included) from a 2-entry-per-row
+// node on the device; colmap / colinv (device, both or neither): relabel
+// column c as colmap[c]
+void bfly_leaf_csr(const double* v, int n, int logs, const int* colmap, const int* colinv, DMat& out,
+                   hipStream_t st);
 // root node (row-major n x n) = A with columns permuted, root(r, c) = A(r, pinv[c]); A column-major (ld lda)
 // or row-major (row_major, ld lda), device memory; pinv_dev device or nullptr
 void bfly_root(const double* A, size_t lda, int row_major, int n, const int* pinv_dev, double* out, hipStream_t st);

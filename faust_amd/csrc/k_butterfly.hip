@@ -21,6 +21,7 @@
 // All arithmetic is FP64.  Singular vectors are defined up to sign; the sign
 // taken here is whatever the computation leaves.
 #include <cfloat>
+#include <cstdlib>
 
 #include "fh_internal.hpp"
 
@@ -281,6 +282,142 @@ __global__ __launch_bounds__(BF_THREADS) void k_bfly_gram(BflyStep s) {
   for (int j = tid; j < d; j += BF_THREADS) bf_put_col(s, tr, t, j, M[j * d + jb] * vs * kc);
 }
 
+// The same computation with the d x d products on FP64 MFMA
+// (v_mfma_f64_16x16x4f64; d = 16..128).  G and every power of it are
+// symmetric, so only the 16 x 16 tiles (ti, tj) with ti <= tj are formed
+// (wave w takes the tiles w, w + 4, ... of that list, NT at most) and each
+// off-diagonal tile is also stored transposed.  Operand lanes: A[i][k] in
+// lane i + 16 k, B[k][j] in lane j + 16 k; output register r of lane l is
+// D[l / 16 + 4 r][l % 16].  Both operands of a square are read from rows of
+// M (M symmetric: column i of M is row i).  M is row-major with row stride
+// d + 1, so the transposed stores of the 16 lanes of a quarter-wave hit 16
+// different banks.
+typedef double bf_d4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ void bf_store_tile(double* M, int ld, int ti, int tj, int li, int lk, const bf_d4& c,
+                                              double f) {
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const double v = c[r] * f;
+    M[(16 * ti + lk + 4 * r) * ld + 16 * tj + li] = v;
+    if (ti != tj) M[(16 * tj + li) * ld + 16 * ti + lk + 4 * r] = v;
+  }
+}
+
+template <int NT>
+__global__ __launch_bounds__(BF_THREADS) void k_bfly_gram_mfma(BflyStep s) {
+  extern __shared__ double lds[];
+  __shared__ double red[BF_THREADS / 64];
+  const int t = bf_block_index(blockIdx.x, gridDim.x), tid = threadIdx.x;
+  const int wave = tid >> 6, lane = tid & 63, li = lane & 15, lk = lane >> 4;
+  const bool tr = s.logBx < s.logBy;
+  const int logD = tr ? s.logBy : s.logBx, logd = tr ? s.logBx : s.logBy;
+  const int D = 1 << logD, d = 1 << logd, ld = d + 1;
+  double* M = lds;           // d x d, row stride ld
+  double* B = lds + d * ld;  // 16 x d staged rows of W0
+  const int td = d >> 4, nup = td * (td + 1) / 2;
+  // this wave's upper tiles
+  int tI[NT], tJ[NT];
+#pragma unroll
+  for (int q = 0; q < NT; ++q) {
+    int u = wave + 4 * q, ti = 0;
+    while (ti < td && u >= td - ti) {
+      u -= td - ti;
+      ++ti;
+    }
+    tI[q] = ti;
+    tJ[q] = ti + u;
+  }
+  bf_d4 acc[NT];
+#pragma unroll
+  for (int q = 0; q < NT; ++q) acc[q] = bf_d4{0.0, 0.0, 0.0, 0.0};
+  // G = W0^T W0: G[p][q] = sum over the rows r of W0 of B[r][p] B[r][q]
+  for (int i0 = 0; i0 < D; i0 += 16) {
+    for (int e = tid; e < 16 * d; e += BF_THREADS) {
+      const int r = e >> logd, j = e & (d - 1);
+      B[r * d + j] = bf_w0(s, tr, t, j, i0 + r);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NT; ++q) {
+      if (wave + 4 * q < nup) {
+#pragma unroll
+        for (int ks = 0; ks < 4; ++ks) {
+          const double* row = B + (4 * ks + lk) * d;
+          acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(row[16 * tI[q] + li], row[16 * tJ[q] + li], acc[q], 0, 0, 0);
+        }
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int q = 0; q < NT; ++q)
+    if (wave + 4 * q < nup) bf_store_tile(M, ld, tI[q], tJ[q], li, lk, acc[q], 1.0);
+  __syncthreads();
+  double tau = 0.0;
+  for (int j = 0; j < d; ++j) tau += M[j * ld + j];
+  if (tau > 0.0) {
+    double scale = 1.0 / tau;  // M holds the current matrix / scale
+    for (int k = 0; k < BF_MAX_SQ; ++k) {
+      const double sc2 = scale * scale;
+#pragma unroll
+      for (int q = 0; q < NT; ++q) {
+        if (wave + 4 * q < nup) {
+          const double* ca = M + 16 * tI[q] + li;
+          const double* cb = M + 16 * tJ[q] + li;
+          bf_d4 c = {0.0, 0.0, 0.0, 0.0};
+          for (int kk = 0; kk < d; kk += 4) {
+            const int o = (kk + lk) * ld;
+            c = __builtin_amdgcn_mfma_f64_16x16x4f64(ca[o], cb[o], c, 0, 0, 0);
+          }
+          acc[q] = c;
+        }
+      }
+      __syncthreads();  // every tile of the square is formed before M is overwritten
+#pragma unroll
+      for (int q = 0; q < NT; ++q)
+        if (wave + 4 * q < nup) bf_store_tile(M, ld, tI[q], tJ[q], li, lk, acc[q], sc2);
+      __syncthreads();
+      double tr2 = 0.0;
+      for (int j = 0; j < d; ++j) tr2 += M[j * ld + j];
+      scale = 1.0 / tr2;
+      if (tr2 >= 1.0 - 16.0 * DBL_EPSILON) break;  // tr(M^2) = 1: M is rank one
+    }
+  }
+  __syncthreads();
+  // v: the column of largest diagonal entry, normalised
+  int jb = 0;
+  for (int j = 1; j < d; ++j)
+    if (M[j * ld + j] > M[jb * ld + jb]) jb = j;
+  double vn = 0.0;
+  for (int j = 0; j < d; ++j) vn = fma(M[j * ld + jb], M[j * ld + jb], vn);
+  const double vs = vn > 0.0 ? 1.0 / sqrt(vn) : 0.0;
+  // w = W0 v, sigma = |w|
+  double part = 0.0;
+  for (int i = tid; i < D; i += BF_THREADS) {
+    double wi = 0.0;
+    for (int j = 0; j < d; ++j) wi = fma(bf_w0(s, tr, t, j, i), M[j * ld + jb] * vs, wi);
+    part = fma(wi, wi, part);
+  }
+  part = wave_sum(part);
+  if (lane == 0) red[wave] = part;
+  __syncthreads();
+  double lam = 0.0;
+  for (int q = 0; q < BF_THREADS / 64; ++q) lam += red[q];
+  double kl = 0.0, kc = 0.0;
+  if (lam > 0.0 && vn > 0.0) {
+    const double sigma = sqrt(lam), rs = sqrt(sigma);
+    kl = 1.0 / rs;
+    kc = rs;
+  }
+  for (int i = tid; i < D; i += BF_THREADS) {
+    double wi = 0.0;
+    for (int j = 0; j < d; ++j) wi = fma(bf_w0(s, tr, t, j, i), M[j * ld + jb] * vs, wi);
+    bf_put_len(s, tr, t, i, wi * kl);
+  }
+  for (int j = tid; j < d; j += BF_THREADS) bf_put_col(s, tr, t, j, M[j * ld + jb] * vs * kc);
+}
+
 // The root node (the whole matrix, row-major: value (r, c) at r n + c) from
 // the caller's matrix with its columns permuted: root(r, c) = A(r, pinv[c])
 // (pinv == nullptr: no permutation).  Column-major A goes through 32 x 32
@@ -324,6 +461,90 @@ __global__ void k_bfly_scatter(const double* v, int n, int logs, double* out) {
   out[(size_t)r + (size_t)(c0 + (1 << logs)) * n] = v[2 * (size_t)r + 1];
 }
 
+// CSR (with its CSC twin and both 8-slot ELL views) of a 2-entry-per-row
+// leaf node, built in place: row r holds v[2r] at column c0(r) and v[2r + 1]
+// at c0(r) + s, c0(r) = (r & ~(2s - 1)) + (r & (s - 1)).  The support is
+// symmetric, so column c (before relabelling) holds rows c0(c) and c0(c) + s,
+// and row i's value there is v[2i + ((c & s) != 0)].  With colmap (a
+// permutation, colinv its inverse) old column c becomes column colmap[c];
+// each row's two entries are then sorted by their new column.
+__global__ void k_bfly_leaf_csr(const double* v, int n, int logs, const int* colmap, const int* colinv, int* rp,
+                                int* ci, double* cv_row, int* cp, int* ri, double* cv_col, int* ell_ci,
+                                double* ell_v, int* tell_ci, double* tell_v) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const int s = 1 << logs;
+  // row r
+  int c0 = (r & ~(2 * s - 1)) + (r & (s - 1)), c1 = c0 + s;
+  double v0 = v[2 * (size_t)r], v1 = v[2 * (size_t)r + 1];
+  if (colmap) {
+    c0 = colmap[c0];
+    c1 = colmap[c1];
+    if (c1 < c0) {
+      const int tc = c0;
+      c0 = c1;
+      c1 = tc;
+      const double tv = v0;
+      v0 = v1;
+      v1 = tv;
+    }
+  }
+  rp[r] = 2 * r;
+  ci[2 * (size_t)r] = c0;
+  ci[2 * (size_t)r + 1] = c1;
+  cv_row[2 * (size_t)r] = v0;
+  cv_row[2 * (size_t)r + 1] = v1;
+  // column r (new label)
+  const int oc = colinv ? colinv[r] : r;
+  const int rb = (oc & ~(2 * s - 1)) + (oc & (s - 1));
+  const int slot = (oc & s) ? 1 : 0;
+  const double w0 = v[2 * (size_t)rb + slot], w1 = v[2 * (size_t)(rb + s) + slot];
+  cp[r] = 2 * r;
+  ri[2 * (size_t)r] = rb;
+  ri[2 * (size_t)r + 1] = rb + s;
+  cv_col[2 * (size_t)r] = w0;
+  cv_col[2 * (size_t)r + 1] = w1;
+  for (int q = 0; q < 8; ++q) {
+    const size_t e = (size_t)r * 8 + q;
+    ell_ci[e] = q == 0 ? c0 : q == 1 ? c1 : -1;
+    ell_v[e] = q == 0 ? v0 : q == 1 ? v1 : 0.0;
+    tell_ci[e] = q == 0 ? rb : q == 1 ? rb + s : -1;
+    tell_v[e] = q == 0 ? w0 : q == 1 ? w1 : 0.0;
+  }
+  if (r == 0) {
+    rp[n] = 2 * n;
+    cp[n] = 2 * n;
+  }
+}
+
+void bfly_leaf_csr(const double* v, int n, int logs, const int* colmap, const int* colinv, DMat& M, hipStream_t st) {
+  FH_CHECK(n >= 4 && (n & (n - 1)) == 0 && logs >= 0 && (2 << logs) <= n, "butterfly: bad leaf node");
+  FH_CHECK((colmap == nullptr) == (colinv == nullptr), "butterfly: leaf relabelling needs both maps");
+  const size_t nz = 2 * (size_t)n, ne = 8 * (size_t)n;
+  M.kind = K_CSR;
+  M.dtype = FH_F64;
+  M.m = M.n = n;
+  M.nnz = (int64_t)nz;
+  M.rp.alloc(sizeof(int) * (n + 1));
+  M.ci.alloc(sizeof(int) * nz);
+  M.v.alloc(sizeof(double) * nz);
+  M.cp.alloc(sizeof(int) * (n + 1));
+  M.ri.alloc(sizeof(int) * nz);
+  M.cv.alloc(sizeof(double) * nz);
+  M.has_csc = true;
+  M.max_row = M.max_col = 2;
+  M.csr_ell_w = M.csr_tell_w = 8;  // the FP64 gather kernels' slot quantum
+  M.csr_ell_ci.alloc(sizeof(int) * ne);
+  M.csr_ell_v.alloc(sizeof(double) * ne);
+  M.csr_tell_ci.alloc(sizeof(int) * ne);
+  M.csr_tell_v.alloc(sizeof(double) * ne);
+  hipLaunchKernelGGL(k_bfly_leaf_csr, dim3((n + 255) / 256), dim3(256), 0, st, v, n, logs, colmap, colinv,
+                     M.rp.as<int>(), M.ci.as<int>(), M.v.as<double>(), M.cp.as<int>(), M.ri.as<int>(),
+                     M.cv.as<double>(), M.csr_ell_ci.as<int>(), M.csr_ell_v.as<double>(), M.csr_tell_ci.as<int>(),
+                     M.csr_tell_v.as<double>());
+  FH_LAUNCH_CHECK();
+}
+
 static size_t bfly_lds_budget() {
   static size_t budget = 0;
   if (budget) return budget;
@@ -332,19 +553,32 @@ static size_t bfly_lds_budget() {
   if (hipGetDevice(&dev) == hipSuccess &&
       hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && v > 64 * 1024) {
     const int want = v - 1024;
-    if (hipFuncSetAttribute((const void*)k_bfly_gram<4>, hipFuncAttributeMaxDynamicSharedMemorySize, want) ==
-            hipSuccess &&
-        hipFuncSetAttribute((const void*)k_bfly_gram<8>, hipFuncAttributeMaxDynamicSharedMemorySize, want) ==
-            hipSuccess)
-      b = (size_t)want;
+    const void* fns[] = {(const void*)k_bfly_gram<4>, (const void*)k_bfly_gram<8>,
+                         (const void*)k_bfly_gram_mfma<1>, (const void*)k_bfly_gram_mfma<3>,
+                         (const void*)k_bfly_gram_mfma<9>};
+    bool ok = true;
+    for (const void* f : fns)
+      ok = ok && hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, want) == hipSuccess;
+    if (ok) b = (size_t)want;
   }
   (void)hipGetLastError();  // a refused attribute leaves the 64 KiB budget, not an error
   budget = b;
   return budget;
 }
 
+// d x d products on MFMA for d >= 16 (FH_BFLY_MFMA=0: the VALU kernel)
+static bool bfly_use_mfma(int logd) {
+  static const int env = [] {
+    const char* e = getenv("FH_BFLY_MFMA");
+    return e && e[0] == '0' ? 0 : 1;
+  }();
+  return env && logd >= 4;
+}
+
 size_t bfly_block_lds_bytes(int logBx, int logBy) {
-  const size_t d = (size_t)1 << std::min(logBx, logBy), D = (size_t)1 << std::max(logBx, logBy);
+  const int logd = std::min(logBx, logBy);
+  const size_t d = (size_t)1 << logd, D = (size_t)1 << std::max(logBx, logBy);
+  if (bfly_use_mfma(logd)) return sizeof(double) * (d * (d + 1) + 16 * d);  // padded M and 16 staged rows
   return sizeof(double) * (d * d + std::min<size_t>(BF_CHUNK, D) * d);  // M and the staged rows
 }
 
@@ -366,7 +600,16 @@ void bfly_split(const BflyStep& s, hipStream_t st) {
   } else {
     FH_CHECK(bfly_block_fits(s.logBx, s.logBy), "butterfly: block too large for one workgroup's LDS");
     const size_t lds = bfly_block_lds_bytes(s.logBx, s.logBy);
-    if (std::min(s.logBx, s.logBy) <= 6)
+    const int logd = std::min(s.logBx, s.logBy);
+    if (bfly_use_mfma(logd)) {
+      // upper 16 x 16 tiles per wave: 1 (d <= 32), 3 (d = 64), 9 (d = 128)
+      if (logd <= 5)
+        hipLaunchKernelGGL(k_bfly_gram_mfma<1>, dim3(s.n), dim3(BF_THREADS), lds, st, s);
+      else if (logd == 6)
+        hipLaunchKernelGGL(k_bfly_gram_mfma<3>, dim3(s.n), dim3(BF_THREADS), lds, st, s);
+      else
+        hipLaunchKernelGGL(k_bfly_gram_mfma<9>, dim3(s.n), dim3(BF_THREADS), lds, st, s);
+    } else if (logd <= 6)
       hipLaunchKernelGGL(k_bfly_gram<4>, dim3(s.n), dim3(BF_THREADS), lds, st, s);
     else
       hipLaunchKernelGGL(k_bfly_gram<8>, dim3(s.n), dim3(BF_THREADS), lds, st, s);

@@ -116,6 +116,43 @@ def test_perm_vs_oracle(gpu, type, mul_perm):
         assert F._facs[-1].info()[0] == _lib.FH_MAT_CSR
 
 
+@pytest.mark.parametrize("perm", [False, True])
+def test_leaf_factor_views(gpu, perm):
+    """The CSR leaf factors are built on the device with their CSC twin and
+    the ELL views of both: rows sorted, 2 entries per row, and F @ X, F.T @ X
+    (rows of the twin) agree with the product of the host copies."""
+    rng = np.random.default_rng(31)
+    n = 256
+    M = rng.standard_normal((n, n))
+    p = rng.permutation(n) if perm else None
+    F = fact.butterfly(M, type="bbtree", perm=p)
+    facs = _host(F)
+    for k, f in enumerate(facs):
+        assert F._facs[k].info()[0] == _lib.FH_MAT_CSR
+        S = F._facs[k].to_host()
+        assert sp.issparse(S) and S.nnz == 2 * n and S.has_sorted_indices
+        assert np.array_equal(np.diff(S.indptr), np.full(n, 2))
+        assert np.array_equal(S.toarray(), f)
+        assert F._facs[k].info()[4] == 2 * n
+    D = ob.product(facs)
+    X = rng.standard_normal((n, 5))
+    for got, ref in ((F @ X, D @ X), (F.T @ X, D.T @ X)):
+        assert np.linalg.norm(np.asarray(got) - ref) <= 1e-12 * np.linalg.norm(ref)
+
+
+def test_dense_leaf_nnz(gpu):
+    """Dense leaf factors (left / right) report their nonzero count without
+    scanning the n x n array: it equals the count of the dense copy."""
+    rng = np.random.default_rng(8)
+    n = 64
+    M = rng.standard_normal((n, n))
+    M[:, 3] = 0.0  # exact zeros in some node values
+    for type in ("right", "left"):
+        F = fact.butterfly(M, type=type)
+        for k, f in enumerate(_host(F)):
+            assert F._facs[k].info()[4] == np.count_nonzero(_dense(f)), (type, k)
+
+
 def test_named_perms(gpu):
     """'bitrev' = its index vector; 'default_8' keeps the best of the eight."""
     rng = np.random.default_rng(5)
