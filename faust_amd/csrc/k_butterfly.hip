@@ -331,22 +331,38 @@ __global__ __launch_bounds__(BF_THREADS) void k_bfly_gram_mfma(BflyStep s) {
   bf_d4 acc[NT];
 #pragma unroll
   for (int q = 0; q < NT; ++q) acc[q] = bf_d4{0.0, 0.0, 0.0, 0.0};
-  // G = W0^T W0: G[p][q] = sum over the rows r of W0 of B[r][p] B[r][q]
+  // G = W0^T W0: G[p][q] = sum over the rows r of W0 of B[r][p] B[r][q].
+  // Entry e = r d + j of a 16-row chunk is staged by thread e % 256; the next
+  // chunk's loads are issued before the current chunk's products.
+  constexpr int PF = 16 * 128 / BF_THREADS;  // staged entries per thread at d = 128
+  double pre[PF];
+#pragma unroll
+  for (int k = 0; k < PF; ++k) {
+    const int e = tid + k * BF_THREADS;
+    if (e < 16 * d) pre[k] = bf_w0(s, tr, t, e & (d - 1), e >> logd);
+  }
   for (int i0 = 0; i0 < D; i0 += 16) {
-    for (int e = tid; e < 16 * d; e += BF_THREADS) {
-      const int r = e >> logd, j = e & (d - 1);
-      B[r * d + j] = bf_w0(s, tr, t, j, i0 + r);
+#pragma unroll
+    for (int k = 0; k < PF; ++k) {
+      const int e = tid + k * BF_THREADS;
+      if (e < 16 * d) B[e] = pre[k];
     }
     __syncthreads();
+    if (i0 + 16 < D) {
 #pragma unroll
-    for (int q = 0; q < NT; ++q) {
-      if (wave + 4 * q < nup) {
-#pragma unroll
-        for (int ks = 0; ks < 4; ++ks) {
-          const double* row = B + (4 * ks + lk) * d;
-          acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(row[16 * tI[q] + li], row[16 * tJ[q] + li], acc[q], 0, 0, 0);
-        }
+      for (int k = 0; k < PF; ++k) {
+        const int e = tid + k * BF_THREADS;
+        if (e < 16 * d) pre[k] = bf_w0(s, tr, t, e & (d - 1), i0 + 16 + (e >> logd));
       }
+    }
+    // tiles inner: the wave's NT accumulation chains interleave
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks) {
+      const double* row = B + (4 * ks + lk) * d;
+#pragma unroll
+      for (int q = 0; q < NT; ++q)
+        if (wave + 4 * q < nup)
+          acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(row[16 * tI[q] + li], row[16 * tJ[q] + li], acc[q], 0, 0, 0);
     }
     __syncthreads();
   }
@@ -361,17 +377,13 @@ __global__ __launch_bounds__(BF_THREADS) void k_bfly_gram_mfma(BflyStep s) {
     for (int k = 0; k < BF_MAX_SQ; ++k) {
       const double sc2 = scale * scale;
 #pragma unroll
-      for (int q = 0; q < NT; ++q) {
-        if (wave + 4 * q < nup) {
-          const double* ca = M + 16 * tI[q] + li;
-          const double* cb = M + 16 * tJ[q] + li;
-          bf_d4 c = {0.0, 0.0, 0.0, 0.0};
-          for (int kk = 0; kk < d; kk += 4) {
-            const int o = (kk + lk) * ld;
-            c = __builtin_amdgcn_mfma_f64_16x16x4f64(ca[o], cb[o], c, 0, 0, 0);
-          }
-          acc[q] = c;
-        }
+      for (int q = 0; q < NT; ++q) acc[q] = bf_d4{0.0, 0.0, 0.0, 0.0};
+      for (int kk = 0; kk < d; kk += 4) {
+        const double* row = M + (kk + lk) * ld + li;
+#pragma unroll
+        for (int q = 0; q < NT; ++q)
+          if (wave + 4 * q < nup)
+            acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(row[16 * tI[q]], row[16 * tJ[q]], acc[q], 0, 0, 0);
       }
       __syncthreads();  // every tile of the square is formed before M is overwritten
 #pragma unroll
@@ -392,9 +404,12 @@ __global__ __launch_bounds__(BF_THREADS) void k_bfly_gram_mfma(BflyStep s) {
   double vn = 0.0;
   for (int j = 0; j < d; ++j) vn = fma(M[j * ld + jb], M[j * ld + jb], vn);
   const double vs = vn > 0.0 ? 1.0 / sqrt(vn) : 0.0;
-  // w = W0 v, sigma = |w|
-  double part = 0.0;
-  for (int i = tid; i < D; i += BF_THREADS) {
+  // w = W0 v, sigma = |w|; entry i = tid is kept for the store below
+  double w = 0.0;
+  if (tid < D)
+    for (int j = 0; j < d; ++j) w = fma(bf_w0(s, tr, t, j, tid), M[j * ld + jb] * vs, w);
+  double part = w * w;
+  for (int i = tid + BF_THREADS; i < D; i += BF_THREADS) {
     double wi = 0.0;
     for (int j = 0; j < d; ++j) wi = fma(bf_w0(s, tr, t, j, i), M[j * ld + jb] * vs, wi);
     part = fma(wi, wi, part);
@@ -410,7 +425,8 @@ __global__ __launch_bounds__(BF_THREADS) void k_bfly_gram_mfma(BflyStep s) {
     kl = 1.0 / rs;
     kc = rs;
   }
-  for (int i = tid; i < D; i += BF_THREADS) {
+  if (tid < D) bf_put_len(s, tr, t, tid, w * kl);
+  for (int i = tid + BF_THREADS; i < D; i += BF_THREADS) {
     double wi = 0.0;
     for (int j = 0; j < d; ++j) wi = fma(bf_w0(s, tr, t, j, i), M[j * ld + jb] * vs, wi);
     bf_put_len(s, tr, t, i, wi * kl);
