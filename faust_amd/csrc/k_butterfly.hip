@@ -304,6 +304,15 @@ __device__ __forceinline__ void bf_store_tile(double* M, int ld, int ti, int tj,
   }
 }
 
+// trace of the d x d matrix M (row stride ld): lane-strided partial sums and
+// a xor-butterfly wave sum, which leaves the same bits in every lane of every
+// wave, so the stopping test stays uniform over the workgroup
+__device__ __forceinline__ double bf_trace(const double* M, int ld, int d, int lane) {
+  double v = 0.0;
+  for (int j = lane; j < d; j += 64) v += M[j * ld + j];
+  return wave_sum(v);
+}
+
 template <int NT>
 __global__ __launch_bounds__(BF_THREADS) void k_bfly_gram_mfma(BflyStep s) {
   extern __shared__ double lds[];
@@ -370,8 +379,7 @@ __global__ __launch_bounds__(BF_THREADS) void k_bfly_gram_mfma(BflyStep s) {
   for (int q = 0; q < NT; ++q)
     if (wave + 4 * q < nup) bf_store_tile(M, ld, tI[q], tJ[q], li, lk, acc[q], 1.0);
   __syncthreads();
-  double tau = 0.0;
-  for (int j = 0; j < d; ++j) tau += M[j * ld + j];
+  double tau = bf_trace(M, ld, d, lane);
   if (tau > 0.0) {
     double scale = 1.0 / tau;  // M holds the current matrix / scale
     for (int k = 0; k < BF_MAX_SQ; ++k) {
@@ -390,8 +398,7 @@ __global__ __launch_bounds__(BF_THREADS) void k_bfly_gram_mfma(BflyStep s) {
       for (int q = 0; q < NT; ++q)
         if (wave + 4 * q < nup) bf_store_tile(M, ld, tI[q], tJ[q], li, lk, acc[q], sc2);
       __syncthreads();
-      double tr2 = 0.0;
-      for (int j = 0; j < d; ++j) tr2 += M[j * ld + j];
+      const double tr2 = bf_trace(M, ld, d, lane);
       scale = 1.0 / tr2;
       if (tr2 >= 1.0 - 16.0 * DBL_EPSILON) break;  // tr(M^2) = 1: M is rank one
     }
