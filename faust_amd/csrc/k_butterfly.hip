@@ -591,11 +591,8 @@ static size_t bfly_lds_budget() {
 
 // d x d products on MFMA for d >= 16 (FH_BFLY_MFMA=0: the VALU kernel)
 static bool bfly_use_mfma(int logd) {
-  static const int env = [] {
-    const char* e = getenv("FH_BFLY_MFMA");
-    return e && e[0] == '0' ? 0 : 1;
-  }();
-  return env && logd >= 4;
+  const char* e = getenv("FH_BFLY_MFMA");  // read per split: a caller may switch between factorizations
+  return !(e && e[0] == '0') && logd >= 4;
 }
 
 size_t bfly_block_lds_bytes(int logBx, int logBy) {

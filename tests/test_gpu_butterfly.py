@@ -153,6 +153,23 @@ def test_dense_leaf_nnz(gpu):
             assert F._facs[k].info()[4] == np.count_nonzero(_dense(f)), (type, k)
 
 
+@pytest.mark.parametrize("n", [64, 1024, 4096])
+def test_valu_gram_kernel(gpu, n, monkeypatch):
+    """FH_BFLY_MFMA=0 runs the VALU Gram kernel at every block size: same
+    bars against the oracle, and the same product as the MFMA kernel within
+    the product bar."""
+    rng = np.random.default_rng(300 + n)
+    M = rng.standard_normal((n, n))
+    Fm = fact.butterfly(M, type="bbtree")
+    monkeypatch.setenv("FH_BFLY_MFMA", "0")
+    Fv = fact.butterfly(M, type="bbtree")
+    monkeypatch.delenv("FH_BFLY_MFMA")
+    if n <= 1024:
+        _compare(Fv, ob.butterfly(M, ob.BALANCED))
+    pm, pv = Fm.toarray(), Fv.toarray()
+    assert np.linalg.norm(pm - pv) <= PROD_TOL * np.linalg.norm(pm)
+
+
 def test_named_perms(gpu):
     """'bitrev' = its index vector; 'default_8' keeps the best of the eight."""
     rng = np.random.default_rng(5)
