@@ -205,13 +205,19 @@ static bool chain_is_zero(const std::vector<DMatP>& f, hipStream_t st) {
   return false;
 }
 
-// Transform::spectralNorm (faust_Transform.hpp:684-709)
-double faust_spectral_norm_f64(const std::vector<DMatP>& f, int maxit, double thr, int* flag, hipStream_t st) {
-  if (f.empty()) return 1.0;
-  if (chain_is_zero(f, st)) return 0.0;
+// the power-iteration chain of F^H F (or F F^H, the smaller side); false when
+// the norm is known without it (*known: 1 for an empty Faust, 0 for a zero one)
+static bool spectral_chain(const std::vector<DMatP>& f, std::vector<ChainOp>& ops, int& n, double* known,
+                           hipStream_t st) {
+  if (f.empty()) {
+    *known = 1.0;
+    return false;
+  }
+  if (chain_is_zero(f, st)) {
+    *known = 0.0;
+    return false;
+  }
   const int nr = f.front()->m, nc = f.back()->n;
-  std::vector<ChainOp> ops;
-  int n;
   if (nc < nr) {
     for (int i = (int)f.size() - 1; i >= 0; --i) ops.push_back({f[i].get(), 'T'});
     for (auto& x : f) ops.push_back({x.get(), 'N'});
@@ -221,8 +227,37 @@ double faust_spectral_norm_f64(const std::vector<DMatP>& f, int maxit, double th
     for (int i = (int)f.size() - 1; i >= 0; --i) ops.push_back({f[i].get(), 'T'});
     n = nr;
   }
+  return true;
+}
+
+// Transform::spectralNorm (faust_Transform.hpp:684-709)
+double faust_spectral_norm_f64(const std::vector<DMatP>& f, int maxit, double thr, int* flag, hipStream_t st) {
+  std::vector<ChainOp> ops;
+  int n = 0;
+  double known;
+  if (!spectral_chain(f, ops, n, &known, st)) return known;
   double lam = power_iteration_f64(ops, n, maxit, thr, flag, st);
   return std::fabs(std::sqrt(lam));
+}
+
+// the 2-norms of two Fausts (PALM's R and L of one factor update), their
+// power iterations sharing a launch; the same values as two calls of
+// faust_spectral_norm_f64 (f1 first), flag from the last one computed
+void faust_spectral_norm_pair_f64(const std::vector<DMatP>& f1, const std::vector<DMatP>& f2, int maxit, double thr,
+                                  int* flag, double* n1, double* n2, hipStream_t st) {
+  std::vector<ChainOp> o1, o2;
+  int m1 = 0, m2 = 0;
+  double k1, k2;
+  const bool p1 = spectral_chain(f1, o1, m1, &k1, st), p2 = spectral_chain(f2, o2, m2, &k2, st);
+  if (p1 && p2) {
+    double l1, l2;
+    power_iteration_pair_f64(o1, m1, o2, m2, maxit, thr, &l1, &l2, flag, st);
+    *n1 = std::fabs(std::sqrt(l1));
+    *n2 = std::fabs(std::sqrt(l2));
+    return;
+  }
+  *n1 = p1 ? std::fabs(std::sqrt(power_iteration_f64(o1, m1, maxit, thr, flag, st))) : k1;
+  *n2 = p2 ? std::fabs(std::sqrt(power_iteration_f64(o2, m2, maxit, thr, flag, st))) : k2;
 }
 
 }  // namespace fh

@@ -106,6 +106,8 @@ void faust_multiply_f64(const std::vector<DMatP>& f, char op, const double* X, i
                         DBuf* ws, FaustTimers* tm = nullptr, int* launches = nullptr);
 DMatP faust_get_product_f64(const std::vector<DMatP>& f, hipStream_t st);
 double faust_spectral_norm_f64(const std::vector<DMatP>& f, int maxit, double thr, int* flag, hipStream_t st);
+void faust_spectral_norm_pair_f64(const std::vector<DMatP>& f1, const std::vector<DMatP>& f2, int maxit, double thr,
+                                  int* flag, double* n1, double* n2, hipStream_t st);
 // FP32 / BF16 paths (k_mixed.hip, k_bsr.hip)
 void mixed_to_dense(const DMat& M, void* dense_dev, hipStream_t st);
 void bsr_to_host(const DMat& M, void* bdata, int32_t* browptr, int32_t* bcolinds);

@@ -206,6 +206,11 @@ struct ChainOp {
 };
 double power_iteration_f64(const std::vector<ChainOp>& ops, int n, int maxit, double thr, int* flag,
                            hipStream_t st);
+// two independent chains at once (one workgroup each when both fit the
+// persistent kernel, else one after the other); the same bits as two calls of
+// power_iteration_f64; flag receives the second chain's
+void power_iteration_pair_f64(const std::vector<ChainOp>& ops1, int n1, const std::vector<ChainOp>& ops2, int n2,
+                              int maxit, double thr, double* lam1, double* lam2, int* flag, hipStream_t st);
 
 // host-side start vector: srand(0xF4+'u'+57), Random on [-1, 1]
 const std::vector<double>& power_x0(int n);

@@ -411,8 +411,12 @@ static void palm4msa2(Palm& P, Chain& S, double& lambda, hipStream_t st) {
       if (p.constant_step_size)
         c = 1 / p.step_size;
       else {
-        if (!pR[f_id].empty()) nR = faust_spectral_norm_f64(pR[f_id], p.norm2_max_iter, p.norm2_threshold, &flag, st);
-        if (!pL[f_id].empty()) nL = faust_spectral_norm_f64(pL[f_id], p.norm2_max_iter, p.norm2_threshold, &flag, st);
+        if (!pR[f_id].empty() && !pL[f_id].empty())
+          faust_spectral_norm_pair_f64(pR[f_id], pL[f_id], p.norm2_max_iter, p.norm2_threshold, &flag, &nR, &nL, st);
+        else if (!pR[f_id].empty())
+          nR = faust_spectral_norm_f64(pR[f_id], p.norm2_max_iter, p.norm2_threshold, &flag, st);
+        else if (!pL[f_id].empty())
+          nL = faust_spectral_norm_f64(pL[f_id], p.norm2_max_iter, p.norm2_threshold, &flag, st);
         if (std::isnan(nL) || std::isnan(nR))
           throw Error(FH_ERR_NUMERIC, "2-norm computation error: R or L 2-norm is NaN.");
         if ((nR == 0 && (it > 0 || !p.is_update_way_R2L)) || (nL == 0 && (it > 0 || p.is_update_way_R2L)))

@@ -61,9 +61,23 @@ __device__ double pw_dot(const double* x, const double* y, int n, double* red, d
 
 // out[0] = lambda, out[1] = flag (iterations, or -1), out[2] = status
 // (0 ok, 1 zero-norm vector, 2 A x == 0)
-__global__ __launch_bounds__(PW_THREADS) void k_power_iteration(const PwOp* __restrict__ ops, int nops, int n,
-                                                                 const double* __restrict__ x0, int maxit,
-                                                                 double thr, double* __restrict__ out) {
+// One chain per workgroup: workgroup b runs jobs[b] (independent chains, e.g.
+// the L and R norms of one PALM factor update, share a launch).
+struct PwJob {
+  const PwOp* ops;
+  int nops;
+  int n;
+  const double* x0;
+  double* out;
+};
+
+__global__ __launch_bounds__(PW_THREADS) void k_power_iteration(const PwJob* __restrict__ jobs, int maxit,
+                                                                 double thr) {
+  const PwJob job = jobs[blockIdx.x];
+  const PwOp* __restrict__ ops = job.ops;
+  const int nops = job.nops, n = job.n;
+  const double* __restrict__ x0 = job.x0;
+  double* __restrict__ out = job.out;
   __shared__ double xa[PW_VMAX], xb[PW_VMAX], xn[PW_VMAX];
   __shared__ double red[RED_LANES];
   __shared__ double tot;
@@ -409,9 +423,7 @@ static double power_iteration_wide(const std::vector<PwOp>& h, int n, const std:
   return hs.status == 2 ? 0.0 : hs.lambda;
 }
 
-double power_iteration_f64(const std::vector<ChainOp>& ops, int n, int maxit, double thr, int* flag,
-                           hipStream_t st) {
-  FH_CHECK(maxit > 0, "power_iteration: nbr_iter_max <= 0");
+static std::vector<PwOp> pw_ops(const std::vector<ChainOp>& ops) {
   FH_CHECK(!ops.empty(), "power_iteration: empty chain");
   std::vector<PwOp> h(ops.size());
   for (size_t s = 0; s < ops.size(); ++s) {
@@ -445,23 +457,80 @@ double power_iteration_f64(const std::vector<ChainOp>& ops, int n, int maxit, do
       o.nnz = 0;
     }
   }
-  const std::vector<double>& x0 = power_x0(n);
-  // the persistent kernel keeps every vector in LDS: up to PW_VMAX entries
+  return h;
+}
+
+// the persistent kernel keeps every vector in LDS: up to PW_VMAX entries
+static bool pw_persistent(const std::vector<PwOp>& h, int n) {
   bool fits = n <= PW_VMAX;
   for (const PwOp& o : h) fits = fits && o.rows <= PW_VMAX && o.cols <= PW_VMAX;
-  if (!fits || use_wide_form(h)) return power_iteration_wide(h, n, x0, maxit, thr, flag, st);
-  DBuf dops(sizeof(PwOp) * h.size()), dx(sizeof(double) * n), dout(sizeof(double) * 3);
-  FH_HIP(hipMemcpyAsync(dops.p, h.data(), sizeof(PwOp) * h.size(), hipMemcpyHostToDevice, st));
-  FH_HIP(hipMemcpyAsync(dx.p, x0.data(), sizeof(double) * n, hipMemcpyHostToDevice, st));
-  hipLaunchKernelGGL(k_power_iteration, dim3(1), dim3(PW_THREADS), 0, st, dops.as<PwOp>(), (int)h.size(), n,
-                     dx.as<double>(), maxit, thr, dout.as<double>());
+  return fits && !use_wide_form(h);
+}
+
+// chains[k] (ops, n) in one launch, one workgroup each; lam[k] and flags[k]
+// receive each chain's result.  A zero-norm start vector throws, for the
+// first such chain in order.
+static void power_iteration_persistent(const std::vector<std::vector<PwOp>>& hs, const std::vector<int>& ns,
+                                       int maxit, double thr, double* lam, int* flags, hipStream_t st) {
+  const int J = (int)hs.size();
+  size_t nops = 0, nx = 0;
+  for (int k = 0; k < J; ++k) {
+    nops += hs[k].size();
+    nx += ns[k];
+  }
+  DBuf dops(sizeof(PwOp) * nops), dx(sizeof(double) * nx), dout(sizeof(double) * 3 * J), djobs(sizeof(PwJob) * J);
+  std::vector<PwOp> allops;
+  std::vector<PwJob> jobs(J);
+  size_t oo = 0, ox = 0;
+  for (int k = 0; k < J; ++k) {
+    allops.insert(allops.end(), hs[k].begin(), hs[k].end());
+    FH_HIP(hipMemcpyAsync(dx.as<double>() + ox, power_x0(ns[k]).data(), sizeof(double) * ns[k],
+                          hipMemcpyHostToDevice, st));
+    jobs[k] = PwJob{dops.as<PwOp>() + oo, (int)hs[k].size(), ns[k], dx.as<double>() + ox, dout.as<double>() + 3 * k};
+    oo += hs[k].size();
+    ox += ns[k];
+  }
+  FH_HIP(hipMemcpyAsync(dops.p, allops.data(), sizeof(PwOp) * nops, hipMemcpyHostToDevice, st));
+  FH_HIP(hipMemcpyAsync(djobs.p, jobs.data(), sizeof(PwJob) * J, hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(k_power_iteration, dim3(J), dim3(PW_THREADS), 0, st, djobs.as<PwJob>(), maxit, thr);
   FH_LAUNCH_CHECK();
-  double res[3];
-  FH_HIP(hipMemcpyAsync(res, dout.p, sizeof(res), hipMemcpyDeviceToHost, st));
+  std::vector<double> res(3 * J);
+  FH_HIP(hipMemcpyAsync(res.data(), dout.p, sizeof(double) * 3 * J, hipMemcpyDeviceToHost, st));
   FH_HIP(hipStreamSynchronize(st));
-  if (res[2] == 1.0) throw Error(FH_ERR_NUMERIC, "Can't normalize a zero-norm vector.");
-  if (flag) *flag = (int)res[1];
-  return res[0];
+  for (int k = 0; k < J; ++k) {
+    if (res[3 * k + 2] == 1.0) throw Error(FH_ERR_NUMERIC, "Can't normalize a zero-norm vector.");
+    lam[k] = res[3 * k];
+    flags[k] = (int)res[3 * k + 1];
+  }
+}
+
+double power_iteration_f64(const std::vector<ChainOp>& ops, int n, int maxit, double thr, int* flag,
+                           hipStream_t st) {
+  FH_CHECK(maxit > 0, "power_iteration: nbr_iter_max <= 0");
+  std::vector<PwOp> h = pw_ops(ops);
+  if (!pw_persistent(h, n)) return power_iteration_wide(h, n, power_x0(n), maxit, thr, flag, st);
+  double lam;
+  int fl;
+  power_iteration_persistent({h}, {n}, maxit, thr, &lam, &fl, st);
+  if (flag) *flag = fl;
+  return lam;
+}
+
+void power_iteration_pair_f64(const std::vector<ChainOp>& ops1, int n1, const std::vector<ChainOp>& ops2, int n2,
+                              int maxit, double thr, double* lam1, double* lam2, int* flag, hipStream_t st) {
+  FH_CHECK(maxit > 0, "power_iteration: nbr_iter_max <= 0");
+  std::vector<PwOp> h1 = pw_ops(ops1), h2 = pw_ops(ops2);
+  if (!pw_persistent(h1, n1) || !pw_persistent(h2, n2)) {
+    *lam1 = power_iteration_f64(ops1, n1, maxit, thr, flag, st);
+    *lam2 = power_iteration_f64(ops2, n2, maxit, thr, flag, st);
+    return;
+  }
+  double lam[2];
+  int fl[2];
+  power_iteration_persistent({h1, h2}, {n1, n2}, maxit, thr, lam, fl, st);
+  *lam1 = lam[0];
+  *lam2 = lam[1];
+  if (flag) *flag = fl[1];
 }
 
 }  // namespace fh
