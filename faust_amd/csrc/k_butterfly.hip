@@ -322,8 +322,8 @@ __global__ __launch_bounds__(BF_THREADS) void k_bfly_gram_mfma(BflyStep s) {
   const bool tr = s.logBx < s.logBy;
   const int logD = tr ? s.logBy : s.logBx, logd = tr ? s.logBx : s.logBy;
   const int D = 1 << logD, d = 1 << logd, ld = d + 1;
-  double* M = lds;           // d x d, row stride ld
-  double* B = lds + d * ld;  // 16 x d staged rows of W0
+  double* M = lds;  // d x d, row stride ld
+  double* B = lds;  // 16 x d staged rows of W0: only live before M is first written
   const int td = d >> 4, nup = td * (td + 1) / 2;
   // this wave's upper tiles
   int tI[NT], tJ[NT];
@@ -598,7 +598,7 @@ static bool bfly_use_mfma(int logd) {
 size_t bfly_block_lds_bytes(int logBx, int logBy) {
   const int logd = std::min(logBx, logBy);
   const size_t d = (size_t)1 << logd, D = (size_t)1 << std::max(logBx, logBy);
-  if (bfly_use_mfma(logd)) return sizeof(double) * (d * (d + 1) + 16 * d);  // padded M and 16 staged rows
+  if (bfly_use_mfma(logd)) return sizeof(double) * d * (d + 1);  // padded M (the staged rows share it)
   return sizeof(double) * (d * d + std::min<size_t>(BF_CHUNK, D) * d);  // M and the staged rows
 }
 
