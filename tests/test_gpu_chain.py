@@ -9,14 +9,14 @@ import faust_amd
 pytestmark = pytest.mark.gpu
 
 
-def _cfg1_factors(seed=0xFA057, n=1024, nnz=8, J=5):
+def _csr_chain_factors(seed=0xFA057, n=1024, nnz=8, J=5):
     # 5 CSR factors 1024x1024, exactly 8 distinct uniform columns per row, N(0,1)
     return [faust_amd.rand_csr(n, n, nnz, np.random.default_rng(seed + f)) for f in range(J)]
 
 
 @pytest.mark.parametrize("op", ["N", "T"])
-def test_cfg1_chain_bitexact(gpu, op):
-    facs = _cfg1_factors()
+def test_csr1024x5_chain_bitexact(gpu, op):
+    facs = _csr_chain_factors()
     X = np.asfortranarray(np.random.default_rng(0xFA057 + 100).standard_normal((1024, 64)))
     F = faust_amd.Faust(facs)
     Y = (F @ X) if op == "N" else (F.T @ X)
@@ -54,7 +54,7 @@ def test_empty_rows_and_vector(gpu):
 
 
 def test_get_product_and_norm2(gpu):
-    facs = _cfg1_factors(n=256, nnz=4, J=4)
+    facs = _csr_chain_factors(n=256, nnz=4, J=4)
     F = faust_amd.Faust(facs)
     assert np.array_equal(F.toarray(), oracle.get_product(facs))
     nrm, _ = oracle.spectral_norm(facs)
@@ -79,7 +79,7 @@ def power_form(request, monkeypatch):
 def test_norm2_both_forms_bitexact(gpu, power_form, case):
     rng = np.random.default_rng(17)
     if case == "sparse256":
-        facs = _cfg1_factors(n=256, nnz=4, J=4)
+        facs = _csr_chain_factors(n=256, nnz=4, J=4)
     elif case == "dense_rect":
         facs = [np.asfortranarray(rng.standard_normal((300, 200))), np.asfortranarray(rng.standard_normal((200, 310)))]
     elif case == "mixed":
@@ -159,7 +159,7 @@ def test_long_chains_fuse_without_ell_views(gpu, monkeypatch):
             Y = G @ X
             assert _launches(G) == launches
             assert np.array_equal(Y, oracle.chain_multiply(facs, X, op))
-    F5 = faust_amd.Faust(_cfg1_factors())
+    F5 = faust_amd.Faust(_csr_chain_factors())
     F5 @ X
     assert _launches(F5) == 5
 
@@ -169,7 +169,7 @@ def test_optimize_time_fp64(gpu):
     # form of an FP64 chain gives the same bits
     import ctypes as C
     from faust_amd import _lib
-    facs = _cfg1_factors()
+    facs = _csr_chain_factors()
     F = faust_amd.Faust(facs)
     X = np.asfortranarray(np.random.default_rng(8).standard_normal((1024, 64)))
     ref = oracle.chain_multiply(facs, X, "N")
@@ -246,10 +246,10 @@ def test_spectral_norm_beyond_2048_bitexact(gpu, dense):
     rng = np.random.default_rng(15)
     if dense:
         n = 2304
-        facs = [_cfg1_factors(seed=0xD3, n=n, nnz=6, J=1)[0], np.asfortranarray(rng.standard_normal((n, n)) / 48.0),
-                _cfg1_factors(seed=0xD4, n=n, nnz=6, J=1)[0]]
+        facs = [_csr_chain_factors(seed=0xD3, n=n, nnz=6, J=1)[0], np.asfortranarray(rng.standard_normal((n, n)) / 48.0),
+                _csr_chain_factors(seed=0xD4, n=n, nnz=6, J=1)[0]]
     else:
-        facs = _cfg1_factors(seed=0xB16, n=4096, nnz=8, J=3)
+        facs = _csr_chain_factors(seed=0xB16, n=4096, nnz=8, J=3)
     F = faust_amd.Faust(facs)
     nrm, _ = oracle.spectral_norm(facs)
     assert F.norm(2) == nrm

@@ -54,8 +54,8 @@ def test_bsr_fp32_split_single_product(gpu):
     assert np.linalg.norm(Y - ref) < np.linalg.norm(Yb - ref)
 
 
-def test_bsr_chain_cfg3a_shape_error(gpu):
-    # 3 factors 2048^2 of the cfg3a kind; error vs FP64 oracle (same BF16 inputs) stated
+def test_bsr_chain_32x32_blocks_error(gpu):
+    # 3 factors 2048^2, 32x32 blocks, 4 blocks per block row; error vs FP64 oracle (same BF16 inputs) stated
     rng = np.random.default_rng(3)
     facs = [_bsr_rounded(rand_bsr(2048, 2048, 32, 4, rng, scale=1 / np.sqrt(128))) for _ in range(3)]
     X = bf16_round(rng.standard_normal((2048, 512)))
@@ -221,7 +221,7 @@ def test_bsr_persistent_chain_matches_per_factor(gpu, monkeypatch, dtype, op):
         assert np.array_equal(Y1, Y2)
 
 
-def test_cfg3a_full_size_against_oracle_columns(gpu):
+def test_bsr_8192x6_full_size_against_oracle_columns(gpu):
     # the benchmarked Faust itself (6 BSR factors 8192^2, 4 blocks per block
     # row, BF16): the full product on X 8192x512, its first 16 columns against
     # the FP64 oracle, and linearity of the whole product
@@ -239,7 +239,7 @@ def test_cfg3a_full_size_against_oracle_columns(gpu):
     assert np.array_equal(Y8, Y[:, 100:108])
 
 
-def test_cfg3b_full_size_against_oracle_columns(gpu):
+def test_csr_fp32_16384x6_full_size_against_oracle_columns(gpu):
     rng = np.random.default_rng(0xC5F)
     facs = [faust_amd.rand_csr(16384, 16384, 12, np.random.default_rng(0xC5F + f)).astype(np.float32)
             .astype(np.float64) for f in range(6)]
