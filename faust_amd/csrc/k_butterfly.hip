@@ -87,6 +87,31 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+// top eigenpair of the 2 x 2 Gram matrix [[a, g], [g, b]] as the scales of
+// the rank-1 pair: the length side gets (e0 w0 + e1 w1) kl, the column side
+// (G e) kc
+__device__ __forceinline__ void bf_d2_pair(double a, double b, double g, double& e0, double& e1, double& kl,
+                                           double& kc) {
+  double lam;
+  if (g == 0.0) {
+    if (a >= b) { e0 = 1.0; e1 = 0.0; lam = a; }
+    else { e0 = 0.0; e1 = 1.0; lam = b; }
+  } else {
+    double c, sn, tn;
+    bf_rot(a, b, g, c, sn, tn);
+    const double ap = a - tn * g, bp = b + tn * g;
+    if (bp >= ap) { e0 = sn; e1 = c; lam = bp; }
+    else { e0 = c; e1 = -sn; lam = ap; }
+  }
+  kl = 0.0;
+  kc = 0.0;
+  if (lam > 0.0) {
+    const double sigma = sqrt(lam), rs = sqrt(sigma);
+    kl = 1.0 / rs;
+    kc = 1.0 / (sigma * rs);
+  }
+}
+
 // ---------------------------------------------------------------- d = 2
 __global__ __launch_bounds__(BF_THREADS) void k_bfly_d2(BflyStep s) {
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -104,24 +129,8 @@ __global__ __launch_bounds__(BF_THREADS) void k_bfly_d2(BflyStep s) {
   a = wave_sum(a);
   b = wave_sum(b);
   g = wave_sum(g);
-  // top eigenpair (lam, (e0, e1)) of [[a, g], [g, b]]
-  double e0, e1, lam;
-  if (g == 0.0) {
-    if (a >= b) { e0 = 1.0; e1 = 0.0; lam = a; }
-    else { e0 = 0.0; e1 = 1.0; lam = b; }
-  } else {
-    double c, sn, tn;
-    bf_rot(a, b, g, c, sn, tn);
-    const double ap = a - tn * g, bp = b + tn * g;
-    if (bp >= ap) { e0 = sn; e1 = c; lam = bp; }
-    else { e0 = c; e1 = -sn; lam = ap; }
-  }
-  double kl = 0.0, kc = 0.0;
-  if (lam > 0.0) {
-    const double sigma = sqrt(lam), rs = sqrt(sigma);
-    kl = 1.0 / rs;
-    kc = 1.0 / (sigma * rs);
-  }
+  double e0, e1, kl, kc;
+  bf_d2_pair(a, b, g, e0, e1, kl, kc);
   // w* = e0 w0 + e1 w1 (= sigma times the unit singular vector on the long
   // side); the short side is W0^T w* = G e
   for (int i = lane; i < D; i += 64) {
@@ -130,6 +139,49 @@ __global__ __launch_bounds__(BF_THREADS) void k_bfly_d2(BflyStep s) {
   }
   if (lane == 0) bf_put_col(s, tr, t, 0, (e0 * a + e1 * g) * kc);
   if (lane == 1) bf_put_col(s, tr, t, 1, (e0 * g + e1 * b) * kc);
+}
+
+// d = 2 with a long side of 2048..8192: one workgroup per block, which keeps
+// its D x 2 values in registers (REG = D / 256 pairs per thread), so the
+// block is read from HBM once.  The three sums are wave sums added in wave
+// order, the same in every thread.
+template <int REG>
+__global__ __launch_bounds__(BF_THREADS) void k_bfly_d2_wide(BflyStep s) {
+  __shared__ double red[3][BF_THREADS / 64];
+  const int t = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const bool tr = s.logBx < s.logBy;
+  double c0[REG], c1[REG];
+  double a = 0.0, b = 0.0, g = 0.0;
+#pragma unroll
+  for (int k = 0; k < REG; ++k) {
+    const int i = tid + k * BF_THREADS;
+    c0[k] = bf_w0(s, tr, t, 0, i);
+    c1[k] = bf_w0(s, tr, t, 1, i);
+    a = fma(c0[k], c0[k], a);
+    b = fma(c1[k], c1[k], b);
+    g = fma(c0[k], c1[k], g);
+  }
+  a = wave_sum(a);
+  b = wave_sum(b);
+  g = wave_sum(g);
+  if (lane == 0) {
+    red[0][wave] = a;
+    red[1][wave] = b;
+    red[2][wave] = g;
+  }
+  __syncthreads();
+  a = b = g = 0.0;
+  for (int q = 0; q < BF_THREADS / 64; ++q) {
+    a += red[0][q];
+    b += red[1][q];
+    g += red[2][q];
+  }
+  double e0, e1, kl, kc;
+  bf_d2_pair(a, b, g, e0, e1, kl, kc);
+#pragma unroll
+  for (int k = 0; k < REG; ++k) bf_put_len(s, tr, t, tid + k * BF_THREADS, (e0 * c0[k] + e1 * c1[k]) * kl);
+  if (tid == 0) bf_put_col(s, tr, t, 0, (e0 * a + e1 * g) * kc);
+  if (tid == 1) bf_put_col(s, tr, t, 1, (e0 * g + e1 * b) * kc);
 }
 
 // ---------------------------------------------------------------- d >= 4
@@ -615,6 +667,18 @@ void bfly_split(const BflyStep& s, hipStream_t st) {
   FH_CHECK(s.logBx + s.logsx == s.in_logB + s.in_logs && s.logsy == s.in_logs && s.logBy + s.logsy == s.logsx,
            "butterfly: split does not match its node");
   if (std::min(s.logBx, s.logBy) == 1) {
+    const int logD = std::max(s.logBx, s.logBy);
+    const char* e = getenv("FH_BFLY_D2_WIDE");  // 0: one wave per block at every length
+    if (!(e && e[0] == '0') && logD >= 11 && logD <= 13) {
+      if (logD == 11)
+        hipLaunchKernelGGL(k_bfly_d2_wide<8>, dim3(s.n), dim3(BF_THREADS), 0, st, s);
+      else if (logD == 12)
+        hipLaunchKernelGGL(k_bfly_d2_wide<16>, dim3(s.n), dim3(BF_THREADS), 0, st, s);
+      else
+        hipLaunchKernelGGL(k_bfly_d2_wide<32>, dim3(s.n), dim3(BF_THREADS), 0, st, s);
+      FH_LAUNCH_CHECK();
+      return;
+    }
     const int per = BF_THREADS / 64;
     hipLaunchKernelGGL(k_bfly_d2, dim3((s.n + per - 1) / per), dim3(BF_THREADS), 0, st, s);
   } else {

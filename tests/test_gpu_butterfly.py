@@ -170,6 +170,29 @@ def test_valu_gram_kernel(gpu, n, monkeypatch):
     assert np.linalg.norm(pm - pv) <= PROD_TOL * np.linalg.norm(pm)
 
 
+@pytest.mark.parametrize("type", ["right", "left"])
+@pytest.mark.parametrize("n", [4096, 8192])
+def test_long_d2_blocks(gpu, type, n, monkeypatch):
+    """Blocks with a side of 2 and a long side of 2048..8192 (the first splits
+    of right / left at these orders) run one workgroup per block with the
+    block in registers: exact butterflies are recovered to 1e-12, and a
+    Gaussian matrix's factorization has the same product as with the
+    one-wave kernel (FH_BFLY_D2_WIDE=0), within the product bar."""
+    from faust_amd.butterfly import rand_butterfly_factors
+    rng = np.random.default_rng(n)
+    print(f"long d2 blocks {type} n={n}", flush=True)
+    B = faust_amd.Faust(rand_butterfly_factors(n, rng=rng)).toarray()
+    F = fact.butterfly(B, type=type)
+    assert np.linalg.norm(F.toarray() - B) <= 1e-12 * np.linalg.norm(B)
+    M = rng.standard_normal((n, n))
+    Fw = fact.butterfly(M, type=type)
+    monkeypatch.setenv("FH_BFLY_D2_WIDE", "0")
+    Fn = fact.butterfly(M, type=type)
+    monkeypatch.delenv("FH_BFLY_D2_WIDE")
+    pw, pn = Fw.toarray(), Fn.toarray()
+    assert np.linalg.norm(pw - pn) <= PROD_TOL * np.linalg.norm(pn)
+
+
 def test_named_perms(gpu):
     """'bitrev' = its index vector; 'default_8' keeps the best of the eight."""
     rng = np.random.default_rng(5)
