@@ -22,7 +22,7 @@ def FH_MUL_BSR_KERNEL(k):
     return 16 + k
 
 
-PROX = {"sp": 0, "spcol": 1, "splin": 2, "splincol": 3, "supp": 4}
+PROX = {"sp": 0, "spcol": 1, "splin": 2, "splincol": 3, "supp": 4, "normcol": 5, "normlin": 6}
 
 
 class FaustHipError(RuntimeError):
@@ -33,7 +33,7 @@ class FaustHipError(RuntimeError):
 
 class fh_constraint(C.Structure):
     _fields_ = [("name", C.c_int), ("k", C.c_int64), ("nrows", C.c_int), ("ncols", C.c_int),
-                ("normalized", C.c_int), ("pos", C.c_int), ("supp", C.POINTER(C.c_double))]
+                ("normalized", C.c_int), ("pos", C.c_int), ("supp", C.POINTER(C.c_double)), ("s", C.c_double)]
 
 
 class fh_stop_crit(C.Structure):
@@ -87,6 +87,7 @@ _SIGS = {
     "fh_faust_timers": (C.c_int, [VP, VP, VP]),
     "fh_faust_spectral_norm": (C.c_int, [VP, C.c_int, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_int), VP]),
     "fh_prox": (C.c_int, [C.c_int, C.c_int, VP, C.c_int, C.c_int, C.c_int64, C.c_int, C.c_int, VP, VP]),
+    "fh_prox_real": (C.c_int, [C.c_int, C.c_int, VP, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, VP]),
     "fh_norm_frob": (C.c_int, [C.c_int, VP, C.c_int64, C.POINTER(C.c_double), VP]),
     "fh_dot": (C.c_int, [C.c_int, VP, VP, C.c_int64, C.POINTER(C.c_double), VP]),
     "fh_palm4msa": (C.c_int, [VP, C.c_int, C.c_int, C.POINTER(fh_constraint), C.c_int, C.POINTER(fh_stop_crit),

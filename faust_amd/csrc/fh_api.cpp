@@ -570,6 +570,15 @@ int fh_prox(int dtype, int name, void* M, int nrows, int ncols, int64_t k, int n
   FH_API_END
 }
 
+int fh_prox_real(int dtype, int name, void* M, int nrows, int ncols, double s, int normalized, int pos,
+                 void* stream) {
+  FH_API_BEGIN
+  FH_CHECK(dtype == FH_F64, "fh_prox_real: FP64 only");
+  FH_CHECK(M && nrows >= 0 && ncols >= 0, "bad argument");
+  prox_real_f64(name, (double*)M, nrows, ncols, s, normalized != 0, pos != 0, (hipStream_t)stream);
+  FH_API_END
+}
+
 int fh_norm_frob(int dtype, const void* M, int64_t n, double* out, void* stream) {
   FH_API_BEGIN
   FH_CHECK(dtype == FH_F64 && out, "fh_norm_frob: FP64 only");

@@ -290,13 +290,16 @@ static void autotune(Batch<T>& B, hipStream_t st) {
       }
       DBuf dt(sizeof(int3) * tl.size());
       FH_HIP(hipMemcpy(dt.p, tl.data(), sizeof(int3) * tl.size(), hipMemcpyHostToDevice));
-      float ms = 0.f;
+      // the fastest of 3 timed launches, so that one slow launch does not decide the tile
+      float ms = 1e30f;
       for (int rep = 0; rep < 3; ++rep) {
         FH_HIP(hipEventRecord(e0, st));
         launch_bgemm<T>(g->jobs.template as<BGemmJob<T>>(), dt.as<int3>(), (int)tl.size(), code, sc, st);
         FH_HIP(hipEventRecord(e1, st));
         FH_HIP(hipEventSynchronize(e1));
-        FH_HIP(hipEventElapsedTime(&ms, e0, e1));
+        float t = 0.f;
+        FH_HIP(hipEventElapsedTime(&t, e0, e1));
+        ms = std::min(ms, t);
       }
       if (ms < best) {
         best = ms;

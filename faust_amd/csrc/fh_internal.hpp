@@ -198,6 +198,8 @@ void eye_f64(double* x, int m, int n, hipStream_t st);
 // projectors on dense col-major M (k per row/col/global); exact partial_sort tie semantics
 void prox_f64(int name, double* M, int m, int n, int64_t k, bool normalized, bool pos, const double* supp,
               hipStream_t st);
+// normcol / normlin: every column / row scaled to 2-norm s (faust_prox.hpp:205-255)
+void prox_real_f64(int name, double* M, int m, int n, double s, bool normalized, bool pos, hipStream_t st);
 
 // power iteration over a chain of (factor, op) applied last to first.
 struct ChainOp {

@@ -313,7 +313,15 @@ struct DevCons {
   DBuf supp;  // device copy of the support matrix (FH_PROX_SUPP)
 };
 
-// ConstraintInt/ConstraintMat::project_gen (faust_prox_gen.hpp:118-210)
+// ConstraintInt / ConstraintMat / ConstraintFPP::project on a dense device matrix
+static void apply_cons(const fh_constraint& c, const double* dsupp, double* M, hipStream_t st) {
+  if (c.name == FH_PROX_NORMCOL || c.name == FH_PROX_NORMLIN)
+    prox_real_f64(c.name, M, c.nrows, c.ncols, c.s, c.normalized != 0, c.pos != 0, st);
+  else
+    prox_f64(c.name, M, c.nrows, c.ncols, c.k, c.normalized != 0, c.pos != 0, dsupp, st);
+}
+
+// ConstraintInt/ConstraintMat/ConstraintFPP::project_gen (faust_prox_gen.hpp:1-25,118-210)
 static DMatP project_gen(const fh_constraint& c, const double* dsupp, DMatP D, int factor_format, hipStream_t st) {
   const int64_t m = c.nrows, n = c.ncols;
   bool out_dense = false;
@@ -324,10 +332,12 @@ static DMatP project_gen(const fh_constraint& c, const double* dsupp, DMatP D, i
     case FH_PROX_SPLIN: out_dense = sparse_size(c.k * n, m) > dense_size(m, n); break;
     default: break;
   }
-  prox_f64(c.name, D->d.as<double>(), c.nrows, c.ncols, c.k, c.normalized != 0, c.pos != 0, dsupp, st);
+  apply_cons(c, dsupp, D->d.as<double>(), st);
   D->nnz = -1;
   if (factor_format == 2) {
-    if (c.name == FH_PROX_SUPP) out_dense = sparse_size(nnz_of(*D, st), m) > dense_size(m, n);
+    // supp, normcol, normlin: the format follows the real nonzero count
+    if (c.name == FH_PROX_SUPP || c.name == FH_PROX_NORMCOL || c.name == FH_PROX_NORMLIN)
+      out_dense = sparse_size(nnz_of(*D, st), m) > dense_size(m, n);
     if (out_dense) return D;
     return to_sparse(*D, st);
   }
@@ -429,7 +439,14 @@ static void palm4msa2(Palm& P, Chain& S, double& lambda, hipStream_t st) {
         grad1(pL[f_id], pR[f_id], S[f_id], A, lambda, c, *D, P.sc, error, st);
       else
         grad2(pL[f_id], pR[f_id], A, lambda, c, *S[f_id], *D, P.sc, error, st);
-      S[f_id] = project_gen(cons[f_id], P.cons[f_id]->supp.as<double>(), D, p.factor_format, st);
+      DMatP upd = project_gen(cons[f_id], P.cons[f_id]->supp.as<double>(), D, p.factor_format, st);
+      if (p.factor_format == 2)
+        S[f_id] = upd;  // AllDynamic: S.replace, the partial products keep the old object
+      else
+        // AllDense / AllSparse: S.update (faust_TransformHelper.hpp:848-868)
+        // overwrites the factor object, so every partial product holding it
+        // (unpacked chains, a packed chain of one dense factor) sees the update
+        *S[f_id] = std::move(*upd);
       if (p.is_update_way_R2L) {
         if (f_id > 0) {
           Chain c2{S[f_id]};
@@ -625,8 +642,7 @@ struct Palm16 {
       const fh_constraint& cc = cons[f]->c;
       sub_f64(S[f].M->d.as<double>(), g->d.as<double>(), (int64_t)cc.nrows * cc.ncols, st);
       S[f].zeros = S[f].ident = false;
-      prox_f64(cc.name, S[f].M->d.as<double>(), cc.nrows, cc.ncols, cc.k, cc.normalized != 0, cc.pos != 0,
-               cons[f]->supp.as<double>(), st);
+      apply_cons(cc, cons[f]->supp.as<double>(), S[f].M->d.as<double>(), st);
       S[f].M->nnz = -1;
       LorR = R2L ? mul16(S[f], LorR, st) : mul16(LorR, S[f], st);
     }
@@ -651,7 +667,7 @@ static std::vector<std::unique_ptr<DevCons>> upload_cons(const fh_constraint* c,
   for (int i = 0; i < n; ++i) {
     auto d = std::make_unique<DevCons>();
     d->c = c[i];
-    FH_CHECK(c[i].name >= FH_PROX_SP && c[i].name <= FH_PROX_SUPP, "unknown constraint name");
+    FH_CHECK(c[i].name >= FH_PROX_SP && c[i].name <= FH_PROX_NORMLIN, "unknown constraint name");
     if (c[i].name == FH_PROX_SUPP) {
       FH_CHECK(c[i].supp != nullptr, "supp constraint needs its support matrix");
       size_t b = sizeof(double) * (size_t)c[i].nrows * c[i].ncols;

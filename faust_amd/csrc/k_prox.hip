@@ -1,4 +1,4 @@
-// k_prox.hip -- sp / spcol / splin / splincol / supp projectors (FP64).
+// k_prox.hip -- sp / spcol / splin / splincol / supp / normcol / normlin projectors (FP64).
 //
 // The selected support must equal the reference's std::partial_sort result
 // bit for bit, ties included.  A workgroup radix select finds the k-th largest
@@ -204,9 +204,49 @@ __global__ void k_union(const double* __restrict__ Mc, const double* __restrict_
   }
 }
 
+__global__ void k_fill_zero(double* __restrict__ M, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) M[i] = 0.0;
+}
+
 __global__ void k_hadamard(double* __restrict__ M, const double* __restrict__ S, int64_t n) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) M[i] = M[i] * S[i];
+}
+
+// normcol / normlin (faust_prox.hpp:205-255): one 256-lane workgroup per
+// line scales it to 2-norm s.  The line's squared norm is the canonical dot
+// (4096-element chunks, lane t takes t, t+256, ..., a halving tree per chunk,
+// chunk partials added in order), so the scale s/||line|| has the oracle's bits.
+__global__ __launch_bounds__(256) void k_line_norm_scale(double* __restrict__ M, int m, int n, int along_rows,
+                                                         double s) {
+  __shared__ double p[256];
+  const int line = blockIdx.x, t = threadIdx.x;
+  const int len = along_rows ? n : m;
+  const int64_t stride = along_rows ? m : 1;
+  double* x = along_rows ? M + line : M + (int64_t)line * m;
+  double tot = 0.0;
+  for (int c0 = 0; c0 < len; c0 += RED_CHUNK) {
+    double acc = 0.0;
+    for (int j = 0; j < RED_CHUNK / RED_LANES; ++j) {
+      const int i = c0 + j * RED_LANES + t;
+      if (i < len) {
+        const double v = x[i * stride];
+        acc = fma(v, v, acc);
+      }
+    }
+    p[t] = acc;
+    __syncthreads();
+    for (int h = RED_LANES / 2; h >= 1; h >>= 1) {
+      if (t < h) p[t] = p[t] + p[t + h];
+      __syncthreads();
+    }
+    tot = tot + p[0];
+    __syncthreads();
+  }
+  double scale = sqrt(tot);
+  if (scale != 0.0) scale = s / scale;
+  for (int i = t; i < len; i += RED_LANES) x[i * stride] = x[i * stride] * scale;
 }
 
 static inline dim3 g1(int64_t n) { return dim3((unsigned)((n + 255) / 256)); }
@@ -309,6 +349,25 @@ void prox_f64(int name, double* M, int m, int n, int64_t k, bool normalized, boo
     default:
       throw Error(FH_ERR_ARG, "prox: unknown constraint");
   }
+  if (normalized) normalize_inplace(M, N, st);
+}
+
+void prox_real_f64(int name, double* M, int m, int n, double s, bool normalized, bool pos, hipStream_t st) {
+  const int64_t N = (int64_t)m * n;
+  FH_CHECK(name == FH_PROX_NORMCOL || name == FH_PROX_NORMLIN, "prox: unknown real-parameter constraint");
+  if (N == 0) return;
+  if (pos) {
+    hipLaunchKernelGGL(k_pos, g1(N), dim3(256), 0, st, M, N);
+    FH_LAUNCH_CHECK();
+  }
+  if (s < 0) throw Error(FH_ERR_ARG, "Faust::prox_normcol : s < 0 ");
+  if (s == 0) {
+    hipLaunchKernelGGL(k_fill_zero, g1(N), dim3(256), 0, st, M, N);
+  } else {
+    const bool rows = name == FH_PROX_NORMLIN;
+    hipLaunchKernelGGL(k_line_norm_scale, dim3(rows ? m : n), dim3(256), 0, st, M, m, n, rows ? 1 : 0, s);
+  }
+  FH_LAUNCH_CHECK();
   if (normalized) normalize_inplace(M, N, st);
 }
 

@@ -1,6 +1,7 @@
 """Parameter classes of the factorization algorithms (pyfaust.factparams
 counterpart; wrapper/python/pyfaust/factparams.py).  Only the constraint
-types on the hot path are provided: sp, spcol, splin, splincol, supp."""
+types on the hot path are provided: sp, spcol, splin, splincol, supp, and
+normcol / normlin (the reference's PALM4MSA known-answer test uses normcol)."""
 import os
 
 import numpy as np
@@ -39,12 +40,14 @@ class StoppingCriterion:
 
 class ConstraintName:
     SP, SPCOL, SPLIN, SPLINCOL, SUPP = "sp", "spcol", "splin", "splincol", "supp"
+    NORMCOL, NORMLIN = "normcol", "normlin"
 
     def __init__(self, name):
         if isinstance(name, ConstraintName):
             name = name.name
         if name not in PROX:
-            raise ValueError(f"constraint {name!r} is not supported (sp, spcol, splin, splincol, supp)")
+            raise ValueError(f"constraint {name!r} is not supported "
+                             "(sp, spcol, splin, splincol, supp, normcol, normlin)")
         self.name = name
 
 
@@ -77,6 +80,9 @@ class ConstraintGeneric:
             keep.append(S)
             c.supp = S.ctypes.data_as(C.POINTER(C.c_double))
             c.k = 0
+        elif self._name.name in ("normcol", "normlin"):
+            c.k = 0
+            c.s = float(self._cons_value)
         else:
             c.k = int(self._cons_value)
         return c
@@ -89,6 +95,16 @@ class ConstraintInt(ConstraintGeneric):
         super().__init__(name, num_rows, num_cols, int(cons_value), normalized, pos)
 
 
+class ConstraintReal(ConstraintGeneric):
+    """normcol / normlin with a real s, the 2-norm imposed on every column /
+    row (factparams.py:700-745; not normalized by default)."""
+
+    def __init__(self, name, num_rows, num_cols, cons_value, normalized=False, pos=False):
+        super().__init__(name, num_rows, num_cols, float(cons_value), normalized, pos)
+        if self._name.name not in ("normcol", "normlin"):
+            raise ValueError("ConstraintReal is for normcol / normlin")
+
+
 class ConstraintMat(ConstraintGeneric):
     """supp with a 0/1 support matrix."""
 
@@ -98,14 +114,17 @@ class ConstraintMat(ConstraintGeneric):
 
 
 class ConstraintList:
-    """ConstraintList('splin', 5, 500, 32, 'normcol'...) subset: (name, k, nrows, ncols) tuples."""
+    """ConstraintList('splin', 5, 500, 32, 'normcol', 1.0, 32, 32, ...): (name, value, nrows, ncols) groups."""
 
     def __init__(self, *args):
         self.clist = []
         i = 0
         while i < len(args):
             name, k, nr, nc = args[i:i + 4]
-            self.clist.append(ConstraintInt(name, nr, nc, k))
+            if name in ("normcol", "normlin"):
+                self.clist.append(ConstraintReal(name, nr, nc, k))
+            else:
+                self.clist.append(ConstraintInt(name, nr, nc, k))
             i += 4
 
     def __iter__(self):

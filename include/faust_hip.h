@@ -150,9 +150,17 @@ int fh_faust_spectral_norm(fh_faust_t F, int max_iter, double threshold, double 
  *  splincol and supp are native here, the reference falls back to the CPU:
  *  faust_prox_gpu.hpp:103-108).  Tie handling reproduces std::partial_sort
  *  with comparator |a| > |b| (faust_prox.hpp:53-69). */
-enum { FH_PROX_SP = 0, FH_PROX_SPCOL = 1, FH_PROX_SPLIN = 2, FH_PROX_SPLINCOL = 3, FH_PROX_SUPP = 4 };
+enum { FH_PROX_SP = 0, FH_PROX_SPCOL = 1, FH_PROX_SPLIN = 2, FH_PROX_SPLINCOL = 3, FH_PROX_SUPP = 4,
+       FH_PROX_NORMCOL = 5, FH_PROX_NORMLIN = 6 };
 int fh_prox(int dtype, int name, void *M, int nrows, int ncols, int64_t k, int normalized, int pos,
             const void *supp, void *stream);
+/* normcol / normlin (Faust::prox_normcol / prox_normlin, faust_prox.hpp:205-255;
+ * the reference's GPU2 form runs them on the CPU, faust_prox_gpu.hpp:137-150):
+ * every column (row) scaled to 2-norm s, zero lines left zero, s == 0 zeroes
+ * the matrix, s < 0 is an FH_ERR_ARG; then the optional Frobenius
+ * normalisation.  FP64 only. */
+int fh_prox_real(int dtype, int name, void *M, int nrows, int ncols, double s, int normalized, int pos,
+                 void *stream);
 
 /* ---- canonical FP64 reductions (dsm_funcs->norm_frob / dot); synchronize. */
 int fh_norm_frob(int dtype, const void *M, int64_t n, double *out, void *stream);
@@ -164,9 +172,10 @@ typedef struct {
   int name;          /* FH_PROX_* */
   int64_t k;
   int nrows, ncols;
-  int normalized;    /* ConstraintInt default: 1 */
+  int normalized;    /* ConstraintInt default: 1; ConstraintReal (normcol/normlin): 0 */
   int pos;           /* default: 0 */
   const double *supp;/* FH_PROX_SUPP: host nrows*ncols col-major */
+  double s;          /* FH_PROX_NORMCOL / FH_PROX_NORMLIN: the 2-norm of every column / row */
 } fh_constraint;
 
 typedef struct {

@@ -21,7 +21,9 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "liboracle.so")
 
 OR_DENSE, OR_SPARSE, OR_BSR = 0, 1, 2
-CONS = {"sp": 0, "spcol": 1, "splin": 2, "splincol": 3, "supp": 4}
+CONS = {"sp": 0, "spcol": 1, "splin": 2, "splincol": 3, "supp": 4, "normcol": 5, "normlin": 6}
+# projectors whose parameter is a real (the norm imposed on each column / row)
+REAL_CONS = ("normcol", "normlin")
 
 
 class or_fac(C.Structure):
@@ -37,7 +39,7 @@ class or_fac(C.Structure):
 class or_cons(C.Structure):
     _fields_ = [("name", C.c_int), ("k", C.c_int), ("nrows", C.c_int),
                 ("ncols", C.c_int), ("normalized", C.c_int), ("pos", C.c_int),
-                ("supp", C.POINTER(C.c_double))]
+                ("supp", C.POINTER(C.c_double)), ("s", C.c_double)]
 
 
 class or_stop(C.Structure):
@@ -193,8 +195,13 @@ def power_x0(n):
 
 
 def make_cons(name, k, nrows, ncols, normalized=True, pos=False, supp=None):
+    """k is the integer parameter, or the real norm s for normcol / normlin."""
     c = or_cons()
-    c.name, c.k, c.nrows, c.ncols = CONS[name], int(k), nrows, ncols
+    c.name, c.nrows, c.ncols = CONS[name], nrows, ncols
+    if name in REAL_CONS:
+        c.k, c.s = 0, float(k)
+    else:
+        c.k = int(k)
     c.normalized, c.pos = int(normalized), int(pos)
     keep = None
     if supp is not None:

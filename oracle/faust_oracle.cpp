@@ -512,6 +512,34 @@ void prox_supp(std::vector<double>& M, const double* S, bool normalized, bool po
   if (normalized) normalize(M);
 }
 
+// prox_normcol (faust_prox.hpp:205-247): every column scaled to 2-norm s
+// (a zero column stays zero), then the optional Frobenius normalisation.
+// The column norm is Vect::norm, here the canonical dot's square root.
+void prox_normcol(std::vector<double>& M, int m, int n, double s, bool normalized, bool pos) {
+  if (pos) pre_prox_pos(M);
+  if (s < 0) throw std::runtime_error("Faust::prox_normcol : s < 0 ");
+  if (s == 0)
+    std::fill(M.begin(), M.end(), 0.0);
+  else
+    for (int j = 0; j < n; ++j) {
+      double* col = M.data() + (size_t)j * m;
+      double scale = std::sqrt(rdot(col, col, m));
+      if (scale != 0.0) scale = s / scale;
+      for (int i = 0; i < m; ++i) col[i] *= scale;
+    }
+  if (normalized) normalize(M);
+}
+
+// prox_normlin (faust_prox.hpp:249-255): normcol of the transpose
+void prox_normlin(std::vector<double>& M, int m, int n, double s, bool normalized, bool pos) {
+  std::vector<double> T((size_t)m * n);
+  for (int j = 0; j < n; ++j)
+    for (int i = 0; i < m; ++i) T[(size_t)i * n + j] = M[(size_t)j * m + i];
+  prox_normcol(T, n, m, s, normalized, pos);
+  for (int j = 0; j < n; ++j)
+    for (int i = 0; i < m; ++i) M[(size_t)j * m + i] = T[(size_t)i * n + j];
+}
+
 void apply_prox(const or_cons& c, std::vector<double>& M) {
   switch (c.name) {
     case OR_SP: prox_sp(M, c.nrows, c.ncols, c.k, c.normalized, c.pos); break;
@@ -522,6 +550,8 @@ void apply_prox(const or_cons& c, std::vector<double>& M) {
       if (!c.supp) throw std::runtime_error("supp constraint without support matrix");
       prox_supp(M, c.supp, c.normalized, c.pos);
       break;
+    case OR_NORMCOL: prox_normcol(M, c.nrows, c.ncols, c.s, c.normalized, c.pos); break;
+    case OR_NORMLIN: prox_normlin(M, c.nrows, c.ncols, c.s, c.normalized, c.pos); break;
     default: throw std::runtime_error("unknown constraint");
   }
 }
@@ -542,7 +572,9 @@ MatP project_gen(const or_cons& c, Mat D) {
     default: out_dense = false; break;  // decided after projection from the real nnz
   }
   apply_prox(c, M);
-  if (c.name == OR_SUPP) out_dense = sparse_size(D.nnz(), m) > dense_size(m, n);
+  // supp, normcol, normlin (faust_prox_gen.hpp:159-170, 1-25): from the real nnz
+  if (c.name == OR_SUPP || c.name == OR_NORMCOL || c.name == OR_NORMLIN)
+    out_dense = sparse_size(D.nnz(), m) > dense_size(m, n);
   if (out_dense) return std::make_shared<Mat>(D);
   return std::make_shared<Mat>(sparse_of_dense(D));
 }
@@ -923,11 +955,18 @@ void palm4msa2(Palm& P, Chain& S, double& lambda) {
       if (p.factor_format == 2) {
         S.f[f_id] = project_gen(cons, std::move(D));
       } else {
+        // AllDense / AllSparse: S.update (faust_TransformHelper.hpp:848-868)
+        // overwrites the factor object itself.  The left/right partial
+        // products that hold this object (unpacked chains, and a packed chain
+        // of one dense factor, which pack_factors leaves as it is,
+        // faust_TransformHelper.hpp:1680-1690) see the new values; AllDynamic
+        // (above) swaps in a new object and leaves them on the old one.
         apply_prox(cons, D.d);
+        Mat& target = const_cast<Mat&>(*S.f[f_id]);  // created non-const by make_shared<Mat>
         if (p.factor_format == AllSparse)
-          S.f[f_id] = std::make_shared<Mat>(sparse_of_dense(D));
+          target = sparse_of_dense(D);
         else
-          S.f[f_id] = std::make_shared<Mat>(D);
+          target = std::move(D);
       }
       // next_fid
       if (p.is_update_way_R2L) {

@@ -45,7 +45,7 @@ typedef struct {
 } or_fac;
 
 /* constraint names (subset of faust_ConstraintGeneric.h:49-70) */
-enum { OR_SP = 0, OR_SPCOL = 1, OR_SPLIN = 2, OR_SPLINCOL = 3, OR_SUPP = 4 };
+enum { OR_SP = 0, OR_SPCOL = 1, OR_SPLIN = 2, OR_SPLINCOL = 3, OR_SUPP = 4, OR_NORMCOL = 5, OR_NORMLIN = 6 };
 
 typedef struct {
   int name;
@@ -54,6 +54,7 @@ typedef struct {
   int normalized;
   int pos;
   const double *supp; /* OR_SUPP only, nrows*ncols col-major */
+  double s;           /* OR_NORMCOL / OR_NORMLIN: the 2-norm imposed on every column / row */
 } or_cons;
 
 typedef struct {
