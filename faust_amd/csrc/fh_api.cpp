@@ -116,22 +116,37 @@ DMatP make_dense(int dtype, int m, int n, const void* data, int ld, bool on_devi
   return M;
 }
 
-// Y = op(F) X for an FP64 chain; X, Y col-major with ld = rows.
-// ws: two reusable intermediates, grown on demand (hipMalloc only when a
-// larger product than any before is requested).
-void faust_multiply_f64(const std::vector<DMatP>& f, char op, const double* X, int p, double* Y, hipStream_t st,
-                        DBuf* ws, FaustTimers* tm, int* launches) {
-  FH_CHECK(!f.empty(), "Faust::multiply: empty Faust");
-  const int J = (int)f.size();
+// Bytes of each of the (at most two) intermediates of an FP64 chain product;
+// *nbuf: how many of them the product uses.
+size_t faust_ws_need_f64(const std::vector<DMatP>& f, char op, int p, int* nbuf) {
   int maxr = 0;
   for (auto& x : f) maxr = std::max(maxr, op == 'N' ? x->m : x->n);
-  const size_t need = sizeof(double) * std::max<size_t>((size_t)maxr * p, 1);
-  for (int b = 0; b < 2 && b < J - 1; ++b)
+  *nbuf = std::min(2, (int)f.size() - 1);
+  return sizeof(double) * std::max<size_t>((size_t)maxr * p, 1);
+}
+
+// Grow a Faust's own intermediates to `need` bytes (hipMalloc only when a
+// larger product than any before is requested) and return their addresses.
+ChainWs faust_ws_grow(DBuf* ws, size_t need, int nbuf, hipStream_t st) {
+  ChainWs w{{nullptr, nullptr}};
+  for (int b = 0; b < nbuf; ++b) {
     if (ws[b].bytes < need) {
       FH_HIP(hipStreamSynchronize(st));  // the old buffer may still be in use
       ws[b].alloc(need);
     }
-  DBuf &t1 = ws[0], &t2 = ws[1];
+    w.p[b] = ws[b].p;
+  }
+  return w;
+}
+
+// Y = op(F) X for an FP64 chain; X, Y col-major with ld = rows; ws holds the
+// intermediates (faust_ws_need_f64 bytes each).
+void faust_multiply_f64(const std::vector<DMatP>& f, char op, const double* X, int p, double* Y, hipStream_t st,
+                        const ChainWs& ws, FaustTimers* tm, int* launches) {
+  FH_CHECK(!f.empty(), "Faust::multiply: empty Faust");
+  const int J = (int)f.size();
+  double* const t1 = static_cast<double*>(ws.p[0]);
+  double* const t2 = static_cast<double*>(ws.p[1]);
   const double* in = X;
   int in_rows = (op == 'N') ? f.back()->n : f.front()->m;
   if (launches) *launches = J;
@@ -167,7 +182,7 @@ void faust_multiply_f64(const std::vector<DMatP>& f, char op, const double* X, i
     const DMat& S = (op == 'N') ? *f[J - 1 - i] : *f[i];
     const int out_rows = (op == 'N') ? S.m : S.n;
     FH_CHECK(((op == 'N') ? S.n : S.m) == in_rows, "Faust::multiply: factor dimensions do not chain");
-    double* out = (i == J - 1) ? Y : ((i & 1) ? t2.as<double>() : t1.as<double>());
+    double* out = (i == J - 1) ? Y : ((i & 1) ? t2 : t1);
     FH_CHECK(S.dtype == FH_F64, "Faust::multiply: FP64 chain expected");
     const size_t slot = (op == 'N') ? (size_t)(J - 1 - i) : (size_t)i;
     if (tm) tm->begin(slot, st);
@@ -500,10 +515,16 @@ int fh_faust_multiply(fh_faust_t F, char op, const void* X, int p, void* Y, void
   }
   FH_CHECK(X && Y, "null argument");
   hipStream_t st = (hipStream_t)stream;
+  int nbuf = 0;
   if (F->f.front()->dtype == FH_F64) {
-    faust_multiply_f64(F->f, op, (const double*)X, p, (double*)Y, st, F->ws, &F->timers, &F->last_launches);
+    const size_t need = faust_ws_need_f64(F->f, op, p, &nbuf);
+    const ChainWs ws = faust_ws_grow(F->ws, need, nbuf, st);
+    faust_multiply_f64(F->f, op, (const double*)X, p, (double*)Y, st, ws, &F->timers, &F->last_launches);
   } else {
-    F->last_launches = faust_multiply_mixed(fp32_chain_factors(F->f, st), op, X, p, Y, st, F->ws, &F->timers);
+    const std::vector<DMatP> f = fp32_chain_factors(F->f, st);
+    const size_t need = faust_ws_need_mixed(f, op, p, &nbuf);
+    const ChainWs ws = faust_ws_grow(F->ws, need, nbuf, st);
+    F->last_launches = faust_multiply_mixed(f, op, X, p, Y, st, ws, &F->timers);
   }
   FH_API_END
 }

@@ -102,8 +102,15 @@ namespace fh {
 DMatP make_csr_from_host(int dtype, int m, int n, int nnz, const int32_t* rp, const int32_t* ci, const void* vals,
                          hipStream_t st);
 DMatP make_dense(int dtype, int m, int n, const void* data, int ld, bool on_device, hipStream_t st);
+// The intermediates of one chain product: device addresses, each at least the
+// bytes faust_ws_need_* gives (unused entries may be null).
+struct ChainWs {
+  void* p[2];
+};
+size_t faust_ws_need_f64(const std::vector<DMatP>& f, char op, int p, int* nbuf);
+ChainWs faust_ws_grow(DBuf* ws, size_t need, int nbuf, hipStream_t st);
 void faust_multiply_f64(const std::vector<DMatP>& f, char op, const double* X, int p, double* Y, hipStream_t st,
-                        DBuf* ws, FaustTimers* tm = nullptr, int* launches = nullptr);
+                        const ChainWs& ws, FaustTimers* tm = nullptr, int* launches = nullptr);
 DMatP faust_get_product_f64(const std::vector<DMatP>& f, hipStream_t st);
 double faust_spectral_norm_f64(const std::vector<DMatP>& f, int maxit, double thr, int* flag, hipStream_t st);
 void faust_spectral_norm_pair_f64(const std::vector<DMatP>& f1, const std::vector<DMatP>& f2, int maxit, double thr,
@@ -113,6 +120,7 @@ void mixed_to_dense(const DMat& M, void* dense_dev, hipStream_t st);
 void bsr_to_host(const DMat& M, void* bdata, int32_t* browptr, int32_t* bcolinds);
 void mixed_mul_dense(const DMat& S, char op, const void* X, int ldx, int ncols, void* Y, int ldy, const void* alpha,
                      const void* beta, hipStream_t st);
+size_t faust_ws_need_mixed(const std::vector<DMatP>& f, char op, int p, int* nbuf);
 int faust_multiply_mixed(const std::vector<DMatP>& f, char op, const void* X, int p, void* Y, hipStream_t st,
-                          DBuf* ws, FaustTimers* tm = nullptr);
+                          const ChainWs& ws, FaustTimers* tm = nullptr);
 }  // namespace fh

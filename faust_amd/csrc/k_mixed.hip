@@ -987,10 +987,10 @@ void mixed_mul_dense(const DMat& S, char op, const void* X, int ldx, int ncols, 
   FH_HIP(hipStreamSynchronize(st));  // R returns to the pool
 }
 
-// Chain product for BSR (BF16 activations) or FP32 CSR chains.  Returns the
-// number of product kernels launched.
-int faust_multiply_mixed(const std::vector<DMatP>& f, char op, const void* X, int p, void* Y, hipStream_t st,
-                          DBuf* ws, FaustTimers* tm) {
+// Bytes of each intermediate of a BSR or FP32 CSR chain product (*nbuf of
+// them): BSR intermediates are tiled, p padded to whole 32-column groups;
+// FP32 CSR ones are row-major with ld = p rounded up to 4.
+size_t faust_ws_need_mixed(const std::vector<DMatP>& f, char op, int p, int* nbuf) {
   const int J = (int)f.size();
   const bool bsr = f[0]->kind == K_BSR;
   for (auto& x : f) {
@@ -999,26 +999,29 @@ int faust_multiply_mixed(const std::vector<DMatP>& f, char op, const void* X, in
   }
   int maxr = 0;
   for (auto& x : f) maxr = std::max(maxr, op == 'N' ? x->m : x->n);
-  const int in_rows0 = (op == 'N') ? f.back()->n : f.front()->m;
-  maxr = std::max(maxr, in_rows0);
+  maxr = std::max(maxr, (op == 'N') ? f.back()->n : f.front()->m);
+  *nbuf = bsr ? std::min(2, J - 1) : std::min(2, J);
+  const int ldr = (p + 3) & ~3;
+  return bsr ? sizeof(u16) * std::max<size_t>((size_t)maxr * (((size_t)p + 31) / 32 * 32), 1)
+             : sizeof(float) * std::max<size_t>((size_t)maxr * ldr, 1);
+}
+
+// Chain product for BSR (BF16 activations) or FP32 CSR chains, intermediates
+// in ws (faust_ws_need_mixed bytes each).  Returns the number of product
+// kernels launched.
+int faust_multiply_mixed(const std::vector<DMatP>& f, char op, const void* X, int p, void* Y, hipStream_t st,
+                          const ChainWs& ws, FaustTimers* tm) {
+  const int J = (int)f.size();
+  const bool bsr = f[0]->kind == K_BSR;
   const int ldr = bsr ? 0 : (p + 3) & ~3;
-  // BSR intermediates are tiled: p padded to whole 32-column groups
-  const size_t need = bsr ? sizeof(u16) * std::max<size_t>((size_t)maxr * (((size_t)p + 31) / 32 * 32), 1)
-                          : sizeof(float) * std::max<size_t>((size_t)maxr * ldr, 1);
-  const int nbuf = bsr ? std::min(2, J - 1) : std::min(2, J);
-  for (int b = 0; b < nbuf; ++b)
-    if (ws[b].bytes < need) {
-      FH_HIP(hipStreamSynchronize(st));
-      ws[b].alloc(need);
-    }
-  int in_rows = in_rows0;
+  int in_rows = (op == 'N') ? f.back()->n : f.front()->m;
   if (bsr) {
     const void* in = X;
     for (int i = 0; i < J; ++i) {
       const DMat& S = (op == 'N') ? *f[J - 1 - i] : *f[i];
       const int out_rows = (op == 'N') ? S.m : S.n;
       FH_CHECK(((op == 'N') ? S.n : S.m) == in_rows, "Faust::multiply: factor dimensions do not chain");
-      void* out = (i == J - 1) ? Y : ws[i & 1].p;
+      void* out = (i == J - 1) ? Y : ws.p[i & 1];
       const size_t slot = (op == 'N') ? (size_t)(J - 1 - i) : (size_t)i;
       if (tm) tm->begin(slot, st);
       // intermediates stay in the tiled layout; X and Y are column-major
@@ -1031,15 +1034,15 @@ int faust_multiply_mixed(const std::vector<DMatP>& f, char op, const void* X, in
   }
   // FP32 CSR: transpose the input once, chain in row-major, last factor writes column-major
   if (tm) tm->begin(J, st);
-  col2row(in_rows, p, (const float*)X, in_rows, ws[0].as<float>(), ldr, st);
+  col2row(in_rows, p, (const float*)X, in_rows, static_cast<float*>(ws.p[0]), ldr, st);
   if (tm) tm->end(J, st);
-  const float* in = ws[0].as<float>();
+  const float* in = static_cast<const float*>(ws.p[0]);
   for (int i = 0; i < J; ++i) {
     const DMat& S = (op == 'N') ? *f[J - 1 - i] : *f[i];
     const int out_rows = (op == 'N') ? S.m : S.n;
     FH_CHECK(((op == 'N') ? S.n : S.m) == in_rows, "Faust::multiply: factor dimensions do not chain");
     const bool last = (i == J - 1);
-    float* out = last ? (float*)Y : ws[(i + 1) & 1].as<float>();
+    float* out = last ? (float*)Y : static_cast<float*>(ws.p[(i + 1) & 1]);
     const size_t slot = (op == 'N') ? (size_t)(J - 1 - i) : (size_t)i;
     if (tm) tm->begin(slot, st);
     csr_f32_mul_rm(S, op, in, ldr, p, out, last ? out_rows : ldr, last, 1.0f, 0.0f, st);

@@ -107,6 +107,20 @@ int fh_faust_free(fh_faust_t F);
  * per-factor op for 'T'/'H', two ping-pong intermediates.  p = 0 is a no-op
  * (X and Y may then be null). */
 int fh_faust_multiply(fh_faust_t F, char op, const void *X, int p, void *Y, void *stream);
+/* fh_faust_multiply without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The two ping-pong intermediates live in a fixed per-GPU device array of
+ * FH_NOALLOC_SCRATCH_BYTES (held from library load to process end) instead of
+ * the Faust's own buffers; the kernels, and so the results, are those of
+ * fh_faust_multiply.  A product whose intermediates need more than the array
+ * (each rounded up to 256 bytes) is refused with FH_ERR_ARG.  FP32 chains with
+ * dense FP32 factors need their CSR twins built by one fh_faust_multiply call
+ * first (FH_ERR_UNSUPPORTED otherwise).  Per-factor timers are not recorded.
+ * Every call on one GPU shares the array: calls that overlap on one GPU (other
+ * streams, other threads) race and must be serialized by the caller. */
+#define FH_NOALLOC_SCRATCH_BYTES (32u << 20)
+int fh_faust_multiply_noalloc(fh_faust_t F, char op, const void *X, int p, void *Y, void *stream);
 /* Number of product kernels the last fh_faust_multiply on F launched (one
  * per factor, plus the input transpose of FP32 CSR chains).  No counterpart
  * in the reference; for benchmarks and tests. */
