@@ -127,14 +127,21 @@ size_t faust_ws_need_f64(const std::vector<DMatP>& f, char op, int p, int* nbuf)
 
 // Grow a Faust's own intermediates to `need` bytes (hipMalloc only when a
 // larger product than any before is requested) and return their addresses.
-ChainWs faust_ws_grow(DBuf* ws, size_t need, int nbuf, hipStream_t st) {
-  ChainWs w{{nullptr, nullptr}};
+ChainWs faust_ws_grow(DBuf* ws, size_t need, int nbuf, hipStream_t st, DBuf* ctl) {
+  ChainWs w{{nullptr, nullptr}, nullptr};
   for (int b = 0; b < nbuf; ++b) {
     if (ws[b].bytes < need) {
       FH_HIP(hipStreamSynchronize(st));  // the old buffer may still be in use
       ws[b].alloc(need);
     }
     w.p[b] = ws[b].p;
+  }
+  if (ctl) {
+    if (ctl->bytes < bsr_chain_ctl_bytes()) {
+      ctl->alloc(bsr_chain_ctl_bytes());
+      bsr_chain_ctl_clear(ctl->as<int>(), st);
+    }
+    w.ctl = ctl->as<int>();
   }
   return w;
 }
@@ -523,7 +530,7 @@ int fh_faust_multiply(fh_faust_t F, char op, const void* X, int p, void* Y, void
   } else {
     const std::vector<DMatP> f = fp32_chain_factors(F->f, st);
     const size_t need = faust_ws_need_mixed(f, op, p, &nbuf);
-    const ChainWs ws = faust_ws_grow(F->ws, need, nbuf, st);
+    const ChainWs ws = faust_ws_grow(F->ws, need, nbuf, st, f.front()->kind == K_BSR ? &F->ctl : nullptr);
     F->last_launches = faust_multiply_mixed(f, op, X, p, Y, st, ws, &F->timers);
   }
   FH_API_END

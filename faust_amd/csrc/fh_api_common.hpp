@@ -80,6 +80,7 @@ struct FaustTimers {
 struct fh_faust_s {
   std::vector<fh::DMatP> f;
   fh::DBuf ws[2];         // ping-pong activations
+  fh::DBuf ctl;           // control block of the one-launch BSR chain
   int last_launches = 0;  // kernel launches of the last product
   int mul_mode = 0;       // FH_MUL_*
   fh::FaustTimers timers;
@@ -106,9 +107,10 @@ DMatP make_dense(int dtype, int m, int n, const void* data, int ld, bool on_devi
 // bytes faust_ws_need_* gives (unused entries may be null).
 struct ChainWs {
   void* p[2];
+  int* ctl;  // control block of the one-launch BSR chain (bsr_chain_ctl_bytes; zeroed when allocated), or null
 };
 size_t faust_ws_need_f64(const std::vector<DMatP>& f, char op, int p, int* nbuf);
-ChainWs faust_ws_grow(DBuf* ws, size_t need, int nbuf, hipStream_t st);
+ChainWs faust_ws_grow(DBuf* ws, size_t need, int nbuf, hipStream_t st, DBuf* ctl = nullptr);
 void faust_multiply_f64(const std::vector<DMatP>& f, char op, const double* X, int p, double* Y, hipStream_t st,
                         const ChainWs& ws, FaustTimers* tm = nullptr, int* launches = nullptr);
 DMatP faust_get_product_f64(const std::vector<DMatP>& f, hipStream_t st);
@@ -121,6 +123,14 @@ void bsr_to_host(const DMat& M, void* bdata, int32_t* browptr, int32_t* bcolinds
 void mixed_mul_dense(const DMat& S, char op, const void* X, int ldx, int ncols, void* Y, int ldy, const void* alpha,
                      const void* beta, hipStream_t st);
 size_t faust_ws_need_mixed(const std::vector<DMatP>& f, char op, int p, int* nbuf);
+// one-launch BSR chain (k_bsr_chain.hip)
+size_t bsr_chain_ctl_bytes();
+void bsr_chain_ctl_clear(int* ctl, hipStream_t st);
+int bsr_chain_report(int* ctl, hipStream_t st);
+int bsr_chain_sync_mode();
+bool bsr_chain_fits(const std::vector<DMatP>& f, char op);
+void bsr_chain_launch(const std::vector<DMatP>& f, char op, const void* X, int p, void* Y, void* T0, void* T1,
+                      int* ctl, hipStream_t st);
 int faust_multiply_mixed(const std::vector<DMatP>& f, char op, const void* X, int p, void* Y, hipStream_t st,
                           const ChainWs& ws, FaustTimers* tm = nullptr);
 }  // namespace fh

@@ -6,12 +6,14 @@
 // entry points give the same bits.
 //
 // `scratch` is one FH_NOALLOC_SCRATCH_BYTES array per GPU, held from module
-// load to process end.  In host code its name is only a host-side stand-in
+// load to process end; BSR chains keep the control block of the one-launch
+// chain at its start.  In host code its name is only a host-side stand-in
 // for the symbol: the device address comes from hipGetSymbolAddress, once per
 // device (with that device current), and only that address is handed to
 // kernels.  Every call on one GPU shares the array; calls that overlap on one
 // GPU (two streams, two threads) race on it.  The intermediates are written
-// before they are read within a call, so nothing needs clearing between calls.
+// before they are read within a call, and the chain's counters are cleared by
+// the chain itself at its end, so nothing needs clearing between calls.
 #include <mutex>
 
 #include "fh_api_common.hpp"
@@ -78,12 +80,17 @@ extern "C" int fh_faust_multiply_noalloc(fh_faust_t F, char op, const void* X, i
   int nbuf = 0;
   const size_t need = f64 ? faust_ws_need_f64(f, op, p, &nbuf) : faust_ws_need_mixed(f, op, p, &nbuf);
   const size_t slot = align256(need);
-  if ((size_t)nbuf * slot > kScratchBytes)
+  // BSR chains also keep the one-launch chain's control block at the start of
+  // the array (zero at load; every chain leaves its counters cleared)
+  const bool bsr = f.front()->kind == K_BSR;
+  const size_t ctl_bytes = bsr ? align256(bsr_chain_ctl_bytes()) : 0;
+  if (ctl_bytes + (size_t)nbuf * slot > kScratchBytes)
     throw Error(FH_ERR_ARG, "fh_faust_multiply_noalloc: this product needs " + std::to_string(nbuf) + " x " +
                                 std::to_string(need) + " bytes of intermediates, more than the " +
                                 std::to_string(kScratchBytes) + "-byte scratch array");
-  char* base = nbuf > 0 ? static_cast<char*>(scratch_address()) : nullptr;
-  const ChainWs ws{{base, nbuf > 1 ? base + slot : nullptr}};
+  char* base = (nbuf > 0 || bsr) ? static_cast<char*>(scratch_address()) : nullptr;
+  const ChainWs ws{{nbuf > 0 ? base + ctl_bytes : nullptr, nbuf > 1 ? base + ctl_bytes + slot : nullptr},
+                   bsr ? reinterpret_cast<int*>(base) : nullptr};
   if (f64)
     faust_multiply_f64(f, op, (const double*)X, p, (double*)Y, st, ws, nullptr, &F->last_launches);
   else

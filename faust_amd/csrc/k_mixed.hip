@@ -33,25 +33,11 @@
 #include <algorithm>
 #include <vector>
 
+#include "bsr_frag.hpp"
 #include "fh_api_common.hpp"
 
 namespace fh {
 
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef float f32x16 __attribute__((ext_vector_type(16)));
-typedef unsigned short u16;
-typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
-
-__device__ __forceinline__ float bf2f(u16 h) { return __uint_as_float(((unsigned)h) << 16); }
-// Two FP32 -> packed BF16 (lo = a, hi = b), round to nearest even: one
-// v_cvt_pk_bf16_f32 on gfx950 (inf stays inf, NaN stays NaN).
-typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
-__device__ __forceinline__ unsigned f2bf2(float a, float b) {
-  bf16x2 r;
-  r[0] = (__bf16)a;
-  r[1] = (__bf16)b;
-  return __builtin_bit_cast(unsigned, r);
-}
 static inline u16 host_f2bf(float f) {
   unsigned u;
   std::memcpy(&u, &f, 4);
@@ -64,52 +50,6 @@ static inline float host_bf2f(u16 h) {
   float f;
   std::memcpy(&f, &u, 4);
   return f;
-}
-
-// ---------------------------------------------------------------- BSR MFMA
-// Activation layouts: column-major (ld), or "tiled" (fragment order): the
-// columns are cut into G = ceil(p/32) groups of 32 (the last one padded), and
-// the 32 x 32 tile of block-row rb, column group g is 1024 consecutive
-// entries at (rb*G + g)*1024, entry (r, c) of the tile at
-//   ((r / 16)*64 + ((r / 8) % 2)*32 + c)*8 + r % 8.
-// That is the MFMA B-operand order: for the K half h the 64 lanes read 16
-// bytes each at (h*64 + lane)*8, one contiguous 1 KiB, and the accumulator of
-// a 32x32 output tile leaves as four contiguous 512-byte stores.  Chains keep
-// their intermediates tiled; only the first factor reads and the last factor
-// writes column-major.  Padding columns (p..32G) are never written and only
-// feed accumulator columns that are never stored.
-__device__ __forceinline__ size_t frag_tile(int rb, int G, int g) { return ((size_t)rb * G + g) * 1024; }
-// offset, inside a tile, of the 4 rows 8*gg + 4*kh .. +3 of column c (one
-// accumulator quarter of lane (c, kh))
-__device__ __forceinline__ int frag_store_off(int gg, int kh, int c) {
-  return (((gg >> 1) * 64 + (gg & 1) * 32 + c) << 3) + 4 * kh;
-}
-// Wave work: one block-row x 64 columns (two 32x32 accumulators sharing the
-// A operand); workgroup: 4 block-rows x 64 columns.
-// Epilogue of 4 consecutive rows of one column: y = bf16(alpha*acc) or, with
-// beta != 0 (a wave-uniform test), bf16(fma(beta, y_old, alpha*acc)); the
-// old values are one 8-byte load.
-template <bool NT = false>
-__device__ __forceinline__ void bsr_store4(u16* yq, float a0, float a1, float a2, float a3, float alpha,
-                                           float beta) {
-  float v0 = alpha * a0, v1 = alpha * a1, v2 = alpha * a2, v3 = alpha * a3;
-  if (beta != 0.0f) {
-    const uint2 old = *reinterpret_cast<const uint2*>(yq);
-    v0 = fmaf(beta, __uint_as_float(old.x << 16), v0);
-    v1 = fmaf(beta, __uint_as_float(old.x & 0xffff0000u), v1);
-    v2 = fmaf(beta, __uint_as_float(old.y << 16), v2);
-    v3 = fmaf(beta, __uint_as_float(old.y & 0xffff0000u), v3);
-  }
-  const unsigned wx = f2bf2(v0, v1), wy = f2bf2(v2, v3);
-  if (NT) {
-    u32x2 wv = {wx, wy};
-    __builtin_nontemporal_store(wv, reinterpret_cast<u32x2*>(yq));
-  } else {
-    uint2 w;
-    w.x = wx;
-    w.y = wy;
-    *reinterpret_cast<uint2*>(yq) = w;
-  }
 }
 
 template <bool SPLIT, bool IN_TILED, bool OUT_TILED>
@@ -987,6 +927,34 @@ void mixed_mul_dense(const DMat& S, char op, const void* X, int ldx, int ncols, 
   FH_HIP(hipStreamSynchronize(st));  // R returns to the pool
 }
 
+// The one-launch BSR chain is checked on its first use in the process, and
+// after every use with FH_BSR_CHAIN_CHECK=1
+// (then a bad report is an error); a variant whose check fails is not used
+// again.  The check synchronizes, so it is skipped while the stream is being
+// captured into a graph.
+static int g_chain_state[1] = {0};  // 0 unchecked, 1 verified, -1 disabled
+
+static bool chain_usable(int sync) { return g_chain_state[sync] >= 0; }
+
+static bool chain_verified(int sync, int* ctl, hipStream_t st) {
+  const char* e = getenv("FH_BSR_CHAIN_CHECK");
+  const bool always = e && e[0] == '1';
+  if (g_chain_state[sync] == 1 && !always) return true;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  FH_HIP(hipStreamIsCapturing(st, &cs));
+  if (cs != hipStreamCaptureStatusNone) return true;
+  const int code = bsr_chain_report(ctl, st);
+  if (code != 0) {
+    if (always)
+      throw Error(FH_ERR_NUMERIC, "one-launch BSR chain reported code " + std::to_string(code) +
+                                      " (1: a wait ran out, 2: an XCD team did not run)");
+    g_chain_state[sync] = -1;
+    return false;
+  }
+  g_chain_state[sync] = 1;
+  return true;
+}
+
 // Bytes of each intermediate of a BSR or FP32 CSR chain product (*nbuf of
 // them): BSR intermediates are tiled, p padded to whole 32-column groups;
 // FP32 CSR ones are row-major with ld = p rounded up to 4.
@@ -1015,6 +983,16 @@ int faust_multiply_mixed(const std::vector<DMatP>& f, char op, const void* X, in
   const bool bsr = f[0]->kind == K_BSR;
   const int ldr = bsr ? 0 : (p + 3) & ~3;
   int in_rows = (op == 'N') ? f.back()->n : f.front()->m;
+  if (bsr && ws.ctl) {
+    const int sync = bsr_chain_sync_mode();
+    if (sync >= 0 && bsr_chain_fits(f, op) && chain_usable(sync)) {
+      if (tm) tm->begin(0, st);
+      bsr_chain_launch(f, op, X, p, Y, ws.p[0], ws.p[1], ws.ctl, st);
+      if (tm) tm->end(0, st);
+      if (chain_verified(sync, ws.ctl, st)) return 1;
+      // the chain reported an invalid output: recompute factor by factor
+    }
+  }
   if (bsr) {
     const void* in = X;
     for (int i = 0; i < J; ++i) {

@@ -193,9 +193,6 @@ def test_bsr_chain_launch_count(gpu, dtype):
     assert n.value == 3
 
 
-@pytest.mark.skip(reason="the single-launch persistent BSR chain (per-XCD work claiming, bounded waits) did not "
-                         "finish on the GPU in its first run and no cause was shown; the path was removed and "
-                         "chains run one kernel per factor")
 @pytest.mark.parametrize("dtype", ["bf16", np.float32])
 @pytest.mark.parametrize("op", ["N", "T"])
 def test_bsr_persistent_chain_matches_per_factor(gpu, monkeypatch, dtype, op):
@@ -209,12 +206,14 @@ def test_bsr_persistent_chain_matches_per_factor(gpu, monkeypatch, dtype, op):
     G = F if op == "N" else F.T
     for p in (512, 200, 8):
         X = bf16_round(rng.standard_normal((2048, p)))
-        monkeypatch.setenv("FH_BSR_PERSIST", "1")
+        # the one-launch chain (k_bsr_chain.hip), every call checked
+        monkeypatch.setenv("FH_BSR_CHAIN", "1")
+        monkeypatch.setenv("FH_BSR_CHAIN_CHECK", "1")
         Y1 = G @ X
         n = C.c_int()
         _lib.check(_lib.lib().fh_faust_last_launches(G.h, C.byref(n)))
         assert n.value == 1
-        monkeypatch.setenv("FH_BSR_PERSIST", "0")
+        monkeypatch.setenv("FH_BSR_CHAIN", "0")
         monkeypatch.setenv("FH_BSR_KERNEL", "2")
         Y2 = G @ X
         monkeypatch.delenv("FH_BSR_KERNEL")

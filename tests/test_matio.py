@@ -1,6 +1,7 @@
 """Faust .mat files on the host: the reference layout (faust_factors cell;
-BSR as the 5-cell of faust_MatBSR.hpp:387-480) read back by the reference's
-own reader, restated here from pyfaust.py:2620-2635."""
+BSR as the 5-cell written by MatBSR::toMatIOVar, faust_MatBSR.hpp:387-480)
+checked against a decoder written from that writer's description, element
+by element."""
 import numpy as np
 import pytest
 import scipy.sparse as sp
@@ -14,29 +15,35 @@ def _dense(m):
     return m.toarray() if sp.issparse(m) else np.asarray(m)
 
 
-def _reference_reader(path):
-    # pyfaust Faust.load: cells as returned by loadmat, BSR cells rebuilt
-    # from their column-major blocks.  The isinstance guard keeps the check
-    # to cell entries: scipy >= 1.15 returns sparse cells as coo_array, which
-    # the reference's unguarded factors[i][0][0] cannot index.
-    factors = loadmat(path)["faust_factors"][0].tolist()
-    for i in range(len(factors)):
-        is_cell = isinstance(factors[i], np.ndarray) and factors[i].dtype == object
-        if is_cell and factors[i][0][0].dtype == "<U3" and factors[i][0][0][0] == "bsr":
-            nrows, ncols, bnnz = factors[i][0][1][0]
-            bcolinds = factors[i][0][2][0]
-            browptr = factors[i][0][3][0]
-            bdata = factors[i][0][4][0]
-            bnrows = int(nrows / (browptr.shape[0] - 1))
-            bncols = int(bdata.shape[0] / bnrows / bnnz)
-            bdata_ = np.empty((bnnz, bnrows, bncols))
-            for bi in range(bnnz):
-                bdata_[bi] = bdata.reshape(bnnz, bncols, bnrows)[bi].T
-            factors[i] = sp.bsr_matrix((bdata_, bcolinds, browptr), shape=(nrows, ncols))
-    return factors
+def _decode_cell(c):
+    """Dense matrix of one faust_factors cell.  A BSR cell holds 'bsr',
+    int32 [nrows, ncols, bnnz], int32 block columns (1 x bnnz), int32 block
+    row pointers (1 x nbr+1) and the blocks as one bm x (bnnz*bn)
+    column-major matrix (the Eigen map of the block data in toMatIOVar):
+    entry (i, j) of block b sits at index (b*bn + j)*bm + i."""
+    if sp.issparse(c):
+        return c.toarray()
+    if isinstance(c, np.ndarray) and c.dtype == object:
+        cells = c.reshape(-1)
+        assert str(cells[0].reshape(-1)[0]) == "bsr"
+        nrows, ncols, bnnz = (int(v) for v in cells[1].reshape(-1))
+        bcol = cells[2].reshape(-1)
+        bptr = cells[3].reshape(-1)
+        data = cells[4].reshape(-1)
+        nbr = bptr.shape[0] - 1
+        bm = nrows // nbr
+        bn = data.shape[0] // (bm * bnnz) if bnnz else 0
+        out = np.zeros((nrows, ncols), dtype=data.dtype)
+        for br in range(nbr):
+            for b in range(int(bptr[br]), int(bptr[br + 1])):
+                for j in range(bn):
+                    for i in range(bm):
+                        out[br * bm + i, int(bcol[b]) * bn + j] = data[(b * bn + j) * bm + i]
+        return out
+    return np.asarray(c)
 
 
-def test_round_trip_and_reference_reader(tmp_path):
+def test_round_trip_and_cell_layout(tmp_path):
     rng = np.random.default_rng(0)
     facs = [sp.random(20, 30, density=0.2, random_state=1, format="csr"),
             rng.standard_normal((30, 64)),
@@ -49,10 +56,9 @@ def test_round_trip_and_reference_reader(tmp_path):
     for a, b in zip(facs, back):
         assert np.array_equal(_dense(a), _dense(b))
     assert back[2].blocksize == (32, 32) and back[3].blocksize == (2, 4)
-    ref = _reference_reader(path)
-    for a, b in zip(facs, ref):
-        assert np.array_equal(_dense(a), _dense(b.toarray() if sp.issparse(b) else b))
     raw = loadmat(path)["faust_factors"]
+    for a, c in zip(facs, raw.reshape(-1)):
+        assert np.array_equal(_dense(a), _decode_cell(c))
     assert raw.shape == (1, 4)
     assert sp.issparse(raw[0, 0]) and raw[0, 0].dtype == np.float64
     params = raw[0, 2][0, 1]
