@@ -246,7 +246,9 @@ class Faust:
         """Execution forms worth timing for this chain (fh_faust_set_mul_mode)."""
         kinds = [f.info()[:2] for f in self._facs]
         if any(k == _lib.FH_MAT_BSR for k, _ in kinds):
-            return {f"BSR_KERNEL_{k}": _lib.FH_MUL_BSR_KERNEL(k) for k in (5, 1, 3, 2, 4)}
+            c = {f"BSR_KERNEL_{k}": _lib.FH_MUL_BSR_KERNEL(k) for k in (5, 1, 3, 2, 4)}
+            c["FUSED"] = _lib.FH_MUL_FUSED  # the one-launch chain (square factors of one order)
+            return c
         if all(d == _lib.FH_F64 for _, d in kinds):
             if all(k == _lib.FH_MAT_CSR for k, _ in kinds):
                 return {"PER_FACTOR": _lib.FH_MUL_PER_FACTOR, "FUSED": _lib.FH_MUL_FUSED}
