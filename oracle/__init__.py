@@ -281,12 +281,14 @@ def _cons_array(cons_list):
     return arr, keep
 
 
-def palm4msa(A, cons, num_its=500, lambda_init=1.0, init_facts=None, record_traj=False, tol=None, **params):
-    """cons: list of (name, k, nrows, ncols[, normalized[, pos[, supp]]])."""
+def palm4msa(A, cons, num_its=500, lambda_init=1.0, init_facts=None, record_traj=False, tol=None, maxiter=10000,
+             **params):
+    """cons: list of (name, k, nrows, ncols[, normalized[, pos[, supp]]]).
+    tol: error threshold (StoppingCriterion's error branch) with maxiter."""
     A = np.asfortranarray(A, dtype=np.float64)
     ca, keep = _cons_array(cons)
     p = _params(**params)
-    s = _stop(num_its, tol)
+    s = _stop(num_its, tol, maxiter)
     res = C.c_void_p()
     if init_facts:
         pk = _FacPack(init_facts)

@@ -100,3 +100,25 @@ def test_prox_normcol_normlin(gpu, name):
             got = P(M)
             want = oracle.prox(name, M, s, normalized=normalized, pos=pos)
             assert np.array_equal(got, want), (shape, s, normalized, pos)
+
+
+@pytest.mark.parametrize("rel_tol,use_grad1,its", [(0.9, False, 2), (0.7, False, 6), (0.62, True, 14),
+                                                 (1e-9, False, 25)])
+def test_palm4msa_error_stopping_criterion(gpu, rel_tol, use_grad1, its):
+    # StoppingCriterion's error branch (faust_StoppingCriterion.hpp:113-150):
+    # stop once the residual norm measured at the last factor update drops
+    # below tol, else after maxiter iterations; tol relative to ||A||
+    # (relerr=True).  The GPU stops at the same iteration as the oracle, with
+    # the same bits.
+    rng = np.random.default_rng(31)
+    A = np.asfortranarray(rng.standard_normal((32, 24)))
+    cons = [factparams.ConstraintInt("splin", 32, 16, 4), factparams.ConstraintInt("sp", 16, 16, 64),
+            factparams.ConstraintInt("spcol", 16, 24, 3)]
+    sc = factparams.StoppingCriterion(tol=rel_tol, maxiter=25, relerr=True, relmat=A)
+    p = factparams.ParamsPalm4MSA(cons, sc)
+    p.use_grad1 = use_grad1
+    res = fact.palm4msa(A, p, ret_raw=True)
+    ref = oracle.palm4msa(A, [("splin", 4, 32, 16), ("sp", 64, 16, 16), ("spcol", 3, 16, 24)],
+                          tol=rel_tol * np.linalg.norm(A), maxiter=25, record_traj=True, use_grad1=use_grad1)
+    _same(res, ref)
+    assert res.traj.shape[0] == its
