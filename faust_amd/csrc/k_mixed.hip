@@ -1188,6 +1188,7 @@ static DMatP bsr_as_csr(int dtype, int m, int n, int bm, int bn, int bnnz, const
   const int nbr = m / bm;
   std::vector<int32_t> rp(m + 1, 0), ci;
   std::vector<float> v;
+  std::vector<double> v64;  // FP64 blocks keep their values (and the canonical FP64 products)
   for (int br = 0; br < nbr; ++br) {
     std::vector<int> order;
     for (int b = browptr[br]; b < browptr[br + 1]; ++b) order.push_back(b);
@@ -1196,13 +1197,18 @@ static DMatP bsr_as_csr(int dtype, int m, int n, int bm, int bn, int bnnz, const
       for (int b : order)
         for (int k = 0; k < bn; ++k) {
           const size_t e = (size_t)b * bm * bn + (size_t)k * bm + i;
-          float x = dtype == FH_BF16 ? host_bf2f(static_cast<const u16*>(bdata)[e]) : static_cast<const float*>(bdata)[e];
           ci.push_back(bcolinds[b] * bn + k);
-          v.push_back(x);
+          if (dtype == FH_F64)
+            v64.push_back(static_cast<const double*>(bdata)[e]);
+          else
+            v.push_back(dtype == FH_BF16 ? host_bf2f(static_cast<const u16*>(bdata)[e])
+                                         : static_cast<const float*>(bdata)[e]);
         }
       rp[br * bm + i + 1] = (int32_t)ci.size();
     }
   }
+  if (dtype == FH_F64)
+    return make_csr_from_host(FH_F64, m, n, (int)ci.size(), rp.data(), ci.data(), v64.data(), st);
   return make_csr_from_host(FH_F32, m, n, (int)ci.size(), rp.data(), ci.data(), v.data(), st);
 }
 
@@ -1232,7 +1238,7 @@ extern "C" int fh_mat_bsr_create(int dtype, int nrows, int ncols, int bm, int bn
     bcolinds = hc.data();
     bdata = hd.data();
   }
-  if (bm == 32 && bn == 32)
+  if (bm == 32 && bn == 32 && dtype != FH_F64)  // FP64 BSR factors run as CSR (canonical FP64 order)
     *out = new fh_mat_s{make_bsr(dtype, nrows, ncols, bnnz, browptr, bcolinds, bdata, nullptr)};
   else
     *out = new fh_mat_s{bsr_as_csr(dtype, nrows, ncols, bm, bn, bnnz, browptr, bcolinds, bdata, nullptr)};

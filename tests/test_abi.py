@@ -6,17 +6,20 @@ import re
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def _declared():
-    txt = open(os.path.join(ROOT, "include", "faust_hip.h")).read()
+def _declared(header="faust_hip.h"):
+    txt = open(os.path.join(ROOT, "include", header)).read()
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
-    return sorted(set(re.findall(r"\b(fh_[a-z0-9_]+)\s*\(", txt)))
+    # functions only: type names (fh_..._t) in function-pointer members are not symbols
+    return sorted(n for n in set(re.findall(r"\b(fh_[a-z0-9_]+)\s*\(", txt)) if not n.endswith("_t"))
 
 
 def test_header_symbols_exported():
     from faust_amd import _lib
     lib = C.CDLL(_lib.LIB_PATH)
-    names = _declared()
-    assert len(names) >= 30
+    headers = sorted(f for f in os.listdir(os.path.join(ROOT, "include")) if f.endswith(".h"))
+    assert headers == ["faust_gm.h", "faust_hip.h"]
+    names = [n for h in headers for n in _declared(h)]
+    assert len(names) >= 38
     missing = [n for n in names if not hasattr(lib, n)]
     assert not missing, missing
 
