@@ -70,6 +70,8 @@ _SIGS = {
     "fh_gemm": (C.c_int, [C.c_int, C.c_char, C.c_char, C.c_int, C.c_int, C.c_int, VP, VP, C.c_int, VP, C.c_int, VP,
                           VP, C.c_int, VP]),
     "fh_faust_multiply_noalloc": (C.c_int, [VP, C.c_char, VP, C.c_int, VP, VP]),
+    "fh_faust_set_act_dtype": (C.c_int, [VP, C.c_int]),
+    "fh_faust_get_act_dtype": (C.c_int, [VP, C.POINTER(C.c_int)]),
     "fh_faust_create": (C.c_int, [C.POINTER(VP)]),
     "fh_faust_push_back": (C.c_int, [VP, VP]),
     "fh_faust_size": (C.c_int, [VP, C.POINTER(C.c_int)]),

@@ -472,6 +472,32 @@ int fh_faust_set_mul_mode(fh_faust_t F, int mode) {
   FH_API_END
 }
 
+int fh_faust_set_act_dtype(fh_faust_t F, int dtype) {
+  FH_API_BEGIN
+  FH_CHECK(F, "null handle");
+  FH_CHECK(dtype == FH_BF16 || dtype == FH_F32, "activation dtype must be FH_BF16 or FH_F32");
+  if (dtype == FH_F32)
+    for (auto& x : F->f)
+      if (x->kind != K_BSR || x->dtype != FH_F32)
+        throw Error(FH_ERR_UNSUPPORTED, "FP32 activations are offered for chains of FP32 BSR factors");
+  for (auto& x : F->f) FH_CHECK(x->kind == K_BSR, "the activation dtype is chosen for BSR chains");
+  F->act_dtype = dtype;
+  FH_API_END
+}
+
+int fh_faust_get_act_dtype(fh_faust_t F, int* dtype) {
+  FH_API_BEGIN
+  FH_CHECK(F && dtype, "null argument");
+  FH_CHECK(!F->f.empty(), "empty Faust");
+  if (F->f.front()->dtype == FH_F64)
+    *dtype = FH_F64;
+  else if (F->f.front()->kind == K_BSR)
+    *dtype = F->act_dtype == FH_F32 ? FH_F32 : FH_BF16;
+  else
+    *dtype = FH_F32;
+  FH_API_END
+}
+
 int fh_faust_get_mul_mode(fh_faust_t F, int* mode) {
   FH_API_BEGIN
   FH_CHECK(F && mode, "null argument");
@@ -529,9 +555,11 @@ int fh_faust_multiply(fh_faust_t F, char op, const void* X, int p, void* Y, void
     faust_multiply_f64(F->f, op, (const double*)X, p, (double*)Y, st, ws, &F->timers, &F->last_launches);
   } else {
     const std::vector<DMatP> f = fp32_chain_factors(F->f, st);
-    const size_t need = faust_ws_need_mixed(f, op, p, &nbuf);
-    const ChainWs ws = faust_ws_grow(F->ws, need, nbuf, st, f.front()->kind == K_BSR ? &F->ctl : nullptr);
-    F->last_launches = faust_multiply_mixed(f, op, X, p, Y, st, ws, &F->timers);
+    const bool act_f32 = F->act_dtype == FH_F32;
+    const size_t need = faust_ws_need_mixed(f, op, p, &nbuf, act_f32);
+    const ChainWs ws =
+        faust_ws_grow(F->ws, need, nbuf, st, (f.front()->kind == K_BSR && !act_f32) ? &F->ctl : nullptr);
+    F->last_launches = faust_multiply_mixed(f, op, X, p, Y, st, ws, &F->timers, act_f32);
   }
   FH_API_END
 }

@@ -83,6 +83,7 @@ struct fh_faust_s {
   fh::DBuf ctl;           // control block of the one-launch BSR chain
   int last_launches = 0;  // kernel launches of the last product
   int mul_mode = 0;       // FH_MUL_*
+  int act_dtype = -1;     // activations of a BSR chain: FH_BF16 (default) or FH_F32 (FP32 BSR factors)
   fh::FaustTimers timers;
 };
 
@@ -122,7 +123,7 @@ void mixed_to_dense(const DMat& M, void* dense_dev, hipStream_t st);
 void bsr_to_host(const DMat& M, void* bdata, int32_t* browptr, int32_t* bcolinds);
 void mixed_mul_dense(const DMat& S, char op, const void* X, int ldx, int ncols, void* Y, int ldy, const void* alpha,
                      const void* beta, hipStream_t st);
-size_t faust_ws_need_mixed(const std::vector<DMatP>& f, char op, int p, int* nbuf);
+size_t faust_ws_need_mixed(const std::vector<DMatP>& f, char op, int p, int* nbuf, bool act_f32 = false);
 // one-launch BSR chain (k_bsr_chain.hip)
 size_t bsr_chain_ctl_bytes();
 void bsr_chain_ctl_clear(int* ctl, hipStream_t st);
@@ -132,5 +133,8 @@ bool bsr_chain_fits(const std::vector<DMatP>& f, char op);
 void bsr_chain_launch(const std::vector<DMatP>& f, char op, const void* X, int p, void* Y, void* T0, void* T1,
                       int* ctl, hipStream_t st);
 int faust_multiply_mixed(const std::vector<DMatP>& f, char op, const void* X, int p, void* Y, hipStream_t st,
-                          const ChainWs& ws, FaustTimers* tm = nullptr);
+                          const ChainWs& ws, FaustTimers* tm = nullptr, bool act_f32 = false);
+// Y = alpha op(B) X + beta Y, B an FP32 BSR factor, X / Y FP32 column-major
+void bsr_mul_f32(const DMat& B, char op, const float* X, int ldx, int p, float* Y, int ldy, float alpha, float beta,
+                 hipStream_t st);
 }  // namespace fh

@@ -153,8 +153,12 @@ fh_gm_mat_t mul_ext(fh_gm_mat_t S, fh_gm_mat_t X, fh_gm_mat_t out, int opS, int 
       else
         gemm_f64(os == 'N' ? 'N' : 'T', 'N', rows, cols, inner, (double)a, Sm.d.as<double>(), Sm.m,
                  xin->d.as<double>(), std::max(inner, 1), (double)b, o->m->d.as<double>(), rows, nullptr);
+    } else if (Sm.kind == K_BSR && Sm.dtype == FH_F32) {
+      // FP32 blocks, FP32 activations (MatBSR<float>::multiply's precision)
+      bsr_mul_f32(Sm, os, xin->d.as<float>(), std::max(inner, 1), cols, o->m->d.as<float>(), rows, (float)a,
+                  (float)b, nullptr);
     } else if (Sm.kind == K_BSR) {
-      // FP32 / BF16 blocks: BF16 activations on MFMA, converted around the product
+      // BF16 blocks: BF16 activations on MFMA, converted around the product
       DBuf xb(sizeof(uint16_t) * std::max<size_t>((size_t)inner * cols, 1)),
           yb(sizeof(uint16_t) * std::max<size_t>((size_t)rows * cols, 1));
       f32_to_bf16(xin->d.as<float>(), (int64_t)inner * cols, xb.as<uint16_t>(), nullptr);

@@ -119,6 +119,10 @@ struct DMat {
   // ell_w == 0 when the widest block row is too wide for it.
   int ell_w = 0, tell_w = 0;
   DBuf ell, tell;
+  // FP32 handles also keep their blocks in FP32 (k_bsr_f32's operand order:
+  // lane l = 32*h + r holds row r, columns 16*h .. 16*h + 15 of the block, 16
+  // consecutive floats), in bdat's (and tbdat's) block order, zero block last
+  DBuf f32dat, tf32dat;
   // the blocks again in ELL order: slot (br, q) at (br * ell_w + q) blocks
   // (zero blocks in empty slots), so a kernel finds a slot's block without
   // loading the slot first; tebdat likewise for the transpose

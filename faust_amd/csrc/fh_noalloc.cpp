@@ -78,11 +78,12 @@ extern "C" int fh_faust_multiply_noalloc(fh_faust_t F, char op, const void* X, i
         x = x->csr_twin;
       }
   int nbuf = 0;
-  const size_t need = f64 ? faust_ws_need_f64(f, op, p, &nbuf) : faust_ws_need_mixed(f, op, p, &nbuf);
+  const bool act_f32 = F->act_dtype == FH_F32;
+  const size_t need = f64 ? faust_ws_need_f64(f, op, p, &nbuf) : faust_ws_need_mixed(f, op, p, &nbuf, act_f32);
   const size_t slot = align256(need);
   // BSR chains also keep the one-launch chain's control block at the start of
   // the array (zero at load; every chain leaves its counters cleared)
-  const bool bsr = f.front()->kind == K_BSR;
+  const bool bsr = f.front()->kind == K_BSR && !act_f32;
   const size_t ctl_bytes = bsr ? align256(bsr_chain_ctl_bytes()) : 0;
   if (ctl_bytes + (size_t)nbuf * slot > kScratchBytes)
     throw Error(FH_ERR_ARG, "fh_faust_multiply_noalloc: this product needs " + std::to_string(nbuf) + " x " +
@@ -94,6 +95,6 @@ extern "C" int fh_faust_multiply_noalloc(fh_faust_t F, char op, const void* X, i
   if (f64)
     faust_multiply_f64(f, op, (const double*)X, p, (double*)Y, st, ws, nullptr, &F->last_launches);
   else
-    F->last_launches = faust_multiply_mixed(f, op, X, p, Y, st, ws, nullptr);
+    F->last_launches = faust_multiply_mixed(f, op, X, p, Y, st, ws, nullptr, act_f32);
   FH_API_END
 }

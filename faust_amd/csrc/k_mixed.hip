@@ -534,6 +534,76 @@ static void bsr_mul(const DMat& B, char op, const void* X, int ldx, int p, void*
   FH_LAUNCH_CHECK();
 }
 
+// ---------------------------------------------------------------- FP32 BSR, FP32 activations
+// Y = alpha*op(B)*X + beta*Y with FP32 blocks and FP32 X / Y (column-major),
+// the arithmetic of MatBSR<float>::multiply (faust_MatBSR.hpp:828-836) on
+// v_mfma_f32_32x32x2_f32: a wave owns one block row x one 32-column tile and
+// runs 16 K-steps per block; in step s lane (h, r) feeds A = B(r, 16h + s)
+// and B = X(16h + s, tile column r), so the blocks (stored that way, 16
+// consecutive floats per lane) and the X column segments are 4 float4 loads
+// per lane.  Blocks in row order (b ascending, then step by step), FP32
+// accumulation.
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+__global__ __launch_bounds__(256) void k_bsr_f32(int nbr, int p, const int* __restrict__ brp,
+                                                 const int* __restrict__ bci, const float* __restrict__ blk,
+                                                 const float* __restrict__ X, int ldx, float* __restrict__ Y, int ldy,
+                                                 float alpha, float beta, int ntiles) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int tile = blockIdx.x % ntiles, br = (blockIdx.x / ntiles) * 4 + wave;
+  if (br >= nbr) return;
+  const int h = lane >> 5, r = lane & 31;
+  const int jc = tile * 32 + r;
+  const bool jok = jc < p;
+  const int j = jok ? jc : 0;  // columns past p read column 0; their outputs are not stored
+  f32x16 acc = {};
+  const int b0 = brp[br], b1 = brp[br + 1];
+  for (int b = b0; b < b1; ++b) {
+    const int bc = bci[b];
+    const f32x4* ap = reinterpret_cast<const f32x4*>(blk + (size_t)b * 1024 + lane * 16);
+    const f32x4* xp = reinterpret_cast<const f32x4*>(X + (size_t)j * ldx + bc * 32 + h * 16);
+    f32x4 a[4], x[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      a[q] = ap[q];
+      x[q] = xp[q];
+    }
+#pragma unroll
+    for (int s4 = 0; s4 < 16; ++s4)
+      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a[s4 >> 2][s4 & 3], x[s4 >> 2][s4 & 3], acc, 0, 0, 0);
+  }
+  if (!jok) return;
+#pragma unroll
+  for (int gg = 0; gg < 4; ++gg) {
+    f32x4* yq = reinterpret_cast<f32x4*>(Y + (size_t)j * ldy + br * 32 + 8 * gg + 4 * h);
+    f32x4 v = {alpha * acc[4 * gg], alpha * acc[4 * gg + 1], alpha * acc[4 * gg + 2], alpha * acc[4 * gg + 3]};
+    if (beta != 0.0f) {
+      const f32x4 old = *yq;
+      v[0] = fmaf(beta, old[0], v[0]);
+      v[1] = fmaf(beta, old[1], v[1]);
+      v[2] = fmaf(beta, old[2], v[2]);
+      v[3] = fmaf(beta, old[3], v[3]);
+    }
+    *yq = v;
+  }
+}
+
+void bsr_mul_f32(const DMat& B, char op, const float* X, int ldx, int p, float* Y, int ldy, float alpha, float beta,
+                 hipStream_t st) {
+  FH_CHECK(B.kind == K_BSR && B.dtype == FH_F32 && B.f32dat.p, "FP32 activations need an FP32 BSR factor");
+  const bool tr = op != 'N';
+  const int rows = tr ? B.n : B.m;
+  const int nbr = rows / 32;
+  FH_CHECK(ldx % 4 == 0 && ldy % 4 == 0 && ((uintptr_t)X & 15) == 0 && ((uintptr_t)Y & 15) == 0,
+           "FP32 BSR path: X and Y must be 16-byte aligned with leading dimensions that are multiples of 4");
+  if (nbr == 0 || p == 0) return;
+  const int ntiles = (p + 31) / 32;
+  const dim3 grid((unsigned)(ntiles * ((nbr + 3) / 4)));
+  hipLaunchKernelGGL(k_bsr_f32, grid, dim3(256), 0, st, nbr, p, (tr ? B.tbrp : B.brp).as<int>(),
+                     (tr ? B.tbci : B.bci).as<int>(), (tr ? B.tf32dat : B.f32dat).as<float>(), X, ldx, Y, ldy, alpha,
+                     beta, ntiles);
+  FH_LAUNCH_CHECK();
+}
+
 // ---------------------------------------------------------------- FP32 CSR
 // A row's batch of NB (<= 16) index/value pairs reaches its 8 lanes with two
 // loads per lane: lane s of the row's group loads entries s and 8 + s (entry
@@ -958,9 +1028,16 @@ static bool chain_verified(int sync, int* ctl, hipStream_t st) {
 // Bytes of each intermediate of a BSR or FP32 CSR chain product (*nbuf of
 // them): BSR intermediates are tiled, p padded to whole 32-column groups;
 // FP32 CSR ones are row-major with ld = p rounded up to 4.
-size_t faust_ws_need_mixed(const std::vector<DMatP>& f, char op, int p, int* nbuf) {
+size_t faust_ws_need_mixed(const std::vector<DMatP>& f, char op, int p, int* nbuf, bool act_f32) {
   const int J = (int)f.size();
   const bool bsr = f[0]->kind == K_BSR;
+  if (act_f32) {
+    for (auto& x : f) FH_CHECK(x->kind == K_BSR && x->dtype == FH_F32, "FP32 activations need FP32 BSR factors");
+    int maxr = 0;
+    for (auto& x : f) maxr = std::max(maxr, op == 'N' ? x->m : x->n);
+    *nbuf = std::min(2, J - 1);
+    return sizeof(float) * std::max<size_t>((size_t)maxr * p, 1);  // column-major, ld = rows
+  }
   for (auto& x : f) {
     FH_CHECK((x->kind == K_BSR) == bsr, "Faust::multiply: BSR and CSR factors cannot be mixed in one FP32/BF16 chain");
     if (!bsr) FH_CHECK(x->kind == K_CSR && x->dtype == FH_F32, "Faust::multiply: FP32 chains need FP32 CSR factors");
@@ -978,11 +1055,29 @@ size_t faust_ws_need_mixed(const std::vector<DMatP>& f, char op, int p, int* nbu
 // in ws (faust_ws_need_mixed bytes each).  Returns the number of product
 // kernels launched.
 int faust_multiply_mixed(const std::vector<DMatP>& f, char op, const void* X, int p, void* Y, hipStream_t st,
-                          const ChainWs& ws, FaustTimers* tm) {
+                          const ChainWs& ws, FaustTimers* tm, bool act_f32) {
   const int J = (int)f.size();
   const bool bsr = f[0]->kind == K_BSR;
   const int ldr = bsr ? 0 : (p + 3) & ~3;
   int in_rows = (op == 'N') ? f.back()->n : f.front()->m;
+  if (act_f32) {
+    // FP32 BSR factors with FP32 activations: one k_bsr_f32 launch per factor,
+    // column-major FP32 intermediates
+    const float* in = static_cast<const float*>(X);
+    for (int i = 0; i < J; ++i) {
+      const DMat& S = (op == 'N') ? *f[J - 1 - i] : *f[i];
+      const int out_rows = (op == 'N') ? S.m : S.n;
+      FH_CHECK(((op == 'N') ? S.n : S.m) == in_rows, "Faust::multiply: factor dimensions do not chain");
+      float* out = (i == J - 1) ? static_cast<float*>(Y) : static_cast<float*>(ws.p[i & 1]);
+      const size_t slot = (op == 'N') ? (size_t)(J - 1 - i) : (size_t)i;
+      if (tm) tm->begin(slot, st);
+      bsr_mul_f32(S, op, in, in_rows, p, out, out_rows, 1.0f, 0.0f, st);
+      if (tm) tm->end(slot, st);
+      in = out;
+      in_rows = out_rows;
+    }
+    return J;
+  }
   if (bsr && ws.ctl) {
     const int sync = bsr_chain_sync_mode();
     if (sync >= 0 && bsr_chain_fits(f, op) && chain_usable(sync)) {
@@ -1070,6 +1165,17 @@ void bsr_to_host(const DMat& M, void* bdata, int32_t* browptr, int32_t* bcolinds
   if (browptr) FH_HIP(hipMemcpy(browptr, M.brp.p, sizeof(int) * (nbr + 1), hipMemcpyDeviceToHost));
   if (bcolinds && bnnz) FH_HIP(hipMemcpy(bcolinds, M.bci.p, sizeof(int) * bnnz, hipMemcpyDeviceToHost));
   if (!bdata || !bnnz) return;
+  if (M.dtype == FH_F32 && M.f32dat.p) {  // the exact FP32 blocks (operand order -> column-major)
+    std::vector<float> f((size_t)bnnz * 1024);
+    FH_HIP(hipMemcpy(f.data(), M.f32dat.p, sizeof(float) * f.size(), hipMemcpyDeviceToHost));
+    for (size_t b = 0; b < (size_t)bnnz; ++b)
+      for (int l = 0; l < 64; ++l)
+        for (int q = 0; q < 16; ++q) {
+          const int r = l & 31, c = (l >> 5) * 16 + q;
+          static_cast<float*>(bdata)[b * 1024 + (size_t)c * 32 + r] = f[b * 1024 + l * 16 + q];
+        }
+    return;
+  }
   std::vector<u16> dat((size_t)bnnz * 1024 * planes);
   FH_HIP(hipMemcpy(dat.data(), M.bdat.p, sizeof(u16) * dat.size(), hipMemcpyDeviceToHost));
   for (size_t b = 0; b < (size_t)bnnz; ++b) {
@@ -1146,7 +1252,31 @@ DMatP make_bsr(int dtype, int m, int n, int bnnz, const int32_t* browptr, const 
     d.alloc(std::max<size_t>(bytes, 4));
     if (bytes) FH_HIP(hipMemcpyAsync(d.p, h, bytes, hipMemcpyHostToDevice, st));
   };
+  std::vector<float> f32, tf32;
+  if (dtype == FH_F32) {
+    // exact FP32 blocks for the FP32-activation product, same block order
+    f32.assign((size_t)(bnnz + 1) * 1024, 0.0f);
+    tf32.assign(f32.size(), 0.0f);
+    std::vector<int> tslot_of(std::max(bnnz, 1));
+    {
+      std::vector<int32_t> tf(tbrp.begin(), tbrp.end() - 1);
+      for (int br = 0; br < nbr; ++br)
+        for (int b = browptr[br]; b < browptr[br + 1]; ++b) tslot_of[b] = tf[bcolinds[b]]++;
+    }
+    const float* src = static_cast<const float*>(bdata);
+    for (int b = 0; b < bnnz; ++b)
+      for (int l = 0; l < 64; ++l)
+        for (int q = 0; q < 16; ++q) {
+          const int r = l & 31, c = (l >> 5) * 16 + q;
+          f32[(size_t)b * 1024 + l * 16 + q] = src[(size_t)b * 1024 + (size_t)c * 32 + r];          // B(r, c)
+          tf32[(size_t)tslot_of[b] * 1024 + l * 16 + q] = src[(size_t)b * 1024 + (size_t)r * 32 + c];  // B(c, r)
+        }
+  }
   up(M->brp, browptr, sizeof(int) * (nbr + 1));
+  if (dtype == FH_F32) {
+    up(M->f32dat, f32.data(), sizeof(float) * f32.size());
+    up(M->tf32dat, tf32.data(), sizeof(float) * tf32.size());
+  }
   up(M->bci, bcolinds, sizeof(int) * bnnz);
   up(M->bdat, dat.data(), sizeof(u16) * (size_t)(bnnz + 1) * 1024 * planes);
   up(M->tbrp, tbrp.data(), sizeof(int) * (nbc + 1));

@@ -172,7 +172,7 @@ class Faust:
     Faust(filepath=...) loads a Faust .mat file (pyfaust.py:102, Faust.load).
     """
 
-    def __init__(self, factors=None, dtype=np.float64, _facs=None, _transposed=False, filepath=None):
+    def __init__(self, factors=None, dtype=np.float64, _facs=None, _transposed=False, filepath=None, act_dtype=None):
         _require_torch_cuda()
         L = lib()
         if isinstance(factors, str):
@@ -190,19 +190,26 @@ class Faust:
         self._dt = dtype
         kinds = [f.info()[:2] for f in self._facs]
         # activation (X, Y) dtype of the chain: FP64 chains -> float64, BSR chains
-        # (BF16 or FP32 values) -> bfloat16, FP32 CSR chains -> float32
+        # (BF16 or FP32 values) -> bfloat16 unless act_dtype asks float32 for FP32
+        # BSR factors (FP32 blocks on FP32 MFMA, the reference's MatBSR<float>
+        # precision), FP32 CSR chains -> float32
+        self.h = C.c_void_p()
+        check(L.fh_faust_create(C.byref(self.h)))
+        for f in self._facs:
+            check(L.fh_faust_push_back(self.h, f.h))
         if any(k == _lib.FH_MAT_BSR for k, _ in kinds):
             self.act_dtype = torch.bfloat16
+            if act_dtype is not None and act_dtype not in ("bf16", torch.bfloat16):
+                if act_dtype not in (np.float32, np.dtype(np.float32), "float32", torch.float32):
+                    raise ValueError("act_dtype of a BSR chain: 'bf16' or float32")
+                check(L.fh_faust_set_act_dtype(self.h, _lib.FH_F32))
+                self.act_dtype = torch.float32
         elif all(d == _lib.FH_F64 for _, d in kinds):
             self.act_dtype = torch.float64
         else:
             self.act_dtype = torch.float32
         self.dtype = {torch.float64: np.dtype(np.float64), torch.float32: np.dtype(np.float32),
                       torch.bfloat16: np.dtype(np.float32)}[self.act_dtype]
-        self.h = C.c_void_p()
-        check(L.fh_faust_create(C.byref(self.h)))
-        for f in self._facs:
-            check(L.fh_faust_push_back(self.h, f.h))
         self._mul_mode = _lib.FH_MUL_AUTO
 
     def __del__(self):
@@ -234,7 +241,7 @@ class Faust:
 
     @property
     def T(self):
-        G = Faust(_facs=self._facs, dtype=self._dt, _transposed=not self._transposed)
+        G = Faust(_facs=self._facs, dtype=self._dt, _transposed=not self._transposed, act_dtype=self.act_dtype)
         G._set_mul_mode(self._mul_mode)
         return G
 
@@ -287,7 +294,8 @@ class Faust:
         finally:
             G._set_mul_mode(saved)
         best = min(times, key=times.get)
-        target = self if inplace else Faust(_facs=self._facs, dtype=self._dt, _transposed=self._transposed)
+        target = self if inplace else Faust(_facs=self._facs, dtype=self._dt, _transposed=self._transposed,
+                                            act_dtype=self.act_dtype)
         target._set_mul_mode(self._mul_candidates()[best])
         target.mul_mode_name = best
         target.mul_mode_times_ms = times

@@ -30,9 +30,9 @@
  * All work is ordered on the null stream; tocpu waits for it.
  *
  * Coverage: double tables hold every role below; the float tables hold
- * handles and products (FP32 CSR on the GPU; FP32 BSR with BF16 activations,
- * converted from and to FP32 around the product), and their gemm / prox /
- * norm entries fail with FH_ERR_UNSUPPORTED.
+ * handles and products (FP32 CSR; FP32 BSR blocks on FP32 MFMA with FP32
+ * activations), and their gemm / prox / norm entries fail with
+ * FH_ERR_UNSUPPORTED.
  */
 #ifndef FAUST_GM_H
 #define FAUST_GM_H

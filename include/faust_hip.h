@@ -67,9 +67,8 @@ int fh_mat_info(fh_mat_t M, int *kind, int *dtype, int *nrows, int *ncols, int64
  * rowptr[nrows+1], colind[nnz], vals[nnz]; BSR (32x32 blocks): browptr
  * [nrows/32+1], bcolinds[bnnz], bdata[bnnz*1024] with the blocks column-major
  * one after the other, as BF16 bit patterns for BF16 handles and as FP32 for
- * FP32 handles, whose values come back within 2^-16 relative of the uploaded
- * ones (they are stored as a BF16 hi/lo pair)).  For BSR handles `nnz` of
- * fh_mat_info is bnnz*1024.  Any output pointer may be NULL to skip it. */
+ * FP32 handles, whose uploaded values come back exactly).  For BSR handles
+ * `nnz` of fh_mat_info is bnnz*1024.  Any output pointer may be NULL to skip it. */
 int fh_mat_tocpu(fh_mat_t M, void *data_or_vals, int32_t *rowptr, int32_t *colind);
 /* Densify into a device buffer (col-major, ld = nrows): FP64 for FP64
  * matrices, FP32 for FP32 matrices and for BSR matrices of either value type,
@@ -140,6 +139,16 @@ int fh_faust_last_launches(fh_faust_t F, int *launches);
 #define FH_MUL_CSR_ROWS1 32
 #define FH_MUL_CSR_ROWS2 33
 int fh_faust_set_mul_mode(fh_faust_t F, int mode);
+/* Activations of a BSR chain (no counterpart needed in the reference, whose
+ * MatBSR<float> products are FP32 throughout, faust_MatBSR.hpp:828-836):
+ * FH_BF16 (the default: BF16 X / Y and intermediates, FP32 accumulation on
+ * BF16 MFMA, FP32 values as a BF16 hi/lo pair) or FH_F32 (FP32 X / Y and
+ * intermediates, FP32 blocks on v_mfma_f32_32x32x2_f32: the reference's
+ * precision), the latter for chains of FP32 BSR factors only
+ * (FH_ERR_UNSUPPORTED otherwise).  fh_faust_get_act_dtype gives the dtype X
+ * and Y have in fh_faust_multiply for any chain. */
+int fh_faust_set_act_dtype(fh_faust_t F, int dtype);
+int fh_faust_get_act_dtype(fh_faust_t F, int *dtype);
 int fh_faust_get_mul_mode(fh_faust_t F, int *mode);
 /* Per-factor device timers: HIP events recorded on the launch stream around
  * each factor's kernel of every fh_faust_multiply while enabled (the

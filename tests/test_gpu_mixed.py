@@ -259,7 +259,9 @@ def test_optimize_time_bsr(gpu):
     F = faust_amd.Faust(facs, dtype="bf16")
     X = bf16_round(rng.standard_normal((1024, 256)))
     G = F.optimize_time(nsamples=2, mat=X)
-    assert G.mul_mode_name.startswith("BSR_KERNEL_") and len(G.mul_mode_times_ms) == 5
+    # five per-factor kernels and the one-launch chain
+    assert len(G.mul_mode_times_ms) == 6 and "FUSED" in G.mul_mode_times_ms
+    assert G.mul_mode_name.startswith("BSR_KERNEL_") or G.mul_mode_name == "FUSED"
     ref = oracle.chain_multiply(facs, X, "N")
     for H in (G, F):
         Y = H @ X
@@ -328,3 +330,69 @@ def test_bsr_chain_with_empty_factor(gpu, dtype):
     ref = oracle.chain_multiply([sp.csr_matrix(bf16_bits_to_f32(f32_to_bf16_bits(A.toarray())).astype(np.float64))
                                  if dtype == "bf16" else sp.csr_matrix(A)] * 2, np.asfortranarray(X), "N")
     assert np.linalg.norm(G @ X - ref) <= 2e-2 * np.linalg.norm(ref)  # BF16 activations
+
+
+def _f32(B):
+    B = sp.bsr_matrix(B)
+    return sp.bsr_matrix((B.data.astype(np.float32).astype(np.float64), B.indices, B.indptr), shape=B.shape)
+
+
+@pytest.mark.parametrize("op", ["N", "T"])
+@pytest.mark.parametrize("p", [512, 40, 1])
+def test_bsr_fp32_activations_single_product(gpu, op, p):
+    # FP32 blocks x FP32 X on v_mfma_f32_32x32x2_f32 (MatBSR<float>::multiply's
+    # precision): FP32 rounding of products summed over 96 terms, each
+    # |err| <= 96 * 2^-24 * sum |b_ik x_kj| (a standard FP32 dot-product bound)
+    rng = np.random.default_rng(21)
+    B = _f32(rand_bsr(1024, 768, 32, 3, rng, scale=1 / np.sqrt(96)))
+    rows_in = B.shape[1] if op == "N" else B.shape[0]
+    X = rng.standard_normal((rows_in, p)).astype(np.float32).astype(np.float64)
+    F = faust_amd.Faust([B], dtype=np.float32, act_dtype=np.float32)
+    assert F.act_dtype == torch.float32
+    Y = (F @ X) if op == "N" else (F.T @ X)
+    ref = oracle.chain_multiply([B], X, op)
+    absref = oracle.chain_multiply([abs(B)], np.abs(X), op)
+    assert np.all(np.abs(Y - ref) <= 100 * 2.0 ** -24 * absref + 1e-30)
+
+
+def test_bsr_fp32_chain_8192x6_full_size_against_oracle_columns(gpu):
+    # config 3a's FP32 leg at full size with FP32 activations: 6 FP32 BSR
+    # factors 8192^2 (32x32 blocks, 4 per block row, N(0, 1/128)), X 8192x512
+    # FP32; 16 columns against the FP64 oracle.  FP32 level: relative
+    # Frobenius error below 1e-5 (the BF16-activation form is at ~1e-2)
+    rng = np.random.default_rng(0xF32)
+    facs = [_f32(rand_bsr(8192, 8192, 32, 4, np.random.default_rng(0xF32 + f), scale=1 / np.sqrt(128)))
+            for f in range(6)]
+    X = rng.standard_normal((8192, 512)).astype(np.float32).astype(np.float64)
+    F = faust_amd.Faust(facs, dtype=np.float32, act_dtype=np.float32)
+    Y = F @ X
+    ref = oracle.chain_multiply(facs, X[:, :16], "N")
+    rel = np.linalg.norm(Y[:, :16] - ref) / np.linalg.norm(ref)
+    assert rel < 1e-5, rel
+    Yt = F.T @ X[:, :16]
+    reft = oracle.chain_multiply(facs, X[:, :16], "T")
+    assert np.linalg.norm(Yt - reft) / np.linalg.norm(reft) < 1e-5
+    # the same chain with BF16 activations (config 3a's stated X) is coarser
+    Fb = faust_amd.Faust(facs, dtype=np.float32)
+    Yb = Fb @ X[:, :16]
+    assert np.linalg.norm(Yb - ref) / np.linalg.norm(ref) > 10 * rel
+
+
+def test_bsr_fp32_handle_values_exact_and_act_dtype_rules(gpu):
+    import ctypes as C
+    from faust_amd import _lib
+    rng = np.random.default_rng(23)
+    B = sp.bsr_matrix(rand_bsr(128, 64, 32, 1, rng)).astype(np.float32)
+    F = faust_amd.Faust([B], dtype=np.float32)
+    L = _lib.lib()
+    h = F._facs[0].h
+    nb = B.indices.shape[0]
+    out = np.empty(nb * 1024, np.float32)
+    _lib.check(L.fh_mat_tocpu(h, out.ctypes.data_as(C.c_void_p), None, None))
+    want = np.ascontiguousarray(np.transpose(B.data, (0, 2, 1))).ravel()  # column-major blocks
+    assert np.array_equal(out, want)
+    dt = C.c_int()
+    _lib.check(L.fh_faust_get_act_dtype(F.h, C.byref(dt)))
+    assert dt.value == _lib.FH_BF16
+    Fb = faust_amd.Faust([sp.bsr_matrix(rand_bsr(64, 64, 32, 1, rng))], dtype="bf16")
+    assert L.fh_faust_set_act_dtype(Fb.h, _lib.FH_F32) == _lib.FH_ERR_UNSUPPORTED
