@@ -52,6 +52,8 @@ def _parse():
                          "the butterfly factorization")
     ap.add_argument("--hadamard", action="store_true",
                     help="also time hierarchical PALM4MSA on the 1024 Hadamard matrix, 100 iterations per level")
+    ap.add_argument("--hadamard-its", type=int, default=100, metavar="ITS",
+                    help="iterations per level of the --hadamard run (fewer for profiler runs)")
     ap.add_argument("--warmstart", type=int, default=0, metavar="NTRAIN",
                     help="also run the learned warm-start experiment (256x256 butterfly targets) with NTRAIN "
                          "training targets and 1000 held-out ones")
@@ -77,22 +79,29 @@ def _bsr_factors(seed, J=6, n=8192, bs=32, bpr=4):
     return [rand_bsr(n, n, bs, bpr, np.random.default_rng(seed + f), scale=1 / np.sqrt(128)) for f in range(J)]
 
 
-def _bsr_workload(values):
+def _bsr_workload(values, act="bf16"):
     """6 BSR factors 8192x8192, 32x32 blocks, 4 distinct uniform block columns
-    per block-row, values N(0, 1/128) stored BF16 (or FP32 as BF16 hi/lo
-    pairs); X 8192x512 BF16 N(0,1); FP32 accumulation on MFMA."""
+    per block-row, values N(0, 1/128); X 8192x512 N(0,1); FP32 accumulation
+    on MFMA.  values "bf16": BF16 blocks, BF16 activations; "f32" with act
+    "bf16": FP32 blocks (kept as a BF16 hi/lo pair on the BF16 MFMA), BF16
+    activations; "f32" with act "f32": FP32 blocks on the FP32 MFMA with FP32
+    activations (the reference's MatBSR<float> precision)."""
     n, J, bs, bpr, p = 8192, 6, 32, 4, 512
     facs = _bsr_factors(0xB5B, J, n, bs, bpr)
     X = np.asfortranarray(np.random.default_rng(0xB5B + 100).standard_normal((n, p)))
     vbytes = 2 if values == "bf16" else 4
+    abytes = 2 if act == "bf16" else 4
     nb = (n // bs) * bpr
-    per_factor = nb * bs * bs * vbytes + nb * 4 + (n // bs + 1) * 4 + n * p * 2 + n * p * 2
-    return dict(name=f"faust_bsr_{values}_8192x8192_J6_b32x32_4blk_X8192x512_bf16", facs=facs, X=X,
-                dtype="bf16", fac_dtype=("bf16" if values == "bf16" else np.float32), ops=("N",),
+    per_factor = nb * bs * bs * vbytes + nb * 4 + (n // bs + 1) * 4 + n * p * abytes * 2
+    if act == "f32":
+        kernel = "k_bsr_f32"
+    else:
+        kernel = "k_bsr_shared<false,*,*>" if values == "bf16" else "k_bsr_shared<true,*,*>"
+    return dict(name=f"faust_bsr_{values}_8192x8192_J6_b32x32_4blk_X8192x512_{act}", facs=facs, X=X,
+                dtype=act, fac_dtype=("bf16" if values == "bf16" else np.float32),
+                act_dtype=(np.float32 if act == "f32" else None), ops=("N",),
                 bytes_per_launch=per_factor, launches_per_step=J, bytes_per_step=J * per_factor,
-                flops_per_step=J * 2 * nb * bs * bs * p,
-                kernel=("k_bsr_shared<false,*,*>" if values == "bf16" else "k_bsr_shared<true,*,*>"),
-                cpu_reps_cap=40)
+                flops_per_step=J * 2 * nb * bs * bs * p, kernel=kernel, cpu_reps_cap=40)
 
 
 def workload_chain_bsr_bf16():
@@ -101,6 +110,10 @@ def workload_chain_bsr_bf16():
 
 def workload_chain_bsr_f32():
     return _bsr_workload("f32")
+
+
+def workload_chain_bsr_f32_act32():
+    return _bsr_workload("f32", "f32")
 
 
 def workload_chain_csr_f32():
@@ -121,7 +134,8 @@ def workload_chain_csr_f32():
 
 
 WORKLOADS = {"chain_csr_f64": workload_chain_csr_f64, "chain_bsr_bf16": workload_chain_bsr_bf16,
-             "chain_bsr_f32": workload_chain_bsr_f32, "chain_csr_f32": workload_chain_csr_f32}
+             "chain_bsr_f32": workload_chain_bsr_f32, "chain_bsr_f32_act32": workload_chain_bsr_f32_act32,
+             "chain_csr_f32": workload_chain_csr_f32}
 
 
 # ----------------------------------------------------------------- PALM4MSA
@@ -202,16 +216,16 @@ def palm_batched_vs_plain(cpu=True):
     return out
 
 
-def palm_hadamard():
+def palm_hadamard(its=100):
     """Hierarchical PALM4MSA of the 1024 Hadamard matrix into 10 factors
-    (splincol, 100 iterations per level, plain FP64 GPU form)."""
+    (splincol, `its` iterations per level, plain FP64 GPU form)."""
     import oracle
     from faust_amd import fact, factparams
     H = oracle.hadamard(1024)
     t = time.perf_counter()
-    res = fact.hierarchical(H, factparams.ParamsHierarchicalWHT(10, num_its=100), ret_raw=True)
+    res = fact.hierarchical(H, factparams.ParamsHierarchicalWHT(10, num_its=its), ret_raw=True)
     dt = time.perf_counter() - t
-    return {"workload": "hierarchical_hadamard_1024_10facs_splincol_100its", "seconds": round(dt, 3),
+    return {"workload": f"hierarchical_hadamard_1024_10facs_splincol_{its}its", "seconds": round(dt, 3),
             "relerr": _relerr(res.lam, res.dense_factors(), H)}
 
 
@@ -291,6 +305,34 @@ def notebook_cases():
 
 
 # ----------------------------------------------------------------- helpers
+def accuracy(wl, F, Xd, out_N, ncols=16):
+    """Error of the benchmarked product against the FP64 oracle on its first
+    `ncols` columns (outside the timed region): `same_inputs` feeds the oracle
+    the values the GPU holds (factor values and X after their rounding to the
+    stored precision), so it measures the arithmetic (activation roundings,
+    FP32 accumulation); `fp64_inputs` feeds it the unrounded FP64 factors and
+    X, the whole error of the reduced-precision form.  FP64 workloads are also
+    compared bit for bit."""
+    import torch
+    import oracle
+    k = min(ncols, wl["X"].shape[1])
+    Y = out_N[:k].to(torch.float64).cpu().numpy().T  # out_N holds the output column-major
+    Xg = Xd[:k].to(torch.float64).cpu().numpy().T
+    held = [f.to_host() for f in F._facs]
+
+    def err(facs, X):
+        ref = oracle.chain_multiply(facs, np.asfortranarray(X), "N")
+        return {"rel_fro": float(np.linalg.norm(Y - ref) / np.linalg.norm(ref)),
+                "max_abs_over_max_ref": float(np.abs(Y - ref).max() / np.abs(ref).max())}, ref
+
+    same, ref_same = err(held, Xg)
+    full, _ = err(wl["facs"], wl["X"][:, :k])
+    out = {"columns": k, "same_inputs": same, "fp64_inputs": full}
+    if wl["dtype"] == "f64":
+        out["bitexact_vs_oracle"] = bool(np.array_equal(Y, ref_same))
+    return out
+
+
 def measured_traffic(wl):
     """HBM-side bytes per launch of the dominant kernel from the rocprofv3
     FETCH_SIZE / WRITE_SIZE runs recorded in profiles/traffic.json (each
@@ -341,7 +383,7 @@ def main():
     torch.cuda.set_device(0)
     wl = WORKLOADS[args.workload]()
     tdt = {"f64": torch.float64, "f32": torch.float32, "bf16": torch.bfloat16}[wl["dtype"]]
-    F = faust_amd.Faust(wl["facs"], dtype=wl.get("fac_dtype", np.float64))
+    F = faust_amd.Faust(wl["facs"], dtype=wl.get("fac_dtype", np.float64), act_dtype=wl.get("act_dtype"))
     Xd = colmajor_to_device(wl["X"], tdt)
     FT = F.T
     outs = {op: torch.empty((Xd.shape[0], (F if op == "N" else FT).shape[0]), dtype=tdt, device="cuda")
@@ -449,6 +491,8 @@ def main():
                       "ms_per_step_eager": round(ms_eager, 5),
                       "ms_per_step_graph": None if ms_graph is None else round(ms_graph, 5)},
            "roofline": roof}
+    if "N" in wl["ops"]:
+        out["accuracy_vs_fp64_oracle"] = accuracy(wl, F, Xd, outs["N"])
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(wl, args.cpu_seconds)
     if not args.no_palm:
@@ -456,7 +500,7 @@ def main():
     if not args.no_butterfly:
         out["butterfly"] = butterfly_measure(cpu=not args.no_cpu_baseline)
     if args.hadamard:
-        out["palm_hadamard"] = palm_hadamard()
+        out["palm_hadamard"] = palm_hadamard(args.hadamard_its)
     if not args.no_notebook:
         out["notebook"] = notebook_cases()
     if args.butterfly_init:
