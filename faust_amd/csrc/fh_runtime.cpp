@@ -25,17 +25,28 @@ size_t size_class(size_t b) {
 }
 }  // namespace
 
+// FH_POOL_POISON=1 (a checking mode for tests): every block handed out is
+// first filled with 0xFF bytes (NaN as FP64/FP32, -1 as int32) on the null
+// stream, so a kernel that reads memory it did not write gives a visibly
+// different result.  Code that captures graphs must not run in this mode.
+static void poison_block(void* p, size_t c) {
+  const char* e = getenv("FH_POOL_POISON");
+  if (e && e[0] == '1') FH_HIP(hipMemsetAsync(p, 0xFF, c, nullptr));
+}
+
 void* dev_alloc(size_t bytes) {
   int dev = 0;
   FH_HIP(hipGetDevice(&dev));
   size_t c = size_class(bytes);
   {
-    std::lock_guard<std::mutex> lk(g_pool_mu);
+    std::unique_lock<std::mutex> lk(g_pool_mu);
     auto it = g_free.find({dev, c});
     if (it != g_free.end()) {
       void* p = it->second;
       g_free.erase(it);
       g_live[p] = {dev, c};
+      lk.unlock();
+      poison_block(p, c);
       return p;
     }
   }
@@ -45,8 +56,11 @@ void* dev_alloc(size_t bytes) {
     pool_trim();
     FH_HIP(hipMalloc(&p, c));
   }
-  std::lock_guard<std::mutex> lk(g_pool_mu);
-  g_live[p] = {dev, c};
+  {
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    g_live[p] = {dev, c};
+  }
+  poison_block(p, c);
   return p;
 }
 

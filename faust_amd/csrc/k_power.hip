@@ -8,6 +8,21 @@
 // Mat-vecs are one fma chain per output row (k ascending); norms and dots use
 // the canonical 256-lane chunk reduction, so the result matches the oracle
 // bit for bit.
+//
+// The chains of PALM4MSA's Lipschitz constants are long strings of small
+// sparse factors (a global update of a 10-factor Faust applies up to 18
+// factors of 2 entries per row each round, for 100 rounds when the estimate
+// does not settle), so a round is a string of dependent latencies, not of
+// bytes.  The kernel therefore copies the chain's row pointers into LDS once
+// per call, reads the operator descriptors through scalar loads, and loads the
+// entries of a thread's rows for the operator after next while it computes the
+// current one (rows of up to PW_PF entries; longer rows read theirs in the
+// loop), with barriers that order only the LDS vectors, so those loads stay in
+// flight across them.  Measured on the 1024 Hadamard hierarchy (100
+// iterations per PALM4MSA): 0.71 us per operator and 3.3 us of norm,
+// normalisation and Rayleigh quotient per round (11.5 s in this kernel); the
+// whole hierarchy takes 24.1 s, against 26.8 s when each thread read its row
+// pointers and entries from global memory as it needed them.
 #include <cstdlib>
 #include <map>
 #include <mutex>
@@ -27,16 +42,26 @@ struct PwOp {
   const int* ci;
   const double* v;  // sparse values or dense data (ld = rows of A)
   int lda;
+  int rpo;      // sparse only: offset of the row pointers' copy in dynamic LDS, or -1
   int64_t nnz;  // sparse only
 };
 
+constexpr int PW_MAX_OPS = 48;        // longest chain of the persistent form
+constexpr int PW_PF = 4;              // entries per row loaded ahead
+constexpr int PW_RP_LDS = 96 * 1024;  // dynamic LDS for cached row pointers (bytes)
+
+__device__ __forceinline__ int pw_sgpr(int x) { return __builtin_amdgcn_readfirstlane(x); }
+
 // canonical reduction of dot(x, y) over n values held in LDS; every thread
 // returns the same value.  red: 256 doubles of LDS scratch; tot: 1 double.
+// Per 4096-value chunk: lane t (< 256) sums x[i] y[i] for i = t, t + 256, ...
+// in order, then the halving tree red[t] += red[t + h], h = 128 ... 1; the
+// last six levels run inside wave 0 through lane shuffles (the same pairs,
+// in the same order).  Chunk sums are added in chunk order.
 __device__ double pw_dot(const double* x, const double* y, int n, double* red, double* tot) {
   const int t = threadIdx.x;
   const int nch = (n + RED_CHUNK - 1) / RED_CHUNK;
-  if (t == 0) *tot = 0.0;
-  __syncthreads();
+  double acc_tot = 0.0;
   for (int c = 0; c < nch; ++c) {
     if (t < RED_LANES) {
       double acc = 0.0;
@@ -47,22 +72,189 @@ __device__ double pw_dot(const double* x, const double* y, int n, double* red, d
       red[t] = acc;
     }
     __syncthreads();
-    for (int s = RED_LANES / 2; s >= 1; s >>= 1) {
-      if (t < s) red[t] = red[t] + red[t + s];
-      __syncthreads();
-    }
-    if (t == 0) *tot = *tot + red[0];
+    if (t < 128) red[t] = red[t] + red[t + 128];
     __syncthreads();
+    if (t < 64) {
+      double v = red[t] + red[t + 64];
+#pragma unroll
+      for (int h = 32; h >= 1; h >>= 1) v = v + __shfl_down(v, h, 64);
+      if (t == 0) acc_tot = acc_tot + v;
+    }
   }
+  if (t == 0) *tot = acc_tot;
+  __syncthreads();
   double r = *tot;
   __syncthreads();
-  return r;
+  // the same on every lane; said so to the compiler, which then keeps the
+  // loop tests that depend on it (and everything inside) scalar
+  const unsigned long long u = __builtin_bit_cast(unsigned long long, r);
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)u), hi = __builtin_amdgcn_readfirstlane((unsigned)(u >> 32));
+  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
+// Entries of one row of a sparse operator, loaded ahead of their use.
+struct PwPre {
+  double v[PW_PF];
+  int c[PW_PF];
+  int p, n;  // first entry and count; n > PW_PF: read in the loop
+};
+
+typedef unsigned int pw_u4 __attribute__((ext_vector_type(4)));
+typedef double pw_d2 __attribute__((ext_vector_type(2)));
+constexpr int PW_RSRC_WORD3 = 0x00020000;
+
+// Global-memory views of the operator arrays for the loops over long rows and
+// dense operators (vector memory loads, counted on vmcnt only).
+typedef const __attribute__((address_space(1))) double* pw_gd;
+typedef const __attribute__((address_space(1))) int* pw_gi;
+// Operator descriptors: read-only for the whole launch, read word by word
+// through the constant address space (scalar loads).
+typedef const __attribute__((address_space(4))) int* pw_cop;
+struct PwOpWords {
+  int w[sizeof(PwOp) / 4];
+};
+static_assert(sizeof(PwOp) % 4 == 0, "PwOp is read as words");
+__device__ __forceinline__ PwOp pw_op(pw_cop ops, int s) {
+  PwOpWords x;
+#pragma unroll
+  for (int i = 0; i < (int)(sizeof(PwOp) / 4); ++i) x.w[i] = ops[s * (int)(sizeof(PwOp) / 4) + i];
+  return __builtin_bit_cast(PwOp, x);
+}
+
+// Barrier between operators: orders the LDS vectors only, without waiting
+// for the entry loads of the next operators.
+__device__ __forceinline__ void pw_lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// Loads the first PW_PF entries of row r: two 16-byte buffer loads of values
+// and one of column indices, whatever the row and the operator.  The buffers
+// cover exactly the operator's entries (none for a dense operator), so reads
+// past them return zeros; rows past the operator and dense operators get
+// n = 0.  Every thread thus issues the same three loads for every operator,
+// and the wait before their use covers only the loads issued before them.
+// RP_LDS: the row pointers are in LDS (a dense operator's rpo points at valid
+// LDS), else they are read from global memory.
+// The descriptor fields pw_fetch needs, read from LDS into scalar registers
+// (the same for every lane), so the buffer resources are scalar.
+struct PwU {
+  const double* v;
+  const int* ci;
+  const int* rp;
+  int kind, rows, rpo, nnz;
+};
+template <class T>
+__device__ __forceinline__ T* pw_sptr(T* p) {
+  const unsigned long long u = (unsigned long long)p;
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)u), hi = __builtin_amdgcn_readfirstlane((unsigned)(u >> 32));
+  return (T*)(((unsigned long long)hi << 32) | lo);
+}
+__device__ __forceinline__ PwU pw_uniform(const PwOp o) {
+  PwU u;
+  u.v = pw_sptr(o.v);
+  u.ci = pw_sptr(o.ci);
+  u.rp = pw_sptr(o.rp);
+  u.kind = pw_sgpr(o.kind);
+  u.rows = pw_sgpr(o.rows);
+  u.rpo = pw_sgpr(o.rpo);
+  u.nnz = pw_sgpr((int)o.nnz);
+  return u;
+}
+
+template <bool RP_LDS>
+__device__ __forceinline__ void pw_fetch(const PwOp od, const int* rpl, int r, PwPre& f) {
+  const PwU o = pw_uniform(od);
+  const bool sp = o.kind == 1;
+  const int rr = min(r, o.rows - 1);
+  int p0, p1;
+  if (RP_LDS) {
+    p0 = rpl[o.rpo + rr];
+    p1 = rpl[o.rpo + rr + 1];
+  } else {
+    const pw_gi rp = (pw_gi)o.rp;
+    p0 = sp ? rp[rr] : 0;
+    p1 = sp ? rp[rr + 1] : 0;
+  }
+  f.p = p0;
+  f.n = (sp && r < o.rows) ? p1 - p0 : 0;
+  const int vb = sp ? o.nnz * 8 : 0, cb = sp ? o.nnz * 4 : 0;
+  const __amdgpu_buffer_rsrc_t rv = __builtin_amdgcn_make_buffer_rsrc((void*)o.v, 0, vb, PW_RSRC_WORD3);
+  const __amdgpu_buffer_rsrc_t rc = __builtin_amdgcn_make_buffer_rsrc((void*)o.ci, 0, cb, PW_RSRC_WORD3);
+  const pw_d2 v01 = __builtin_bit_cast(pw_d2, __builtin_amdgcn_raw_buffer_load_b128(rv, p0 * 8, 0, 0));
+  const pw_d2 v23 = __builtin_bit_cast(pw_d2, __builtin_amdgcn_raw_buffer_load_b128(rv, p0 * 8 + 16, 0, 0));
+  const pw_u4 c = __builtin_amdgcn_raw_buffer_load_b128(rc, p0 * 4, 0, 0);
+  f.v[0] = v01.x;
+  f.v[1] = v01.y;
+  f.v[2] = v23.x;
+  f.v[3] = v23.y;
+  f.c[0] = (int)c.x;
+  f.c[1] = (int)c.y;
+  f.c[2] = (int)c.z;
+  f.c[3] = (int)c.w;
+}
+
+// row r of op(o) applied to cur (LDS), entries from f when they were loaded
+__device__ __forceinline__ double pw_row(const PwOp o, const PwPre& f, int r, const double* cur) {
+  double acc = 0.0;
+  if (o.kind == 1) {
+    if (f.n <= PW_PF) {
+#pragma unroll
+      for (int k = 0; k < PW_PF; ++k)
+        if (k < f.n) acc = fma(f.v[k], cur[f.c[k]], acc);
+    } else {
+      const pw_gd v = (pw_gd)o.v;
+      const pw_gi ci = (pw_gi)o.ci;
+      for (int p = f.p, pe = f.p + f.n; p < pe; ++p) acc = fma(v[p], cur[ci[p]], acc);
+    }
+  } else if (!o.trans) {
+    const pw_gd a = (pw_gd)o.v + r;
+    for (int kk = 0; kk < o.cols; ++kk) acc = fma(a[(size_t)kk * o.lda], cur[kk], acc);
+  } else {
+    const pw_gd a = (pw_gd)o.v + (size_t)r * o.lda;
+    for (int kk = 0; kk < o.cols; ++kk) acc = fma(a[kk], cur[kk], acc);
+  }
+  return acc;
+}
+
+// One operator of the chain: row t (and t + PW_THREADS when TWO_ROWS) of
+// operator s from cur into nxt with the entries in (f0, f1), which are then
+// reloaded with those of operator s - 2; ends with the LDS barrier.
+// Marks the prefetched registers as read on every path, so that the wait for
+// their loads sits here (with the next operator's loads still in flight) and
+// the registers can be reused afterwards without a wait for everything.
+__device__ __forceinline__ void pw_settle(const PwPre& f) {
+  asm volatile("" ::"v"(f.v[0]), "v"(f.v[1]), "v"(f.v[2]), "v"(f.v[3]), "v"(f.c[0]), "v"(f.c[1]), "v"(f.c[2]),
+               "v"(f.c[3]));
+}
+
+template <bool RP_LDS, bool TWO_ROWS>
+__device__ __forceinline__ void pw_apply(pw_cop sops, const int* rpl, int s, const double* cur, double* nxt,
+                                         PwPre& f0, PwPre& f1) {
+  const int t = threadIdx.x;
+  pw_settle(f0);
+  if (TWO_ROWS) pw_settle(f1);
+  s = pw_sgpr(s);  // the same on every lane: descriptors come through scalar loads
+  const PwOp o = pw_op(sops, s);
+  const int rows = o.rows;
+  double y0 = 0.0, y1 = 0.0;
+  if (t < rows) y0 = pw_row(o, f0, t, cur);
+  if (TWO_ROWS && t + PW_THREADS < rows) y1 = pw_row(o, f1, t + PW_THREADS, cur);
+  // (operators 1 and 0 reload operator 0: unused, keeps the loads unconditional)
+  const PwOp o2 = pw_op(sops, max(s - 2, 0));
+  pw_fetch<RP_LDS>(o2, rpl, t, f0);
+  if (TWO_ROWS) pw_fetch<RP_LDS>(o2, rpl, t + PW_THREADS, f1);
+  if (t < rows) nxt[t] = y0;
+  if (TWO_ROWS && t + PW_THREADS < rows) nxt[t + PW_THREADS] = y1;
+  pw_lds_barrier();
 }
 
 // out[0] = lambda, out[1] = flag (iterations, or -1), out[2] = status
 // (0 ok, 1 zero-norm vector, 2 A x == 0)
 // One chain per workgroup: workgroup b runs jobs[b] (independent chains, e.g.
-// the L and R norms of one PALM factor update, share a launch).
+// the L and R norms of one PALM factor update, share a launch).  A chain has
+// at most PW_MAX_OPS operators.
 struct PwJob {
   const PwOp* ops;
   int nops;
@@ -71,23 +263,35 @@ struct PwJob {
   double* out;
 };
 
+template <bool RP_LDS, bool TWO_ROWS>
 __global__ __launch_bounds__(PW_THREADS) void k_power_iteration(const PwJob* __restrict__ jobs, int maxit,
                                                                  double thr) {
+  extern __shared__ int rpl[];  // cached row pointers (PwOp::rpo)
   const PwJob job = jobs[blockIdx.x];
-  const PwOp* __restrict__ ops = job.ops;
   const int nops = job.nops, n = job.n;
   const double* __restrict__ x0 = job.x0;
   double* __restrict__ out = job.out;
   __shared__ double xa[PW_VMAX], xb[PW_VMAX], xn[PW_VMAX];
   __shared__ double red[RED_LANES];
   __shared__ double tot;
+  // operator descriptors: read-only and indexed by loop counters, so they
+  // come through scalar loads
+  const pw_cop sops = (pw_cop)job.ops;
   const int t = threadIdx.x;
   for (int i = t; i < n; i += PW_THREADS) xa[i] = x0[i];
+  for (int s = 0; s < nops; ++s) {
+    const PwOp o = pw_op(sops, s);
+    if (RP_LDS && o.kind == 1)
+      for (int i = t; i <= o.rows; i += PW_THREADS) rpl[o.rpo + i] = o.rp[i];
+  }
   __syncthreads();
   double lambda_old = 1.0, lambda = 0.0;
   int it = 0;
   int status = 0;
   const double athr = fabs(thr);
+  // entries of rows t and t + PW_THREADS (rows <= PW_VMAX = 2 x PW_THREADS) of
+  // the operators applied 1st, 3rd, ... (a) and 2nd, 4th, ... (b)
+  PwPre a0, a1, b0, b1;
   while ((fabs(lambda_old - lambda) > athr || fabs(lambda) <= athr) && it < maxit) {
     it++;
     lambda_old = lambda;
@@ -96,31 +300,26 @@ __global__ __launch_bounds__(PW_THREADS) void k_power_iteration(const PwJob* __r
       status = 1;
       break;
     }
+    pw_fetch<RP_LDS>(pw_op(sops, nops - 1), rpl, t, a0);
+    if (TWO_ROWS) pw_fetch<RP_LDS>(pw_op(sops, nops - 1), rpl, t + PW_THREADS, a1);
+    pw_fetch<RP_LDS>(pw_op(sops, max(nops - 2, 0)), rpl, t, b0);
+    if (TWO_ROWS) pw_fetch<RP_LDS>(pw_op(sops, max(nops - 2, 0)), rpl, t + PW_THREADS, b1);
     const double inv = 1 / nrm;
     for (int i = t; i < n; i += PW_THREADS) xn[i] = xa[i] * inv;
-    __syncthreads();
-    // apply ops last to first: cur -> nxt
+    pw_lds_barrier();
+    // apply ops last to first, ping-pong between xb and xa (xn stays the
+    // normalised start vector)
     const double* cur = xn;
     double* nxt = xb;
     int len = n;
-    for (int s = nops - 1; s >= 0; --s) {
-      const PwOp o = ops[s];
-      for (int r = t; r < o.rows; r += PW_THREADS) {
-        double acc = 0.0;
-        if (o.kind == 1) {
-          for (int p = o.rp[r]; p < o.rp[r + 1]; ++p) acc = fma(o.v[p], cur[o.ci[p]], acc);
-        } else if (!o.trans) {
-          const double* a = o.v + r;
-          for (int kk = 0; kk < o.cols; ++kk) acc = fma(a[(size_t)kk * o.lda], cur[kk], acc);
-        } else {
-          const double* a = o.v + (size_t)r * o.lda;
-          for (int kk = 0; kk < o.cols; ++kk) acc = fma(a[kk], cur[kk], acc);
-        }
-        nxt[r] = acc;
-      }
-      len = o.rows;
-      __syncthreads();
-      // ping-pong between xb and xa (xn stays the normalised start vector)
+    for (int s = nops - 1; s >= 0; s -= 2) {
+      pw_apply<RP_LDS, TWO_ROWS>(sops, rpl, s, cur, nxt, a0, a1);
+      len = pw_op(sops, s).rows;
+      cur = nxt;
+      nxt = (nxt == xb) ? xa : xb;
+      if (s == 0) break;
+      pw_apply<RP_LDS, TWO_ROWS>(sops, rpl, s - 1, cur, nxt, b0, b1);
+      len = pw_op(sops, s - 1).rows;
       cur = nxt;
       nxt = (nxt == xb) ? xa : xb;
     }
@@ -430,6 +629,7 @@ static std::vector<PwOp> pw_ops(const std::vector<ChainOp>& ops) {
     const DMat& M = *ops[s].M;
     const bool tr = ops[s].op != 'N';
     PwOp& o = h[s];
+    o.rpo = -1;
     o.rows = tr ? M.n : M.m;
     o.cols = tr ? M.m : M.n;
     if (M.kind == K_CSR) {
@@ -451,8 +651,9 @@ static std::vector<PwOp> pw_ops(const std::vector<ChainOp>& ops) {
       FH_CHECK(M.kind == K_DENSE && M.dtype == FH_F64, "power_iteration: FP64 dense or CSR factors only");
       o.kind = 0;
       o.trans = tr ? 1 : 0;
-      o.rp = o.ci = nullptr;
       o.v = M.d.as<double>();
+      // read (and unused) by the persistent kernel's unconditional loads
+      o.rp = o.ci = reinterpret_cast<const int*>(o.v);
       o.lda = M.m;
       o.nnz = 0;
     }
@@ -462,21 +663,76 @@ static std::vector<PwOp> pw_ops(const std::vector<ChainOp>& ops) {
 
 // the persistent kernel keeps every vector in LDS: up to PW_VMAX entries
 static bool pw_persistent(const std::vector<PwOp>& h, int n) {
-  bool fits = n <= PW_VMAX;
-  for (const PwOp& o : h) fits = fits && o.rows <= PW_VMAX && o.cols <= PW_VMAX;
+  bool fits = n <= PW_VMAX && h.size() <= (size_t)PW_MAX_OPS;
+  for (const PwOp& o : h) fits = fits && o.rows >= 1 && o.rows <= PW_VMAX && o.cols <= PW_VMAX;
   return fits && !use_wide_form(h);
+}
+
+// Dynamic LDS the persistent kernel may take for row pointers: PW_RP_LDS
+// once the attribute is granted, else none (every operator then reads its
+// row pointers from global memory).  Queried once per process.
+static size_t pw_rp_budget() {
+  static int budget = -1;
+  if (budget < 0)
+    budget = hipFuncSetAttribute((const void*)k_power_iteration<true, false>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, PW_RP_LDS) == hipSuccess &&
+                     hipFuncSetAttribute((const void*)k_power_iteration<true, true>,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, PW_RP_LDS) == hipSuccess
+                 ? PW_RP_LDS
+                 : 0;
+  (void)hipGetLastError();
+  return (size_t)budget;
+}
+
+// Row-pointer copies of a chain's sparse operators in LDS (an array shared by
+// two operators is copied once); returns the ints used, or -1 (and no
+// copies) when they do not all fit the budget.
+static long pw_assign_rp(std::vector<PwOp>& ops, size_t budget_ints) {
+  size_t used = 0, maxrows = 0;
+  for (size_t s = 0; s < ops.size(); ++s) {
+    PwOp& o = ops[s];
+    o.rpo = -1;
+    maxrows = std::max(maxrows, (size_t)o.rows);
+    if (o.kind != 1) {
+      o.rpo = 0;  // a dense operator's (unused) row-pointer reads stay inside the array
+      continue;
+    }
+    for (size_t q = 0; q < s; ++q)
+      if (ops[q].kind == 1 && ops[q].rp == o.rp) o.rpo = ops[q].rpo;
+    if (o.rpo >= 0) continue;
+    o.rpo = (int)used;
+    used += o.rows + 1;
+  }
+  used = std::max(used, maxrows + 1);
+  if (used > budget_ints) {
+    for (PwOp& o : ops) o.rpo = -1;
+    return -1;
+  }
+  return (long)used;
 }
 
 // chains[k] (ops, n) in one launch, one workgroup each; lam[k] and flags[k]
 // receive each chain's result.  A zero-norm start vector throws, for the
 // first such chain in order.
-static void power_iteration_persistent(const std::vector<std::vector<PwOp>>& hs, const std::vector<int>& ns,
+static void power_iteration_persistent(const std::vector<std::vector<PwOp>>& hs_in, const std::vector<int>& ns,
                                        int maxit, double thr, double* lam, int* flags, hipStream_t st) {
-  const int J = (int)hs.size();
+  const int J = (int)hs_in.size();
+  std::vector<std::vector<PwOp>> hs = hs_in;
   size_t nops = 0, nx = 0;
+  long dyn_ints = 0;
+  const size_t budget = pw_rp_budget() / sizeof(int);
+  bool rp_lds = true;
   for (int k = 0; k < J; ++k) {
     nops += hs[k].size();
     nx += ns[k];
+    const long used = pw_assign_rp(hs[k], budget);
+    rp_lds = rp_lds && used >= 0;
+    dyn_ints = std::max(dyn_ints, used);
+  }
+  if (!rp_lds) {
+    for (auto& h : hs)
+      for (PwOp& o : h) o.rpo = -1;
+    dyn_ints = 0;
   }
   DBuf dops(sizeof(PwOp) * nops), dx(sizeof(double) * nx), dout(sizeof(double) * 3 * J), djobs(sizeof(PwJob) * J);
   std::vector<PwOp> allops;
@@ -492,7 +748,15 @@ static void power_iteration_persistent(const std::vector<std::vector<PwOp>>& hs,
   }
   FH_HIP(hipMemcpyAsync(dops.p, allops.data(), sizeof(PwOp) * nops, hipMemcpyHostToDevice, st));
   FH_HIP(hipMemcpyAsync(djobs.p, jobs.data(), sizeof(PwJob) * J, hipMemcpyHostToDevice, st));
-  hipLaunchKernelGGL(k_power_iteration, dim3(J), dim3(PW_THREADS), 0, st, djobs.as<PwJob>(), maxit, thr);
+  bool two = false;  // some vector or operator longer than one row per thread
+  for (int k = 0; k < J; ++k) {
+    two = two || ns[k] > PW_THREADS;
+    for (const PwOp& o : hs[k]) two = two || o.rows > PW_THREADS;
+  }
+  const size_t dyn = rp_lds ? sizeof(int) * dyn_ints : 0;
+  auto kern = rp_lds ? (two ? k_power_iteration<true, true> : k_power_iteration<true, false>)
+                     : (two ? k_power_iteration<false, true> : k_power_iteration<false, false>);
+  hipLaunchKernelGGL(kern, dim3(J), dim3(PW_THREADS), dyn, st, djobs.as<PwJob>(), maxit, thr);
   FH_LAUNCH_CHECK();
   std::vector<double> res(3 * J);
   FH_HIP(hipMemcpyAsync(res.data(), dout.p, sizeof(double) * 3 * J, hipMemcpyDeviceToHost, st));
