@@ -69,8 +69,10 @@ def workload_chain_csr_f64():
     facs = [faust_amd.rand_csr(n, n, nnz, np.random.default_rng(0xFA057 + f)) for f in range(J)]
     X = np.asfortranarray(np.random.default_rng(0xFA057 + 100).standard_normal((n, p)))
     per_factor = nnz * n * (8 + 4) + (n + 1) * 4 + n * p * 8 + n * p * 8
+    chain = J * (nnz * n * (8 + 4) + (n + 1) * 4) + 2 * n * p * 8  # factors, X, Y of one product
     return dict(name="faust_csr_fp64_1024x1024_J5_nnz8_X1024x64", facs=facs, X=X, dtype="f64", ops=("N", "T"),
                 bytes_per_launch=per_factor, launches_per_step=2 * J, bytes_per_step=2 * J * per_factor,
+                chain_bytes_per_product=chain,
                 flops_per_step=2 * J * 2 * n * nnz * p, kernel="k_csr_spmm_f64")
 
 
@@ -397,18 +399,24 @@ def main():
         step()
     torch.cuda.synchronize()
     L = _lib.lib()
-    # small FP64 CSR chains run as one fused launch per product: the unit of
-    # the roofline figure is then the whole chain
+    # FP64 CSR chains: the autotuner keeps, per product, the one-launch chain
+    # or one launch per factor.  A one-launch product's algorithmic bytes are
+    # its factors, X and Y (the intermediates stay in LDS); a per-factor
+    # launch's are the workload's per-factor bytes.  `value` keeps the
+    # factor-by-factor byte count of the workload.
     if wl["dtype"] == "f64":
         n_l = C.c_int()
-        total = 0
+        per_op, total, nbytes = {}, 0, 0
         for op in wl["ops"]:
             _lib.check(L.fh_faust_last_launches((F if op == "N" else FT).h, C.byref(n_l)))
+            per_op[op] = n_l.value
             total += n_l.value
-        if total == len(wl["ops"]):
+            nbytes += wl["chain_bytes_per_product"] if n_l.value == 1 else n_l.value * wl["bytes_per_launch"]
+        if total != wl["launches_per_step"]:
             wl["launches_per_step"] = total
-            wl["bytes_per_launch"] = wl["bytes_per_step"] // total
-            wl["kernel"] = "k_chain_csr_f64 (whole chain, one launch)"
+            wl["bytes_per_launch"] = nbytes // total
+            wl["kernel"] = " + ".join(f"{op}: " + ("k_chain_pf_f64 (one launch)" if n == 1 else f"k_csr_spmm_f64 x{n}")
+                                      for op, n in per_op.items())
     # timed region, two ways over the same K steps: (a) eager launches on the
     # stream, queued behind a GPU spin so that host enqueue time is off the
     # clock; (b) the K steps captured once in a HIP graph and replayed.  The

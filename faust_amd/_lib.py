@@ -53,6 +53,7 @@ VP = C.c_void_p
 _SIGS = {
     "fh_last_error": (C.c_char_p, []),
     "fh_version": (C.c_char_p, []),
+    "fh_autotune_clear": (C.c_int, []),
     "fh_dev_count": (C.c_int, [C.POINTER(C.c_int)]),
     "fh_set_dev": (C.c_int, [C.c_int]),
     "fh_cur_dev": (C.c_int, [C.POINTER(C.c_int)]),

@@ -1,5 +1,8 @@
 // fh_api.cpp -- C ABI: matrix handles, products, Faust chains, projectors.
 #include <algorithm>
+#include <map>
+#include <mutex>
+#include <string>
 #include <cstring>
 #include <vector>
 
@@ -148,21 +151,88 @@ ChainWs faust_ws_grow(DBuf* ws, size_t need, int nbuf, hipStream_t st, DBuf* ctl
 
 // Y = op(F) X for an FP64 chain; X, Y col-major with ld = rows; ws holds the
 // intermediates (faust_ws_need_f64 bytes each).
+// ---------------------------------------------------------------- autotune
+// Execution forms of an FP64 CSR chain, all with the same bits: one launch
+// per factor, the staged one-launch chain (wc columns per workgroup), the
+// prefetched one-launch chain (wc columns, pf entries per row ahead).
+struct ChainForm {
+  int kind;  // 0 per factor, 1 staged, 2 prefetched
+  int wc, pf;
+};
+
+namespace {
+std::mutex g_tune_mu;
+std::map<std::string, ChainForm> g_tune;  // chain signature -> quickest form measured
+}  // namespace
+
+// FH_AUTOTUNE=0 turns the measurement off (the default forms are used).
+static bool autotune_on() {
+  const char* e = getenv("FH_AUTOTUNE");
+  return !(e && e[0] == '0');
+}
+
+static bool stream_capturing(hipStream_t st) {
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  const hipError_t e = hipStreamIsCapturing(st, &cs);
+  (void)hipGetLastError();
+  return e != hipSuccess || cs != hipStreamCaptureStatusNone;
+}
+
+// What the quickest form depends on: direction, columns, each op'd factor's
+// shape and entries, and the device.
+static std::string chain_sig(char op, int p, int in_rows, const std::vector<ChainCsrOp>& ops) {
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  std::string k = std::to_string(dev) + op + ":" + std::to_string(p) + ":" + std::to_string(in_rows);
+  for (const auto& o : ops) k += "," + std::to_string(o.rows) + "/" + std::to_string(o.nnz);
+  return k;
+}
+
+// Y = op(F) X with the FP64 chain.  AUTO mode on an all-CSR chain uses the
+// form measured quickest for this chain signature (measured on its first
+// call outside graph capture: every candidate once untimed, then the minimum
+// of three timed runs, HIP events on the launch stream); otherwise, and when
+// measuring is off or impossible, the prefetched one-launch chain where the
+// shape allows it, the staged one where the planner takes it, else one launch
+// per factor.  `tune` false (allocation-free callers) never measures.
 void faust_multiply_f64(const std::vector<DMatP>& f, char op, const double* X, int p, double* Y, hipStream_t st,
-                        const ChainWs& ws, FaustTimers* tm, int* launches) {
+                        const ChainWs& ws, FaustTimers* tm, int* launches, bool tune) {
   FH_CHECK(!f.empty(), "Faust::multiply: empty Faust");
   const int J = (int)f.size();
   double* const t1 = static_cast<double*>(ws.p[0]);
   double* const t2 = static_cast<double*>(ws.p[1]);
-  const double* in = X;
-  int in_rows = (op == 'N') ? f.back()->n : f.front()->m;
-  if (launches) *launches = J;
-  // all-CSR chains whose intermediates fit a workgroup's LDS column panel run
-  // in one launch (same bits as the factor-by-factor path)
+  const int in_rows0 = (op == 'N') ? f.back()->n : f.front()->m;
+
+  auto per_factor = [&](FaustTimers* tmr) {
+    const double* in = X;
+    int in_rows = in_rows0;
+    for (int i = 0; i < J; ++i) {
+      const DMat& S = (op == 'N') ? *f[J - 1 - i] : *f[i];
+      const int out_rows = (op == 'N') ? S.m : S.n;
+      FH_CHECK(((op == 'N') ? S.n : S.m) == in_rows, "Faust::multiply: factor dimensions do not chain");
+      double* out = (i == J - 1) ? Y : ((i & 1) ? t2 : t1);
+      FH_CHECK(S.dtype == FH_F64, "Faust::multiply: FP64 chain expected");
+      const size_t slot = (op == 'N') ? (size_t)(J - 1 - i) : (size_t)i;
+      if (tmr) tmr->begin(slot, st);
+      if (S.kind == K_CSR)
+        csr_spmm_f64(S, op, in, in_rows, p, out, out_rows, 1.0, 0.0, st);
+      else if (S.kind == K_DENSE)
+        gemm_f64(op == 'N' ? 'N' : 'T', 'N', out_rows, p, in_rows, 1.0, S.d.as<double>(), S.m, in, in_rows, 0.0,
+                 out, out_rows, st);
+      else
+        throw Error(FH_ERR_UNSUPPORTED, "Faust::multiply: BSR factors need the FP32/BF16 chain");
+      if (tmr) tmr->end(slot, st);
+      in = out;
+      in_rows = out_rows;
+    }
+  };
+
+  // the chain as op'd CSR factors in application order, when it is one
+  std::vector<ChainCsrOp> ops;
+  int r = in_rows0, widest = 0, nnz_max = 0;
+  double mean_max = 0.0;
+  bool ok = true;
   {
-    std::vector<ChainCsrOp> ops;
-    int r = in_rows;
-    bool ok = true;
     const char* e_ell = getenv("FH_CSR_ELL");
     const bool ell_on = !(e_ell && e_ell[0] == '0');
     for (int i = 0; i < J && ok; ++i) {
@@ -170,40 +240,107 @@ void faust_multiply_f64(const std::vector<DMatP>& f, char op, const double* X, i
       ok = S.kind == K_CSR && S.dtype == FH_F64 && ((op == 'N') ? S.n : S.m) == r && (op == 'N' || S.has_csc);
       if (!ok) break;
       const bool tr = op != 'N';
+      const int wr = tr ? S.max_col : S.max_row;
+      widest = std::max(widest, wr < 0 ? (1 << 30) : wr);
+      const int rows = tr ? S.n : S.m;
+      mean_max = std::max(mean_max, (double)S.nnz / std::max(rows, 1));
+      nnz_max = std::max(nnz_max, (int)S.nnz);
       const int ew = tr ? S.csr_tell_w : S.csr_ell_w;
       ops.push_back({tr ? S.cp.as<int>() : S.rp.as<int>(), tr ? S.ri.as<int>() : S.ci.as<int>(),
-                     tr ? S.cv.as<double>() : S.v.as<double>(), tr ? S.n : S.m, (int)S.nnz,
+                     tr ? S.cv.as<double>() : S.v.as<double>(), rows, (int)S.nnz,
                      ell_on && ew > 0 && ew % 8 == 0 && ew <= 32 ? 1 : 0});
-      r = tr ? S.n : S.m;
+      r = rows;
     }
-    const int wc = ok ? csr_chain_plan_f64(ops, in_rows, p) : 0;
-    if (wc > 0) {
-      if (tm) tm->begin(0, st);
-      csr_chain_run_f64(ops, in_rows, wc, X, in_rows, p, Y, r, 1.0, 0.0, st);
-      if (tm) tm->end(0, st);
-      if (launches) *launches = 1;
+  }
+  const int out_rows = r;
+  auto run = [&](const ChainForm& c, FaustTimers* tmr) {
+    if (c.kind == 0) {
+      per_factor(tmr);
       return;
     }
-  }
-  for (int i = 0; i < J; ++i) {
-    const DMat& S = (op == 'N') ? *f[J - 1 - i] : *f[i];
-    const int out_rows = (op == 'N') ? S.m : S.n;
-    FH_CHECK(((op == 'N') ? S.n : S.m) == in_rows, "Faust::multiply: factor dimensions do not chain");
-    double* out = (i == J - 1) ? Y : ((i & 1) ? t2 : t1);
-    FH_CHECK(S.dtype == FH_F64, "Faust::multiply: FP64 chain expected");
-    const size_t slot = (op == 'N') ? (size_t)(J - 1 - i) : (size_t)i;
-    if (tm) tm->begin(slot, st);
-    if (S.kind == K_CSR)
-      csr_spmm_f64(S, op, in, in_rows, p, out, out_rows, 1.0, 0.0, st);
-    else if (S.kind == K_DENSE)
-      gemm_f64(op == 'N' ? 'N' : 'T', 'N', out_rows, p, in_rows, 1.0, S.d.as<double>(), S.m, in, in_rows, 0.0, out,
-               out_rows, st);
+    if (tmr) tmr->begin(0, st);
+    if (c.kind == 1)
+      csr_chain_run_f64(ops, in_rows0, c.wc, X, in_rows0, p, Y, out_rows, 1.0, 0.0, st);
     else
-      throw Error(FH_ERR_UNSUPPORTED, "Faust::multiply: BSR factors need the FP32/BF16 chain");
-    if (tm) tm->end(slot, st);
-    in = out;
-    in_rows = out_rows;
+      csr_chain_pf_run_f64(ops, in_rows0, c.wc, c.pf, X, in_rows0, p, Y, out_rows, 1.0, 0.0, st);
+    if (tmr) tmr->end(0, st);
+  };
+  auto done = [&](const ChainForm& c) {
+    if (launches) *launches = c.kind == 0 ? J : 1;
+  };
+
+  if (ok) {
+    // default form: prefetched (8 entries ahead unless the rows are long on
+    // average and some exceed 8), else the staged plan, else per factor
+    ChainForm def{0, 0, 0};
+    const int pwc = csr_chain_pf_plan_f64(ops, in_rows0, p);
+    const int swc = pwc > 0 ? 0 : csr_chain_plan_f64(ops, in_rows0, p);
+    if (pwc > 0)
+      def = {2, pwc, (widest <= 8 || mean_max <= 4.0) ? 8 : 16};
+    else if (swc > 0)
+      def = {1, swc, 0};
+    ChainForm use = def;
+    const bool tunable = mul_mode() == FH_MUL_AUTO && autotune_on() && !getenv("FH_CHAIN_FUSED") &&
+                         !(getenv("FH_CHAIN_PF") && getenv("FH_CHAIN_PF")[0] == '0');
+    if (tunable) {
+      const std::string key = chain_sig(op, p, in_rows0, ops);
+      bool known = false;
+      {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tune.find(key);
+        if (it != g_tune.end()) {
+          use = it->second;
+          known = true;
+        }
+      }
+      if (!known && tune && !stream_capturing(st)) {
+        std::vector<ChainForm> cand{{0, 0, 0}};
+        const int fwc = csr_chain_fit_wc(ops, in_rows0, p);
+        if (fwc > 0) cand.push_back({1, fwc, 0});
+        if (csr_chain_pf_fits(ops, in_rows0, p))
+          for (int pf : {8, 16}) {
+            if (pf == 16 && widest <= 8) continue;
+            for (int wc : {1, 2, 4})
+              if (wc == 1 || (p + wc - 1) / wc >= 8) cand.push_back({2, wc, pf});
+          }
+        hipEvent_t e0, e1;
+        FH_HIP(hipEventCreate(&e0));
+        FH_HIP(hipEventCreate(&e1));
+        float best_ms = 1e30f;
+        for (const ChainForm& c : cand) {
+          run(c, nullptr);
+          float m = 1e30f;
+          for (int rep = 0; rep < 3; ++rep) {
+            FH_HIP(hipEventRecord(e0, st));
+            run(c, nullptr);
+            FH_HIP(hipEventRecord(e1, st));
+            FH_HIP(hipEventSynchronize(e1));
+            float ms = 0.f;
+            FH_HIP(hipEventElapsedTime(&ms, e0, e1));
+            m = std::min(m, ms);
+          }
+          if (m < best_ms) {
+            best_ms = m;
+            use = c;
+          }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tune[key] = use;
+      }
+    }
+    run(use, tm);
+    done(use);
+    return;
   }
+  per_factor(tm);
+  if (launches) *launches = J;
+}
+
+void autotune_clear() {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_tune.clear();
 }
 
 // Transform::get_product (faust_Transform.hpp:599-626)

@@ -112,8 +112,10 @@ struct ChainWs {
 };
 size_t faust_ws_need_f64(const std::vector<DMatP>& f, char op, int p, int* nbuf);
 ChainWs faust_ws_grow(DBuf* ws, size_t need, int nbuf, hipStream_t st, DBuf* ctl = nullptr);
+// Forgets the forms measured by faust_multiply_f64's autotuner.
+void autotune_clear();
 void faust_multiply_f64(const std::vector<DMatP>& f, char op, const double* X, int p, double* Y, hipStream_t st,
-                        const ChainWs& ws, FaustTimers* tm = nullptr, int* launches = nullptr);
+                        const ChainWs& ws, FaustTimers* tm = nullptr, int* launches = nullptr, bool tune = true);
 DMatP faust_get_product_f64(const std::vector<DMatP>& f, hipStream_t st);
 double faust_spectral_norm_f64(const std::vector<DMatP>& f, int maxit, double thr, int* flag, hipStream_t st);
 void faust_spectral_norm_pair_f64(const std::vector<DMatP>& f1, const std::vector<DMatP>& f2, int maxit, double thr,

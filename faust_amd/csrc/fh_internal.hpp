@@ -176,6 +176,19 @@ struct ChainCsrOp {
 int csr_chain_plan_f64(const std::vector<ChainCsrOp>& ops, int in_rows, int p);
 void csr_chain_run_f64(const std::vector<ChainCsrOp>& ops, int in_rows, int wc, const double* X, int ldx, int p,
                        double* Y, int ldy, double alpha, double beta, hipStream_t st);
+// The same chain in one launch with every row of every factor and of X at
+// most 1024 long (one thread per row, the first pf = 8 or 16 entries of a
+// row loaded two factors ahead): fits says whether the shape allows it,
+// plan returns the default columns per workgroup or 0 (mode / environment),
+// default_wc the default columns per workgroup for p columns.
+bool csr_chain_pf_fits(const std::vector<ChainCsrOp>& ops, int in_rows, int p);
+int csr_chain_pf_plan_f64(const std::vector<ChainCsrOp>& ops, int in_rows, int p);
+int csr_chain_pf_default_wc(int p);
+void csr_chain_pf_run_f64(const std::vector<ChainCsrOp>& ops, int in_rows, int wc, int pf, const double* X, int ldx,
+                          int p, double* Y, int ldy, double alpha, double beta, hipStream_t st);
+// Columns per workgroup of the staged one-launch chain for this shape
+// (whatever the chain length), or 0 when it does not fit.
+int csr_chain_fit_wc(const std::vector<ChainCsrOp>& ops, int in_rows, int p);
 // C = alpha*op(A)*op(S) + beta*C, A dense col-major, S CSR.
 void dense_csr_f64(const double* A, int lda, char opA, int m, const DMat& S, char opS, double* C, int ldc,
                    double alpha, double beta, hipStream_t st);

@@ -93,7 +93,7 @@ extern "C" int fh_faust_multiply_noalloc(fh_faust_t F, char op, const void* X, i
   const ChainWs ws{{nbuf > 0 ? base + ctl_bytes : nullptr, nbuf > 1 ? base + ctl_bytes + slot : nullptr},
                    bsr ? reinterpret_cast<int*>(base) : nullptr};
   if (f64)
-    faust_multiply_f64(f, op, (const double*)X, p, (double*)Y, st, ws, nullptr, &F->last_launches);
+    faust_multiply_f64(f, op, (const double*)X, p, (double*)Y, st, ws, nullptr, &F->last_launches, false);
   else
     F->last_launches = faust_multiply_mixed(f, op, X, p, Y, st, ws, nullptr, act_f32);
   FH_API_END

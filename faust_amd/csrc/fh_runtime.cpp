@@ -101,6 +101,12 @@ int fh_set_dev(int dev) {
   FH_API_END
 }
 
+int fh_autotune_clear(void) {
+  FH_API_BEGIN
+  fh::autotune_clear();
+  FH_API_END
+}
+
 int fh_cur_dev(int* dev) {
   FH_API_BEGIN
   FH_HIP(hipGetDevice(dev));

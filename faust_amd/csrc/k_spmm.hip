@@ -394,6 +394,268 @@ void csr_chain_run_f64(const std::vector<ChainCsrOp>& ops, int in_rows, int wc, 
   FH_LAUNCH_CHECK();
 }
 
+// ------------------------------------------------------ prefetched chain
+// FP64 CSR chains whose factors and input have at most 1024 rows, in one
+// launch of 1024 threads per workgroup: thread t owns row t of every factor,
+// a workgroup WC columns of X.  Every factor's row pointers are copied into
+// LDS at the start; a thread loads the first PF entries (values, columns) of
+// its row of factor f + 2 while it computes factor f (two register sets,
+// loads through buffer resources that return zeros past the factor's
+// entries), and the barriers between factors order only the LDS activations,
+// so those loads stay in flight across them.  Rows longer than PF read the
+// rest of their entries in batches of 8.  Each output entry is the fma chain of
+// k_csr_spmm_f64 (k ascending from 0; the last factor applies alpha / beta),
+// so the bits are those of the factor-by-factor path.
+constexpr int PFC_THREADS = 1024;
+constexpr int PFC_RP_INTS = 16 * 1024;  // LDS ints for the row pointers of the whole chain
+
+typedef unsigned int pfc_u4 __attribute__((ext_vector_type(4)));
+typedef double pfc_d2 __attribute__((ext_vector_type(2)));
+typedef const __attribute__((address_space(1))) double* pfc_gd;
+typedef const __attribute__((address_space(1))) int* pfc_gi;
+
+template <int PF>
+struct PfcRow {
+  double v[PF];
+  int c[PF];
+  int p, n;  // first entry, count
+};
+
+__device__ __forceinline__ void pfc_lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// loads the first PF entries of row r of factor S (rows past the factor get
+// n = 0 and read entries that are not used)
+template <int PF>
+__device__ __forceinline__ void pfc_fetch(const ChainCsrOp& S, const int* rpl, int rpo, int r, PfcRow<PF>& f) {
+  const int rr = min(r, S.rows - 1);
+  const int p0 = rpl[rpo + rr], p1 = rpl[rpo + rr + 1];
+  f.p = p0;
+  f.n = r < S.rows ? p1 - p0 : 0;
+  const __amdgpu_buffer_rsrc_t rv = __builtin_amdgcn_make_buffer_rsrc((void*)S.v, 0, S.nnz * 8, 0x00020000);
+  const __amdgpu_buffer_rsrc_t rc = __builtin_amdgcn_make_buffer_rsrc((void*)S.ci, 0, S.nnz * 4, 0x00020000);
+#pragma unroll
+  for (int k = 0; k < PF; k += 2) {
+    const pfc_d2 w = __builtin_bit_cast(pfc_d2, __builtin_amdgcn_raw_buffer_load_b128(rv, (p0 + k) * 8, 0, 0));
+    f.v[k] = w.x;
+    f.v[k + 1] = w.y;
+  }
+#pragma unroll
+  for (int k = 0; k < PF; k += 4) {
+    const pfc_u4 c = __builtin_amdgcn_raw_buffer_load_b128(rc, (p0 + k) * 4, 0, 0);
+    f.c[k] = (int)c.x;
+    f.c[k + 1] = (int)c.y;
+    f.c[k + 2] = (int)c.z;
+    f.c[k + 3] = (int)c.w;
+  }
+}
+
+// the prefetched registers are read here on every path (see k_power.hip)
+template <int PF>
+__device__ __forceinline__ void pfc_settle(const PfcRow<PF>& f) {
+#pragma unroll
+  for (int k = 0; k < PF; ++k) asm volatile("" ::"v"(f.v[k]), "v"(f.c[k]));
+}
+
+// one factor: row t of S from cur (R x WC in LDS) into acc, entries from f
+template <int PF, int WC>
+__device__ __forceinline__ void pfc_row(const ChainCsrOp& S, const PfcRow<PF>& f, const double* cur, int R,
+                                        double (&acc)[WC]) {
+#pragma unroll
+  for (int c = 0; c < WC; ++c) acc[c] = 0.0;
+#pragma unroll
+  for (int k = 0; k < PF; ++k)
+    if (k < f.n) {
+#pragma unroll
+      for (int c = 0; c < WC; ++c) acc[c] = fma(f.v[k], cur[c * R + f.c[k]], acc[c]);
+    }
+  if (f.n > PF) {
+    // the rest of the row in batches of 8: loads issued together (indices
+    // clamped inside the row), then the fma chain in k order
+    const pfc_gd v = (pfc_gd)S.v;
+    const pfc_gi ci = (pfc_gi)S.ci;
+    const int qe = f.p + f.n;
+    for (int base = f.p + PF; base < qe; base += 8) {
+      double w[8];
+      int k[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int q = min(base + u, qe - 1);
+        w[u] = v[q];
+        k[u] = ci[q];
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (base + u < qe) {
+#pragma unroll
+          for (int c = 0; c < WC; ++c) acc[c] = fma(w[u], cur[c * R + k[u]], acc[c]);
+        }
+    }
+  }
+}
+
+template <int PF, int WC>
+__device__ __forceinline__ void pfc_factor(const ChainArgsF64& a, const int* rpl, const int* rpo, int f,
+                                           const double* cur, double* nxt, PfcRow<PF>& fr, int c0, int p,
+                                           double* __restrict__ Y, int ldy, double alpha, double beta) {
+  const int t = threadIdx.x;
+  pfc_settle(fr);
+  f = __builtin_amdgcn_readfirstlane(f);
+  const ChainCsrOp S = a.f[f];
+  double acc[WC];
+  if (t < S.rows) pfc_row<PF, WC>(S, fr, cur, a.R, acc);
+  // entries of the factor after next into the registers just consumed
+  const int f2 = min(f + 2, a.J - 1);
+  pfc_fetch<PF>(a.f[f2], rpl, rpo[f2], t, fr);
+  if (t < S.rows) {
+    if (f < a.J - 1) {
+#pragma unroll
+      for (int c = 0; c < WC; ++c) nxt[c * a.R + t] = acc[c];
+    } else {
+#pragma unroll
+      for (int c = 0; c < WC; ++c)
+        if (c0 + c < p) {
+          double* y = Y + (size_t)(c0 + c) * ldy + t;
+          const double tt = alpha * acc[c];
+          *y = (beta == 0.0) ? tt : fma(beta, *y, tt);
+        }
+    }
+  }
+  pfc_lds_barrier();
+}
+
+template <int PF, int WC>
+__global__ __launch_bounds__(PFC_THREADS) void k_chain_pf_f64(ChainArgsF64 a, const double* __restrict__ X, int ldx,
+                                                              int p, double* __restrict__ Y, int ldy, double alpha,
+                                                              double beta) {
+  extern __shared__ double lds[];  // cur, nxt (R x WC each), then the row pointers
+  __shared__ int rpo[CHAIN_MAX];
+  const int t = threadIdx.x;
+  const int c0 = blockIdx.x * WC;
+  const int R = a.R;
+  double* cur = lds;
+  double* nxt = lds + (size_t)R * WC;
+  int* rpl = reinterpret_cast<int*>(lds + (size_t)2 * R * WC);
+  int off = 0;
+  for (int f = 0; f < a.J; ++f) {
+    const ChainCsrOp S = a.f[f];
+    if (t == 0) rpo[f] = off;
+    for (int i = t; i <= S.rows; i += PFC_THREADS) rpl[off + i] = S.rp[i];
+    off += S.rows + 1;
+  }
+  const int n0 = a.in_rows;
+  for (int i = t; i < n0 * WC; i += PFC_THREADS) {
+    const int c = i / n0, r = i - c * n0;
+    cur[(size_t)c * R + r] = (c0 + c < p) ? X[(size_t)(c0 + c) * ldx + r] : 0.0;
+  }
+  __syncthreads();
+  PfcRow<PF> fa, fb;  // entries of the factors applied 1st, 3rd, ... and 2nd, 4th, ...
+  pfc_fetch<PF>(a.f[0], rpl, rpo[0], t, fa);
+  pfc_fetch<PF>(a.f[min(1, a.J - 1)], rpl, rpo[min(1, a.J - 1)], t, fb);
+  for (int f = 0; f < a.J; f += 2) {
+    pfc_factor<PF, WC>(a, rpl, rpo, f, cur, nxt, fa, c0, p, Y, ldy, alpha, beta);
+    double* sw = cur;
+    cur = nxt;
+    nxt = sw;
+    if (f + 1 == a.J) break;
+    pfc_factor<PF, WC>(a, rpl, rpo, f + 1, cur, nxt, fb, c0, p, Y, ldy, alpha, beta);
+    sw = cur;
+    cur = nxt;
+    nxt = sw;
+  }
+}
+
+// Columns per workgroup of the prefetched chain: FH_CHAIN_PF_WC (1/2/4), else
+// 1 up to 128 columns, 2 up to 512, 4 beyond (a workgroup reads every factor
+// once whatever its width, so narrow panels spread that over more CUs while
+// there are few columns).
+static int pfc_wc(int p) {
+  if (const char* e = getenv("FH_CHAIN_PF_WC")) {
+    const int w = atoi(e);
+    if (w == 1 || w == 2 || w == 4) return w;
+  }
+  return p <= 128 ? 1 : p <= 512 ? 2 : 4;
+}
+
+bool csr_chain_pf_fits(const std::vector<ChainCsrOp>& ops, int in_rows, int p) {
+  const int J = (int)ops.size();
+  if (J < 2 || J > CHAIN_MAX || p <= 0 || in_rows < 1 || in_rows > PFC_THREADS) return false;
+  size_t rp_ints = 0;
+  for (const auto& o : ops) {
+    if (o.rows < 1 || o.rows > PFC_THREADS || (int64_t)o.nnz * 8 > INT32_MAX) return false;
+    rp_ints += (size_t)o.rows + 1;
+  }
+  return rp_ints <= (size_t)PFC_RP_INTS;
+}
+
+int csr_chain_pf_default_wc(int p) { return pfc_wc(p); }
+
+int csr_chain_pf_plan_f64(const std::vector<ChainCsrOp>& ops, int in_rows, int p) {
+  if (mul_mode() != FH_MUL_AUTO) return 0;
+  // FH_CHAIN_PF=0, or FH_CHAIN_FUSED set (either way: the staged one-launch
+  // chain or the factor-by-factor launches are asked for), turns it off
+  if (const char* e = getenv("FH_CHAIN_PF"))
+    if (e[0] == '0') return 0;
+  if (getenv("FH_CHAIN_FUSED")) return 0;
+  return csr_chain_pf_fits(ops, in_rows, p) ? pfc_wc(p) : 0;
+}
+
+int csr_chain_fit_wc(const std::vector<ChainCsrOp>& ops, int in_rows, int p) {
+  const int J = (int)ops.size();
+  if (J < 2 || J > CHAIN_MAX || p <= 0) return 0;
+  int R = in_rows, max_nnz = 0;
+  for (const auto& o : ops) {
+    R = std::max(R, o.rows);
+    max_nnz = std::max(max_nnz, o.nnz);
+  }
+  if (R <= 0) return 0;
+  const int wc = chain_wc(R, max_nnz, p);
+  return (wc == 0 || (p + wc - 1) / wc > 1024) ? 0 : wc;
+}
+
+template <int PF, int WC>
+static void pfc_go(const ChainArgsF64& a, size_t lds, const double* X, int ldx, int p, double* Y, int ldy,
+                   double alpha, double beta, hipStream_t st) {
+  static bool attr = false;
+  if (!attr) {
+    FH_HIP(hipFuncSetAttribute((const void*)k_chain_pf_f64<PF, WC>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)(sizeof(double) * 2 * PFC_THREADS * 4 + sizeof(int) * PFC_RP_INTS)));
+    attr = true;
+  }
+  const dim3 grid((unsigned)((p + WC - 1) / WC));
+  hipLaunchKernelGGL((k_chain_pf_f64<PF, WC>), grid, dim3(PFC_THREADS), lds, st, a, X, ldx, p, Y, ldy, alpha, beta);
+}
+
+void csr_chain_pf_run_f64(const std::vector<ChainCsrOp>& ops, int in_rows, int wc, int pf, const double* X, int ldx,
+                          int p, double* Y, int ldy, double alpha, double beta, hipStream_t st) {
+  const int J = (int)ops.size();
+  FH_CHECK(J >= 2 && J <= CHAIN_MAX && (wc == 1 || wc == 2 || wc == 4) && (pf == 8 || pf == 16),
+           "prefetched chain: bad plan");
+  ChainArgsF64 a{};
+  a.J = J;
+  a.in_rows = in_rows;
+  a.R = in_rows;
+  size_t rp_ints = 0;
+  for (int i = 0; i < J; ++i) {
+    a.f[i] = ops[i];
+    a.R = std::max(a.R, ops[i].rows);
+    rp_ints += (size_t)ops[i].rows + 1;
+  }
+  FH_CHECK(a.R <= PFC_THREADS && rp_ints <= (size_t)PFC_RP_INTS, "prefetched chain: shape out of range");
+  const size_t lds = sizeof(double) * 2 * (size_t)a.R * wc + sizeof(int) * rp_ints;
+#define FH_PFC(PF, WC) pfc_go<PF, WC>(a, lds, X, ldx, p, Y, ldy, alpha, beta, st)
+  if (pf == 8) {
+    if (wc == 1) FH_PFC(8, 1); else if (wc == 2) FH_PFC(8, 2); else FH_PFC(8, 4);
+  } else {
+    if (wc == 1) FH_PFC(16, 1); else if (wc == 2) FH_PFC(16, 2); else FH_PFC(16, 4);
+  }
+#undef FH_PFC
+  FH_LAUNCH_CHECK();
+}
+
 // C(i,j) = alpha * sum_{k in col j of op(S)} op(A)(i,k) * op(S)(k,j) + beta*C(i,j)
 // (cp, ri, cv) is the column-compressed form of op(S): CSC of S for op N,
 // CSR of S for op T.

@@ -258,7 +258,9 @@ class Faust:
             return c
         if all(d == _lib.FH_F64 for _, d in kinds):
             if all(k == _lib.FH_MAT_CSR for k, _ in kinds):
-                return {"PER_FACTOR": _lib.FH_MUL_PER_FACTOR, "FUSED": _lib.FH_MUL_FUSED}
+                # AUTO: the one-launch chain with entries loaded ahead when
+                # every row fits one thread per row, else the default form
+                return {"AUTO": _lib.FH_MUL_AUTO, "PER_FACTOR": _lib.FH_MUL_PER_FACTOR, "FUSED": _lib.FH_MUL_FUSED}
             return {"PER_FACTOR": _lib.FH_MUL_PER_FACTOR}
         return {"CSR_ROWS1": _lib.FH_MUL_CSR_ROWS1, "CSR_ROWS2": _lib.FH_MUL_CSR_ROWS2}
 

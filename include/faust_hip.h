@@ -44,6 +44,13 @@ const char *fh_last_error(void);
 /* Library version string. */
 const char *fh_version(void);
 
+/* Forget the execution forms the autotuner measured (FP64 CSR chains in
+ * FH_MUL_AUTO mode time their one-launch and per-factor forms on the first
+ * product of each chain signature outside graph capture, and keep the
+ * quickest; FH_AUTOTUNE=0 turns it off).  No counterpart in the reference,
+ * whose optimize_multiply is an explicit call (faust_TransformHelper.hpp:436-512). */
+int fh_autotune_clear(void);
+
 /* ---- device selection: gm_GenPurposeFunc dev_count/set_dev/cur_dev
  *      (src/faust_linear_operator/GPU2/faust_cuda_device.cpp:6-30) */
 int fh_dev_count(int *count);
@@ -127,8 +134,9 @@ int fh_faust_last_launches(fh_faust_t F, int *launches);
 /* Execution form of F's products (TransformHelper::set_FM_mul_mode,
  * faust_TransformHelper.hpp:436-512 optimize_multiply, which times the modes
  * and keeps the quickest; faust_amd.Faust.optimize_time does the same on the
- * GPU): FH_MUL_AUTO (defaults), FH_MUL_PER_FACTOR / FH_MUL_FUSED (FP64 CSR
- * chains: one launch per factor / the one-launch chain when it fits LDS),
+ * GPU): FH_MUL_AUTO (defaults; for FP64 CSR chains the autotuned form, see
+ * fh_autotune_clear), FH_MUL_PER_FACTOR / FH_MUL_FUSED (FP64 CSR chains: one
+ * launch per factor / the staged one-launch chain when it fits LDS),
  * FH_MUL_BSR_KERNEL(k) (BSR chains: kernel k = 0..5, as FH_BSR_KERNEL),
  * FH_MUL_CSR_ROWS1 / FH_MUL_CSR_ROWS2 (FP32 CSR chains: rows per thread).
  * A mode that does not apply to F's chain kind falls back to the default. */

@@ -132,6 +132,70 @@ def test_fused_chain_matches_per_factor(gpu, monkeypatch, wc, op):
         assert np.array_equal(Y1, oracle.chain_multiply(facs, X, op))
 
 
+@pytest.mark.parametrize("wc", ["1", "2", "4"])
+@pytest.mark.parametrize("op", ["N", "T"])
+def test_prefetched_chain_matches_per_factor(gpu, monkeypatch, wc, op):
+    # the one-launch chain with one thread per row (every row <= 1024) and the
+    # entries loaded two factors ahead, against the factor-by-factor launches
+    # and the oracle, bit for bit: rectangular factors, an empty row, rows of
+    # 0 to 40 entries (past the 8 / 16 prefetched: the loop reads the rest),
+    # odd chain lengths, column counts that leave partial panels
+    monkeypatch.setenv("FH_CHAIN_PF_WC", wc)
+    monkeypatch.setenv("FH_AUTOTUNE", "0")  # the default form, not the measured one
+    monkeypatch.delenv("FH_CHAIN_FUSED", raising=False)
+    rng = np.random.default_rng(23)
+    for shapes in ([(300, 700), (700, 129), (129, 1000), (1000, 500)],
+                   [(1024, 1024)] * 5,
+                   [(64, 1024), (1024, 1024), (1024, 17)]):
+        facs = []
+        for r, c in shapes:
+            a = sp.random(r, c, density=min(1.0, 9.0 / c), random_state=rng, format="csr").toarray()
+            a[3, : min(c, 40)] = rng.standard_normal(min(c, 40))  # a row of 40 entries
+            a[min(4, r - 1), :] = 0.0  # an empty row
+            facs.append(sp.csr_matrix(a))
+        F = faust_amd.Faust(facs)
+        G = F if op == "N" else F.T
+        rows_in = shapes[-1][1] if op == "N" else shapes[0][0]
+        for p in (1, 7, 64, 200):
+            X = np.asfortranarray(rng.standard_normal((rows_in, p)))
+            monkeypatch.delenv("FH_CHAIN_PF", raising=False)
+            Y1 = G @ X
+            assert _launches(G) == 1
+            monkeypatch.setenv("FH_CHAIN_PF", "0")
+            monkeypatch.setenv("FH_CHAIN_FUSED", "0")
+            Y2 = G @ X
+            monkeypatch.delenv("FH_CHAIN_FUSED")
+            assert _launches(G) == len(facs)
+            assert np.array_equal(Y1, Y2)
+            assert np.array_equal(Y1, oracle.chain_multiply(facs, X, op))
+
+
+@pytest.mark.parametrize("op", ["N", "T"])
+def test_autotuned_chain_form(gpu, monkeypatch, op):
+    # AUTO mode measures the forms of an FP64 CSR chain on its first product
+    # and keeps the quickest: every product, before and after, has the
+    # oracle's bits; clearing the measurements measures again; FH_AUTOTUNE=0
+    # gives the default (one-launch) form
+    from faust_amd import _lib
+    monkeypatch.delenv("FH_CHAIN_FUSED", raising=False)
+    monkeypatch.delenv("FH_CHAIN_PF", raising=False)
+    monkeypatch.delenv("FH_AUTOTUNE", raising=False)
+    facs = _csr_chain_factors()
+    F = faust_amd.Faust(facs)
+    G = F if op == "N" else F.T
+    X = np.asfortranarray(np.random.default_rng(9).standard_normal((1024, 48)))
+    ref = oracle.chain_multiply(facs, X, op)
+    _lib.check(_lib.lib().fh_autotune_clear())
+    for _ in range(3):
+        assert np.array_equal(G @ X, ref)
+        assert _launches(G) in (1, len(facs))
+    _lib.check(_lib.lib().fh_autotune_clear())
+    assert np.array_equal(G @ X, ref)
+    monkeypatch.setenv("FH_AUTOTUNE", "0")
+    assert np.array_equal(G @ X, ref)
+    assert _launches(G) == 1
+
+
 def test_fused_chain_falls_back_when_too_tall(gpu):
     # 12000-row intermediates do not fit two LDS buffers even at one column
     rng = np.random.default_rng(22)
@@ -145,23 +209,28 @@ def test_fused_chain_falls_back_when_too_tall(gpu):
 
 def test_long_chains_fuse_without_ell_views(gpu, monkeypatch):
     # the 10-factor Hadamard Faust of order 1024 (2 nnz per row and column):
-    # one launch per factor through the ELL views by default, one launch for
-    # the whole chain when the views are off; the 5-factor chain of 1024^2
-    # factors is never fused by default.  Same bits every way.
+    # one launch (entries loaded ahead) by default; without that form, one
+    # launch per factor through the ELL views, one launch for the whole chain
+    # (staged factors) when the views are off; the 5-factor chain of 1024^2
+    # factors is then never fused.  Same bits every way.
     monkeypatch.delenv("FH_CHAIN_FUSED", raising=False)
     monkeypatch.delenv("FH_CHAIN_WC", raising=False)
+    monkeypatch.setenv("FH_AUTOTUNE", "0")
     facs = [sp.csr_matrix(f) for f in faust_amd.wht_factors(1024)]
     F = faust_amd.Faust(facs)
     X = np.asfortranarray(np.random.default_rng(7).standard_normal((1024, 33)))
-    for ell, launches in (("1", 10), ("0", 1)):
+    for pf, ell, launches in (("1", "1", 1), ("0", "1", 10), ("0", "0", 1)):
+        monkeypatch.setenv("FH_CHAIN_PF", pf)
         monkeypatch.setenv("FH_CSR_ELL", ell)
         for G, op in ((F, "N"), (F.T, "T")):
             Y = G @ X
             assert _launches(G) == launches
             assert np.array_equal(Y, oracle.chain_multiply(facs, X, op))
     F5 = faust_amd.Faust(_csr_chain_factors())
-    F5 @ X
-    assert _launches(F5) == 5
+    for pf, launches in (("1", 1), ("0", 5)):
+        monkeypatch.setenv("FH_CHAIN_PF", pf)
+        F5 @ X
+        assert _launches(F5) == launches
 
 
 def test_optimize_time_fp64(gpu):
@@ -174,8 +243,8 @@ def test_optimize_time_fp64(gpu):
     X = np.asfortranarray(np.random.default_rng(8).standard_normal((1024, 64)))
     ref = oracle.chain_multiply(facs, X, "N")
     G = F.optimize_time(nsamples=3, mat=X)
-    assert G is not F and G.mul_mode_name in ("PER_FACTOR", "FUSED")
-    assert set(G.mul_mode_times_ms) == {"PER_FACTOR", "FUSED"}
+    assert G is not F and G.mul_mode_name in ("AUTO", "PER_FACTOR", "FUSED")
+    assert set(G.mul_mode_times_ms) == {"AUTO", "PER_FACTOR", "FUSED"}
     assert np.array_equal(G @ X, ref)
     m = C.c_int()
     _lib.check(_lib.lib().fh_faust_get_mul_mode(F.h, C.byref(m)))
