@@ -13,7 +13,7 @@ class fh_batch_params(C.Structure):
     _fields_ = [("dtype", C.c_int), ("num_its", C.c_int), ("refine_its", C.c_int), ("prox", C.c_int),
                 ("warm_power", C.c_int), ("norm2_max_iter", C.c_int), ("norm2_threshold", C.c_double),
                 ("use_graph", C.c_int), ("autotune", C.c_int), ("check_every", C.c_int),
-                ("target_err", C.c_double)]
+                ("target_err", C.c_double), ("target_errs", C.c_void_p)]
 
 
 class BatchResult:
@@ -32,7 +32,9 @@ def palm4msa_batched(targets, nfacts, k=None, supports=None, init=None, lambda_i
                      target_err=0.0):
     """targets: list of square FP64 arrays.  k: sp parameter per target (list or
     int) for the sp projector; supports: per target a list of nfacts 0/1 masks
-    (prox supp) instead.  init: per target a list of nfacts initial factors."""
+    (prox supp) instead.  init: per target a list of nfacts initial factors.
+    target_err: the FP64 relative error to reach (checked every check_every
+    iterations), one for all targets or one per target."""
     L = lib()
     nt = len(targets)
     As = [np.asfortranarray(a, dtype=np.float64) for a in targets]
@@ -49,7 +51,15 @@ def palm4msa_batched(targets, nfacts, k=None, supports=None, init=None, lambda_i
     p.num_its, p.refine_its = int(num_its), int(refine_its)
     p.warm_power, p.use_graph, p.autotune = int(warm_power), int(use_graph), int(autotune)
     p.norm2_max_iter, p.norm2_threshold = int(norm2_max_iter), float(norm2_threshold)
-    p.check_every, p.target_err = int(check_every), float(target_err)
+    p.check_every = int(check_every)
+    if np.isscalar(target_err):
+        p.target_err = float(target_err)
+    else:
+        te = np.ascontiguousarray(target_err, dtype=np.float64)
+        if te.shape != (nt,):
+            raise ValueError("target_err: one value, or one per target")
+        keep.append(te)
+        p.target_errs = te.ctypes.data
     if supports is not None:
         p.prox = _lib.PROX["supp"]
         masks = [np.asfortranarray(m, dtype=np.float64) for s in supports for m in s]

@@ -410,7 +410,7 @@ struct Progress {
 };
 
 template <class T>
-static bool run_checked(Batch<T>& B, int its, int it0, bool graph, int check_every, double target_err,
+static bool run_checked(Batch<T>& B, int its, int it0, bool graph, int check_every, const std::vector<double>& thr,
                         Verifier& ver, Progress& pr, std::chrono::steady_clock::time_point t_start,
                         double& ms_iter, hipStream_t st) {
   if (check_every <= 0) {
@@ -425,7 +425,7 @@ static bool run_checked(Batch<T>& B, int its, int it0, bool graph, int check_eve
     ran += chunk;
     ver.run(B, err.data(), st);
     for (int b = 0; b < B.sp.nt; ++b)
-      if (pr.first_hit[b] < 0 && err[b] <= target_err) {
+      if (pr.first_hit[b] < 0 && err[b] <= thr[b]) {
         pr.first_hit[b] = it0 + ran;
         pr.done++;
       }
@@ -522,6 +522,8 @@ extern "C" int fh_palm4msa_batched(int ntargets, int nfacts, const int* n, const
   }
   std::vector<int64_t> kv;
   if (p->prox == FH_PROX_SP) kv.assign(k, k + ntargets);
+  std::vector<double> thr(ntargets, p->target_err);  // error each target has to reach
+  if (p->target_errs) thr.assign(p->target_errs, p->target_errs + ntargets);
   std::vector<double> lam_out(ntargets);
   if (p->dtype == FH_F32) {
     Batch<float> B;
@@ -551,7 +553,7 @@ extern "C" int fh_palm4msa_batched(int ntargets, int nfacts, const int* n, const
     pr.first_hit.assign(ntargets, -1);
     if (p->check_every > 0) ver.init(sp, Adp, st);
     const auto t_start = std::chrono::steady_clock::now();
-    const bool all = run_checked(B, p->num_its, 0, p->use_graph != 0, p->check_every, p->target_err, ver, pr,
+    const bool all = run_checked(B, p->num_its, 0, p->use_graph != 0, p->check_every, thr, ver, pr,
                                  t_start, r->ms[1], st);
     // frozen-support FP64 refinement
     Batch<double> D;
@@ -579,7 +581,7 @@ extern "C" int fh_palm4msa_batched(int ntargets, int nfacts, const int* n, const
       build_schedule(D, {});
       FH_HIP(hipStreamSynchronize(st));
       const double setup_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t1).count();
-      run_checked(D, p->refine_its, p->num_its, p->use_graph != 0, p->check_every, p->target_err, ver, pr, t_start,
+      run_checked(D, p->refine_its, p->num_its, p->use_graph != 0, p->check_every, thr, ver, pr, t_start,
                   r->ms[2], st);
       r->ms[0] += setup_ms;
       auto t2 = std::chrono::steady_clock::now();
@@ -638,7 +640,7 @@ extern "C" int fh_palm4msa_batched(int ntargets, int nfacts, const int* n, const
     pr.first_hit.assign(ntargets, -1);
     if (p->check_every > 0) ver.init(sp, Adp, st);
     const auto t_start = std::chrono::steady_clock::now();
-    run_checked(B, p->num_its + p->refine_its, 0, p->use_graph != 0, p->check_every, p->target_err, ver, pr, t_start,
+    run_checked(B, p->num_its + p->refine_its, 0, p->use_graph != 0, p->check_every, thr, ver, pr, t_start,
                 r->ms[1], st);
     r->iters_to_target = pr.first_hit;
     r->ms[4] = pr.ms_all;

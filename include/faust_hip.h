@@ -313,6 +313,7 @@ typedef struct {
   int autotune;        /* time 32/64 GEMM tiles per target size at set-up */
   int check_every;     /* > 0: FP64 relative-error check every check_every iterations */
   double target_err;   /* stop when every target's FP64 relative error <= target_err */
+  const double *target_errs; /* per-target thresholds (ntargets of them) in place of target_err, or NULL */
 } fh_batch_params;
 
 typedef struct fh_batch_result_s *fh_batch_result_t;
@@ -329,9 +330,9 @@ int fh_batch_result_lambda(fh_batch_result_t r, double *lambdas);
 int fh_batch_result_relerr(fh_batch_result_t r, double *errs);
 int fh_batch_result_factor(fh_batch_result_t r, int target, int f, double *dense_out);
 /* ms[0] set-up, ms[1] main iterations, ms[2] refinement, ms[3] verification,
- * ms[4] wall time until every target reached target_err (-1: not reached) */
+ * ms[4] wall time until every target reached its threshold (-1: not reached) */
 int fh_batch_result_times(fh_batch_result_t r, double *ms5);
-/* per target: iterations run when its error was first seen <= target_err (-1: never) */
+/* per target: iterations run when its error was first seen <= its threshold (-1: never) */
 int fh_batch_result_iters(fh_batch_result_t r, int *iters);
 int fh_batch_result_free(fh_batch_result_t r);
 
