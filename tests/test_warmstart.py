@@ -1,5 +1,6 @@
 """Butterfly targets and the canonical form of their factorizations (CPU)."""
 import numpy as np
+import pytest
 
 from faust_amd import warmstart as ws
 
@@ -59,3 +60,40 @@ def test_canonical_form_is_invariant_to_diagonal_rescaling():
     for c in can0[:-1]:
         assert np.allclose(np.linalg.norm(c, axis=0), 1.0 / np.sqrt(n))
         assert np.all(np.diag(c) > 0)
+
+
+@pytest.mark.parametrize("n", [16, 32, 64])
+def test_butterfly_log_magnitudes_are_linear_of_rank_n_L_plus_1(n):
+    # Every entry of a full butterfly product B_0 ... B_{L-1} is the product of
+    # exactly one entry of each factor (a single path), so log|A| = M log|F|
+    # for the 0/1 path-incidence matrix M (n^2 x 2nL, L ones per row), and
+    # sign(A) is the parity of the path's signs.  M's rank is n (L + 1): the
+    # 2nL factor values minus the n (L - 1) diagonal rescalings between
+    # consecutive factors.  At n = 256 (L = 8) the map from log|A| to the
+    # factors' log-magnitudes therefore has rank 2304, above the 512-wide
+    # hidden layers of the configured 65536 -> 512 -> 512 -> 4096 network.
+    from faust_amd import warmstart as ws
+    rng = np.random.default_rng(n)
+    masks = ws.butterfly_supports(n)
+    L = len(masks)
+    _, facs = ws.butterfly_sample(n, rng, 0.0)
+    facs = [np.abs(f) + (m != 0) * 0.1 for f, m in zip(facs, masks)]  # strictly positive entries on the masks
+
+    def logA(fs):
+        P = np.eye(n)
+        for f in fs:
+            P = P @ f
+        return np.log(P).ravel()
+
+    v = ws.pack_values(facs, masks)
+    base = logA(facs)
+    M = np.empty((n * n, v.size))
+    for p in range(v.size):  # d log A / d log F_p, exact: log A is linear in log F
+        w = v.copy()
+        w[p] *= np.e
+        M[:, p] = logA(ws.unpack_values(w, masks)) - base
+    assert np.allclose(M, np.round(M), atol=1e-9)
+    M = np.round(M)
+    assert np.all(M.sum(axis=1) == L)  # one entry of every factor on each path
+    assert np.allclose(M @ np.log(v), base, atol=1e-9)
+    assert np.linalg.matrix_rank(M) == n * (L + 1)
