@@ -99,7 +99,7 @@ def _bsr_workload(values, act="bf16"):
     nb = (n // bs) * bpr
     per_factor = nb * bs * bs * vbytes + nb * 4 + (n // bs + 1) * 4 + n * p * abytes * 2
     if act == "f32":
-        kernel = "k_bsr_f32"
+        kernel = "k_bsr_f32s"
     else:
         kernel = "k_bsr_shared<false,*,*>" if values == "bf16" else "k_bsr_shared<true,*,*>"
     return dict(name=f"faust_bsr_{values}_8192x8192_J6_b32x32_4blk_X8192x512_{act}", facs=facs, X=X,

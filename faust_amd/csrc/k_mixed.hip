@@ -587,6 +587,91 @@ __global__ __launch_bounds__(256) void k_bsr_f32(int nbr, int p, const int* __re
   }
 }
 
+// Shared-block form: a workgroup owns one block row x 128 columns (one
+// 32-column tile per wave, as k_bsr_shared does for BF16), the row's blocks
+// (up to 4 at a time, 16 KiB) are staged once in LDS for the four waves, and
+// each wave issues the X loads of all of them before its first MFMA.  The
+// MFMA sequence of every output tile is k_bsr_f32's (blocks ascending, 16
+// K-steps each), so the bits are the same.  Consecutive workgroups walk the
+// column groups first (one group per XCD pair, as in k_bsr_shared).
+constexpr int BF32_CH = 4;  // blocks staged per round
+__global__ __launch_bounds__(256) void k_bsr_f32s(int nbr, int p, const int* __restrict__ brp,
+                                                  const int* __restrict__ bci, const float* __restrict__ blk,
+                                                  const float* __restrict__ X, int ldx, float* __restrict__ Y, int ldy,
+                                                  float alpha, float beta, int ngroups) {
+  __shared__ f32x4 sa[BF32_CH * 256];  // BF32_CH blocks, 1024 floats each
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, t = threadIdx.x;
+  const int g = blockIdx.x % ngroups, br = blockIdx.x / ngroups;
+  if (br >= nbr) return;
+  const int h = lane >> 5, r = lane & 31;
+  const int jc = g * 128 + wave * 32 + r;
+  const bool jok = jc < p;
+  const int j = jok ? jc : 0;  // columns past p read column 0; their outputs are not stored
+  f32x16 acc = {};
+  const int b0 = brp[br], b1 = brp[br + 1];
+  for (int c0 = b0; c0 < b1; c0 += BF32_CH) {
+    const int nb = min(BF32_CH, b1 - c0);
+    if (c0 > b0) __syncthreads();  // the previous round's blocks are consumed
+    if (nb == BF32_CH) {
+      // a full round: the staging loads and this wave's X column segments of
+      // all its blocks are issued together, before any LDS write or MFMA
+      f32x4 stg[BF32_CH], x[BF32_CH][4];
+#pragma unroll
+      for (int q = 0; q < BF32_CH; ++q) stg[q] = reinterpret_cast<const f32x4*>(blk + (size_t)(c0 + q) * 1024)[t];
+#pragma unroll
+      for (int q = 0; q < BF32_CH; ++q) {
+        const f32x4* xp = reinterpret_cast<const f32x4*>(X + (size_t)j * ldx + bci[c0 + q] * 32 + h * 16);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) x[q][u] = xp[u];
+      }
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int q = 0; q < BF32_CH; ++q) sa[q * 256 + t] = stg[q];
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < BF32_CH; ++q) {
+        f32x4 a[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) a[u] = sa[q * 256 + lane * 4 + u];
+#pragma unroll
+        for (int s4 = 0; s4 < 16; ++s4)
+          acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a[s4 >> 2][s4 & 3], x[q][s4 >> 2][s4 & 3], acc, 0, 0, 0);
+      }
+    } else {
+      // the row's last, partial round: one block at a time
+      for (int q = 0; q < nb; ++q) {
+        if (q > 0) __syncthreads();
+        sa[t] = reinterpret_cast<const f32x4*>(blk + (size_t)(c0 + q) * 1024)[t];
+        const f32x4* xp = reinterpret_cast<const f32x4*>(X + (size_t)j * ldx + bci[c0 + q] * 32 + h * 16);
+        f32x4 x[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) x[u] = xp[u];
+        __syncthreads();
+        f32x4 a[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) a[u] = sa[lane * 4 + u];
+#pragma unroll
+        for (int s4 = 0; s4 < 16; ++s4)
+          acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a[s4 >> 2][s4 & 3], x[s4 >> 2][s4 & 3], acc, 0, 0, 0);
+      }
+    }
+  }
+  if (!jok) return;
+#pragma unroll
+  for (int gg = 0; gg < 4; ++gg) {
+    f32x4* yq = reinterpret_cast<f32x4*>(Y + (size_t)j * ldy + br * 32 + 8 * gg + 4 * h);
+    f32x4 v = {alpha * acc[4 * gg], alpha * acc[4 * gg + 1], alpha * acc[4 * gg + 2], alpha * acc[4 * gg + 3]};
+    if (beta != 0.0f) {
+      const f32x4 old = *yq;
+      v[0] = fmaf(beta, old[0], v[0]);
+      v[1] = fmaf(beta, old[1], v[1]);
+      v[2] = fmaf(beta, old[2], v[2]);
+      v[3] = fmaf(beta, old[3], v[3]);
+    }
+    *yq = v;
+  }
+}
+
 void bsr_mul_f32(const DMat& B, char op, const float* X, int ldx, int p, float* Y, int ldy, float alpha, float beta,
                  hipStream_t st) {
   FH_CHECK(B.kind == K_BSR && B.dtype == FH_F32 && B.f32dat.p, "FP32 activations need an FP32 BSR factor");
@@ -596,11 +681,20 @@ void bsr_mul_f32(const DMat& B, char op, const float* X, int ldx, int p, float* 
   FH_CHECK(ldx % 4 == 0 && ldy % 4 == 0 && ((uintptr_t)X & 15) == 0 && ((uintptr_t)Y & 15) == 0,
            "FP32 BSR path: X and Y must be 16-byte aligned with leading dimensions that are multiples of 4");
   if (nbr == 0 || p == 0) return;
-  const int ntiles = (p + 31) / 32;
-  const dim3 grid((unsigned)(ntiles * ((nbr + 3) / 4)));
-  hipLaunchKernelGGL(k_bsr_f32, grid, dim3(256), 0, st, nbr, p, (tr ? B.tbrp : B.brp).as<int>(),
-                     (tr ? B.tbci : B.bci).as<int>(), (tr ? B.tf32dat : B.f32dat).as<float>(), X, ldx, Y, ldy, alpha,
-                     beta, ntiles);
+  const int* brp = (tr ? B.tbrp : B.brp).as<int>();
+  const int* bci = (tr ? B.tbci : B.bci).as<int>();
+  const float* blk = (tr ? B.tf32dat : B.f32dat).as<float>();
+  // FH_BSR_F32_KERNEL=0: one wave per (block row, tile), blocks from global memory
+  const char* e = getenv("FH_BSR_F32_KERNEL");
+  if (e && e[0] == '0') {
+    const int ntiles = (p + 31) / 32;
+    const dim3 grid((unsigned)(ntiles * ((nbr + 3) / 4)));
+    hipLaunchKernelGGL(k_bsr_f32, grid, dim3(256), 0, st, nbr, p, brp, bci, blk, X, ldx, Y, ldy, alpha, beta, ntiles);
+  } else {
+    const int ngroups = (p + 127) / 128;
+    hipLaunchKernelGGL(k_bsr_f32s, dim3((unsigned)(ngroups * nbr)), dim3(256), 0, st, nbr, p, brp, bci, blk, X, ldx,
+                       Y, ldy, alpha, beta, ngroups);
+  }
   FH_LAUNCH_CHECK();
 }
 

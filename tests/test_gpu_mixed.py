@@ -355,6 +355,28 @@ def test_bsr_fp32_activations_single_product(gpu, op, p):
     assert np.all(np.abs(Y - ref) <= 100 * 2.0 ** -24 * absref + 1e-30)
 
 
+@pytest.mark.parametrize("bpr", [3, 4, 5, 9])
+@pytest.mark.parametrize("op", ["N", "T"])
+def test_bsr_fp32_shared_block_kernel_same_bits(gpu, monkeypatch, bpr, op):
+    # k_bsr_f32s (a row's blocks staged in LDS for four column tiles, rounds of
+    # 4 blocks, the rest one by one) against k_bsr_f32 (FH_BSR_F32_KERNEL=0):
+    # the same MFMA sequence per output tile, the same bits; p = 200 leaves a
+    # partial 128-column group
+    rng = np.random.default_rng(40 + bpr)
+    B = _f32(rand_bsr(512, 384, 32, bpr, rng, scale=1 / np.sqrt(32 * bpr)))
+    rows_in = B.shape[1] if op == "N" else B.shape[0]
+    X = rng.standard_normal((rows_in, 200)).astype(np.float32).astype(np.float64)
+    F = faust_amd.Faust([B], dtype=np.float32, act_dtype=np.float32)
+    G = F if op == "N" else F.T
+    monkeypatch.delenv("FH_BSR_F32_KERNEL", raising=False)
+    Y1 = G @ X
+    monkeypatch.setenv("FH_BSR_F32_KERNEL", "0")
+    Y0 = G @ X
+    assert np.array_equal(Y1, Y0)
+    ref = oracle.chain_multiply([B], X, op)
+    assert np.linalg.norm(Y1 - ref) / np.linalg.norm(ref) < 1e-6
+
+
 def test_bsr_fp32_chain_8192x6_full_size_against_oracle_columns(gpu):
     # config 3a's FP32 leg at full size with FP32 activations: 6 FP32 BSR
     # factors 8192^2 (32x32 blocks, 4 per block row, N(0, 1/128)), X 8192x512
