@@ -386,6 +386,30 @@ def measured_traffic(wl):
 
 
 
+def plain_copy_rate(nbytes, reps):
+    """A plain device copy moving `nbytes` per copy (nbytes / 2 read, nbytes / 2
+    written), `reps` dependent copies captured in one HIP graph and replayed,
+    timed with HIP events: the rate a bandwidth-bound launch of that size gets
+    on this GPU with no arithmetic and no gathers.  Returns us per copy, GB/s."""
+    import torch
+    n = max(1, nbytes // 8)
+    a = torch.empty(n, dtype=torch.float32, device="cuda").normal_()
+    b = torch.empty_like(a)
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        for _ in range(reps):
+            b.copy_(a)
+    g.replay()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    g.replay()
+    e1.record()
+    torch.cuda.synchronize()
+    us = e0.elapsed_time(e1) * 1e3 / reps
+    return round(us, 3), round(8 * n / (us * 1e-6) / 1e9, 1)
+
+
 def cpu_baseline(wl, budget_s):
     import oracle
     nthreads = len(os.sched_getaffinity(0))
@@ -522,6 +546,13 @@ def main():
             "per_factor_event_avg_launch_us": round(k_event_ms * 1e3, 3), "launches_timed": int(cnt[:J].sum()),
             "launches_per_step": wl["launches_per_step"],
             "layout_conversion_us_per_step": round(conv_ms_step * 1e3, 3)}
+    # context for `frac`: a plain copy moving the same bytes per launch, and a
+    # 256 MiB copy (the asymptotic copy rate)
+    cu, cr = plain_copy_rate(wl["bytes_per_launch"], 200)
+    lu, lr = plain_copy_rate(1 << 29, 10)
+    roof["plain_copy"] = {"bytes_per_copy": 8 * max(1, wl["bytes_per_launch"] // 8), "us_per_copy": cu,
+                          "GB/s": cr, "frac": round(cr / HBM_PEAK_GBS, 4),
+                          "GB/s_256MiB_copy": lr, "frac_256MiB_copy": round(lr / HBM_PEAK_GBS, 4)}
 
     out = {"metric": "Faust·X HBM GB/s (% peak); PALM4MSA time-to-rel-err, tuned vs plain FP64",
            "value": round(value, 2), "unit": "GB/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,

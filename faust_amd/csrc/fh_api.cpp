@@ -191,7 +191,8 @@ static std::string chain_sig(char op, int p, int in_rows, const std::vector<Chai
 // Y = op(F) X with the FP64 chain.  AUTO mode on an all-CSR chain uses the
 // form measured quickest for this chain signature (measured on its first
 // call outside graph capture: every candidate once untimed, then the minimum
-// of three timed runs, HIP events on the launch stream); otherwise, and when
+// of three timed runs, HIP events on the launch stream; the default form
+// stays unless another is at least 3% quicker); otherwise, and when
 // measuring is off or impossible, the prefetched one-launch chain where the
 // shape allows it, the staged one where the planner takes it, else one launch
 // per factor.  `tune` false (allocation-free callers) never measures.
@@ -294,14 +295,21 @@ void faust_multiply_f64(const std::vector<DMatP>& f, char op, const double* X, i
         }
       }
       if (!known && tune && !stream_capturing(st)) {
-        std::vector<ChainForm> cand{{0, 0, 0}};
+        // the default form first: another replaces the quickest so far only
+        // when it is at least 3% quicker, so near-ties keep the default and
+        // the choice does not flip between processes on timing noise
+        std::vector<ChainForm> cand{def};
+        auto add = [&](ChainForm c) {
+          if (c.kind != def.kind || c.wc != def.wc || c.pf != def.pf) cand.push_back(c);
+        };
+        add({0, 0, 0});
         const int fwc = csr_chain_fit_wc(ops, in_rows0, p);
-        if (fwc > 0) cand.push_back({1, fwc, 0});
+        if (fwc > 0) add({1, fwc, 0});
         if (csr_chain_pf_fits(ops, in_rows0, p))
           for (int pf : {8, 16}) {
             if (pf == 16 && widest <= 8) continue;
             for (int wc : {1, 2, 4})
-              if (wc == 1 || (p + wc - 1) / wc >= 8) cand.push_back({2, wc, pf});
+              if (wc == 1 || (p + wc - 1) / wc >= 8) add({2, wc, pf});
           }
         hipEvent_t e0, e1;
         FH_HIP(hipEventCreate(&e0));
@@ -319,7 +327,7 @@ void faust_multiply_f64(const std::vector<DMatP>& f, char op, const double* X, i
             FH_HIP(hipEventElapsedTime(&ms, e0, e1));
             m = std::min(m, ms);
           }
-          if (m < best_ms) {
+          if (m < 0.97f * best_ms) {
             best_ms = m;
             use = c;
           }
