@@ -217,6 +217,31 @@ def prox(name, M, k=0, normalized=True, pos=False, supp=None):
     return M
 
 
+def mul_all_ends(factors, flags="N", alpha=1.0, beta=0.0, out=None):
+    """multiply_order_opt_all_ends (faust_prod_opt.hpp:49-86): alpha times the
+    product of op(F_i) in the reference's greedy end-pair order, plus beta * out."""
+    if len(flags) != len(factors) and flags != "N":
+        raise ValueError("flags: one per factor, or the single flag 'N'")
+    pk = _FacPack(factors)
+    dims = [(f.shape[1], f.shape[0]) if (flags[i] if len(flags) > i else flags[0]) in "TH" else f.shape
+            for i, f in enumerate(factors)]
+    m, n = dims[0][0], dims[-1][1]
+    o = np.zeros((m, n), order="F") if out is None else np.asfortranarray(out, dtype=np.float64).copy(order="F")
+    fl = flags.encode()
+    _check(lib().or_mul_all_ends(pk.arr, len(factors), fl, len(fl), C.c_double(alpha), C.c_double(beta), m, n, _dp(o)))
+    return o
+
+
+def project_gen_kind(name, M, k=0, normalized=True, pos=False, supp=None):
+    """'dense' or 'sparse': the format ConstraintGeneric::project_gen returns
+    (faust_prox_gen.hpp: sparse_size > dense_size -> dense)."""
+    M = np.asfortranarray(M, dtype=np.float64)
+    c, keep = make_cons(name, k, M.shape[0], M.shape[1], normalized, pos, supp)
+    kind = C.c_int()
+    _check(lib().or_project_gen_kind(C.byref(c), _dp(M), C.byref(kind)))
+    return "dense" if kind.value == OR_DENSE else "sparse"
+
+
 def _params(**kw):
     p = or_params()
     p.is_update_way_R2L = int(kw.get("is_update_way_R2L", False))

@@ -43,13 +43,14 @@
 //                               5930 +- 30) gives 5942.2 with an exact factorization;
 //                               lambda's closed form on the returned factors; the three
 //                               gradient forms agree to rounding; error decreases
+//   multiply_order_opt_all_ends  bits of every end-pair association on one-term-per-
+//                               entry matrices (numpy exact), the rule's pick only
+//   project_gen format          the reference's 204^2, k = 1632 case (sparse); the
+//                               byte-count boundaries of sp/spcol/splincol/splin
 // Parity unpinned (restated from the reference, no independent check):
 //   Eigen's own summation order (replaced by the canonical order);
 //   which of the 2016 backend's MatDense flags are set when (the spectral norm of a
-//   flagged identity is exactly 1, of a flagged zero 0; followed from the source);
-//   the greedy end-pair order of multiply_order_opt_all_ends and the dense/sparse
-//   format rule of prox_*_gen (their values are pinned within rounding through the
-//   products above; the exact order and the returned formats are not).
+//   flagged identity is exactly 1, of a flagged zero 0; followed from the source).
 #include "faust_oracle.h"
 
 #include <algorithm>
@@ -1315,6 +1316,27 @@ int or_dynprog_tables(const or_fac* facs, int nfacs, int32_t* cost, int* split) 
         cost[i * nfacs + j] = c[i][j];
         split[i * nfacs + j] = inds[i][j];
       }
+  })
+}
+
+int or_mul_all_ends(const or_fac* facs, int nfacs, const char* flags, int nflags, double alpha, double beta,
+                    int out_m, int out_n, double* out) {
+  OR_TRY({
+    if (nfacs < 2 || nflags < 1) throw std::runtime_error("mul_all_ends: at least two factors and one flag");
+    Chain c = chain_of(facs, nfacs);
+    std::vector<const Mat*> f;
+    for (auto& x : c.f) f.push_back(x.get());
+    Mat o = dense_from_ptr(out, out_m, out_n);
+    mul_all_ends(f, o, alpha, beta, std::vector<char>(flags, flags + nflags));
+    if (o.m != out_m || o.n != out_n) throw std::runtime_error("mul_all_ends: output shape");
+    std::memcpy(out, o.d.data(), sizeof(double) * o.d.size());
+  })
+}
+
+int or_project_gen_kind(const or_cons* c, const double* M, int* kind) {
+  OR_TRY({
+    MatP r = project_gen(*c, dense_from_ptr(M, c->nrows, c->ncols));
+    *kind = r->kind == Mat::DENSE ? OR_DENSE : OR_SPARSE;
   })
 }
 

@@ -114,6 +114,18 @@ void or_power_iteration_x0(int n, double *out);
 /* proximal projectors on a dense col-major matrix (faust_prox.hpp) */
 int or_prox(const or_cons *c, double *M);
 
+/* multiply_order_opt_all_ends (faust_prod_opt.hpp:49-86), the product order
+ * of PALM4MSA's gradient: out = alpha * op(F_0) ... op(F_{n-1}) + beta * out
+ * (out: out_m x out_n col-major, read for beta); flags as the reference's
+ * transconj_flags: one per factor, or one flag that the intermediate products
+ * apply to every factor and the last product to its left operand only */
+int or_mul_all_ends(const or_fac *facs, int nfacs, const char *flags, int nflags,
+                    double alpha, double beta, int out_m, int out_n, double *out);
+
+/* the format ConstraintInt/ConstraintMat::project_gen returns for M
+ * (faust_prox_gen.hpp: sparse_size vs dense_size): OR_DENSE or OR_SPARSE */
+int or_project_gen_kind(const or_cons *c, const double *M, int *kind);
+
 /* PALM4MSA 2020 / hierarchical results */
 typedef struct or_result or_result;
 int or_palm4msa(const double *A, int m, int n, const or_cons *cons, int ncons,
