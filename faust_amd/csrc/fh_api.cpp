@@ -163,15 +163,29 @@ struct ChainForm {
 namespace {
 std::mutex g_tune_mu;
 std::map<std::string, ChainForm> g_tune;  // chain signature -> quickest form measured
+std::map<std::string, int> g_tune_mode;   // BSR / FP32 CSR chain signature -> FH_MUL_* mode
 }  // namespace
 
 // FH_AUTOTUNE=0 turns the measurement off (the default forms are used).
-static bool autotune_on() {
+bool autotune_on() {
   const char* e = getenv("FH_AUTOTUNE");
   return !(e && e[0] == '0');
 }
 
-static bool stream_capturing(hipStream_t st) {
+bool autotune_get_mode(const std::string& key, int* mode) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  auto it = g_tune_mode.find(key);
+  if (it == g_tune_mode.end()) return false;
+  *mode = it->second;
+  return true;
+}
+
+void autotune_put_mode(const std::string& key, int mode) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_tune_mode[key] = mode;
+}
+
+bool stream_capturing(hipStream_t st) {
   hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
   const hipError_t e = hipStreamIsCapturing(st, &cs);
   (void)hipGetLastError();
@@ -349,6 +363,7 @@ void faust_multiply_f64(const std::vector<DMatP>& f, char op, const double* X, i
 void autotune_clear() {
   std::lock_guard<std::mutex> lk(g_tune_mu);
   g_tune.clear();
+  g_tune_mode.clear();
 }
 
 // Transform::get_product (faust_Transform.hpp:599-626)

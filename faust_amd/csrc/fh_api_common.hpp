@@ -112,8 +112,15 @@ struct ChainWs {
 };
 size_t faust_ws_need_f64(const std::vector<DMatP>& f, char op, int p, int* nbuf);
 ChainWs faust_ws_grow(DBuf* ws, size_t need, int nbuf, hipStream_t st, DBuf* ctl = nullptr);
-// Forgets the forms measured by faust_multiply_f64's autotuner.
+// Forgets the forms measured by the chain autotuners (FP64 and BSR / FP32 CSR).
 void autotune_clear();
+// Shared by the autotuners: FH_AUTOTUNE=0 turns measuring off; a stream being
+// captured into a graph is never measured on; the BSR / FP32 CSR store maps a
+// chain signature to the FH_MUL_* mode measured quickest.
+bool autotune_on();
+bool stream_capturing(hipStream_t st);
+bool autotune_get_mode(const std::string& key, int* mode);
+void autotune_put_mode(const std::string& key, int mode);
 void faust_multiply_f64(const std::vector<DMatP>& f, char op, const double* X, int p, double* Y, hipStream_t st,
                         const ChainWs& ws, FaustTimers* tm = nullptr, int* launches = nullptr, bool tune = true);
 DMatP faust_get_product_f64(const std::vector<DMatP>& f, hipStream_t st);
@@ -135,7 +142,7 @@ bool bsr_chain_fits(const std::vector<DMatP>& f, char op);
 void bsr_chain_launch(const std::vector<DMatP>& f, char op, const void* X, int p, void* Y, void* T0, void* T1,
                       int* ctl, hipStream_t st);
 int faust_multiply_mixed(const std::vector<DMatP>& f, char op, const void* X, int p, void* Y, hipStream_t st,
-                          const ChainWs& ws, FaustTimers* tm = nullptr, bool act_f32 = false);
+                          const ChainWs& ws, FaustTimers* tm = nullptr, bool act_f32 = false, bool tune = true);
 // Y = alpha op(B) X + beta Y, B an FP32 BSR factor, X / Y FP32 column-major
 void bsr_mul_f32(const DMat& B, char op, const float* X, int ldx, int p, float* Y, int ldy, float alpha, float beta,
                  hipStream_t st);

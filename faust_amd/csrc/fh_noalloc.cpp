@@ -95,6 +95,6 @@ extern "C" int fh_faust_multiply_noalloc(fh_faust_t F, char op, const void* X, i
   if (f64)
     faust_multiply_f64(f, op, (const double*)X, p, (double*)Y, st, ws, nullptr, &F->last_launches, false);
   else
-    F->last_launches = faust_multiply_mixed(f, op, X, p, Y, st, ws, nullptr, act_f32);
+    F->last_launches = faust_multiply_mixed(f, op, X, p, Y, st, ws, nullptr, act_f32, false);
   FH_API_END
 }

@@ -1148,8 +1148,8 @@ size_t faust_ws_need_mixed(const std::vector<DMatP>& f, char op, int p, int* nbu
 // Chain product for BSR (BF16 activations) or FP32 CSR chains, intermediates
 // in ws (faust_ws_need_mixed bytes each).  Returns the number of product
 // kernels launched.
-int faust_multiply_mixed(const std::vector<DMatP>& f, char op, const void* X, int p, void* Y, hipStream_t st,
-                          const ChainWs& ws, FaustTimers* tm, bool act_f32) {
+static int multiply_mixed_run(const std::vector<DMatP>& f, char op, const void* X, int p, void* Y, hipStream_t st,
+                              const ChainWs& ws, FaustTimers* tm, bool act_f32) {
   const int J = (int)f.size();
   const bool bsr = f[0]->kind == K_BSR;
   const int ldr = bsr ? 0 : (p + 3) & ~3;
@@ -1218,6 +1218,80 @@ int faust_multiply_mixed(const std::vector<DMatP>& f, char op, const void* X, in
     in_rows = out_rows;
   }
   return J + 1;
+}
+
+// Autotuning of BSR (BF16 activations) and FP32 CSR chains: in AUTO mode, the
+// first product of each chain signature (direction, columns, every op'd
+// factor's kind, shape, entries and ELL width, FP32 split, device) outside
+// graph capture times the execution forms that give the same bits -- BSR
+// kernels 5 (shared-block, the default), 1, 2, 4 (ELL, 1 / 2 / 4 column
+// tiles per wave) and 0 (row pointers), FP32 CSR one or two rows per thread
+// -- each once untimed, then the minimum of three runs (HIP events on the
+// launch stream), and keeps the quickest: the default unless another is at
+// least 3% quicker.  The K-split BSR kernel (other bits) and the one-launch
+// BSR chain (opt-in) are not candidates.  Environment overrides of the
+// kernels (FH_BSR_KERNEL, FH_BSR_XSPLIT, FH_BSR_NT, FH_CSR_RPT, FH_CSR_ELL,
+// FH_BSR_CHAIN) turn it off, and so does FH_AUTOTUNE=0.
+static bool mixed_env_override() {
+  for (const char* v : {"FH_BSR_KERNEL", "FH_BSR_XSPLIT", "FH_BSR_NT", "FH_CSR_RPT", "FH_CSR_ELL", "FH_BSR_CHAIN"})
+    if (getenv(v)) return true;
+  return false;
+}
+
+static std::string mixed_sig(const std::vector<DMatP>& f, char op, int p) {
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  std::string k = "mixed:" + std::to_string(dev) + op + ":" + std::to_string(p);
+  const bool tr = op != 'N';
+  for (const auto& x : f) {
+    const int w = x->kind == K_BSR ? (tr ? x->tell_w : x->ell_w) : (tr ? x->csr_tell_w : x->csr_ell_w);
+    k += "," + std::to_string(x->kind) + "/" + std::to_string(x->m) + "x" + std::to_string(x->n) + "/" +
+         std::to_string(x->kind == K_BSR ? x->bnnz : x->nnz) + "/" + std::to_string(w) + (x->split ? "s" : "");
+  }
+  return k;
+}
+
+int faust_multiply_mixed(const std::vector<DMatP>& f, char op, const void* X, int p, void* Y, hipStream_t st,
+                          const ChainWs& ws, FaustTimers* tm, bool act_f32, bool tune) {
+  if (act_f32 || mul_mode() != FH_MUL_AUTO || !autotune_on() || mixed_env_override() || f.empty() || p <= 0)
+    return multiply_mixed_run(f, op, X, p, Y, st, ws, tm, act_f32);
+  const bool bsr = f[0]->kind == K_BSR;
+  const std::string key = mixed_sig(f, op, p);
+  int mode = FH_MUL_AUTO;
+  if (!autotune_get_mode(key, &mode) && tune && !stream_capturing(st)) {
+    const std::vector<int> cand =
+        bsr ? std::vector<int>{FH_MUL_BSR_KERNEL(5), FH_MUL_BSR_KERNEL(1), FH_MUL_BSR_KERNEL(2),
+                               FH_MUL_BSR_KERNEL(4), FH_MUL_BSR_KERNEL(0)}
+            : std::vector<int>{FH_MUL_CSR_ROWS1, FH_MUL_CSR_ROWS2};
+    hipEvent_t e0, e1;
+    FH_HIP(hipEventCreate(&e0));
+    FH_HIP(hipEventCreate(&e1));
+    float best_ms = 1e30f;
+    mode = cand[0];
+    for (int c : cand) {
+      MulModeScope scope(c);
+      multiply_mixed_run(f, op, X, p, Y, st, ws, nullptr, false);
+      float m = 1e30f;
+      for (int rep = 0; rep < 3; ++rep) {
+        FH_HIP(hipEventRecord(e0, st));
+        multiply_mixed_run(f, op, X, p, Y, st, ws, nullptr, false);
+        FH_HIP(hipEventRecord(e1, st));
+        FH_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        FH_HIP(hipEventElapsedTime(&ms, e0, e1));
+        m = std::min(m, ms);
+      }
+      if (m < 0.97f * best_ms) {
+        best_ms = m;
+        mode = c;
+      }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    autotune_put_mode(key, mode);
+  }
+  MulModeScope scope(mode);
+  return multiply_mixed_run(f, op, X, p, Y, st, ws, tm, false);
 }
 
 // ---------------------------------------------------------------- BSR handle

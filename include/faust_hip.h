@@ -44,9 +44,11 @@ const char *fh_last_error(void);
 /* Library version string. */
 const char *fh_version(void);
 
-/* Forget the execution forms the autotuner measured (FP64 CSR chains in
- * FH_MUL_AUTO mode time their one-launch and per-factor forms on the first
- * product of each chain signature outside graph capture, and keep the
+/* Forget the execution forms the autotuner measured (in FH_MUL_AUTO mode the
+ * first product of each chain signature outside graph capture times the
+ * forms that give the same bits -- FP64 CSR chains: one launch per factor and
+ * the one-launch chains with their tile choices; BSR chains: kernels 5, 1, 2,
+ * 4, 0; FP32 CSR chains: one or two rows per thread -- and keeps the
  * quickest; FH_AUTOTUNE=0 turns it off).  No counterpart in the reference,
  * whose optimize_multiply is an explicit call (faust_TransformHelper.hpp:436-512). */
 int fh_autotune_clear(void);

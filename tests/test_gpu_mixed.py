@@ -418,3 +418,58 @@ def test_bsr_fp32_handle_values_exact_and_act_dtype_rules(gpu):
     assert dt.value == _lib.FH_BF16
     Fb = faust_amd.Faust([sp.bsr_matrix(rand_bsr(64, 64, 32, 1, rng))], dtype="bf16")
     assert L.fh_faust_set_act_dtype(Fb.h, _lib.FH_F32) == _lib.FH_ERR_UNSUPPORTED
+
+
+@pytest.mark.parametrize("dtype", ["bf16", "f32split"])
+@pytest.mark.parametrize("op", ["N", "T"])
+def test_bsr_autotune_candidates_same_bits(gpu, dtype, op):
+    # the BSR kernels the chain autotuner chooses among (5 shared-block, 1 / 2 /
+    # 4 ELL tiles per wave, 0 row pointers) give the same bits, and so does the
+    # autotuned AUTO product: ragged rows (two slot groups), an empty row, a
+    # row wider than the ELL limit (row-pointer fallback), p = 200 / 8 / 512
+    from faust_amd import _lib
+    rng = np.random.default_rng(31)
+    _lib.check(_lib.lib().fh_autotune_clear())
+    for wide in (None, 20):
+        facs = [_ragged_bsr(rng, r, c, wide_row=wide) for r, c in ((25, 24), (24, 25), (25, 24))]
+        if dtype == "bf16":
+            facs = [_bsr_rounded(f) for f in facs]
+            fdt = "bf16"
+        else:
+            facs = [sp.bsr_matrix((f.data.astype(np.float32).astype(np.float64), f.indices, f.indptr), shape=f.shape)
+                    for f in facs]
+            fdt = np.float32
+        for p in (200, 8, 512):
+            X = bf16_round(rng.standard_normal((32 * (24 if op == "N" else 25), p)))
+            outs = {}
+            for k in (5, 1, 2, 4, 0, None):
+                F = faust_amd.Faust(facs, dtype=fdt)
+                if k is not None:
+                    F._set_mul_mode(_lib.FH_MUL_BSR_KERNEL(k))
+                outs[k] = (F @ X) if op == "N" else (F.T @ X)
+            for k, Y in outs.items():
+                assert np.array_equal(Y, outs[5]), (k, p, wide)
+
+
+@pytest.mark.parametrize("op", ["N", "T"])
+def test_csr_fp32_autotune_candidates_same_bits(gpu, op):
+    # one or two rows per thread, and the autotuned AUTO product: same bits
+    from faust_amd import _lib
+    rng = np.random.default_rng(32)
+    _lib.check(_lib.lib().fh_autotune_clear())
+    facs = []
+    for s in [(300, 250), (250, 333), (333, 300)]:
+        a = rng.standard_normal(s) * (rng.random(s) < rng.uniform(0.0, 0.15, size=(s[0], 1)))
+        a[7] = 0.0
+        facs.append(sp.csr_matrix(a).astype(np.float32).astype(np.float64))
+    for p in (70, 33, 1):
+        rows_in = facs[-1].shape[1] if op == "N" else facs[0].shape[0]
+        X = rng.standard_normal((rows_in, p)).astype(np.float32).astype(np.float64)
+        outs = {}
+        for mode in (_lib.FH_MUL_CSR_ROWS1, _lib.FH_MUL_CSR_ROWS2, None):
+            F = faust_amd.Faust(facs, dtype=np.float32)
+            if mode is not None:
+                F._set_mul_mode(mode)
+            outs[mode] = (F @ X) if op == "N" else (F.T @ X)
+        for mode, Y in outs.items():
+            assert np.array_equal(Y, outs[_lib.FH_MUL_CSR_ROWS1]), (mode, p)
