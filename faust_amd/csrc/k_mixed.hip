@@ -52,6 +52,16 @@ static inline float host_bf2f(u16 h) {
   return f;
 }
 
+// Barrier that orders LDS only: the X loads already issued stay in flight
+// across it, so that a wave's first MFMAs start when their own operands
+// arrive instead of after every load of the round (__syncthreads waits for
+// all of them).
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 template <bool SPLIT, bool IN_TILED, bool OUT_TILED>
 __global__ __launch_bounds__(256) void k_bsr_mfma(int nbr, int p, const int* __restrict__ brp,
                                                   const int* __restrict__ bci, const u16* __restrict__ bdat,
@@ -345,12 +355,12 @@ __global__ __launch_bounds__(256) void k_bsr_shared(int nbr, int p, int W, const
         x[q][h] = *reinterpret_cast<const bf16x8*>(src);
       }
     __builtin_amdgcn_sched_barrier(0);
-    if (g > 0) __syncthreads();  // every wave is done reading the previous group's blocks
+    if (g > 0) lds_barrier();  // every wave is done reading the previous group's blocks
 #pragma unroll
     for (int pl = 0; pl < PLANES; ++pl)
 #pragma unroll
       for (int h = 0; h < 2; ++h) sA[wave][pl][h][lane] = a_own[pl][h];
-    __syncthreads();
+    lds_barrier();
 #pragma unroll
     for (int q = 0; q < 4; ++q)
 #pragma unroll
@@ -611,7 +621,7 @@ __global__ __launch_bounds__(256) void k_bsr_f32s(int nbr, int p, const int* __r
   const int b0 = brp[br], b1 = brp[br + 1];
   for (int c0 = b0; c0 < b1; c0 += BF32_CH) {
     const int nb = min(BF32_CH, b1 - c0);
-    if (c0 > b0) __syncthreads();  // the previous round's blocks are consumed
+    if (c0 > b0) lds_barrier();  // the previous round's blocks are consumed
     if (nb == BF32_CH) {
       // a full round: the staging loads and this wave's X column segments of
       // all its blocks are issued together, before any LDS write or MFMA
@@ -627,7 +637,7 @@ __global__ __launch_bounds__(256) void k_bsr_f32s(int nbr, int p, const int* __r
       __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
       for (int q = 0; q < BF32_CH; ++q) sa[q * 256 + t] = stg[q];
-      __syncthreads();
+      lds_barrier();
 #pragma unroll
       for (int q = 0; q < BF32_CH; ++q) {
         f32x4 a[4];

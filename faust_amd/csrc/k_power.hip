@@ -22,7 +22,11 @@
 // iterations per PALM4MSA): 0.71 us per operator and 3.3 us of norm,
 // normalisation and Rayleigh quotient per round (11.5 s in this kernel); the
 // whole hierarchy takes 24.1 s, against 26.8 s when each thread read its row
-// pointers and entries from global memory as it needed them.
+// pointers and entries from global memory as it needed them.  Chains whose
+// operators are all sparse with rows of at most PW_PF entries run a second
+// copy of the round loop without the loops over long rows and dense
+// operators: with those in the loop, the compiler waited for every load in
+// flight at the start of each operator, the prefetched set included.
 #include <cstdlib>
 #include <map>
 #include <mutex>
@@ -163,10 +167,10 @@ __device__ __forceinline__ PwU pw_uniform(const PwOp o) {
   return u;
 }
 
-template <bool RP_LDS>
+template <bool RP_LDS, bool SHORT = false>
 __device__ __forceinline__ void pw_fetch(const PwOp od, const int* rpl, int r, PwPre& f) {
   const PwU o = pw_uniform(od);
-  const bool sp = o.kind == 1;
+  const bool sp = SHORT || o.kind == 1;
   const int rr = min(r, o.rows - 1);
   int p0, p1;
   if (RP_LDS) {
@@ -195,9 +199,37 @@ __device__ __forceinline__ void pw_fetch(const PwOp od, const int* rpl, int r, P
   f.c[3] = (int)c.w;
 }
 
-// row r of op(o) applied to cur (LDS), entries from f when they were loaded
-__device__ __forceinline__ double pw_row(const PwOp o, const PwPre& f, int r, const double* cur) {
+// row r of op(o) applied to cur (LDS), entries from f when they were loaded.
+// SHORT: every operator of the chain is sparse with rows of at most PW_PF
+// entries, so the row is the prefetched slots alone, taken without branches:
+// every slot's column is a valid column of the operator (an entry of this
+// row or a later one, or 0 past the entries) and the terms past the row end
+// are computed and not kept (the same fma chain, the same bits).
+// K (SHORT only): the operator's longest row, the same on every lane, so that
+// rows of at most two entries gather two slots, not four.
+template <bool SHORT>
+__device__ __forceinline__ double pw_row(const PwOp o, const PwPre& f, int r, const double* cur, int K = PW_PF) {
   double acc = 0.0;
+  if (SHORT) {
+    // two slots when no row has more than two entries, else all four; the
+    // gathers of a case are issued before its fma chain
+    if (K <= 2) {
+      const double x0 = cur[f.c[0]], x1 = cur[f.c[1]];
+      const double t0 = fma(f.v[0], x0, acc);
+      acc = (0 < f.n) ? t0 : acc;
+      const double t1 = fma(f.v[1], x1, acc);
+      return (1 < f.n) ? t1 : acc;
+    }
+    double x[PW_PF];
+#pragma unroll
+    for (int k = 0; k < PW_PF; ++k) x[k] = cur[f.c[k]];
+#pragma unroll
+    for (int k = 0; k < PW_PF; ++k) {
+      const double tk = fma(f.v[k], x[k], acc);
+      acc = (k < f.n) ? tk : acc;
+    }
+    return acc;
+  }
   if (o.kind == 1) {
     if (f.n <= PW_PF) {
 #pragma unroll
@@ -229,22 +261,23 @@ __device__ __forceinline__ void pw_settle(const PwPre& f) {
                "v"(f.c[3]));
 }
 
-template <bool RP_LDS, bool TWO_ROWS>
-__device__ __forceinline__ void pw_apply(pw_cop sops, const int* rpl, int s, const double* cur, double* nxt,
-                                         PwPre& f0, PwPre& f1) {
+template <bool RP_LDS, bool TWO_ROWS, bool SHORT>
+__device__ __forceinline__ void pw_apply(pw_cop sops, const int* rpl, const int* opw, int s, const double* cur,
+                                         double* nxt, PwPre& f0, PwPre& f1) {
   const int t = threadIdx.x;
   pw_settle(f0);
   if (TWO_ROWS) pw_settle(f1);
   s = pw_sgpr(s);  // the same on every lane: descriptors come through scalar loads
   const PwOp o = pw_op(sops, s);
   const int rows = o.rows;
+  const int K = SHORT ? pw_sgpr(opw[s]) : PW_PF;
   double y0 = 0.0, y1 = 0.0;
-  if (t < rows) y0 = pw_row(o, f0, t, cur);
-  if (TWO_ROWS && t + PW_THREADS < rows) y1 = pw_row(o, f1, t + PW_THREADS, cur);
+  if (t < rows) y0 = pw_row<SHORT>(o, f0, t, cur, K);
+  if (TWO_ROWS && t + PW_THREADS < rows) y1 = pw_row<SHORT>(o, f1, t + PW_THREADS, cur, K);
   // (operators 1 and 0 reload operator 0: unused, keeps the loads unconditional)
   const PwOp o2 = pw_op(sops, max(s - 2, 0));
-  pw_fetch<RP_LDS>(o2, rpl, t, f0);
-  if (TWO_ROWS) pw_fetch<RP_LDS>(o2, rpl, t + PW_THREADS, f1);
+  pw_fetch<RP_LDS, SHORT>(o2, rpl, t, f0);
+  if (TWO_ROWS) pw_fetch<RP_LDS, SHORT>(o2, rpl, t + PW_THREADS, f1);
   if (t < rows) nxt[t] = y0;
   if (TWO_ROWS && t + PW_THREADS < rows) nxt[t + PW_THREADS] = y1;
   pw_lds_barrier();
@@ -263,6 +296,68 @@ struct PwJob {
   double* out;
 };
 
+// The rounds of one chain (the reference's loop, faust_linear_algebra.hpp:
+// 808-898): normalise xa into xn, apply the operators last to first, Rayleigh
+// quotient.  Instantiated for SHORT chains (every operator sparse, rows of at
+// most PW_PF entries: no loops over entries, so nothing but the prefetch is
+// in flight across the operator barriers) and for the general case.
+template <bool RP_LDS, bool TWO_ROWS, bool SHORT>
+__device__ __forceinline__ void pw_rounds(pw_cop sops, const int* rpl, const int* opw, int nops, int n, int maxit,
+                                          double thr,
+                                          double* xa, double* xb, double* xn, double* red, double* tot,
+                                          double& lambda, int& it, int& status) {
+  const int t = threadIdx.x;
+  double lambda_old = 1.0;
+  const double athr = fabs(thr);
+  // entries of rows t and t + PW_THREADS (rows <= PW_VMAX = 2 x PW_THREADS) of
+  // the operators applied 1st, 3rd, ... (a) and 2nd, 4th, ... (b)
+  PwPre a0, a1, b0, b1;
+  while ((fabs(lambda_old - lambda) > athr || fabs(lambda) <= athr) && it < maxit) {
+    it++;
+    lambda_old = lambda;
+    double nrm = sqrt(pw_dot(xa, xa, n, red, tot));
+    if (nrm == 0.0) {
+      status = 1;
+      break;
+    }
+    pw_fetch<RP_LDS, SHORT>(pw_op(sops, nops - 1), rpl, t, a0);
+    if (TWO_ROWS) pw_fetch<RP_LDS, SHORT>(pw_op(sops, nops - 1), rpl, t + PW_THREADS, a1);
+    pw_fetch<RP_LDS, SHORT>(pw_op(sops, max(nops - 2, 0)), rpl, t, b0);
+    if (TWO_ROWS) pw_fetch<RP_LDS, SHORT>(pw_op(sops, max(nops - 2, 0)), rpl, t + PW_THREADS, b1);
+    const double inv = 1 / nrm;
+    for (int i = t; i < n; i += PW_THREADS) xn[i] = xa[i] * inv;
+    pw_lds_barrier();
+    // apply ops last to first, ping-pong between xb and xa (xn stays the
+    // normalised start vector)
+    const double* cur = xn;
+    double* nxt = xb;
+    int len = n;
+    for (int s = nops - 1; s >= 0; s -= 2) {
+      pw_apply<RP_LDS, TWO_ROWS, SHORT>(sops, rpl, opw, s, cur, nxt, a0, a1);
+      len = pw_op(sops, s).rows;
+      cur = nxt;
+      nxt = (nxt == xb) ? xa : xb;
+      if (s == 0) break;
+      pw_apply<RP_LDS, TWO_ROWS, SHORT>(sops, rpl, opw, s - 1, cur, nxt, b0, b1);
+      len = pw_op(sops, s - 1).rows;
+      cur = nxt;
+      nxt = (nxt == xb) ? xa : xb;
+    }
+    // result vector is `cur`; move it into xa if needed
+    if (cur != xa) {
+      for (int i = t; i < len; i += PW_THREADS) xa[i] = cur[i];
+      __syncthreads();
+    }
+    int nz = 0;
+    for (int i = t; i < n; i += PW_THREADS) nz |= (xa[i] != 0.0);
+    if (!__syncthreads_or(nz)) {
+      status = 2;
+      break;
+    }
+    lambda = pw_dot(xn, xa, n, red, tot);
+  }
+}
+
 template <bool RP_LDS, bool TWO_ROWS>
 __global__ __launch_bounds__(PW_THREADS) void k_power_iteration(const PwJob* __restrict__ jobs, int maxit,
                                                                  double thr) {
@@ -279,63 +374,37 @@ __global__ __launch_bounds__(PW_THREADS) void k_power_iteration(const PwJob* __r
   const pw_cop sops = (pw_cop)job.ops;
   const int t = threadIdx.x;
   for (int i = t; i < n; i += PW_THREADS) xa[i] = x0[i];
+  // copy the row pointers, find each operator's longest row (opw) and whether
+  // every operator is sparse with rows of at most PW_PF entries
+  __shared__ int opw[PW_MAX_OPS];
+  for (int s = t; s < nops; s += PW_THREADS) opw[s] = 0;
+  __syncthreads();
+  int wide_row = 0;
   for (int s = 0; s < nops; ++s) {
     const PwOp o = pw_op(sops, s);
-    if (RP_LDS && o.kind == 1)
-      for (int i = t; i <= o.rows; i += PW_THREADS) rpl[o.rpo + i] = o.rp[i];
+    if (o.kind != 1) {
+      wide_row = 1;
+      continue;
+    }
+    int w = 0;
+    for (int i = t; i <= o.rows; i += PW_THREADS) {
+      const int q = o.rp[i];
+      if (RP_LDS) rpl[o.rpo + i] = q;
+      if (i < o.rows) w = max(w, o.rp[i + 1] - q);
+    }
+    wide_row |= w > PW_PF;
+    if (w > 0) atomicMax(&opw[s], w);
   }
-  __syncthreads();
-  double lambda_old = 1.0, lambda = 0.0;
+  const bool short_chain = !__syncthreads_or(wide_row);
+  double lambda = 0.0;
   int it = 0;
   int status = 0;
-  const double athr = fabs(thr);
-  // entries of rows t and t + PW_THREADS (rows <= PW_VMAX = 2 x PW_THREADS) of
-  // the operators applied 1st, 3rd, ... (a) and 2nd, 4th, ... (b)
-  PwPre a0, a1, b0, b1;
-  while ((fabs(lambda_old - lambda) > athr || fabs(lambda) <= athr) && it < maxit) {
-    it++;
-    lambda_old = lambda;
-    double nrm = sqrt(pw_dot(xa, xa, n, red, &tot));
-    if (nrm == 0.0) {
-      status = 1;
-      break;
-    }
-    pw_fetch<RP_LDS>(pw_op(sops, nops - 1), rpl, t, a0);
-    if (TWO_ROWS) pw_fetch<RP_LDS>(pw_op(sops, nops - 1), rpl, t + PW_THREADS, a1);
-    pw_fetch<RP_LDS>(pw_op(sops, max(nops - 2, 0)), rpl, t, b0);
-    if (TWO_ROWS) pw_fetch<RP_LDS>(pw_op(sops, max(nops - 2, 0)), rpl, t + PW_THREADS, b1);
-    const double inv = 1 / nrm;
-    for (int i = t; i < n; i += PW_THREADS) xn[i] = xa[i] * inv;
-    pw_lds_barrier();
-    // apply ops last to first, ping-pong between xb and xa (xn stays the
-    // normalised start vector)
-    const double* cur = xn;
-    double* nxt = xb;
-    int len = n;
-    for (int s = nops - 1; s >= 0; s -= 2) {
-      pw_apply<RP_LDS, TWO_ROWS>(sops, rpl, s, cur, nxt, a0, a1);
-      len = pw_op(sops, s).rows;
-      cur = nxt;
-      nxt = (nxt == xb) ? xa : xb;
-      if (s == 0) break;
-      pw_apply<RP_LDS, TWO_ROWS>(sops, rpl, s - 1, cur, nxt, b0, b1);
-      len = pw_op(sops, s - 1).rows;
-      cur = nxt;
-      nxt = (nxt == xb) ? xa : xb;
-    }
-    // result vector is `cur`; move it into xa if needed
-    if (cur != xa) {
-      for (int i = t; i < len; i += PW_THREADS) xa[i] = cur[i];
-      __syncthreads();
-    }
-    int nz = 0;
-    for (int i = t; i < n; i += PW_THREADS) nz |= (xa[i] != 0.0);
-    if (!__syncthreads_or(nz)) {
-      status = 2;
-      break;
-    }
-    lambda = pw_dot(xn, xa, n, red, &tot);
-  }
+  if (short_chain)
+    pw_rounds<RP_LDS, TWO_ROWS, true>(sops, rpl, opw, nops, n, maxit, thr, xa, xb, xn, red, &tot, lambda, it,
+                                      status);
+  else
+    pw_rounds<RP_LDS, TWO_ROWS, false>(sops, rpl, opw, nops, n, maxit, thr, xa, xb, xn, red, &tot, lambda, it,
+                                       status);
   if (t == 0) {
     out[0] = (status == 2) ? 0.0 : lambda;
     out[1] = (it < maxit) ? (double)it : -1.0;

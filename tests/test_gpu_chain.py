@@ -75,10 +75,42 @@ def power_form(request, monkeypatch):
     return request.param
 
 
-@pytest.mark.parametrize("case", ["sparse256", "dense_rect", "mixed", "long_rows1024"])
+def _short_factor(m, n, rng, k, wide_row=False):
+    """m x n CSR whose rows and columns have at most k entries (k random partial
+    matchings of rows to columns, each keeping a random half of its pairs), so
+    that the factor and its transpose both have short rows; wide_row: one row
+    gets 5 more entries."""
+    A = np.zeros((m, n))
+    for _ in range(k):
+        q = min(m, n)
+        rows, cols = rng.permutation(m)[:q], rng.permutation(n)[:q]
+        keep = rng.random(q) < 0.5
+        A[rows[keep], cols[keep]] = rng.standard_normal(int(keep.sum()))
+    if wide_row:
+        A[m // 2, rng.choice(n, size=5, replace=False)] = rng.standard_normal(5)
+    return sp.csr_matrix(A)
+
+
+@pytest.mark.parametrize("case", ["sparse256", "dense_rect", "mixed", "long_rows1024", "short_rows",
+                                  "short_rows_two_per_thread", "short_rows_but_one"])
 def test_norm2_both_forms_bitexact(gpu, power_form, case):
+    # short_rows*: every operator of the F^T F chain has rows of at most 4
+    # entries (the persistent kernel's branch-free round loop), on vectors of
+    # up to 1024 and (two rows per thread) 1500 entries; short_rows_but_one:
+    # one row of 5 entries sends the same kind of chain through the general loop
     rng = np.random.default_rng(17)
-    if case == "sparse256":
+    if case.startswith("short_rows"):
+        wide = case.endswith("but_one")
+        if case == "short_rows_two_per_thread":
+            facs = [_short_factor(1500, 1500, rng, 4) for _ in range(3)]
+        else:
+            facs = [_short_factor(700, 600, rng, 4), _short_factor(600, 650, rng, 2, wide_row=wide),
+                    _short_factor(650, 700, rng, 4)]
+        for f in facs:
+            rl = np.diff(f.indptr)
+            cl = np.diff(sp.csc_matrix(f).indptr)
+            assert (max(rl.max(), cl.max()) <= 4) or wide
+    elif case == "sparse256":
         facs = _csr_chain_factors(n=256, nnz=4, J=4)
     elif case == "dense_rect":
         facs = [np.asfortranarray(rng.standard_normal((300, 200))), np.asfortranarray(rng.standard_normal((200, 310)))]
