@@ -26,7 +26,8 @@
 // operators are all sparse with rows of at most PW_PF entries run a second
 // copy of the round loop without the loops over long rows and dense
 // operators: with those in the loop, the compiler waited for every load in
-// flight at the start of each operator, the prefetched set included.
+// flight at the start of each operator, the prefetched set included
+// (hierarchy 24.3 -> 23.6 s).
 #include <cstdlib>
 #include <map>
 #include <mutex>
