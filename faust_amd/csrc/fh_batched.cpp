@@ -34,7 +34,7 @@ struct Batch {
   std::vector<int64_t> powoff;     // per target start of its 2*nf power vectors
   // schedule
   struct Stage {
-    int kind;  // 0 gemm, 1 power(R->nR), 2 power(L->nL), 3 prox, 4 lambda
+    int kind;  // 0 gemm, 1 power(R->nR), 2 power(L->nL), 3 prox, 4 lambda, 5 sparse build, 6 / 7 sparse power (R / L)
     DBuf jobs;
     int njobs = 0;
   };
@@ -46,6 +46,12 @@ struct Batch {
   int maxit = 100;
   double thr = 1e-6;
   int warm = 1;
+  // step-size power iterations on the sparse factors instead of the dense
+  // partial products (FP32 sp batches): every factor's compressed rows and
+  // columns, rebuilt after its projection
+  bool sparse = false;
+  DBuf spmem, spfac;  // the arrays; the BSpFac<T> of target b, factor f at b * nf + f
+  std::vector<BSpFac<T>> spfac_h;
 
   T* mat(int b, int slot) const { return arena.as<T>() + off[b] + (int64_t)slot * sp.n[b] * sp.n[b]; }
   // slot layout: 0 A, 1..nf S, nf+1..2nf-1 PR[0..nf-2], 2nf..3nf-2 PL[1..nf-1], then DR, TMP, TR, SM, MASK[nf]
@@ -159,6 +165,93 @@ static void add_prox(Batch<T>& B, int f, const std::vector<int64_t>& k) {
   B.stages.push_back(std::move(s));
 }
 
+// Room for every factor's compressed rows and columns: cap = max(k_b, n_b)
+// entries (a sp projection keeps at most k_b; the default initial factors
+// hold at most n_b).
+template <class T>
+static void sparse_alloc(Batch<T>& B, const std::vector<int64_t>& k) {
+  const int nt = B.sp.nt, nf = B.sp.nf;
+  auto up = [](size_t x) { return (x + 255) & ~size_t(255); };
+  std::vector<size_t> off((size_t)nt * nf);
+  size_t tot = 0;
+  for (int b = 0; b < nt; ++b) {
+    const size_t n = B.sp.n[b], cap = (size_t)std::max<int64_t>(k[b], B.sp.n[b]);
+    for (int f = 0; f < nf; ++f) {
+      off[(size_t)b * nf + f] = tot;
+      tot += 2 * up(sizeof(int) * (n + 1)) + 2 * up(sizeof(int) * cap) + 2 * up(sizeof(T) * cap);
+    }
+  }
+  B.spmem.alloc(std::max<size_t>(tot, 1));
+  char* base = static_cast<char*>(B.spmem.p);
+  B.spfac_h.resize((size_t)nt * nf);
+  for (int b = 0; b < nt; ++b) {
+    const size_t n = B.sp.n[b], cap = (size_t)std::max<int64_t>(k[b], B.sp.n[b]);
+    for (int f = 0; f < nf; ++f) {
+      char* q = base + off[(size_t)b * nf + f];
+      BSpFac<T> F;
+      F.rp = reinterpret_cast<int*>(q), q += up(sizeof(int) * (n + 1));
+      F.cp = reinterpret_cast<int*>(q), q += up(sizeof(int) * (n + 1));
+      F.ci = reinterpret_cast<int*>(q), q += up(sizeof(int) * cap);
+      F.ri = reinterpret_cast<int*>(q), q += up(sizeof(int) * cap);
+      F.v = reinterpret_cast<T*>(q), q += up(sizeof(T) * cap);
+      F.cv = reinterpret_cast<T*>(q);
+      F.cap = (int)cap;
+      B.spfac_h[(size_t)b * nf + f] = F;
+    }
+  }
+  B.spfac.alloc(sizeof(BSpFac<T>) * B.spfac_h.size());
+  FH_HIP(hipMemcpy(B.spfac.p, B.spfac_h.data(), sizeof(BSpFac<T>) * B.spfac_h.size(), hipMemcpyHostToDevice));
+}
+
+template <class T>
+static void fill_build_jobs(Batch<T>& B, const std::vector<int>& fs, typename Batch<T>::Stage& s) {
+  std::vector<BSpBuildJob<T>> h;
+  for (int b = 0; b < B.sp.nt; ++b)
+    for (int f : fs) h.push_back(BSpBuildJob<T>{B.mat(b, B.sS(f)), B.spfac_h[(size_t)b * B.sp.nf + f], B.sp.n[b]});
+  s.kind = 5;
+  s.njobs = (int)h.size();
+  s.jobs.alloc(sizeof(BSpBuildJob<T>) * h.size());
+  FH_HIP(hipMemcpy(s.jobs.p, h.data(), sizeof(BSpBuildJob<T>) * h.size(), hipMemcpyHostToDevice));
+}
+
+template <class T>
+static void add_sparse_build(Batch<T>& B, int f) {
+  typename Batch<T>::Stage s;
+  fill_build_jobs(B, {f}, s);
+  B.stages.push_back(std::move(s));
+}
+
+// the compressed forms of every factor as they stand (before the first iteration)
+template <class T>
+static void sparse_build_all(Batch<T>& B, hipStream_t st) {
+  typename Batch<T>::Stage s;
+  std::vector<int> fs(B.sp.nf);
+  std::iota(fs.begin(), fs.end(), 0);
+  fill_build_jobs(B, fs, s);
+  launch_bsparse<T>(s.jobs.template as<BSpBuildJob<T>>(), s.njobs, st);
+  FH_HIP(hipStreamSynchronize(st));
+}
+
+// kind 6: R side of factor f (factors f+1 .. nf-1), kind 7: L side (0 .. f-1)
+template <class T>
+static void add_power_sp(Batch<T>& B, int kind, int f, int slot) {
+  const int nf = B.sp.nf;
+  std::vector<BPowSpJob<T>> h(B.sp.nt);
+  double* out = (kind == 6 ? B.nR : B.nL).template as<double>();
+  const BSpFac<T>* fac = B.spfac.template as<BSpFac<T>>();
+  for (int b = 0; b < B.sp.nt; ++b) {
+    const int first = kind == 6 ? f + 1 : 0, m = kind == 6 ? nf - 1 - f : f;
+    h[b] = BPowSpJob<T>{fac + (size_t)b * nf + first, m, B.sp.n[b], b, out,
+                        B.powv.template as<double>() + B.powoff[b] + (int64_t)slot * B.sp.n[b]};
+  }
+  typename Batch<T>::Stage s;
+  s.kind = kind;
+  s.njobs = B.sp.nt;
+  s.jobs.alloc(sizeof(BPowSpJob<T>) * h.size());
+  FH_HIP(hipMemcpy(s.jobs.p, h.data(), sizeof(BPowSpJob<T>) * h.size(), hipMemcpyHostToDevice));
+  B.stages.push_back(std::move(s));
+}
+
 template <class T>
 static void add_lambda(Batch<T>& B) {
   std::vector<BLamJob<T>> h(B.sp.nt);
@@ -188,8 +281,13 @@ static void build_schedule(Batch<T>& B, const std::vector<int64_t>& k) {
     add_gemm<T>(B, [&, i](int b) { return job(b, B.mat(b, B.sS(i + 1)), 0, PRp(b, i + 1), 0, nullptr, B.mat(b, B.sPR(i)), 0, 0); });
   for (int f = 0; f < nf; ++f) {
     const bool hasR = f < nf - 1, hasL = f > 0;
-    add_power<T>(B, 1, [&, f, hasR](int b) -> const T* { return hasR ? PRp(b, f) : nullptr; }, 2 * f);
-    add_power<T>(B, 2, [&, f, hasL](int b) -> const T* { return hasL ? PLp(b, f) : nullptr; }, 2 * f + 1);
+    if (B.sparse) {
+      add_power_sp<T>(B, 6, f, 2 * f);
+      add_power_sp<T>(B, 7, f, 2 * f + 1);
+    } else {
+      add_power<T>(B, 1, [&, f, hasR](int b) -> const T* { return hasR ? PRp(b, f) : nullptr; }, 2 * f);
+      add_power<T>(B, 2, [&, f, hasL](int b) -> const T* { return hasL ? PLp(b, f) : nullptr; }, 2 * f + 1);
+    }
     if (hasR && hasL) {
       add_gemm<T>(B, [&, f](int b) { return job(b, B.mat(b, B.sS(f)), 0, PRp(b, f), 0, nullptr, B.mat(b, B.sDR()), 0, 0); });
       add_gemm<T>(B, [&, f](int b) {
@@ -215,6 +313,7 @@ static void build_schedule(Batch<T>& B, const std::vector<int64_t>& k) {
       });
     }
     add_prox<T>(B, f, k);
+    if (B.sparse) add_sparse_build<T>(B, f);
     if (f >= 1 && f < nf - 1)
       add_gemm<T>(B, [&, f](int b) { return job(b, PLp(b, f), 0, B.mat(b, B.sS(f)), 0, nullptr, B.mat(b, B.sPL(f + 1)), 0, 0); });
   }
@@ -241,6 +340,13 @@ static void run_stage(Batch<T>& B, const typename Batch<T>::Stage& s, hipStream_
       break;
     case 4:
       launch_blambda<T>(s.jobs.template as<BLamJob<T>>(), s.njobs, B.lam.template as<double>(), st);
+      break;
+    case 5:
+      launch_bsparse<T>(s.jobs.template as<BSpBuildJob<T>>(), s.njobs, st);
+      break;
+    case 6:
+    case 7:
+      launch_bpower_sp<T>(s.jobs.template as<BPowSpJob<T>>(), s.njobs, B.maxit, B.thr, B.warm, st);
       break;
   }
 }
@@ -540,9 +646,19 @@ extern "C" int fh_palm4msa_batched(int ntargets, int nfacts, const int* n, const
           std::vector<float> h(supp[(size_t)b * nfacts + f], supp[(size_t)b * nfacts + f] + len);
           FH_HIP(hipMemcpy(B.mat(b, B.sMASK(f)), h.data(), sizeof(float) * len, hipMemcpyHostToDevice));
         }
+    // sp projections keeping at most 1/8 of each factor: step sizes from the
+    // sparse factors (FH_BATCH_SPARSE_POWER=0 keeps the dense products)
+    {
+      const char* e = getenv("FH_BATCH_SPARSE_POWER");
+      bool sparse = B.prox_mode == 0 && !(e && e[0] == '0');
+      for (int b = 0; b < ntargets && sparse; ++b) sparse = kv[b] <= (int64_t)n[b] * n[b] / 8;
+      B.sparse = sparse;
+      if (sparse) sparse_alloc(B, kv);
+    }
     B.tile_of.assign(ntargets, 64);
     build_tiles(B);
     build_schedule(B, kv);
+    if (B.sparse) sparse_build_all(B, st);
     if (p->autotune) {
       autotune(B, st);
       build_tiles(B);

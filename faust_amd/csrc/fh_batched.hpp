@@ -40,6 +40,40 @@ struct BPowJob {
   double* vec;
 };
 
+// A factor of one target as compressed rows and compressed columns (entries
+// of a row in column order, of a column in row order), rebuilt from the
+// dense factor after each projection; cap entries of room each.
+template <class T>
+struct BSpFac {
+  int* rp;
+  int* ci;
+  T* v;
+  int* cp;
+  int* ri;
+  T* cv;
+  int cap;
+};
+
+// Build job: the dense n x n factor M -> F.
+template <class T>
+struct BSpBuildJob {
+  const T* M;
+  BSpFac<T> F;
+  int n;
+};
+
+// sqrt(lambda_max(P P^T)) with P = F_0 F_1 ... F_{m-1} given by its sparse
+// factors (fac[0..m-1]); m == 0 writes 1.  vec: this slot's start vector.
+template <class T>
+struct BPowSpJob {
+  const BSpFac<T>* fac;
+  int m;
+  int n;
+  int target;
+  double* out;
+  double* vec;
+};
+
 template <class T>
 struct BProxJob {
   T* M;
@@ -62,6 +96,10 @@ template <class T>
 void launch_bpower(const BPowJob<T>* jobs, int njobs, int maxit, double thr, int warm, hipStream_t st);
 template <class T>
 void launch_bprox(const BProxJob<T>* jobs, int njobs, int mode, hipStream_t st);
+template <class T>
+void launch_bsparse(const BSpBuildJob<T>* jobs, int njobs, hipStream_t st);
+template <class T>
+void launch_bpower_sp(const BPowSpJob<T>* jobs, int njobs, int maxit, double thr, int warm, hipStream_t st);
 template <class T>
 void launch_blambda(const BLamJob<T>* jobs, int njobs, double* lambda, hipStream_t st);
 template <class T>

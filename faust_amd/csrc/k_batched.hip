@@ -277,6 +277,211 @@ __global__ __launch_bounds__(256) void k_bpower(const BPowJob<T>* __restrict__ j
   if (t == 0) J.out[J.target] = sqrt(fabs(lambda));
 }
 
+// ---------------------------------------------------------------- sparse factors
+// Compressed rows and columns of a dense n x n factor (one 256-thread
+// workgroup per job), entries of a row in column order and of a column in row
+// order whatever the thread timing.  Entries beyond cap are not written (the
+// caller sizes cap to the projection's nonzero count).
+__device__ void block_excl_scan_int(int* a, int n, int* part) {
+  // a[0..n) -> exclusive prefix sums; part: 256 ints of LDS scratch
+  const int t = threadIdx.x;
+  const int per = (n + 255) / 256;
+  const int lo = min(t * per, n), hi = min(lo + per, n);
+  int s = 0;
+  for (int i = lo; i < hi; ++i) s += a[i];
+  part[t] = s;
+  __syncthreads();
+  if (t == 0) {
+    int run = 0;
+    for (int i = 0; i < 256; ++i) {
+      const int v = part[i];
+      part[i] = run;
+      run += v;
+    }
+  }
+  __syncthreads();
+  int run = part[t];
+  for (int i = lo; i < hi; ++i) {
+    const int v = a[i];
+    a[i] = run;
+    run += v;
+  }
+  __syncthreads();
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_bsparse(const BSpBuildJob<T>* __restrict__ jobs) {
+  __shared__ int cc[BATCH_NMAX + 1], rc[BATCH_NMAX + 1];
+  __shared__ int part[256];
+  const BSpBuildJob<T> J = jobs[blockIdx.x];
+  const int n = J.n, t = threadIdx.x;
+  const BSpFac<T> F = J.F;
+  // global (not flat) loads, so that their waits do not also wait for the LDS atomics
+  const __attribute__((address_space(1))) T* M = (const __attribute__((address_space(1))) T*)J.M;
+  for (int i = t; i <= n; i += 256) cc[i] = 0;
+  __syncthreads();
+  // counts: thread t walks rows t, t + 256, ... column by column (the 256
+  // rows of one column step are one coalesced read); row counts in a
+  // register, column counts by LDS atomics (integer sums)
+  constexpr int UB = 16;  // columns loaded together before they are tested
+  for (int i0 = 0; i0 < n; i0 += 256) {
+    const int i = i0 + t;
+    int c = 0;
+    if (i < n)
+      for (int j0 = 0; j0 < n; j0 += UB) {
+        T xv[UB];
+#pragma unroll
+        for (int u = 0; u < UB; ++u) xv[u] = M[(size_t)min(j0 + u, n - 1) * n + i];
+#pragma unroll
+        for (int u = 0; u < UB; ++u)
+          if (j0 + u < n && xv[u] != T(0)) {
+            ++c;
+            atomicAdd(&cc[j0 + u], 1);
+          }
+      }
+    if (i < n) rc[i] = c;
+  }
+  if (t == 0) rc[n] = 0;
+  __syncthreads();
+  block_excl_scan_int(cc, n + 1, part);
+  block_excl_scan_int(rc, n + 1, part);
+  for (int i = t; i <= n; i += 256) {
+    F.cp[i] = cc[i];
+    F.rp[i] = rc[i];
+  }
+  __syncthreads();
+  // fill: a row's entries in column order from the thread's own walk; a
+  // column's entries at slots claimed by LDS atomics (cc becomes the running
+  // end of each column), put in row order below
+  for (int i0 = 0; i0 < n; i0 += 256) {
+    const int i = i0 + t;
+    if (i >= n) continue;
+    int q = rc[i];
+    for (int j0 = 0; j0 < n; j0 += UB) {
+      T xv[UB];
+#pragma unroll
+      for (int u = 0; u < UB; ++u) xv[u] = M[(size_t)min(j0 + u, n - 1) * n + i];
+#pragma unroll
+      for (int u = 0; u < UB; ++u) {
+        const int j = j0 + u;
+        const T x = xv[u];
+        if (j < n && x != T(0)) {
+          if (q < F.cap) {
+            F.ci[q] = j;
+            F.v[q] = x;
+          }
+          ++q;
+          const int s = atomicAdd(&cc[j], 1);
+          if (s < F.cap) {
+            F.ri[s] = i;
+            F.cv[s] = x;
+          }
+        }
+      }
+    }
+  }
+  __syncthreads();
+  // each column's entries by row (insertion sort: a sp factor has a few per column)
+  for (int j = t; j < n; j += 256) {
+    const int a = min(F.cp[j], F.cap), e = min(cc[j], F.cap);
+    for (int p = a + 1; p < e; ++p) {
+      const int r = F.ri[p];
+      const T x = F.cv[p];
+      int q = p - 1;
+      while (q >= a && F.ri[q] > r) {
+        F.ri[q + 1] = F.ri[q];
+        F.cv[q + 1] = F.cv[q];
+        --q;
+      }
+      F.ri[q + 1] = r;
+      F.cv[q + 1] = x;
+    }
+  }
+}
+
+// Power iteration on P = F_0 ... F_{m-1} through its sparse factors (one
+// 256-thread workgroup per job), the rounds of k_bpower: normalise x, y =
+// P^T x (F_0^T first, by compressed columns), z = P y (F_{m-1} first, by
+// compressed rows), lambda = <x, z>.  A product of m sparse factors costs
+// their entries per round instead of the n^2 of the dense product.  Vectors
+// and sums are double, entries widened on load.
+template <class T>
+__device__ __forceinline__ void bsp_apply(const BSpFac<T>& F, bool trans, const double* in, double* out, int n) {
+  const int* p0 = trans ? F.cp : F.rp;
+  const int* ix = trans ? F.ri : F.ci;
+  const T* vv = trans ? F.cv : F.v;
+  for (int i = threadIdx.x; i < n; i += 256) {
+    const int a = p0[i], b = p0[i + 1];
+    double acc = 0.0;
+    for (int q = a; q < b; ++q) acc = fma((double)vv[q], in[ix[q]], acc);
+    out[i] = acc;
+  }
+  __syncthreads();
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_bpower_sp(const BPowSpJob<T>* __restrict__ jobs, int maxit, double thr,
+                                                   int warm) {
+  __shared__ double x[BATCH_NMAX], u[BATCH_NMAX], w[BATCH_NMAX];
+  __shared__ double red[256];
+  const BPowSpJob<T> J = jobs[blockIdx.x];
+  const int n = J.n, t = threadIdx.x, m = J.m;
+  if (m == 0) {
+    if (t == 0) J.out[J.target] = 1.0;
+    return;
+  }
+  for (int i = t; i < n; i += 256) x[i] = J.vec[i];
+  __syncthreads();
+  double lambda_old = 1.0, lambda = 0.0;
+  int it = 0;
+  bool zero = false;
+  while ((fabs(lambda_old - lambda) > fabs(thr) || fabs(lambda) <= fabs(thr)) && it < maxit) {
+    ++it;
+    lambda_old = lambda;
+    double ss = 0.0;
+    for (int i = t; i < n; i += 256) ss = fma(x[i], x[i], ss);
+    const double nrm = sqrt(block_sum_d(ss, red));
+    if (nrm == 0.0) {
+      zero = true;
+      break;
+    }
+    const double inv = 1.0 / nrm;
+    for (int i = t; i < n; i += 256) x[i] = x[i] * inv;
+    __syncthreads();
+    // y = P^T x = F_{m-1}^T ... F_0^T x
+    const double* cur = x;
+    double* nxt = u;
+    for (int c = 0; c < m; ++c) {
+      bsp_apply(J.fac[c], true, cur, nxt, n);
+      cur = nxt;
+      nxt = (nxt == u) ? w : u;
+    }
+    // z = P y = F_0 ... F_{m-1} y
+    for (int c = m - 1; c >= 0; --c) {
+      bsp_apply(J.fac[c], false, cur, nxt, n);
+      cur = nxt;
+      nxt = (nxt == u) ? w : u;
+    }
+    double dot = 0.0;
+    int nz = 0;
+    for (int i = t; i < n; i += 256) {
+      dot = fma(x[i], cur[i], dot);
+      nz |= (cur[i] != 0.0);
+    }
+    lambda = block_sum_d(dot, red);
+    if (!__syncthreads_or(nz)) {
+      lambda = 0.0;
+      zero = true;
+      break;
+    }
+    for (int i = t; i < n; i += 256) x[i] = cur[i];
+    __syncthreads();
+  }
+  if (warm && !zero)
+    for (int i = t; i < n; i += 256) J.vec[i] = x[i];
+  if (t == 0) J.out[J.target] = sqrt(fabs(lambda));
+}
+
 // ---------------------------------------------------------------- projections
 // One 1024-thread workgroup per target.  SP: keep the k largest magnitudes
 // (radix select on the magnitude bits; among entries equal to the k-th
@@ -489,6 +694,20 @@ void launch_bpower(const BPowJob<T>* jobs, int njobs, int maxit, double thr, int
 }
 
 template <class T>
+void launch_bsparse(const BSpBuildJob<T>* jobs, int njobs, hipStream_t st) {
+  if (njobs == 0) return;
+  hipLaunchKernelGGL(k_bsparse<T>, dim3(njobs), dim3(256), 0, st, jobs);
+  FH_LAUNCH_CHECK();
+}
+
+template <class T>
+void launch_bpower_sp(const BPowSpJob<T>* jobs, int njobs, int maxit, double thr, int warm, hipStream_t st) {
+  if (njobs == 0) return;
+  hipLaunchKernelGGL(k_bpower_sp<T>, dim3(njobs), dim3(256), 0, st, jobs, maxit, thr, warm);
+  FH_LAUNCH_CHECK();
+}
+
+template <class T>
 void launch_bprox(const BProxJob<T>* jobs, int njobs, int mode, hipStream_t st) {
   if (njobs == 0) return;
   if (mode == 0)
@@ -523,6 +742,8 @@ void launch_bcopy(T* dst, const T* src, int64_t n, hipStream_t st) {
   template void launch_bgemm<T>(const BGemmJob<T>*, const int3*, int, int, const BScal&, hipStream_t);       \
   template void launch_bpower<T>(const BPowJob<T>*, int, int, double, int, hipStream_t);                    \
   template void launch_bprox<T>(const BProxJob<T>*, int, int, hipStream_t);                                 \
+  template void launch_bsparse<T>(const BSpBuildJob<T>*, int, hipStream_t);                                 \
+  template void launch_bpower_sp<T>(const BPowSpJob<T>*, int, int, double, int, hipStream_t);               \
   template void launch_blambda<T>(const BLamJob<T>*, int, double*, hipStream_t);                            \
   template void launch_brelerr<T>(const BLamJob<T>*, int, const double*, double*, hipStream_t);             \
   template void launch_bcopy<T>(T*, const T*, int64_t, hipStream_t);

@@ -109,3 +109,22 @@ def test_batched_gemm_tiles_agree(gpu, monkeypatch, tile):
     r64 = palm4msa_batched(As, 3, k=ks, dtype="f64", num_its=30, refine_its=0, autotune=False)
     assert np.all(np.isfinite(r32.relerr))
     assert np.all(np.abs(r32.relerr - r64.relerr) <= 0.02 * r64.relerr + 1e-5)
+
+
+def test_batched_fp32_sparse_step_sizes_track_dense(gpu, monkeypatch):
+    # the FP32 engine's step-size power iterations on the compressed sparse
+    # factors (default for sp projections keeping at most 1/8 of a factor)
+    # against the same iterations on the dense partial products: the two
+    # differ only by the FP32 rounding of the dense products, so after 60
+    # iterations the FP64-verified errors agree closely and every target
+    # converges the same way; sizes with rows of up to 2048 / 256 threads
+    sizes = [64, 128, 96, 256, 64, 512]
+    As = _targets(5, sizes)
+    ks = [4 * n for n in sizes]
+    rel = {}
+    for mode in ("1", "0"):
+        monkeypatch.setenv("FH_BATCH_SPARSE_POWER", mode)
+        res = palm4msa_batched(As, 3, k=ks, dtype="f32", num_its=60, refine_its=0, return_factors=False)
+        rel[mode] = np.asarray(res.relerr)
+    assert np.all(np.isfinite(rel["1"])) and np.all(rel["1"] < 1.0)
+    assert np.all(np.abs(rel["1"] - rel["0"]) <= 1e-3 * rel["0"] + 1e-6), (rel["1"], rel["0"])
