@@ -509,14 +509,22 @@ __global__ __launch_bounds__(1024) void k_bprox(const BProxJob<T>* __restrict__ 
   __shared__ unsigned scan[1024];
   __shared__ K s_prefix;
   __shared__ long long s_kk;
+  __shared__ unsigned s_eq;
   const BProxJob<T> J = jobs[blockIdx.x];
   const int t = threadIdx.x;
   const int64_t len = J.len;
+  // global (not flat) accesses: their waits do not wait for the LDS atomics
+  typedef __attribute__((address_space(1))) T* GT;
+  GT M = (GT)J.M;
+  // ss: this thread's sum of squares over i = t, t + 1024, ... of the
+  // projected entries (fma in that order, as the separate pass below)
+  T ss = T(0);
+  bool have_ss = false;
   if (MODE == 1) {
-    for (int64_t i = t; i < len; i += 1024) J.M[i] = J.M[i] * J.mask[i];
+    for (int64_t i = t; i < len; i += 1024) M[i] = M[i] * J.mask[i];
   } else if (J.k < len) {
     if (J.k <= 0) {
-      for (int64_t i = t; i < len; i += 1024) J.M[i] = T(0);
+      for (int64_t i = t; i < len; i += 1024) M[i] = T(0);
     } else {
       K prefix = 0, mask = 0;
       long long kk = J.k;  // rank still to find among keys matching prefix
@@ -524,7 +532,7 @@ __global__ __launch_bounds__(1024) void k_bprox(const BProxJob<T>* __restrict__ 
         for (int b = t; b < 256; b += 1024) hist[b] = 0;
         __syncthreads();
         for (int64_t i = t; i < len; i += 1024) {
-          const K key = KO::key(J.M[i]);
+          const K key = KO::key(M[i]);
           if ((key & mask) == prefix) atomicAdd(&hist[(key >> shift) & 255], 1u);
         }
         __syncthreads();
@@ -537,6 +545,7 @@ __global__ __launch_bounds__(1024) void k_bprox(const BProxJob<T>* __restrict__ 
           }
           s_kk = kk - cum;
           s_prefix = prefix | ((K)b << shift);
+          s_eq = hist[b];  // after the last digit: the entries equal to the k-th largest key
         }
         __syncthreads();
         kk = s_kk;
@@ -545,34 +554,47 @@ __global__ __launch_bounds__(1024) void k_bprox(const BProxJob<T>* __restrict__ 
         __syncthreads();
       }
       // keys > prefix are kept; kk entries equal to prefix are kept, lowest index first
-      const int64_t chunk = (len + 1023) / 1024;
-      const int64_t c0 = t * chunk, c1 = min(len, c0 + chunk);
-      unsigned eq = 0;
-      for (int64_t i = c0; i < c1; ++i) eq += (KO::key(J.M[i]) == prefix);
-      scan[t] = eq;
-      __syncthreads();
-      for (int s = 1; s < 1024; s <<= 1) {
-        unsigned v = (t >= s) ? scan[t - s] : 0u;
-        __syncthreads();
-        scan[t] += v;
-        __syncthreads();
-      }
-      long long before = (long long)scan[t] - eq;
-      for (int64_t i = c0; i < c1; ++i) {
-        const K key = KO::key(J.M[i]);
-        bool keep = key > prefix;
-        if (key == prefix) {
-          keep = before < kk;
-          ++before;
+      if ((long long)s_eq == kk) {
+        // every entry equal to the k-th largest key is kept: no ranking, and
+        // the sum of squares is taken on the way
+        for (int64_t i = t; i < len; i += 1024) {
+          const T x = M[i];
+          const bool keep = KO::key(x) >= prefix;
+          if (!keep) M[i] = T(0);
+          const T y = keep ? x : T(0);
+          ss = fma(y, y, ss);
         }
-        if (!keep) J.M[i] = T(0);
+        have_ss = true;
+      } else {
+        const int64_t chunk = (len + 1023) / 1024;
+        const int64_t c0 = t * chunk, c1 = min(len, c0 + chunk);
+        unsigned eq = 0;
+        for (int64_t i = c0; i < c1; ++i) eq += (KO::key(M[i]) == prefix);
+        scan[t] = eq;
+        __syncthreads();
+        for (int s = 1; s < 1024; s <<= 1) {
+          unsigned v = (t >= s) ? scan[t - s] : 0u;
+          __syncthreads();
+          scan[t] += v;
+          __syncthreads();
+        }
+        long long before = (long long)scan[t] - eq;
+        for (int64_t i = c0; i < c1; ++i) {
+          const K key = KO::key(M[i]);
+          bool keep = key > prefix;
+          if (key == prefix) {
+            keep = before < kk;
+            ++before;
+          }
+          if (!keep) M[i] = T(0);
+        }
       }
     }
   }
   __syncthreads();
   // Frobenius normalisation
-  T ss = T(0);
-  for (int64_t i = t; i < len; i += 1024) ss = fma(J.M[i], J.M[i], ss);
+  if (!have_ss)
+    for (int64_t i = t; i < len; i += 1024) ss = fma(M[i], M[i], ss);
   red[t] = ss;
   __syncthreads();
   for (int s = 512; s >= 1; s >>= 1) {
@@ -582,7 +604,7 @@ __global__ __launch_bounds__(1024) void k_bprox(const BProxJob<T>* __restrict__ 
   const T nrm = sqrt(red[0]);
   if (nrm > T(0)) {
     const T inv = T(1) / nrm;
-    for (int64_t i = t; i < len; i += 1024) J.M[i] = J.M[i] * inv;
+    for (int64_t i = t; i < len; i += 1024) M[i] = M[i] * inv;
   }
 }
 
