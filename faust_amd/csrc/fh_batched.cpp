@@ -34,9 +34,12 @@ struct Batch {
   std::vector<int64_t> powoff;     // per target start of its 2*nf power vectors
   // schedule
   struct Stage {
-    int kind;  // 0 gemm, 1 power(R->nR), 2 power(L->nL), 3 prox, 4 lambda, 5 sparse build, 6 / 7 sparse power (R / L)
+    int kind;  // 0 gemm, 1 power(R->nR), 2 power(L->nL), 3 prox, 4 lambda, 5 sparse build, 6 / 7 sparse power (R / L),
+               // 8 sparse x dense product
     DBuf jobs;
     int njobs = 0;
+    DBuf tiles;  // kind 8: (job, first column, columns) per workgroup
+    int ntiles = 0;
   };
   std::vector<Stage> stages;
   DBuf tiles32, tiles64, tilesM;
@@ -123,10 +126,52 @@ static void build_tiles(Batch<T>& B) {
   up(B.tilesM, B.ntM, tm);
 }
 
+// The factor whose dense storage p is (of target b), or -1.
+template <class T>
+static int factor_of(const Batch<T>& B, int b, const T* p) {
+  for (int f = 0; f < B.sp.nf; ++f)
+    if (p == B.mat(b, B.sS(f))) return f;
+  return -1;
+}
+
 template <class T>
 static void add_gemm(Batch<T>& B, std::function<BGemmJob<T>(int)> mk) {
   std::vector<BGemmJob<T>> h(B.sp.nt);
   for (int b = 0; b < B.sp.nt; ++b) h[b] = mk(b);
+  // a product with a factor as an operand runs through the factor's
+  // compressed form when the batch keeps them (their structure is the
+  // factor's value at that point of the sweep: a factor is only read before
+  // its gradient step or after its projection and rebuild)
+  if (B.sparse) {
+    const int fa = factor_of(B, 0, h[0].A), fb = factor_of(B, 0, h[0].B);
+    const int side = fa >= 0 ? 0 : (fb >= 0 ? 1 : -1);
+    if (side >= 0) {
+      std::vector<BSpmmJob<T>> hs(B.sp.nt);
+      std::vector<int3> tl;
+      for (int b = 0; b < B.sp.nt; ++b) {
+        const BGemmJob<T>& g = h[b];
+        const int f = side == 0 ? factor_of(B, b, g.A) : factor_of(B, b, g.B);
+        FH_CHECK(f >= 0, "batched PALM: inconsistent sparse operand");
+        hs[b] = BSpmmJob<T>{B.spfac_h[(size_t)b * B.sp.nf + f], side == 0 ? g.B : g.A, g.Cin, g.Cout, B.sp.n[b],
+                            side, side == 0 ? g.transA : g.transB, g.alpha_mode, g.beta_mode, g.target};
+        // transposed dense operands do not occur next to a factor in the schedule
+        FH_CHECK((side == 0 ? g.transB : g.transA) == 0, "batched PALM: transposed dense operand of a sparse product");
+        // about 8 K output entries per workgroup (the staged factor serves all of them)
+        const int n = B.sp.n[b], nc = std::max(1, std::min(n, 8192 / n));
+        for (int j0 = 0; j0 < n; j0 += nc) tl.push_back(make_int3(b, j0, std::min(nc, n - j0)));
+      }
+      typename Batch<T>::Stage s;
+      s.kind = 8;
+      s.njobs = B.sp.nt;
+      s.jobs.alloc(sizeof(BSpmmJob<T>) * hs.size());
+      FH_HIP(hipMemcpy(s.jobs.p, hs.data(), sizeof(BSpmmJob<T>) * hs.size(), hipMemcpyHostToDevice));
+      s.ntiles = (int)tl.size();
+      s.tiles.alloc(sizeof(int3) * tl.size());
+      FH_HIP(hipMemcpy(s.tiles.p, tl.data(), sizeof(int3) * tl.size(), hipMemcpyHostToDevice));
+      B.stages.push_back(std::move(s));
+      return;
+    }
+  }
   typename Batch<T>::Stage s;
   s.kind = 0;
   s.njobs = B.sp.nt;
@@ -347,6 +392,9 @@ static void run_stage(Batch<T>& B, const typename Batch<T>::Stage& s, hipStream_
     case 6:
     case 7:
       launch_bpower_sp<T>(s.jobs.template as<BPowSpJob<T>>(), s.njobs, B.maxit, B.thr, B.warm, st);
+      break;
+    case 8:
+      launch_bspmm<T>(s.jobs.template as<BSpmmJob<T>>(), s.tiles.template as<int3>(), s.ntiles, sc, st);
       break;
   }
 }

@@ -74,6 +74,23 @@ struct BPowSpJob {
   double* vec;
 };
 
+// Cout = alpha_b * (op(S) X  or  X op(S)) + beta * Cin on n x n column-major
+// blocks, S a sparse factor (alpha / beta modes as BGemmJob).  side 0: S on
+// the left, output entry (i, j) from row i of op(S) gathering column j of X;
+// side 1: S on the right, column j of the output from column j of op(S)
+// combining columns of X.
+template <class T>
+struct BSpmmJob {
+  BSpFac<T> S;
+  const T* X;
+  const T* Cin;
+  T* Cout;
+  int n;
+  int side, transS;
+  int alpha_mode, beta_mode;
+  int target;
+};
+
 template <class T>
 struct BProxJob {
   T* M;
@@ -98,6 +115,9 @@ template <class T>
 void launch_bprox(const BProxJob<T>* jobs, int njobs, int mode, hipStream_t st);
 template <class T>
 void launch_bsparse(const BSpBuildJob<T>* jobs, int njobs, hipStream_t st);
+// tiles: (job, first column, columns) per workgroup
+template <class T>
+void launch_bspmm(const BSpmmJob<T>* jobs, const int3* tiles, int ntiles, const BScal& sc, hipStream_t st);
 template <class T>
 void launch_bpower_sp(const BPowSpJob<T>* jobs, int njobs, int maxit, double thr, int warm, hipStream_t st);
 template <class T>

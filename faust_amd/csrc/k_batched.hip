@@ -345,9 +345,11 @@ __global__ __launch_bounds__(256) void k_bsparse(const BSpBuildJob<T>* __restric
   __syncthreads();
   block_excl_scan_int(cc, n + 1, part);
   block_excl_scan_int(rc, n + 1, part);
+  // pointers clamped to cap, so that a factor with more entries than room
+  // (which a sp projection does not produce) is cut, never read past its arrays
   for (int i = t; i <= n; i += 256) {
-    F.cp[i] = cc[i];
-    F.rp[i] = rc[i];
+    F.cp[i] = min(cc[i], F.cap);
+    F.rp[i] = min(rc[i], F.cap);
   }
   __syncthreads();
   // fill: a row's entries in column order from the thread's own walk; a
@@ -480,6 +482,157 @@ __global__ __launch_bounds__(256) void k_bpower_sp(const BPowSpJob<T>* __restric
   if (warm && !zero)
     for (int i = t; i < n; i += 256) J.vec[i] = x[i];
   if (t == 0) J.out[J.target] = sqrt(fabs(lambda));
+}
+
+// Batched sparse x dense products (BSpmmJob): a workgroup computes the
+// columns [j0, j0 + nc) of one job's output.  The factor's entry lists (the
+// rows of op(S) for S on the left, its columns for S on the right) are
+// staged in LDS once per workgroup and serve all its columns; with S on the
+// left each column of X is staged too and gathered from LDS, with S on the
+// right the lanes read the combined columns of X directly (coalesced).  Each
+// output entry is one fma chain over the entries in their stored order, FP32
+// (T) accumulation like the dense GEMM it replaces.  Factors with more
+// entries than the LDS room read them from global memory instead.
+constexpr int SPMM_LDS_BYTES = 16 * 1024;
+constexpr int SPMM_XCOLS = 2048;  // entries of X staged per column block (S on the left)
+constexpr int SPMM_CB = 4;        // columns per block with S on the right
+constexpr int SPMM_EB = 4;        // entries of a column loaded together (S on the right)
+
+template <class T>
+struct SpmmCtx {
+  const int* s_p;
+  const int* ix;  // entry indices and values: the LDS copies, or the factor's arrays
+  const T* vv;
+  T* s_x;
+  const __attribute__((address_space(1))) T* X;
+  const __attribute__((address_space(1))) T* Cin;
+  __attribute__((address_space(1))) T* Cout;
+  int n, j0, nc, side, beta_mode;
+  T alpha, beta;
+};
+
+// IX_LDS: the entry lists are in LDS (the pointers of ctx point there), else
+// in global memory; two copies of the loops, so that no load goes through a
+// pointer that may be either (a flat access, which waits with the LDS reads)
+template <class T, bool IX_LDS>
+__device__ __forceinline__ void bspmm_body(const SpmmCtx<T>& C) {
+  typedef const __attribute__((address_space(IX_LDS ? 3 : 1))) int* PI;
+  typedef const __attribute__((address_space(IX_LDS ? 3 : 1))) T* PV;
+  const PI ix = (PI)C.ix;
+  const PV vv = (PV)C.vv;
+  const int n = C.n, j0 = C.j0, nc = C.nc, t = threadIdx.x;
+  auto out = [&](int i, int j, T acc) {
+    const size_t o = (size_t)j * n + i;
+    T v = C.alpha * acc;
+    if (C.beta_mode) v = fma(C.beta, C.Cin[o], v);
+    C.Cout[o] = v;
+  };
+  if (C.side == 0) {
+    // (op(S) X)(i, j) = sum over row i of op(S): v * X(k, j); cb columns of X
+    // staged in LDS at a time, a row's entries read once for all of them
+    const int cb = max(1, min(8, SPMM_XCOLS / n));
+    for (int jb = 0; jb < nc; jb += cb) {
+      const int m = min(cb, nc - jb);
+      for (int e = t; e < m * n; e += 256) C.s_x[e] = C.X[(size_t)(j0 + jb) * n + e];
+      __syncthreads();
+      for (int i = t; i < n; i += 256) {
+        T acc[8];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) acc[c] = T(0);
+        for (int q = C.s_p[i], qe = C.s_p[i + 1]; q < qe; ++q) {
+          const int k = ix[q];
+          const T v = vv[q];
+#pragma unroll
+          for (int c = 0; c < 8; ++c)
+            if (c < m) acc[c] = fma(v, C.s_x[c * n + k], acc[c]);
+        }
+#pragma unroll
+        for (int c = 0; c < 8; ++c)
+          if (c < m) out(i, j0 + jb + c, acc[c]);
+      }
+      __syncthreads();  // s_x is rewritten for the next block
+    }
+  } else {
+    // (X op(S))(i, j) = sum over column j of op(S): X(i, k) * v; the X values of
+    // SPMM_EB entries of each of SPMM_CB columns are loaded before their fmas
+    for (int jb = 0; jb < nc; jb += SPMM_CB) {
+      for (int i = t; i < n; i += 256) {
+        T xv[SPMM_CB][SPMM_EB];
+        int qa[SPMM_CB], qe[SPMM_CB];
+#pragma unroll
+        for (int c = 0; c < SPMM_CB; ++c) {
+          const int j = min(j0 + jb + c, j0 + nc - 1);
+          qa[c] = C.s_p[j];
+          qe[c] = C.s_p[j + 1];
+#pragma unroll
+          for (int u = 0; u < SPMM_EB; ++u) {
+            // slots past the column's entries read some valid row of X (the
+            // index is clamped, also for a factor without entries) and are not used
+            const int q = min(qa[c] + u, max(qe[c] - 1, 0));
+            const int k = min(max(ix[q], 0), n - 1);
+            xv[c][u] = C.X[(size_t)k * n + i];
+          }
+        }
+#pragma unroll
+        for (int c = 0; c < SPMM_CB; ++c) {
+          if (jb + c >= nc) break;
+          T acc = T(0);
+#pragma unroll
+          for (int u = 0; u < SPMM_EB; ++u)
+            if (qa[c] + u < qe[c]) acc = fma(xv[c][u], vv[qa[c] + u], acc);
+          for (int q = qa[c] + SPMM_EB; q < qe[c]; ++q) acc = fma(C.X[(size_t)ix[q] * n + i], vv[q], acc);
+          out(i, j0 + jb + c, acc);
+        }
+      }
+    }
+  }
+}
+
+// at most 128 VGPRs and ~32 KB of LDS: four to five workgroups per CU, whose
+// load latencies overlap
+template <class T>
+__global__ __launch_bounds__(256, 4) void k_bspmm(const BSpmmJob<T>* __restrict__ jobs, const int3* __restrict__ tiles,
+                                                  BScal sc) {
+  constexpr int ENT = SPMM_LDS_BYTES / (sizeof(int) + sizeof(T));
+  __shared__ int s_p[BATCH_NMAX + 1];
+  __shared__ int s_ix[ENT];
+  __shared__ T s_v[ENT];
+  __shared__ T s_x[SPMM_XCOLS];
+  const int3 tl = tiles[blockIdx.x];
+  const BSpmmJob<T> J = jobs[tl.x];
+  const int n = J.n, t = threadIdx.x;
+  const bool rows_of_S = (J.side == 0) != (J.transS != 0);  // the entry lists are rows of S (CSR), else columns (CSC)
+  const int* p0 = rows_of_S ? J.S.rp : J.S.cp;
+  const int* ix = rows_of_S ? J.S.ci : J.S.ri;
+  const T* vv = rows_of_S ? J.S.v : J.S.cv;
+  const int nnz = p0[n];
+  const bool lds = nnz <= ENT;
+  for (int i = t; i <= n; i += 256) s_p[i] = p0[i];
+  if (lds)
+    for (int q = t; q < nnz; q += 256) {
+      s_ix[q] = ix[q];
+      s_v[q] = vv[q];
+    }
+  __syncthreads();
+  SpmmCtx<T> C;
+  C.s_p = s_p;
+  C.ix = lds ? s_ix : ix;
+  C.vv = lds ? s_v : vv;
+  C.s_x = s_x;
+  C.X = (const __attribute__((address_space(1))) T*)J.X;
+  C.Cin = (const __attribute__((address_space(1))) T*)J.Cin;
+  C.Cout = (__attribute__((address_space(1))) T*)J.Cout;
+  C.n = n;
+  C.j0 = tl.y;
+  C.nc = tl.z;
+  C.side = J.side;
+  C.beta_mode = J.beta_mode;
+  C.alpha = job_alpha<T>(J.alpha_mode, J.target, sc);
+  C.beta = J.beta_mode == 0 ? T(0) : (J.beta_mode == 1 ? T(-1) : T(1));
+  if (lds)
+    bspmm_body<T, true>(C);
+  else
+    bspmm_body<T, false>(C);
 }
 
 // ---------------------------------------------------------------- projections
@@ -716,6 +869,13 @@ void launch_bpower(const BPowJob<T>* jobs, int njobs, int maxit, double thr, int
 }
 
 template <class T>
+void launch_bspmm(const BSpmmJob<T>* jobs, const int3* tiles, int ntiles, const BScal& sc, hipStream_t st) {
+  if (ntiles == 0) return;
+  hipLaunchKernelGGL(k_bspmm<T>, dim3(ntiles), dim3(256), 0, st, jobs, tiles, sc);
+  FH_LAUNCH_CHECK();
+}
+
+template <class T>
 void launch_bsparse(const BSpBuildJob<T>* jobs, int njobs, hipStream_t st) {
   if (njobs == 0) return;
   hipLaunchKernelGGL(k_bsparse<T>, dim3(njobs), dim3(256), 0, st, jobs);
@@ -765,6 +925,7 @@ void launch_bcopy(T* dst, const T* src, int64_t n, hipStream_t st) {
   template void launch_bpower<T>(const BPowJob<T>*, int, int, double, int, hipStream_t);                    \
   template void launch_bprox<T>(const BProxJob<T>*, int, int, hipStream_t);                                 \
   template void launch_bsparse<T>(const BSpBuildJob<T>*, int, hipStream_t);                                 \
+  template void launch_bspmm<T>(const BSpmmJob<T>*, const int3*, int, const BScal&, hipStream_t);           \
   template void launch_bpower_sp<T>(const BPowSpJob<T>*, int, int, double, int, hipStream_t);               \
   template void launch_blambda<T>(const BLamJob<T>*, int, double*, hipStream_t);                            \
   template void launch_brelerr<T>(const BLamJob<T>*, int, const double*, double*, hipStream_t);             \
