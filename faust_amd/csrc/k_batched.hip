@@ -409,16 +409,43 @@ __global__ __launch_bounds__(256) void k_bsparse(const BSpBuildJob<T>* __restric
 // and sums are double, entries widened on load.
 template <class T>
 __device__ __forceinline__ void bsp_apply(const BSpFac<T>& F, bool trans, const double* in, double* out, int n) {
-  const int* p0 = trans ? F.cp : F.rp;
-  const int* ix = trans ? F.ri : F.ci;
-  const T* vv = trans ? F.cv : F.v;
+  typedef const __attribute__((address_space(1))) int* GI;
+  typedef const __attribute__((address_space(1))) T* GV;
+  const GI p0 = (GI)(trans ? F.cp : F.rp);
+  const GI ix = (GI)(trans ? F.ri : F.ci);
+  const GV vv = (GV)(trans ? F.cv : F.v);
   for (int i = threadIdx.x; i < n; i += 256) {
     const int a = p0[i], b = p0[i + 1];
     double acc = 0.0;
-    for (int q = a; q < b; ++q) acc = fma((double)vv[q], in[ix[q]], acc);
+    // entries four at a time: their loads issued together (indices clamped
+    // inside the factor's arrays), then the fma chain in order
+    for (int q0 = a; q0 < b; q0 += 4) {
+      int k[4];
+      T v[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int q = min(q0 + u, b - 1);
+        k[u] = ix[q];
+        v[u] = vv[q];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (q0 + u < b) acc = fma((double)v[u], in[k[u]], acc);
+    }
     out[i] = acc;
   }
   __syncthreads();
+}
+
+// block sum for the sparse power iteration: lane shuffles inside each wave,
+// then the four wave sums in order (a fixed order: the same result every run)
+__device__ __forceinline__ double block_sum_w(double v, double* red) {
+  v = wave_sum_d(v);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  const double r = (red[0] + red[1]) + (red[2] + red[3]);
+  __syncthreads();
+  return r;
 }
 
 template <class T>
@@ -442,7 +469,7 @@ __global__ __launch_bounds__(256) void k_bpower_sp(const BPowSpJob<T>* __restric
     lambda_old = lambda;
     double ss = 0.0;
     for (int i = t; i < n; i += 256) ss = fma(x[i], x[i], ss);
-    const double nrm = sqrt(block_sum_d(ss, red));
+    const double nrm = sqrt(block_sum_w(ss, red));
     if (nrm == 0.0) {
       zero = true;
       break;
@@ -470,7 +497,7 @@ __global__ __launch_bounds__(256) void k_bpower_sp(const BPowSpJob<T>* __restric
       dot = fma(x[i], cur[i], dot);
       nz |= (cur[i] != 0.0);
     }
-    lambda = block_sum_d(dot, red);
+    lambda = block_sum_w(dot, red);
     if (!__syncthreads_or(nz)) {
       lambda = 0.0;
       zero = true;
