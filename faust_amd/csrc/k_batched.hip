@@ -690,6 +690,11 @@ __global__ __launch_bounds__(1024) void k_bprox(const BProxJob<T>* __restrict__ 
   __shared__ K s_prefix;
   __shared__ long long s_kk;
   __shared__ unsigned s_eq;
+  // the kept entries of the untied path, so that the normalisation scales
+  // them alone instead of every entry
+  constexpr int KEEP_LIST = 8192;
+  __shared__ int kept[KEEP_LIST];
+  __shared__ int s_nkept;
   const BProxJob<T> J = jobs[blockIdx.x];
   const int t = threadIdx.x;
   const int64_t len = J.len;
@@ -700,6 +705,8 @@ __global__ __launch_bounds__(1024) void k_bprox(const BProxJob<T>* __restrict__ 
   // projected entries (fma in that order, as the separate pass below)
   T ss = T(0);
   bool have_ss = false;
+  if (t == 0) s_nkept = -1;  // no list
+  __syncthreads();
   if (MODE == 1) {
     for (int64_t i = t; i < len; i += 1024) M[i] = M[i] * J.mask[i];
   } else if (J.k < len) {
@@ -737,12 +744,18 @@ __global__ __launch_bounds__(1024) void k_bprox(const BProxJob<T>* __restrict__ 
       if ((long long)s_eq == kk) {
         // every entry equal to the k-th largest key is kept: no ranking, and
         // the sum of squares is taken on the way
+        if (t == 0) s_nkept = 0;
+        __syncthreads();
         for (int64_t i = t; i < len; i += 1024) {
           const T x = M[i];
           const bool keep = KO::key(x) >= prefix;
           if (!keep) M[i] = T(0);
           const T y = keep ? x : T(0);
           ss = fma(y, y, ss);
+          if (keep) {
+            const int slot = atomicAdd(&s_nkept, 1);
+            if (slot < KEEP_LIST) kept[slot] = (int)i;
+          }
         }
         have_ss = true;
       } else {
@@ -784,7 +797,13 @@ __global__ __launch_bounds__(1024) void k_bprox(const BProxJob<T>* __restrict__ 
   const T nrm = sqrt(red[0]);
   if (nrm > T(0)) {
     const T inv = T(1) / nrm;
-    for (int64_t i = t; i < len; i += 1024) M[i] = M[i] * inv;
+    const int nk = s_nkept;
+    if (nk >= 0 && nk <= KEEP_LIST && isfinite(inv)) {
+      // only the kept entries are nonzero (0 * inv = 0 for the others)
+      for (int q = t; q < nk; q += 1024) M[kept[q]] = M[kept[q]] * inv;
+    } else {
+      for (int64_t i = t; i < len; i += 1024) M[i] = M[i] * inv;
+    }
   }
 }
 
