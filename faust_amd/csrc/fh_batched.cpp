@@ -277,21 +277,24 @@ static void sparse_build_all(Batch<T>& B, hipStream_t st) {
   FH_HIP(hipStreamSynchronize(st));
 }
 
-// kind 6: R side of factor f (factors f+1 .. nf-1), kind 7: L side (0 .. f-1)
+// The R side (factors f+1 .. nf-1, norm to nR) and the L side (0 .. f-1, to
+// nL) of factor f's update, in one launch: the two sides of every target are
+// independent jobs (a side without factors writes 1 at once).
 template <class T>
-static void add_power_sp(Batch<T>& B, int kind, int f, int slot) {
+static void add_power_sp(Batch<T>& B, int f) {
   const int nf = B.sp.nf;
-  std::vector<BPowSpJob<T>> h(B.sp.nt);
-  double* out = (kind == 6 ? B.nR : B.nL).template as<double>();
+  std::vector<BPowSpJob<T>> h;
   const BSpFac<T>* fac = B.spfac.template as<BSpFac<T>>();
-  for (int b = 0; b < B.sp.nt; ++b) {
-    const int first = kind == 6 ? f + 1 : 0, m = kind == 6 ? nf - 1 - f : f;
-    h[b] = BPowSpJob<T>{fac + (size_t)b * nf + first, m, B.sp.n[b], b, out,
-                        B.powv.template as<double>() + B.powoff[b] + (int64_t)slot * B.sp.n[b]};
-  }
+  for (int side = 0; side < 2; ++side)
+    for (int b = 0; b < B.sp.nt; ++b) {
+      const int first = side == 0 ? f + 1 : 0, m = side == 0 ? nf - 1 - f : f;
+      double* out = (side == 0 ? B.nR : B.nL).template as<double>();
+      h.push_back(BPowSpJob<T>{fac + (size_t)b * nf + first, m, B.sp.n[b], b, out,
+                               B.powv.template as<double>() + B.powoff[b] + (int64_t)(2 * f + side) * B.sp.n[b]});
+    }
   typename Batch<T>::Stage s;
-  s.kind = kind;
-  s.njobs = B.sp.nt;
+  s.kind = 6;
+  s.njobs = (int)h.size();
   s.jobs.alloc(sizeof(BPowSpJob<T>) * h.size());
   FH_HIP(hipMemcpy(s.jobs.p, h.data(), sizeof(BPowSpJob<T>) * h.size(), hipMemcpyHostToDevice));
   B.stages.push_back(std::move(s));
@@ -327,8 +330,7 @@ static void build_schedule(Batch<T>& B, const std::vector<int64_t>& k) {
   for (int f = 0; f < nf; ++f) {
     const bool hasR = f < nf - 1, hasL = f > 0;
     if (B.sparse) {
-      add_power_sp<T>(B, 6, f, 2 * f);
-      add_power_sp<T>(B, 7, f, 2 * f + 1);
+      add_power_sp<T>(B, f);
     } else {
       add_power<T>(B, 1, [&, f, hasR](int b) -> const T* { return hasR ? PRp(b, f) : nullptr; }, 2 * f);
       add_power<T>(B, 2, [&, f, hasL](int b) -> const T* { return hasL ? PLp(b, f) : nullptr; }, 2 * f + 1);

@@ -1236,10 +1236,11 @@ static int multiply_mixed_run(const std::vector<DMatP>& f, char op, const void* 
 // graph capture times the execution forms that give the same bits -- BSR
 // kernels 5 (shared-block, the default), 1, 2, 4 (ELL, 1 / 2 / 4 column
 // tiles per wave) and 0 (row pointers), FP32 CSR one or two rows per thread
-// -- each once untimed, then the minimum of three runs (HIP events on the
-// launch stream), and keeps the quickest: the default unless another is at
-// least 3% quicker.  The K-split BSR kernel (other bits) and the one-launch
-// BSR chain (opt-in) are not candidates.  Environment overrides of the
+// -- each once untimed, then the minimum of three timings of five
+// back-to-back products (HIP events on the launch stream), and keeps the
+// quickest: the default unless another is at least 3% quicker.  The K-split
+// BSR kernel (other bits) and the one-launch BSR chain (opt-in) are not
+// candidates.  Environment overrides of the
 // kernels (FH_BSR_KERNEL, FH_BSR_XSPLIT, FH_BSR_NT, FH_CSR_RPT, FH_CSR_ELL,
 // FH_BSR_CHAIN) turn it off, and so does FH_AUTOTUNE=0.
 static bool mixed_env_override() {
@@ -1278,13 +1279,17 @@ int faust_multiply_mixed(const std::vector<DMatP>& f, char op, const void* X, in
     FH_HIP(hipEventCreate(&e1));
     float best_ms = 1e30f;
     mode = cand[0];
+    // each timing covers MIXED_TUNE_RUNS back-to-back products, the steady
+    // state a caller's loop or graph sees (single products timed alone favour
+    // kernels that are quicker only from cold caches)
+    constexpr int MIXED_TUNE_RUNS = 5;
     for (int c : cand) {
       MulModeScope scope(c);
       multiply_mixed_run(f, op, X, p, Y, st, ws, nullptr, false);
       float m = 1e30f;
       for (int rep = 0; rep < 3; ++rep) {
         FH_HIP(hipEventRecord(e0, st));
-        multiply_mixed_run(f, op, X, p, Y, st, ws, nullptr, false);
+        for (int r = 0; r < MIXED_TUNE_RUNS; ++r) multiply_mixed_run(f, op, X, p, Y, st, ws, nullptr, false);
         FH_HIP(hipEventRecord(e1, st));
         FH_HIP(hipEventSynchronize(e1));
         float ms = 0.f;
