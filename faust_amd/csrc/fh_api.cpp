@@ -205,8 +205,9 @@ static std::string chain_sig(char op, int p, int in_rows, const std::vector<Chai
 // Y = op(F) X with the FP64 chain.  AUTO mode on an all-CSR chain uses the
 // form measured quickest for this chain signature (measured on its first
 // call outside graph capture: every candidate once untimed, then the minimum
-// of three timed runs, HIP events on the launch stream; the default form
-// stays unless another is at least 3% quicker); otherwise, and when
+// of three timings of five back-to-back products, HIP events on the launch
+// stream; the default form stays unless another is at least 3% quicker);
+// otherwise, and when
 // measuring is off or impossible, the prefetched one-launch chain where the
 // shape allows it, the staged one where the planner takes it, else one launch
 // per factor.  `tune` false (allocation-free callers) never measures.
@@ -329,12 +330,14 @@ void faust_multiply_f64(const std::vector<DMatP>& f, char op, const double* X, i
         FH_HIP(hipEventCreate(&e0));
         FH_HIP(hipEventCreate(&e1));
         float best_ms = 1e30f;
+        // each timing covers five back-to-back products: the steady state a
+        // caller's loop or graph sees
         for (const ChainForm& c : cand) {
           run(c, nullptr);
           float m = 1e30f;
           for (int rep = 0; rep < 3; ++rep) {
             FH_HIP(hipEventRecord(e0, st));
-            run(c, nullptr);
+            for (int r = 0; r < 5; ++r) run(c, nullptr);
             FH_HIP(hipEventRecord(e1, st));
             FH_HIP(hipEventSynchronize(e1));
             float ms = 0.f;

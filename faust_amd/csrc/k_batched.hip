@@ -811,27 +811,51 @@ __global__ __launch_bounds__(1024) void k_bprox(const BProxJob<T>* __restrict__ 
 // lambda_b = <A, S>_F / ||S||_F^2 (update_lambda, faust_palm4msa2020.hpp:442-493)
 
 template <class T>
-__global__ __launch_bounds__(256) void k_blambda(const BLamJob<T>* __restrict__ jobs, double* __restrict__ lambda) {
-  __shared__ double r1[256], r2[256];
+__global__ __launch_bounds__(1024) void k_blambda(const BLamJob<T>* __restrict__ jobs, double* __restrict__ lambda) {
+  // 1024 threads, four independent partial sums per thread over a stride of
+  // 4096 (loads of four iterations in flight), then wave shuffles and the 16
+  // wave sums in order: a fixed order, the same result every run
+  __shared__ double r1[16], r2[16];
   const BLamJob<T> J = jobs[blockIdx.x];
   const int t = threadIdx.x;
-  double tr = 0.0, ss = 0.0;
-  for (int64_t i = t; i < J.len; i += 256) {
-    const double s = (double)J.S[i];
-    tr = fma((double)J.A[i], s, tr);
-    ss = fma(s, s, ss);
-  }
-  r1[t] = tr;
-  r2[t] = ss;
-  __syncthreads();
-  for (int s = 128; s >= 1; s >>= 1) {
-    if (t < s) {
-      r1[t] += r1[t + s];
-      r2[t] += r2[t + s];
+  typedef const __attribute__((address_space(1))) T* G;
+  const G A = (G)J.A, S = (G)J.S;
+  double tr[4] = {0.0, 0.0, 0.0, 0.0}, ss[4] = {0.0, 0.0, 0.0, 0.0};
+  int64_t i = t;
+  for (; i + 3 * 1024 < J.len; i += 4 * 1024) {
+    T a[4], s[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      a[u] = A[i + u * 1024];
+      s[u] = S[i + u * 1024];
     }
-    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      tr[u] = fma((double)a[u], (double)s[u], tr[u]);
+      ss[u] = fma((double)s[u], (double)s[u], ss[u]);
+    }
   }
-  if (t == 0 && r2[0] > 0.0) lambda[J.target] = r1[0] / r2[0];
+  for (; i < J.len; i += 1024) {
+    const double sv = (double)S[i];
+    tr[0] = fma((double)A[i], sv, tr[0]);
+    ss[0] = fma(sv, sv, ss[0]);
+  }
+  double x = (tr[0] + tr[1]) + (tr[2] + tr[3]), y = (ss[0] + ss[1]) + (ss[2] + ss[3]);
+  x = wave_sum_d(x);
+  y = wave_sum_d(y);
+  if ((t & 63) == 0) {
+    r1[t >> 6] = x;
+    r2[t >> 6] = y;
+  }
+  __syncthreads();
+  if (t == 0) {
+    double a1 = 0.0, a2 = 0.0;
+    for (int w = 0; w < 16; ++w) {
+      a1 += r1[w];
+      a2 += r2[w];
+    }
+    if (a2 > 0.0) lambda[J.target] = a1 / a2;
+  }
 }
 
 // rel_err_b = || lambda_b P - A ||_F / ||A||_F in FP64
@@ -948,7 +972,7 @@ void launch_bprox(const BProxJob<T>* jobs, int njobs, int mode, hipStream_t st) 
 template <class T>
 void launch_blambda(const BLamJob<T>* jobs, int njobs, double* lambda, hipStream_t st) {
   if (njobs == 0) return;
-  hipLaunchKernelGGL(k_blambda<T>, dim3(njobs), dim3(256), 0, st, jobs, lambda);
+  hipLaunchKernelGGL(k_blambda<T>, dim3(njobs), dim3(1024), 0, st, jobs, lambda);
   FH_LAUNCH_CHECK();
 }
 
