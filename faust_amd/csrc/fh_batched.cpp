@@ -533,21 +533,39 @@ struct Verifier {
     ready = true;
   }
 
+  // per batch verified (its address): the cast job lists of every factor and
+  // the copy of the GEMM output back to the running product, one launch each
+  const void* cast_for = nullptr;
+  std::vector<DBuf> cast_jobs;  // [f]: factor f of every target -> slot 2 (slot 1 for the last factor)
+  DBuf copy_jobs;
+
+  template <class T>
+  void build_casts(Batch<T>& B) {
+    cast_jobs.clear();
+    cast_jobs.resize(sp.nf);
+    std::vector<BCastJob> h(sp.nt);
+    for (int f = 0; f < sp.nf; ++f) {
+      for (int b = 0; b < sp.nt; ++b)
+        h[b] = BCastJob{V.mat(b, f == sp.nf - 1 ? 1 : 2), B.mat(b, B.sS(f)), (int64_t)sp.n[b] * sp.n[b]};
+      cast_jobs[f].alloc(sizeof(BCastJob) * sp.nt);
+      FH_HIP(hipMemcpy(cast_jobs[f].p, h.data(), sizeof(BCastJob) * sp.nt, hipMemcpyHostToDevice));
+    }
+    for (int b = 0; b < sp.nt; ++b) h[b] = BCastJob{V.mat(b, 1), V.mat(b, 3), (int64_t)sp.n[b] * sp.n[b]};
+    copy_jobs.alloc(sizeof(BCastJob) * sp.nt);
+    FH_HIP(hipMemcpy(copy_jobs.p, h.data(), sizeof(BCastJob) * sp.nt, hipMemcpyHostToDevice));
+    cast_for = &B;
+  }
+
   template <class T>
   void run(Batch<T>& B, double* host_out, hipStream_t st) {
-    auto cast = [&](double* dst, const T* src, int64_t len) {
-      if constexpr (std::is_same<T, float>::value)
-        launch_cast_f32_f64(dst, src, len, st);
-      else
-        launch_bcopy<double>(dst, src, len, st);
-    };
+    if (cast_for != &B) build_casts(B);
+    constexpr bool f32 = std::is_same<T, float>::value;
     BScal sc{V.lam.as<double>(), V.nR.as<double>(), V.nL.as<double>()};
-    for (int b = 0; b < sp.nt; ++b) cast(V.mat(b, 1), B.mat(b, B.sS(sp.nf - 1)), (int64_t)sp.n[b] * sp.n[b]);
+    launch_bcast(cast_jobs[sp.nf - 1].as<BCastJob>(), sp.nt, f32, st);
     for (int f = sp.nf - 2; f >= 0; --f) {
-      for (int b = 0; b < sp.nt; ++b) cast(V.mat(b, 2), B.mat(b, B.sS(f)), (int64_t)sp.n[b] * sp.n[b]);
+      launch_bcast(cast_jobs[f].as<BCastJob>(), sp.nt, f32, st);
       launch_bgemm<double>(gemm_jobs[0].as<BGemmJob<double>>(), V.tiles64.as<int3>(), V.nt64, 64, sc, st);
-      for (int b = 0; b < sp.nt; ++b)
-        launch_bcopy<double>(V.mat(b, 1), V.mat(b, 3), (int64_t)sp.n[b] * sp.n[b], st);
+      launch_bcast(copy_jobs.as<BCastJob>(), sp.nt, false, st);
     }
     launch_brelerr<double>(lam_jobs.as<BLamJob<double>>(), sp.nt, B.lam.template as<double>(), out.as<double>(), st);
     FH_HIP(hipMemcpyAsync(host_out, out.p, sizeof(double) * sp.nt, hipMemcpyDeviceToHost, st));

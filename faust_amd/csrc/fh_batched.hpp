@@ -127,6 +127,13 @@ void launch_brelerr(const BLamJob<T>* jobs, int njobs, const double* lambda, dou
 template <class T>
 void launch_bcopy(T* dst, const T* src, int64_t n, hipStream_t st);
 void launch_cast_f32_f64(double* dst, const float* src, int64_t n, hipStream_t st);
+// dst_j = (double) src_j for every job j in one launch (src FP32 or FP64)
+struct BCastJob {
+  double* dst;
+  const void* src;
+  int64_t len;
+};
+void launch_bcast(const BCastJob* jobs, int njobs, bool src_f32, hipStream_t st);
 void launch_mask_nonzero_f32(double* mask, const float* src, int64_t n, hipStream_t st);
 
 }  // namespace fh

@@ -983,6 +983,24 @@ void launch_brelerr(const BLamJob<T>* jobs, int njobs, const double* lambda, dou
   FH_LAUNCH_CHECK();
 }
 
+template <class S>
+__global__ __launch_bounds__(256) void k_bcast(const BCastJob* __restrict__ jobs) {
+  const BCastJob J = jobs[blockIdx.y];
+  const S* src = static_cast<const S*>(J.src);
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < J.len; i += (int64_t)gridDim.x * 256)
+    J.dst[i] = (double)src[i];
+}
+
+void launch_bcast(const BCastJob* jobs, int njobs, bool src_f32, hipStream_t st) {
+  if (njobs == 0) return;
+  const dim3 grid(64, (unsigned)njobs);
+  if (src_f32)
+    hipLaunchKernelGGL(k_bcast<float>, grid, dim3(256), 0, st, jobs);
+  else
+    hipLaunchKernelGGL(k_bcast<double>, grid, dim3(256), 0, st, jobs);
+  FH_LAUNCH_CHECK();
+}
+
 template <class T>
 void launch_bcopy(T* dst, const T* src, int64_t n, hipStream_t st) {
   if (n <= 0) return;
