@@ -200,7 +200,13 @@ static void add_prox(Batch<T>& B, int f, const std::vector<int64_t>& k) {
   for (int b = 0; b < B.sp.nt; ++b) {
     const int64_t len = (int64_t)B.sp.n[b] * B.sp.n[b];
     h[b] = BProxJob<T>{B.mat(b, B.sS(f)), B.prox_mode == 1 ? B.mat(b, B.sMASK(f)) : nullptr, len,
-                       k.empty() ? len : k[b]};
+                       k.empty() ? len : k[b], nullptr, nullptr, 0};
+    if (B.sparse) {
+      const BSpFac<T>& F = B.spfac_h[(size_t)b * B.sp.nf + f];
+      h[b].klist = F.klist;
+      h[b].kcount = F.kcount;
+      h[b].kcap = F.cap;
+    }
   }
   typename Batch<T>::Stage s;
   s.kind = 3;
@@ -223,10 +229,12 @@ static void sparse_alloc(Batch<T>& B, const std::vector<int64_t>& k) {
     const size_t n = B.sp.n[b], cap = (size_t)std::max<int64_t>(k[b], B.sp.n[b]);
     for (int f = 0; f < nf; ++f) {
       off[(size_t)b * nf + f] = tot;
-      tot += 2 * up(sizeof(int) * (n + 1)) + 2 * up(sizeof(int) * cap) + 2 * up(sizeof(T) * cap);
+      tot += 2 * up(sizeof(int) * (n + 1)) + 3 * up(sizeof(int) * cap) + 2 * up(sizeof(T) * cap) + up(sizeof(int));
     }
   }
   B.spmem.alloc(std::max<size_t>(tot, 1));
+  // every byte 0xFF: the kept-entry counts start at -1 (no list yet)
+  FH_HIP(hipMemset(B.spmem.p, 0xFF, std::max<size_t>(tot, 1)));
   char* base = static_cast<char*>(B.spmem.p);
   B.spfac_h.resize((size_t)nt * nf);
   for (int b = 0; b < nt; ++b) {
@@ -239,7 +247,9 @@ static void sparse_alloc(Batch<T>& B, const std::vector<int64_t>& k) {
       F.ci = reinterpret_cast<int*>(q), q += up(sizeof(int) * cap);
       F.ri = reinterpret_cast<int*>(q), q += up(sizeof(int) * cap);
       F.v = reinterpret_cast<T*>(q), q += up(sizeof(T) * cap);
-      F.cv = reinterpret_cast<T*>(q);
+      F.cv = reinterpret_cast<T*>(q), q += up(sizeof(T) * cap);
+      F.klist = reinterpret_cast<int*>(q), q += up(sizeof(int) * cap);
+      F.kcount = reinterpret_cast<int*>(q);
       F.cap = (int)cap;
       B.spfac_h[(size_t)b * nf + f] = F;
     }

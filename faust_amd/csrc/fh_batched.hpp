@@ -52,6 +52,11 @@ struct BSpFac {
   int* ri;
   T* cv;
   int cap;
+  // the kept entries (column-major positions, in no order) the last sp
+  // projection of this factor listed, and their count, or -1 when it listed
+  // none (ties ranked by index, more than the list holds, or no projection yet)
+  int* klist;
+  int* kcount;
 };
 
 // Build job: the dense n x n factor M -> F.
@@ -97,6 +102,9 @@ struct BProxJob {
   const T* mask;  // SUPP only
   int64_t len;
   int64_t k;
+  int* klist;   // optional: where the kept entries are listed (room for kcap)
+  int* kcount;  // their count, or -1
+  int kcap;
 };
 
 template <class T>

@@ -309,13 +309,90 @@ __device__ void block_excl_scan_int(int* a, int n, int* part) {
   __syncthreads();
 }
 
+// From the projection's list of kept entries (at most BSP_SORT): the list
+// sorted in LDS by column-major position gives the compressed columns in
+// order (entry q of the sorted list is entry q of the columns), sorted again
+// by row-major position the compressed rows; the pointers are lower bounds
+// in the sorted lists.  The result equals the dense scan's.
+constexpr int BSP_SORT = 8192;
+
+__device__ void bitonic_sort_lds(int* a, int P) {  // P a power of two, 256 threads
+  for (int k = 2; k <= P; k <<= 1)
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int i = threadIdx.x; i < P; i += 256) {
+        const int l = i ^ j;
+        if (l > i) {
+          const bool up = (i & k) == 0;
+          const int x = a[i], y = a[l];
+          if ((x > y) == up) {
+            a[i] = y;
+            a[l] = x;
+          }
+        }
+      }
+      __syncthreads();
+    }
+}
+
+__device__ __forceinline__ int lower_bound_lds(const int* a, int cnt, int key) {
+  int lo = 0, hi = cnt;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (a[mid] < key)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+template <class T>
+__device__ void bsparse_from_list(const BSpBuildJob<T>& J, int nk, int* keys) {
+  const int n = J.n, t = threadIdx.x;
+  const BSpFac<T> F = J.F;
+  int P = 1;
+  while (P < nk) P <<= 1;
+  for (int q = t; q < P; q += 256) keys[q] = q < nk ? F.klist[q] : INT_MAX;
+  __syncthreads();
+  bitonic_sort_lds(keys, P);
+  // compressed columns: positions j * n + i ascending
+  for (int j = t; j <= n; j += 256) F.cp[j] = lower_bound_lds(keys, nk, j * n);
+  for (int q = t; q < nk; q += 256) {
+    const int idx = keys[q], j = idx / n;
+    F.ri[q] = idx - j * n;
+    F.cv[q] = J.M[idx];
+  }
+  __syncthreads();
+  // compressed rows: positions i * n + j ascending
+  for (int q = t; q < nk; q += 256) {
+    const int idx = keys[q], j = idx / n, i = idx - j * n;
+    keys[q] = i * n + j;
+  }
+  __syncthreads();
+  bitonic_sort_lds(keys, P);
+  for (int i = t; i <= n; i += 256) F.rp[i] = lower_bound_lds(keys, nk, i * n);
+  for (int q = t; q < nk; q += 256) {
+    const int key = keys[q], i = key / n, j = key - i * n;
+    F.ci[q] = j;
+    F.v[q] = J.M[(size_t)j * n + i];
+  }
+}
+
 template <class T>
 __global__ __launch_bounds__(256) void k_bsparse(const BSpBuildJob<T>* __restrict__ jobs) {
   __shared__ int cc[BATCH_NMAX + 1], rc[BATCH_NMAX + 1];
   __shared__ int part[256];
+  __shared__ int keys[BSP_SORT];
   const BSpBuildJob<T> J = jobs[blockIdx.x];
   const int n = J.n, t = threadIdx.x;
   const BSpFac<T> F = J.F;
+  {
+    const int nk = F.kcount ? *F.kcount : -1;
+    if (nk >= 0 && nk <= F.cap && nk <= BSP_SORT) {
+      bsparse_from_list(J, nk, keys);
+      return;
+    }
+  }
   // global (not flat) loads, so that their waits do not also wait for the LDS atomics
   const __attribute__((address_space(1))) T* M = (const __attribute__((address_space(1))) T*)J.M;
   for (int i = t; i <= n; i += 256) cc[i] = 0;
@@ -795,15 +872,23 @@ __global__ __launch_bounds__(1024) void k_bprox(const BProxJob<T>* __restrict__ 
     __syncthreads();
   }
   const T nrm = sqrt(red[0]);
+  const int nk = s_nkept;
+  const bool listed = nk >= 0 && nk <= KEEP_LIST;
   if (nrm > T(0)) {
     const T inv = T(1) / nrm;
-    const int nk = s_nkept;
-    if (nk >= 0 && nk <= KEEP_LIST && isfinite(inv)) {
+    if (listed && isfinite(inv)) {
       // only the kept entries are nonzero (0 * inv = 0 for the others)
       for (int q = t; q < nk; q += 1024) M[kept[q]] = M[kept[q]] * inv;
     } else {
       for (int64_t i = t; i < len; i += 1024) M[i] = M[i] * inv;
     }
+  }
+  // the kept entries for the compressed form built next
+  if (J.kcount) {
+    const bool ok = listed && nk <= J.kcap;
+    if (ok)
+      for (int q = t; q < nk; q += 1024) J.klist[q] = kept[q];
+    if (t == 0) *J.kcount = ok ? nk : -1;
   }
 }
 
