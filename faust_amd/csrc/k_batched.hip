@@ -625,11 +625,23 @@ __device__ __forceinline__ void bspmm_body(const SpmmCtx<T>& C) {
   const PI ix = (PI)C.ix;
   const PV vv = (PV)C.vv;
   const int n = C.n, j0 = C.j0, nc = C.nc, t = threadIdx.x;
-  auto out = [&](int i, int j, T acc) {
-    const size_t o = (size_t)j * n + i;
-    T v = C.alpha * acc;
-    if (C.beta_mode) v = fma(C.beta, C.Cin[o], v);
-    C.Cout[o] = v;
+  // the epilogue of up to 8 outputs of one row: every Cin value loaded
+  // before the first store (Cin may be Cout: each output still reads its own
+  // old value first), so the loads are in flight together
+  auto out8 = [&](int i, int jbase, int m, const T* acc) {
+    T old[8];
+    if (C.beta_mode) {
+#pragma unroll
+      for (int c = 0; c < 8; ++c)
+        if (c < m) old[c] = C.Cin[(size_t)(jbase + c) * n + i];
+    }
+#pragma unroll
+    for (int c = 0; c < 8; ++c)
+      if (c < m) {
+        T v = C.alpha * acc[c];
+        if (C.beta_mode) v = fma(C.beta, old[c], v);
+        C.Cout[(size_t)(jbase + c) * n + i] = v;
+      }
   };
   if (C.side == 0) {
     // (op(S) X)(i, j) = sum over row i of op(S): v * X(k, j); cb columns of X
@@ -637,7 +649,22 @@ __device__ __forceinline__ void bspmm_body(const SpmmCtx<T>& C) {
     const int cb = max(1, min(8, SPMM_XCOLS / n));
     for (int jb = 0; jb < nc; jb += cb) {
       const int m = min(cb, nc - jb);
-      for (int e = t; e < m * n; e += 256) C.s_x[e] = C.X[(size_t)(j0 + jb) * n + e];
+      {
+        // staging: all of this thread's loads first, then the LDS writes
+        constexpr int PER = SPMM_XCOLS / 256;
+        T st[PER];
+        const int tot = m * n;
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+          const int e = t + 256 * u;
+          if (e < tot) st[u] = C.X[(size_t)(j0 + jb) * n + e];
+        }
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+          const int e = t + 256 * u;
+          if (e < tot) C.s_x[e] = st[u];
+        }
+      }
       __syncthreads();
       for (int i = t; i < n; i += 256) {
         T acc[8];
@@ -650,9 +677,7 @@ __device__ __forceinline__ void bspmm_body(const SpmmCtx<T>& C) {
           for (int c = 0; c < 8; ++c)
             if (c < m) acc[c] = fma(v, C.s_x[c * n + k], acc[c]);
         }
-#pragma unroll
-        for (int c = 0; c < 8; ++c)
-          if (c < m) out(i, j0 + jb + c, acc[c]);
+        out8(i, j0 + jb, m, acc);
       }
       __syncthreads();  // s_x is rewritten for the next block
     }
@@ -660,6 +685,7 @@ __device__ __forceinline__ void bspmm_body(const SpmmCtx<T>& C) {
     // (X op(S))(i, j) = sum over column j of op(S): X(i, k) * v; the X values of
     // SPMM_EB entries of each of SPMM_CB columns are loaded before their fmas
     for (int jb = 0; jb < nc; jb += SPMM_CB) {
+      const int m = min(SPMM_CB, nc - jb);
       for (int i = t; i < n; i += 256) {
         T xv[SPMM_CB][SPMM_EB];
         int qa[SPMM_CB], qe[SPMM_CB];
@@ -677,23 +703,34 @@ __device__ __forceinline__ void bspmm_body(const SpmmCtx<T>& C) {
             xv[c][u] = C.X[(size_t)k * n + i];
           }
         }
+        T acc[8];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) acc[c] = T(0);
 #pragma unroll
         for (int c = 0; c < SPMM_CB; ++c) {
-          if (jb + c >= nc) break;
-          T acc = T(0);
+          if (c >= m) break;
 #pragma unroll
           for (int u = 0; u < SPMM_EB; ++u)
-            if (qa[c] + u < qe[c]) acc = fma(xv[c][u], vv[qa[c] + u], acc);
-          for (int q = qa[c] + SPMM_EB; q < qe[c]; ++q) acc = fma(C.X[(size_t)ix[q] * n + i], vv[q], acc);
-          out(i, j0 + jb + c, acc);
+            if (qa[c] + u < qe[c]) acc[c] = fma(xv[c][u], vv[qa[c] + u], acc[c]);
+          // longer columns: the rest of the entries SPMM_EB at a time, loads first
+          for (int q0 = qa[c] + SPMM_EB; q0 < qe[c]; q0 += SPMM_EB) {
+            T xr[SPMM_EB];
+#pragma unroll
+            for (int u = 0; u < SPMM_EB; ++u) {
+              const int q = min(q0 + u, qe[c] - 1);
+              xr[u] = C.X[(size_t)ix[q] * n + i];
+            }
+#pragma unroll
+            for (int u = 0; u < SPMM_EB; ++u)
+              if (q0 + u < qe[c]) acc[c] = fma(xr[u], vv[q0 + u], acc[c]);
+          }
         }
+        out8(i, j0 + jb, m, acc);
       }
     }
   }
 }
 
-// at most 128 VGPRs and ~32 KB of LDS: four to five workgroups per CU, whose
-// load latencies overlap
 template <class T>
 __global__ __launch_bounds__(256, 4) void k_bspmm(const BSpmmJob<T>* __restrict__ jobs, const int3* __restrict__ tiles,
                                                   BScal sc) {
