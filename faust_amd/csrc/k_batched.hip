@@ -832,7 +832,18 @@ __global__ __launch_bounds__(1024) void k_bprox(const BProxJob<T>* __restrict__ 
       for (int shift = KO::BITS - 8; shift >= 0; shift -= 8) {
         for (int b = t; b < 256; b += 1024) hist[b] = 0;
         __syncthreads();
-        for (int64_t i = t; i < len; i += 1024) {
+        // eight entries per thread loaded before they are binned, so that the
+        // loads are in flight together
+        int64_t i0 = t;
+        for (; i0 + 7 * 1024 < len; i0 += 8 * 1024) {
+          K kv[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) kv[u] = KO::key(M[i0 + u * 1024]);
+#pragma unroll
+          for (int u = 0; u < 8; ++u)
+            if ((kv[u] & mask) == prefix) atomicAdd(&hist[(kv[u] >> shift) & 255], 1u);
+        }
+        for (int64_t i = i0; i < len; i += 1024) {
           const K key = KO::key(M[i]);
           if ((key & mask) == prefix) atomicAdd(&hist[(key >> shift) & 255], 1u);
         }
@@ -860,8 +871,7 @@ __global__ __launch_bounds__(1024) void k_bprox(const BProxJob<T>* __restrict__ 
         // the sum of squares is taken on the way
         if (t == 0) s_nkept = 0;
         __syncthreads();
-        for (int64_t i = t; i < len; i += 1024) {
-          const T x = M[i];
+        auto one = [&](int64_t i, T x) {
           const bool keep = KO::key(x) >= prefix;
           if (!keep) M[i] = T(0);
           const T y = keep ? x : T(0);
@@ -870,7 +880,18 @@ __global__ __launch_bounds__(1024) void k_bprox(const BProxJob<T>* __restrict__ 
             const int slot = atomicAdd(&s_nkept, 1);
             if (slot < KEEP_LIST) kept[slot] = (int)i;
           }
+        };
+        // eight loads in flight, then the entries in order (the order of the
+        // sum of squares is unchanged)
+        int64_t i0 = t;
+        for (; i0 + 7 * 1024 < len; i0 += 8 * 1024) {
+          T xv[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) xv[u] = M[i0 + u * 1024];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) one(i0 + u * 1024, xv[u]);
         }
+        for (int64_t i = i0; i < len; i += 1024) one(i, M[i]);
         have_ss = true;
       } else {
         const int64_t chunk = (len + 1023) / 1024;
