@@ -40,6 +40,8 @@ struct Batch {
     int njobs = 0;
     DBuf tiles;  // kind 8: (job, first column, columns) per workgroup
     int ntiles = 0;
+    int nmax = 0;  // kinds 6 / 7: the largest target edge of the jobs
+    int mc = 0;    // kinds 6 / 7: the factor count all jobs with factors share (1 or 2, edges <= 512), else 0
   };
   std::vector<Stage> stages;
   DBuf tiles32, tiles64, tilesM;
@@ -305,6 +307,15 @@ static void add_power_sp(Batch<T>& B, int f) {
   typename Batch<T>::Stage s;
   s.kind = 6;
   s.njobs = (int)h.size();
+  int mc = -1;
+  for (const auto& j : h) {
+    s.nmax = std::max(s.nmax, j.n);
+    if (j.m > 0) mc = (mc == -1 || mc == j.m) ? j.m : 0;
+  }
+  // FH_BATCH_POWER_REG=0 reads the rows from memory in every round instead
+  const char* reg = std::getenv("FH_BATCH_POWER_REG");
+  const bool reg_ok = reg == nullptr || std::string(reg) != "0";
+  s.mc = (mc == 1 || mc == 2) && s.nmax <= 512 && reg_ok ? mc : 0;
   s.jobs.alloc(sizeof(BPowSpJob<T>) * h.size());
   FH_HIP(hipMemcpy(s.jobs.p, h.data(), sizeof(BPowSpJob<T>) * h.size(), hipMemcpyHostToDevice));
   B.stages.push_back(std::move(s));
@@ -403,7 +414,7 @@ static void run_stage(Batch<T>& B, const typename Batch<T>::Stage& s, hipStream_
       break;
     case 6:
     case 7:
-      launch_bpower_sp<T>(s.jobs.template as<BPowSpJob<T>>(), s.njobs, B.maxit, B.thr, B.warm, st);
+      launch_bpower_sp<T>(s.jobs.template as<BPowSpJob<T>>(), s.njobs, s.nmax, s.mc, B.maxit, B.thr, B.warm, st);
       break;
     case 8:
       launch_bspmm<T>(s.jobs.template as<BSpmmJob<T>>(), s.tiles.template as<int3>(), s.ntiles, sc, st);

@@ -127,7 +127,10 @@ void launch_bsparse(const BSpBuildJob<T>* jobs, int njobs, hipStream_t st);
 template <class T>
 void launch_bspmm(const BSpmmJob<T>* jobs, const int3* tiles, int ntiles, const BScal& sc, hipStream_t st);
 template <class T>
-void launch_bpower_sp(const BPowSpJob<T>* jobs, int njobs, int maxit, double thr, int warm, hipStream_t st);
+// mc: 1 or 2 when every job has mc factors or none and nmax <= 512 (their rows
+// are then held in registers), else 0
+void launch_bpower_sp(const BPowSpJob<T>* jobs, int njobs, int nmax, int mc, int maxit, double thr, int warm,
+                      hipStream_t st);
 template <class T>
 void launch_blambda(const BLamJob<T>* jobs, int njobs, double* lambda, hipStream_t st);
 template <class T>

@@ -483,7 +483,8 @@ __global__ __launch_bounds__(256) void k_bsparse(const BSpBuildJob<T>* __restric
 // P^T x (F_0^T first, by compressed columns), z = P y (F_{m-1} first, by
 // compressed rows), lambda = <x, z>.  A product of m sparse factors costs
 // their entries per round instead of the n^2 of the dense product.  Vectors
-// and sums are double, entries widened on load.
+// and sums are double, entries widened on load.  Each output entry is one
+// fma chain over its row's entries in stored order, in every variant below.
 template <class T>
 __device__ __forceinline__ void bsp_apply(const BSpFac<T>& F, bool trans, const double* in, double* out, int n) {
   typedef const __attribute__((address_space(1))) int* GI;
@@ -514,6 +515,105 @@ __device__ __forceinline__ void bsp_apply(const BSpFac<T>& F, bool trans, const 
   __syncthreads();
 }
 
+// A factor descriptor through the scalar cache (it is uniform over the
+// workgroup); only the fields the products read.
+template <class T>
+__device__ __forceinline__ BSpFac<T> bsp_load(const __attribute__((address_space(4))) BSpFac<T>* p) {
+  BSpFac<T> f;
+  f.rp = p->rp;
+  f.ci = p->ci;
+  f.v = p->v;
+  f.cp = p->cp;
+  f.ri = p->ri;
+  f.cv = p->cv;
+  f.cap = p->cap;
+  f.klist = nullptr;
+  f.kcount = nullptr;
+  return f;
+}
+
+// Rows t and t + 256 (n <= 512) of one factor in one orientation: their
+// first BSP_REG entries, loaded once per job and kept in registers over all
+// the rounds; the rest of a longer row, entries [s, e) of the factor's
+// arrays, is read from memory.  A row with fewer entries is padded with
+// (-0) x in[BSP_PAD], where the slot BSP_PAD of every vector holds +0:
+// -0 x +0 = -0 and acc + (-0) == acc for every acc, so the padding leaves
+// each sum bit-identical.
+constexpr int BSP_REG = 4;
+constexpr int BSP_PAD = 512;
+template <class T>
+struct BspRows {
+  int s[2], e[2];
+  int k[2][BSP_REG];
+  T v[2][BSP_REG];
+  const __attribute__((address_space(1))) int* ix;
+  const __attribute__((address_space(1))) T* vv;
+};
+
+template <class T>
+__device__ __forceinline__ void bsp_rows_load(BspRows<T>& R, const BSpFac<T>& F, bool trans, int n) {
+  typedef const __attribute__((address_space(1))) int* GI;
+  typedef const __attribute__((address_space(1))) T* GV;
+  const GI p0 = (GI)(trans ? F.cp : F.rp);
+  R.ix = (GI)(trans ? F.ri : F.ci);
+  R.vv = (GV)(trans ? F.cv : F.v);
+  int a[2], b[2];
+#pragma unroll
+  for (int r = 0; r < 2; ++r) {
+    const int i = threadIdx.x + 256 * r;
+    a[r] = i < n ? p0[i] : 0;
+    b[r] = i < n ? p0[i + 1] : 0;
+  }
+#pragma unroll
+  for (int r = 0; r < 2; ++r)
+#pragma unroll
+    for (int u = 0; u < BSP_REG; ++u) {
+      const int q = max(min(a[r] + u, b[r] - 1), 0);
+      R.k[r][u] = R.ix[q];
+      R.v[r][u] = R.vv[q];
+    }
+#pragma unroll
+  for (int r = 0; r < 2; ++r) {
+#pragma unroll
+    for (int u = 0; u < BSP_REG; ++u)
+      if (a[r] + u >= b[r]) {
+        R.k[r][u] = BSP_PAD;
+        R.v[r][u] = -T(0);
+      }
+    R.s[r] = a[r] + BSP_REG;
+    R.e[r] = max(b[r], a[r] + BSP_REG);
+  }
+}
+
+template <class T>
+__device__ __forceinline__ void bsp_rows_apply(const BspRows<T>& R, const double* in, double* out, int n) {
+#pragma unroll
+  for (int r = 0; r < 2; ++r) {
+    const int i = threadIdx.x + 256 * r;
+    if (i < n) {
+      double acc = 0.0;
+#pragma unroll
+      for (int u = 0; u < BSP_REG; ++u) {
+        // an empty asm makes the value opaque here, so that its widening is
+        // done at each use instead of being hoisted out of the rounds (the
+        // widened copies would double the registers the rows take)
+        T vu = R.v[r][u];
+        asm volatile("" : "+v"(vu));
+        acc = fma((double)vu, in[R.k[r][u]], acc);
+      }
+      if (R.e[r] > R.s[r]) {
+        // the rest of a long row; its start is made opaque so that its
+        // addresses are formed here rather than kept over the rounds
+        int q = R.s[r];
+        asm volatile("" : "+v"(q));
+        for (; q < R.e[r]; ++q) acc = fma((double)R.vv[q], in[R.ix[q]], acc);
+      }
+      out[i] = acc;
+    }
+  }
+  __syncthreads();
+}
+
 // block sum for the sparse power iteration: lane shuffles inside each wave,
 // then the four wave sums in order (a fixed order: the same result every run)
 __device__ __forceinline__ double block_sum_w(double v, double* red) {
@@ -525,16 +625,22 @@ __device__ __forceinline__ double block_sum_w(double v, double* red) {
   return r;
 }
 
-template <class T>
-__global__ __launch_bounds__(256) void k_bpower_sp(const BPowSpJob<T>* __restrict__ jobs, int maxit, double thr,
-                                                   int warm) {
-  __shared__ double x[BATCH_NMAX], u[BATCH_NMAX], w[BATCH_NMAX];
-  __shared__ double red[256];
-  const BPowSpJob<T> J = jobs[blockIdx.x];
-  const int n = J.n, t = threadIdx.x, m = J.m;
-  if (m == 0) {
-    if (t == 0) J.out[J.target] = 1.0;
-    return;
+// The rounds of one job.  MC > 0: m == MC factors of edge n <= 512 with their
+// rows in registers (cols[c] for P^T, rows[c] for P); MC == 0: any m, rows
+// read from memory each round.
+template <class T, int MC>
+__device__ __forceinline__ void bpower_sp_job(const BPowSpJob<T>& J, int maxit, double thr, int warm, double* x,
+                                              double* u, double* w, double* red) {
+  const int n = J.n, t = threadIdx.x, m = MC > 0 ? MC : J.m;
+  const __attribute__((address_space(4))) BSpFac<T>* fac = (const __attribute__((address_space(4))) BSpFac<T>*)J.fac;
+  BspRows<T> cols[MC > 0 ? MC : 1], rows[MC > 0 ? MC : 1];
+  if (MC > 0) {
+#pragma unroll
+    for (int c = 0; c < (MC > 0 ? MC : 1); ++c) {
+      const BSpFac<T> F = bsp_load(fac + c);
+      bsp_rows_load(cols[c], F, true, n);
+      bsp_rows_load(rows[c], F, false, n);
+    }
   }
   for (int i = t; i < n; i += 256) x[i] = J.vec[i];
   __syncthreads();
@@ -554,19 +660,33 @@ __global__ __launch_bounds__(256) void k_bpower_sp(const BPowSpJob<T>* __restric
     const double inv = 1.0 / nrm;
     for (int i = t; i < n; i += 256) x[i] = x[i] * inv;
     __syncthreads();
-    // y = P^T x = F_{m-1}^T ... F_0^T x
     const double* cur = x;
     double* nxt = u;
-    for (int c = 0; c < m; ++c) {
-      bsp_apply(J.fac[c], true, cur, nxt, n);
-      cur = nxt;
-      nxt = (nxt == u) ? w : u;
-    }
-    // z = P y = F_0 ... F_{m-1} y
-    for (int c = m - 1; c >= 0; --c) {
-      bsp_apply(J.fac[c], false, cur, nxt, n);
-      cur = nxt;
-      nxt = (nxt == u) ? w : u;
+    if (MC > 0) {
+      // y = P^T x = F_{m-1}^T ... F_0^T x, then z = P y = F_0 ... F_{m-1} y
+#pragma unroll
+      for (int c = 0; c < (MC > 0 ? MC : 1); ++c) {
+        bsp_rows_apply(cols[c], cur, nxt, n);
+        cur = nxt;
+        nxt = (nxt == u) ? w : u;
+      }
+#pragma unroll
+      for (int c = (MC > 0 ? MC : 1) - 1; c >= 0; --c) {
+        bsp_rows_apply(rows[c], cur, nxt, n);
+        cur = nxt;
+        nxt = (nxt == u) ? w : u;
+      }
+    } else {
+      for (int c = 0; c < m; ++c) {
+        bsp_apply<T>(bsp_load(fac + c), true, cur, nxt, n);
+        cur = nxt;
+        nxt = (nxt == u) ? w : u;
+      }
+      for (int c = m - 1; c >= 0; --c) {
+        bsp_apply<T>(bsp_load(fac + c), false, cur, nxt, n);
+        cur = nxt;
+        nxt = (nxt == u) ? w : u;
+      }
     }
     double dot = 0.0;
     int nz = 0;
@@ -586,6 +706,37 @@ __global__ __launch_bounds__(256) void k_bpower_sp(const BPowSpJob<T>* __restric
   if (warm && !zero)
     for (int i = t; i < n; i += 256) J.vec[i] = x[i];
   if (t == 0) J.out[J.target] = sqrt(fabs(lambda));
+}
+
+// Any job: rows read from memory each round, vectors in dynamic LDS.
+template <class T>
+__global__ __launch_bounds__(256) void k_bpower_sp(const BPowSpJob<T>* __restrict__ jobs, int maxit, double thr,
+                                                   int warm, int nmax) {
+  extern __shared__ double lds[];  // x, u, w: nmax doubles each
+  __shared__ double red[4];
+  const BPowSpJob<T> J = jobs[blockIdx.x];
+  if (J.m == 0) {
+    if (threadIdx.x == 0) J.out[J.target] = 1.0;
+    return;
+  }
+  bpower_sp_job<T, 0>(J, maxit, thr, warm, lds, lds + nmax, lds + 2 * nmax, red);
+}
+
+// Jobs of MC factors (or none) of edge n <= 512: rows in registers, vectors
+// at fixed LDS offsets (so that every gather is one LDS read at a constant
+// offset from the entry's index).
+template <class T, int MC>
+__global__ __launch_bounds__(256, 4) void k_bpower_sp_reg(const BPowSpJob<T>* __restrict__ jobs, int maxit, double thr,
+                                                       int warm) {
+  __shared__ double x[BSP_PAD + 2], u[BSP_PAD + 2], w[BSP_PAD + 2];
+  __shared__ double red[4];
+  const BPowSpJob<T> J = jobs[blockIdx.x];
+  if (J.m == 0) {
+    if (threadIdx.x == 0) J.out[J.target] = 1.0;
+    return;
+  }
+  if (threadIdx.x == 0) x[BSP_PAD] = u[BSP_PAD] = w[BSP_PAD] = 0.0;
+  bpower_sp_job<T, MC>(J, maxit, thr, warm, x, u, w, red);
 }
 
 // Batched sparse x dense products (BSpmmJob): a workgroup computes the
@@ -1096,9 +1247,18 @@ void launch_bsparse(const BSpBuildJob<T>* jobs, int njobs, hipStream_t st) {
 }
 
 template <class T>
-void launch_bpower_sp(const BPowSpJob<T>* jobs, int njobs, int maxit, double thr, int warm, hipStream_t st) {
+void launch_bpower_sp(const BPowSpJob<T>* jobs, int njobs, int nmax, int mc, int maxit, double thr, int warm,
+                      hipStream_t st) {
   if (njobs == 0) return;
-  hipLaunchKernelGGL(k_bpower_sp<T>, dim3(njobs), dim3(256), 0, st, jobs, maxit, thr, warm);
+  if (nmax < 1 || nmax > BATCH_NMAX) throw Error(FH_ERR_ARG, "sparse power iteration: target edge out of range");
+  if (mc != 0 && nmax > 512) throw Error(FH_ERR_ARG, "sparse power iteration: register rows need edges <= 512");
+  if (mc == 1)
+    hipLaunchKernelGGL((k_bpower_sp_reg<T, 1>), dim3(njobs), dim3(256), 0, st, jobs, maxit, thr, warm);
+  else if (mc == 2)
+    hipLaunchKernelGGL((k_bpower_sp_reg<T, 2>), dim3(njobs), dim3(256), 0, st, jobs, maxit, thr, warm);
+  else
+    hipLaunchKernelGGL(k_bpower_sp<T>, dim3(njobs), dim3(256), (size_t)3 * nmax * sizeof(double), st, jobs, maxit,
+                       thr, warm, nmax);
   FH_LAUNCH_CHECK();
 }
 
@@ -1157,7 +1317,7 @@ void launch_bcopy(T* dst, const T* src, int64_t n, hipStream_t st) {
   template void launch_bprox<T>(const BProxJob<T>*, int, int, hipStream_t);                                 \
   template void launch_bsparse<T>(const BSpBuildJob<T>*, int, hipStream_t);                                 \
   template void launch_bspmm<T>(const BSpmmJob<T>*, const int3*, int, const BScal&, hipStream_t);           \
-  template void launch_bpower_sp<T>(const BPowSpJob<T>*, int, int, double, int, hipStream_t);               \
+  template void launch_bpower_sp<T>(const BPowSpJob<T>*, int, int, int, int, double, int, hipStream_t);    \
   template void launch_blambda<T>(const BLamJob<T>*, int, double*, hipStream_t);                            \
   template void launch_brelerr<T>(const BLamJob<T>*, int, const double*, double*, hipStream_t);             \
   template void launch_bcopy<T>(T*, const T*, int64_t, hipStream_t);

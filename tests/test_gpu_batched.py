@@ -128,3 +128,25 @@ def test_batched_fp32_sparse_step_sizes_track_dense(gpu, monkeypatch):
         rel[mode] = np.asarray(res.relerr)
     assert np.all(np.isfinite(rel["1"])) and np.all(rel["1"] < 1.0)
     assert np.all(np.abs(rel["1"] - rel["0"]) <= 1e-3 * rel["0"] + 1e-6), (rel["1"], rel["0"])
+
+
+@pytest.mark.parametrize("sizes", [[64, 96, 256, 512, 200], [512, 1024, 64]])
+def test_batched_sparse_power_register_rows_same_bits(gpu, monkeypatch, sizes):
+    # edges <= 512: the sparse power iterations keep each thread's rows in
+    # registers (short rows padded with -0 x +0 terms); FH_BATCH_POWER_REG=0
+    # reads them from memory every round.  The two run the same fma chains,
+    # so factors, lambdas and errors must agree bit for bit.  With an edge
+    # above 512 both settings take the memory path.
+    As = _targets(11, sizes)
+    ks = [4 * n for n in sizes]
+    out = {}
+    for mode in ("1", "0"):
+        monkeypatch.setenv("FH_BATCH_POWER_REG", mode)
+        out[mode] = palm4msa_batched(As, 3, k=ks, dtype="f32", num_its=30, refine_its=0, autotune=False)
+    a, b = out["1"], out["0"]
+    assert np.array_equal(np.asarray(a.relerr), np.asarray(b.relerr))
+    assert np.array_equal(np.asarray(a.lambdas), np.asarray(b.lambdas))
+    assert np.all(np.asarray(a.relerr) < 1.0)
+    for t in range(len(sizes)):
+        for f in range(3):
+            assert np.array_equal(a.factors[t][f], b.factors[t][f]), (t, f)
