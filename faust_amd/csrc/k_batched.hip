@@ -39,14 +39,26 @@ __device__ __forceinline__ T job_alpha(int mode, int b, const BScal& s) {
   return isfinite(a) ? a : T(0);  // beyond the FP32 range: no step rather than inf
 }
 
+// Workgroup -> tile.  The hardware deals workgroups to the 8 XCDs in turn
+// (workgroup b to XCD b % 8); with per = ceil(ntiles / 8), XCD x gets the
+// contiguous tiles [x per, (x + 1) per), so that the tiles of one job, which
+// read the same operands, mostly share one XCD's L2.  per = 0 keeps tile =
+// workgroup.  Workgroups past the last tile return at once.
+__device__ __forceinline__ int xcd_tile(int per) {
+  const int b = blockIdx.x;
+  return per > 0 ? (b & 7) * per + (b >> 3) : b;
+}
+
 template <class T, int TILE>
 __global__ __launch_bounds__(256) void k_bgemm(const BGemmJob<T>* __restrict__ jobs, const int3* __restrict__ tiles,
-                                               BScal sc) {
+                                               BScal sc, int ntiles, int per) {
   constexpr int MT = TILE / 16;  // micro-tile edge per thread
   constexpr int KS = 16;
   __shared__ T As[KS][TILE + 1];
   __shared__ T Bs[KS][TILE + 1];
-  const int3 tl = tiles[blockIdx.x];
+  const int tix = xcd_tile(per);
+  if (tix >= ntiles) return;
+  const int3 tl = tiles[tix];
   const BGemmJob<T> J = jobs[tl.x];
   const int t = threadIdx.x, tx = t & 15, ty = t >> 4;
   const int i0 = tl.y * TILE, j0 = tl.z * TILE;
@@ -111,11 +123,14 @@ __global__ __launch_bounds__(256) void k_bgemm(const BGemmJob<T>* __restrict__ j
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 __global__ __launch_bounds__(256) void k_bgemm_mfma_f32(const BGemmJob<float>* __restrict__ jobs,
-                                                        const int3* __restrict__ tiles, BScal sc) {
+                                                        const int3* __restrict__ tiles, BScal sc, int ntiles,
+                                                        int per) {
   constexpr int TILE = 64, KS = 32;
   __shared__ float As[KS][TILE + 4];
   __shared__ float Bs[KS][TILE + 4];
-  const int3 tl = tiles[blockIdx.x];
+  const int tix = xcd_tile(per);
+  if (tix >= ntiles) return;
+  const int3 tl = tiles[tix];
   const BGemmJob<float> J = jobs[tl.x];
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int wr = wave & 1, wc = wave >> 1;
@@ -884,13 +899,15 @@ __device__ __forceinline__ void bspmm_body(const SpmmCtx<T>& C) {
 
 template <class T>
 __global__ __launch_bounds__(256, 4) void k_bspmm(const BSpmmJob<T>* __restrict__ jobs, const int3* __restrict__ tiles,
-                                                  BScal sc) {
+                                                  BScal sc, int ntiles, int per) {
   constexpr int ENT = SPMM_LDS_BYTES / (sizeof(int) + sizeof(T));
   __shared__ int s_p[BATCH_NMAX + 1];
   __shared__ int s_ix[ENT];
   __shared__ T s_v[ENT];
   __shared__ T s_x[SPMM_XCOLS];
-  const int3 tl = tiles[blockIdx.x];
+  const int tix = xcd_tile(per);
+  if (tix >= ntiles) return;
+  const int3 tl = tiles[tix];
   const BSpmmJob<T> J = jobs[tl.x];
   const int n = J.n, t = threadIdx.x;
   const bool rows_of_S = (J.side == 0) != (J.transS != 0);  // the entry lists are rows of S (CSR), else columns (CSC)
@@ -1206,12 +1223,23 @@ void launch_mask_nonzero_f32(double* mask, const float* src, int64_t n, hipStrea
 }
 
 // ---------------------------------------------------------------- host launchers
+// tiles per XCD for xcd_tile, or 0 (tile = workgroup) under FH_BATCH_XCD=0
+static int xcd_per(int ntiles) {
+  static const bool on = [] {
+    const char* e = std::getenv("FH_BATCH_XCD");
+    return e == nullptr || std::string(e) != "0";
+  }();
+  return on ? (ntiles + 7) / 8 : 0;
+}
+
 template <class T>
 void launch_bgemm(const BGemmJob<T>* jobs, const int3* tiles, int ntiles, int tile, const BScal& sc, hipStream_t st) {
   if (ntiles == 0) return;
+  const int per = xcd_per(ntiles);
+  const unsigned grid = per > 0 ? 8u * per : (unsigned)ntiles;
   if (tile == BGEMM_MFMA) {
     if constexpr (std::is_same<T, float>::value) {
-      hipLaunchKernelGGL(k_bgemm_mfma_f32, dim3(ntiles), dim3(256), 0, st, jobs, tiles, sc);
+      hipLaunchKernelGGL(k_bgemm_mfma_f32, dim3(grid), dim3(256), 0, st, jobs, tiles, sc, ntiles, per);
       FH_LAUNCH_CHECK();
       return;
     } else {
@@ -1219,9 +1247,9 @@ void launch_bgemm(const BGemmJob<T>* jobs, const int3* tiles, int ntiles, int ti
     }
   }
   if (tile == 32)
-    hipLaunchKernelGGL((k_bgemm<T, 32>), dim3(ntiles), dim3(256), 0, st, jobs, tiles, sc);
+    hipLaunchKernelGGL((k_bgemm<T, 32>), dim3(grid), dim3(256), 0, st, jobs, tiles, sc, ntiles, per);
   else
-    hipLaunchKernelGGL((k_bgemm<T, 64>), dim3(ntiles), dim3(256), 0, st, jobs, tiles, sc);
+    hipLaunchKernelGGL((k_bgemm<T, 64>), dim3(grid), dim3(256), 0, st, jobs, tiles, sc, ntiles, per);
   FH_LAUNCH_CHECK();
 }
 
@@ -1235,7 +1263,9 @@ void launch_bpower(const BPowJob<T>* jobs, int njobs, int maxit, double thr, int
 template <class T>
 void launch_bspmm(const BSpmmJob<T>* jobs, const int3* tiles, int ntiles, const BScal& sc, hipStream_t st) {
   if (ntiles == 0) return;
-  hipLaunchKernelGGL(k_bspmm<T>, dim3(ntiles), dim3(256), 0, st, jobs, tiles, sc);
+  const int per = xcd_per(ntiles);
+  const unsigned grid = per > 0 ? 8u * per : (unsigned)ntiles;
+  hipLaunchKernelGGL(k_bspmm<T>, dim3(grid), dim3(256), 0, st, jobs, tiles, sc, ntiles, per);
   FH_LAUNCH_CHECK();
 }
 
