@@ -809,12 +809,28 @@ __device__ __forceinline__ void bspmm_body(const SpmmCtx<T>& C) {
         C.Cout[(size_t)(jbase + c) * n + i] = v;
       }
   };
+  // the same with the old values already loaded (issued with the operand
+  // loads, so that their latency is not paid again in the epilogue)
+  auto out_pre = [&](int i, int jbase, int m, const T* acc, const T* old) {
+#pragma unroll
+    for (int c = 0; c < 8; ++c)
+      if (c < m) {
+        T v = C.alpha * acc[c];
+        if (C.beta_mode) v = fma(C.beta, old[c], v);
+        C.Cout[(size_t)(jbase + c) * n + i] = v;
+      }
+  };
   if (C.side == 0) {
     // (op(S) X)(i, j) = sum over row i of op(S): v * X(k, j); cb columns of X
     // staged in LDS at a time, a row's entries read once for all of them
     const int cb = max(1, min(8, SPMM_XCOLS / n));
+    // the old values of this thread's outputs in the block are loaded with the
+    // staging when they fit 8 slots: one row (n <= 256, up to 8 columns) or
+    // two rows of up to 4 columns (n <= 512 with blocks of 4 columns)
+    const bool pre = C.beta_mode && (n <= 256 || (n <= 512 && cb <= 4));
     for (int jb = 0; jb < nc; jb += cb) {
       const int m = min(cb, nc - jb);
+      T old[8];
       {
         // staging: all of this thread's loads first, then the LDS writes
         constexpr int PER = SPMM_XCOLS / 256;
@@ -825,6 +841,20 @@ __device__ __forceinline__ void bspmm_body(const SpmmCtx<T>& C) {
           const int e = t + 256 * u;
           if (e < tot) st[u] = C.X[(size_t)(j0 + jb) * n + e];
         }
+        if (pre) {
+#pragma unroll
+          for (int r = 0; r < 2; ++r)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+              // slot r * 4 + c: row t + 256 r, column c (m <= 4 when there are two rows)
+              const int i = t + 256 * r;
+              if (i < n && c < m) old[r * 4 + c] = C.Cin[(size_t)(j0 + jb + c) * n + i];
+            }
+          if (n <= 256)
+#pragma unroll
+            for (int c = 4; c < 8; ++c)
+              if (t < n && c < m) old[c] = C.Cin[(size_t)(j0 + jb + c) * n + t];
+        }
 #pragma unroll
         for (int u = 0; u < PER; ++u) {
           const int e = t + 256 * u;
@@ -832,6 +862,26 @@ __device__ __forceinline__ void bspmm_body(const SpmmCtx<T>& C) {
         }
       }
       __syncthreads();
+      if (pre) {
+        // at most two rows per thread here
+        for (int r = 0; r < 2; ++r) {
+          const int i = t + 256 * r;
+          if (i >= n) break;
+          T acc[8];
+#pragma unroll
+          for (int c = 0; c < 8; ++c) acc[c] = T(0);
+          for (int q = C.s_p[i], qe = C.s_p[i + 1]; q < qe; ++q) {
+            const int k = ix[q];
+            const T v = vv[q];
+#pragma unroll
+            for (int c = 0; c < 8; ++c)
+              if (c < m) acc[c] = fma(v, C.s_x[c * n + k], acc[c]);
+          }
+          out_pre(i, j0 + jb, m, acc, old + 4 * r);
+        }
+        __syncthreads();  // s_x is rewritten for the next block
+        continue;
+      }
       for (int i = t; i < n; i += 256) {
         T acc[8];
 #pragma unroll
@@ -869,6 +919,13 @@ __device__ __forceinline__ void bspmm_body(const SpmmCtx<T>& C) {
             xv[c][u] = C.X[(size_t)k * n + i];
           }
         }
+        // the old values of the outputs, loaded with the X values
+        T old[8];
+        if (C.beta_mode) {
+#pragma unroll
+          for (int c = 0; c < SPMM_CB; ++c)
+            if (c < m) old[c] = C.Cin[(size_t)(j0 + jb + c) * n + i];
+        }
         T acc[8];
 #pragma unroll
         for (int c = 0; c < 8; ++c) acc[c] = T(0);
@@ -891,7 +948,7 @@ __device__ __forceinline__ void bspmm_body(const SpmmCtx<T>& C) {
               if (q0 + u < qe[c]) acc[c] = fma(xr[u], vv[q0 + u], acc[c]);
           }
         }
-        out8(i, j0 + jb, m, acc);
+        out_pre(i, j0 + jb, m, acc, old);
       }
     }
   }

@@ -117,8 +117,9 @@ def test_batched_fp32_sparse_step_sizes_track_dense(gpu, monkeypatch):
     # against the same iterations on the dense partial products: the two
     # differ only by the FP32 rounding of the dense products, so after 60
     # iterations the FP64-verified errors agree closely and every target
-    # converges the same way; sizes with rows of up to 2048 / 256 threads
-    sizes = [64, 128, 96, 256, 64, 512]
+    # converges the same way; sizes with rows of up to 2048 / 256 threads,
+    # and 300 (sparse x dense blocks of 6 columns over two rows per thread)
+    sizes = [64, 128, 96, 256, 64, 512, 300]
     As = _targets(5, sizes)
     ks = [4 * n for n in sizes]
     rel = {}
