@@ -640,6 +640,20 @@ __device__ __forceinline__ double block_sum_w(double v, double* red) {
   return r;
 }
 
+// two block sums at once (each in the order of block_sum_w); red: 8 doubles
+__device__ __forceinline__ void block_sum_w2(double& a, double& b, double* red) {
+  a = wave_sum_d(a);
+  b = wave_sum_d(b);
+  if ((threadIdx.x & 63) == 0) {
+    red[threadIdx.x >> 6] = a;
+    red[4 + (threadIdx.x >> 6)] = b;
+  }
+  __syncthreads();
+  a = (red[0] + red[1]) + (red[2] + red[3]);
+  b = (red[4] + red[5]) + (red[6] + red[7]);
+  __syncthreads();
+}
+
 // The rounds of one job.  MC > 0: m == MC factors of edge n <= 512 with their
 // rows in registers (cols[c] for P^T, rows[c] for P); MC == 0: any m, rows
 // read from memory each round.
@@ -659,21 +673,31 @@ __device__ __forceinline__ void bpower_sp_job(const BPowSpJob<T>& J, int maxit, 
   }
   for (int i = t; i < n; i += 256) x[i] = J.vec[i];
   __syncthreads();
+  // Each round: x = src / ||src||, z = P P^T x, lambda = <x, z>, and the
+  // next round's src = z.  The sum of squares of z is taken in the pass of
+  // the dot product and reduced with it, and z is normalised straight from
+  // its buffer: the same sums in the same order as normalising a copy of z
+  // at the start of the next round.
+  double sstot;
+  {
+    double ss = 0.0;
+    for (int i = t; i < n; i += 256) ss = fma(x[i], x[i], ss);
+    sstot = block_sum_w(ss, red);
+  }
+  const double* src = x;
   double lambda_old = 1.0, lambda = 0.0;
   int it = 0;
   bool zero = false;
   while ((fabs(lambda_old - lambda) > fabs(thr) || fabs(lambda) <= fabs(thr)) && it < maxit) {
     ++it;
     lambda_old = lambda;
-    double ss = 0.0;
-    for (int i = t; i < n; i += 256) ss = fma(x[i], x[i], ss);
-    const double nrm = sqrt(block_sum_w(ss, red));
+    const double nrm = sqrt(sstot);
     if (nrm == 0.0) {
       zero = true;
       break;
     }
     const double inv = 1.0 / nrm;
-    for (int i = t; i < n; i += 256) x[i] = x[i] * inv;
+    for (int i = t; i < n; i += 256) x[i] = src[i] * inv;
     __syncthreads();
     const double* cur = x;
     double* nxt = u;
@@ -703,23 +727,26 @@ __device__ __forceinline__ void bpower_sp_job(const BPowSpJob<T>& J, int maxit, 
         nxt = (nxt == u) ? w : u;
       }
     }
-    double dot = 0.0;
+    double dot = 0.0, ss = 0.0;
     int nz = 0;
     for (int i = t; i < n; i += 256) {
-      dot = fma(x[i], cur[i], dot);
-      nz |= (cur[i] != 0.0);
+      const double z = cur[i];
+      dot = fma(x[i], z, dot);
+      ss = fma(z, z, ss);
+      nz |= (z != 0.0);
     }
-    lambda = block_sum_w(dot, red);
+    block_sum_w2(dot, ss, red);
+    lambda = dot;
+    sstot = ss;
     if (!__syncthreads_or(nz)) {
       lambda = 0.0;
       zero = true;
       break;
     }
-    for (int i = t; i < n; i += 256) x[i] = cur[i];
-    __syncthreads();
+    src = cur;
   }
   if (warm && !zero)
-    for (int i = t; i < n; i += 256) J.vec[i] = x[i];
+    for (int i = t; i < n; i += 256) J.vec[i] = src[i];
   if (t == 0) J.out[J.target] = sqrt(fabs(lambda));
 }
 
@@ -728,7 +755,7 @@ template <class T>
 __global__ __launch_bounds__(256) void k_bpower_sp(const BPowSpJob<T>* __restrict__ jobs, int maxit, double thr,
                                                    int warm, int nmax) {
   extern __shared__ double lds[];  // x, u, w: nmax doubles each
-  __shared__ double red[4];
+  __shared__ double red[8];
   const BPowSpJob<T> J = jobs[blockIdx.x];
   if (J.m == 0) {
     if (threadIdx.x == 0) J.out[J.target] = 1.0;
@@ -744,7 +771,7 @@ template <class T, int MC>
 __global__ __launch_bounds__(256, 4) void k_bpower_sp_reg(const BPowSpJob<T>* __restrict__ jobs, int maxit, double thr,
                                                        int warm) {
   __shared__ double x[BSP_PAD + 2], u[BSP_PAD + 2], w[BSP_PAD + 2];
-  __shared__ double red[4];
+  __shared__ double red[8];
   const BPowSpJob<T> J = jobs[blockIdx.x];
   if (J.m == 0) {
     if (threadIdx.x == 0) J.out[J.target] = 1.0;
