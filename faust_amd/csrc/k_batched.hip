@@ -1046,11 +1046,14 @@ template <> struct KeyOf<double> {
   __device__ static K key(double v) { return (unsigned long long)__double_as_longlong(v) & 0x7fffffffffffffffull; }
 };
 
+constexpr int BSEL_BITS = 11;  // radix-select digit width (bins in LDS: 2048)
+
 template <class T, int MODE>  // MODE 0: SP, 1: SUPP
 __global__ __launch_bounds__(1024) void k_bprox(const BProxJob<T>* __restrict__ jobs) {
   using KO = KeyOf<T>;
   using K = typename KO::K;
-  __shared__ unsigned hist[256];
+  __shared__ unsigned hist[1 << BSEL_BITS];
+  __shared__ unsigned wtot[16];
   __shared__ T red[1024];
   __shared__ unsigned scan[1024];
   __shared__ K s_prefix;
@@ -1081,8 +1084,13 @@ __global__ __launch_bounds__(1024) void k_bprox(const BProxJob<T>* __restrict__ 
     } else {
       K prefix = 0, mask = 0;
       long long kk = J.k;  // rank still to find among keys matching prefix
-      for (int shift = KO::BITS - 8; shift >= 0; shift -= 8) {
-        for (int b = t; b < 256; b += 1024) hist[b] = 0;
+      // digits of up to BSEL_BITS bits from the top: 3 passes for FP32 keys,
+      // 6 for FP64
+      for (int hi = KO::BITS; hi > 0; hi -= BSEL_BITS) {
+        const int shift = hi > BSEL_BITS ? hi - BSEL_BITS : 0;
+        const int nb = 1 << (hi - shift);
+        const K dmask = (K)(nb - 1);
+        for (int b = t; b < nb; b += 1024) hist[b] = 0;
         __syncthreads();
         // eight entries per thread loaded before they are binned, so that the
         // loads are in flight together
@@ -1093,17 +1101,43 @@ __global__ __launch_bounds__(1024) void k_bprox(const BProxJob<T>* __restrict__ 
           for (int u = 0; u < 8; ++u) kv[u] = KO::key(M[i0 + u * 1024]);
 #pragma unroll
           for (int u = 0; u < 8; ++u)
-            if ((kv[u] & mask) == prefix) atomicAdd(&hist[(kv[u] >> shift) & 255], 1u);
+            if ((kv[u] & mask) == prefix) atomicAdd(&hist[(kv[u] >> shift) & dmask], 1u);
         }
         for (int64_t i = i0; i < len; i += 1024) {
           const K key = KO::key(M[i]);
-          if ((key & mask) == prefix) atomicAdd(&hist[(key >> shift) & 255], 1u);
+          if ((key & mask) == prefix) atomicAdd(&hist[(key >> shift) & dmask], 1u);
         }
         __syncthreads();
+        // the bin holding the kk-th largest key: thread t owns bins
+        // [lo, lo + per); suffix sums over the threads (lane shuffles, then
+        // the 16 wave totals) give each thread the count above its bins, and
+        // the one thread whose bins hold rank kk walks them from the top
+        const int per = nb >= 1024 ? nb / 1024 : 1;
+        const int lo = t * per;
+        unsigned own = 0;
+        if (lo < nb)
+          for (int q = 0; q < per; ++q) own += hist[lo + q];
+        unsigned v = own;
+        for (int off = 1; off < 64; off <<= 1) {
+          const unsigned o = __shfl_down(v, off);
+          if ((t & 63) + off < 64) v += o;
+        }
+        if ((t & 63) == 0) wtot[t >> 6] = v;
         if (t == 0) {
-          long long cum = 0;
-          int b = 255;
-          for (; b > 0; --b) {
+          // the bin search below always finds the bin (kk is at least 1 and at
+          // most the keys matching prefix); these values are only a default
+          s_kk = kk;
+          s_prefix = prefix;
+          s_eq = hist[0];
+        }
+        __syncthreads();
+        unsigned above = 0;
+        for (int w2 = (t >> 6) + 1; w2 < 16; ++w2) above += wtot[w2];
+        const long long incl = (long long)v + above, excl = incl - own;
+        if (lo < nb && excl < kk && kk <= incl) {
+          long long cum = excl;
+          int b = lo + per - 1;
+          for (; b > lo; --b) {
             if (cum + hist[b] >= kk) break;
             cum += hist[b];
           }
@@ -1114,7 +1148,7 @@ __global__ __launch_bounds__(1024) void k_bprox(const BProxJob<T>* __restrict__ 
         __syncthreads();
         kk = s_kk;
         prefix = s_prefix;
-        mask |= ((K)255 << shift);
+        mask |= (dmask << shift);
         __syncthreads();
       }
       // keys > prefix are kept; kk entries equal to prefix are kept, lowest index first
